@@ -1,0 +1,138 @@
+"""ORACLE — test infrastructure only.
+
+ctypes wrapper over oracle/build/liboracle2d.so, the CPU restatement of the
+reference's 2D elastic P-SV forward path (oracle/cpu_ref2d.cpp).  Only
+tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may import
+this module, and only as the checker / CPU baseline; the specfempp_amd
+package never does.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle2d.so")
+_lib = None
+
+SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
+
+
+def build(force: bool = False) -> str:
+    """Compiles the oracle with make (g++, no GPU needed)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
+    return _LIB_PATH
+
+
+def _load():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        build()
+    lib = ctypes.CDLL(_LIB_PATH)
+    P = ctypes.c_void_p
+    i, f, d = ctypes.c_int, ctypes.c_float, ctypes.c_double
+    lib.oracle2d_create.restype = P
+    lib.oracle2d_create.argtypes = [i, i, i, i, P, P, P, P, P, i, P, P, f, i, i, P, P, P, P, P, P, i, P, P]
+    lib.oracle2d_destroy.argtypes = [P]
+    lib.oracle2d_nglob.argtypes = [P]
+    lib.oracle2d_t0.argtypes = [P]
+    lib.oracle2d_t0.restype = d
+    lib.oracle2d_run_forward.argtypes = [P, i, i, i, i, P, P]
+    lib.oracle2d_get.argtypes = [P, i, P]
+    lib.oracle2d_get.restype = ctypes.c_long
+    lib.oracle2d_set_field.argtypes = [P, i, P]
+    lib.oracle2d_apply_stiffness.argtypes = [P, i]
+    lib.oracle2d_last_error.restype = ctypes.c_char_p
+    _lib = lib
+    return lib
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None
+
+
+GET = {"minv": 0, "u": 1, "v": 2, "a": 3, "ibool": 4, "jacobian": 5, "xix": 6, "xiz": 7,
+       "gammax": 8, "gammaz": 9, "xi": 10, "w": 11, "hprime": 12, "stf": 13,
+       "source_array": 14, "receiver_h": 15, "source_ispec": 16, "receiver_ispec": 17,
+       "receiver_xi": 18, "receiver_gamma": 19, "stacey_flag": 20, "stacey_n0": 21,
+       "stacey_n1": 22, "stacey_w": 23}
+
+
+class Oracle2D:
+    """One forward simulation on the CPU (precision 4 = FP32, 8 = FP64)."""
+
+    def __init__(self, mesh, sources, receivers, dt, nstep, precision=4):
+        lib = _load()
+        self._keep = []
+
+        def arr(x, dt_):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dt_))
+            self._keep.append(a)
+            return a
+
+        coord = arr(mesh.coord, np.float64)
+        knods = arr(mesh.knods, np.int32)
+        kappa, mu, rho = (arr(mesh.kappa, np.float32), arr(mesh.mu, np.float32),
+                          arr(mesh.rho, np.float32))
+        ai, at = arr(mesh.abs_ispec, np.int32), arr(mesh.abs_type, np.int32)
+        sx, sz = arr(sources.x, np.float64), arr(sources.z, np.float64)
+        ang, f0 = arr(sources.angle, np.float32), arr(sources.f0, np.float32)
+        tsh, fac = arr(sources.tshift, np.float32), arr(sources.factor, np.float32)
+        rx, rz = arr(receivers.x, np.float64), arr(receivers.z, np.float64)
+        self.nrec = len(rx)
+        self.nstep = int(nstep)
+        self.h = lib.oracle2d_create(precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
+                                     _ptr(coord), _ptr(knods), _ptr(kappa), _ptr(mu), _ptr(rho),
+                                     len(ai), _ptr(ai), _ptr(at), float(dt), int(nstep),
+                                     len(sx), _ptr(sx), _ptr(sz), _ptr(ang), _ptr(f0), _ptr(tsh),
+                                     _ptr(fac), self.nrec, _ptr(rx), _ptr(rz))
+        if not self.h:
+            raise RuntimeError("oracle2d_create failed: " + lib.oracle2d_last_error().decode())
+        self.lib = lib
+        self.istep = 0
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle2d_destroy(self.h)
+            self.h = None
+
+    @property
+    def nglob(self):
+        return self.lib.oracle2d_nglob(self.h)
+
+    @property
+    def t0(self):
+        return self.lib.oracle2d_t0(self.h)
+
+    def get(self, what):
+        n = self.lib.oracle2d_get(self.h, GET[what], None)
+        out = np.empty(n, dtype=np.float64)
+        self.lib.oracle2d_get(self.h, GET[what], _ptr(out))
+        return out
+
+    def set_field(self, what, values):
+        v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).ravel())
+        self.lib.oracle2d_set_field(self.h, {"u": 1, "v": 2, "a": 3}[what], _ptr(v))
+
+    def apply_stiffness(self, with_stacey=True):
+        self.lib.oracle2d_apply_stiffness(self.h, int(with_stacey))
+        return self.get("a")
+
+    def run(self, nstep=None, nsbs=1, types=("displacement",)):
+        """Advances nstep steps; returns seismograms (nsig, ntypes, nrec, 2)."""
+        nstep = self.nstep - self.istep if nstep is None else int(nstep)
+        t = np.array([SEIS_TYPES[s] for s in types], dtype=np.int32)
+        nsig = (nstep + nsbs - 1) // nsbs
+        out = np.zeros((nsig, len(t), self.nrec, 2), dtype=np.float64)
+        got = self.lib.oracle2d_run_forward(self.h, self.istep, nstep, nsbs, len(t), _ptr(t), _ptr(out))
+        if got < 0:
+            raise RuntimeError(self.lib.oracle2d_last_error().decode())
+        self.istep += nstep
+        return out[:got]

@@ -1,0 +1,112 @@
+"""Sources, receivers and the seeded benchmark configurations.
+
+Sources are vertical/oblique point forces with a Ricker time function, the
+reference's ``force`` source (core/specfem/source/dim2/vector_source/force_source.cpp:22-73,
+core/specfem/source_time_functions/ricker.cpp:9-66).  Receivers are station
+coordinates (core/specfem/receivers/dim2/receiver.hpp).  The seeded
+configurations follow the measured workloads: an 80x60 box of 50 m quads
+(single homogeneous medium, free top, Stacey elsewhere) and an 800x600 box with
+six horizontal layers.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from .mesh import Mesh2D, box_mesh
+
+
+@dataclass
+class Sources2D:
+    x: np.ndarray
+    z: np.ndarray
+    angle: np.ndarray    # degrees, clockwise from +z (0 = vertical force)
+    f0: np.ndarray
+    tshift: np.ndarray
+    factor: np.ndarray
+
+    @staticmethod
+    def ricker(x, z, f0, factor=1e10, angle=0.0, tshift=0.0):
+        x = np.atleast_1d(np.asarray(x, dtype=np.float64))
+        n = len(x)
+
+        def full(v, dt):
+            v = np.asarray(v, dtype=dt)
+            return np.broadcast_to(v, (n,)).copy()
+
+        return Sources2D(x=x, z=full(z, np.float64), angle=full(angle, np.float32),
+                         f0=full(f0, np.float32), tshift=full(tshift, np.float32),
+                         factor=full(factor, np.float32))
+
+    def __len__(self):
+        return len(self.x)
+
+
+@dataclass
+class Receivers2D:
+    x: np.ndarray
+    z: np.ndarray
+    names: tuple = ()
+
+    def __len__(self):
+        return len(self.x)
+
+
+@dataclass
+class Config2D:
+    name: str
+    mesh: Mesh2D
+    sources: Sources2D
+    receivers: Receivers2D
+    dt: float
+    nstep: int
+    nsbs: int = 1
+    seismo_types: tuple = ("displacement",)
+    meta: dict = None
+
+
+def homogeneous_box(nx=80, nz=60, h=50.0, nstep=2000, dt=5e-4, nrec=50) -> Config2D:
+    """4000x3000 m box (80x60 quads of 50 m), rho=2700, vp=3000, vs=1732, free
+    top, Stacey on the other three sides; vertical Ricker force (f0=10 Hz,
+    factor 1e10) at (2000 m, 2000 m); receivers at 100 m depth below the top,
+    x = 40 + 80 i (chosen spacing)."""
+    xmax, zmax = nx * h, nz * h
+    mesh = box_mesh(0.0, xmax, 0.0, zmax, nx, nz, ngnod=4,
+                    absorbing=("bottom", "right", "left"),
+                    rho=2700.0, vp=3000.0, vs=1732.0)
+    src = Sources2D.ricker([0.5 * xmax], [zmax - 1000.0], f0=10.0, factor=1e10)
+    rx = 40.0 + (xmax - 80.0) / max(nrec - 1, 1) * np.arange(nrec)
+    rec = Receivers2D(x=rx, z=np.full(nrec, zmax - 100.0))
+    return Config2D("homogeneous_80x60", mesh, src, rec, dt, nstep,
+                    meta={"nx": nx, "nz": nz, "h": h})
+
+
+def layered_box(nx=800, nz=600, h=50.0, nstep=5000, dt=4e-4, nsrc=8, nrec=500,
+                nlayers=6, seed=1001, f0=5.0) -> Config2D:
+    """Layered box: ``nlayers`` equal horizontal layers with vp ~ U[2000, 6000]
+    m/s, vs = vp/sqrt(3), rho = 310 vp^0.25 (Gardner), ``nsrc`` vertical Ricker
+    sources (f0 = 5 Hz) at seeded interior positions and ``nrec`` surface
+    receivers; free top, Stacey on the other three sides."""
+    rng = np.random.default_rng(seed)
+    vps = rng.uniform(2000.0, 6000.0, size=nlayers)
+    xmax, zmax = nx * h, nz * h
+    per = max(nz // nlayers, 1)
+
+    def layer(jz):
+        return min((nz - 1 - jz) // per, nlayers - 1)  # layer 0 at the top
+
+    mesh = box_mesh(0.0, xmax, 0.0, zmax, nx, nz, ngnod=4,
+                    absorbing=("bottom", "right", "left"),
+                    rho=lambda jx, jz: 310.0 * vps[layer(jz)] ** 0.25,
+                    vp=lambda jx, jz: vps[layer(jz)],
+                    vs=lambda jx, jz: vps[layer(jz)] / np.sqrt(3.0))
+    sx = rng.uniform(0.05 * xmax, 0.95 * xmax, size=nsrc)
+    sz = rng.uniform(0.067 * zmax, 0.933 * zmax, size=nsrc)
+    src = Sources2D.ricker(sx, sz, f0=f0, factor=1e10)
+    rx = 40.0 + (xmax - 80.0) / max(nrec - 1, 1) * np.arange(nrec)
+    rec = Receivers2D(x=rx, z=np.full(nrec, zmax))
+    return Config2D(f"layered_{nx}x{nz}", mesh, src, rec, dt, nstep,
+                    meta={"nx": nx, "nz": nz, "h": h, "vp_layers": vps.tolist(),
+                          "seed": seed})

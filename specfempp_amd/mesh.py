@@ -1,0 +1,177 @@
+"""Host mesh model for the 2D elastic P-SV path.
+
+A mesh is what the reference reads from an xmeshfem2D database
+(src/io/mesh/impl/fortran/dim2/mesh.cpp:26-215): control-node coordinates,
+per-element control-node connectivity (4- or 9-node quads, node order of
+core/specfem/shape_function/dim2/shape_functions2d.cpp), material properties
+and the list of absorbing edges (include/enumerations/dim2/mesh_entities.hpp:29-38
+numbering: bottom=1, right=2, top=3, left=4).  The structured box generator
+below replaces the Fortran mesher for the rectangular domains this path is
+measured on.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+NGLL = 5
+
+BOTTOM, RIGHT, TOP, LEFT = 1, 2, 3, 4
+EDGE_TYPES = {"bottom": BOTTOM, "right": RIGHT, "top": TOP, "left": LEFT}
+
+
+def elastic_moduli(rho, vp, vs):
+    """(kappa, mu, rho) from (rho, vp, vs) in FP32.
+
+    Follows the promotions of the reference material constructor
+    (include/medium/dim2/elastic/isotropic/material.hpp:62-69):
+    ``mu = rho*cs*cs`` in float, ``kappa = rho*(cp*cp - (4.0/3.0)*cs*cs)``
+    with ``cp*cp`` in float and the rest in double, rounded to float.
+    """
+    rho = np.asarray(rho, dtype=np.float32)
+    vp = np.asarray(vp, dtype=np.float32)
+    vs = np.asarray(vs, dtype=np.float32)
+    mu = (rho * vs) * vs  # float32 arithmetic
+    cp2 = (vp * vp).astype(np.float64)
+    kappa = (rho.astype(np.float64) *
+             (cp2 - ((4.0 / 3.0) * vs.astype(np.float64)) * vs.astype(np.float64)))
+    return kappa.astype(np.float32), mu.astype(np.float32), rho.copy()
+
+
+@dataclass
+class Mesh2D:
+    coord: np.ndarray          # (nnodes, 2) float64, control-node (x, z)
+    knods: np.ndarray          # (nspec, ngnod) int32
+    kappa: np.ndarray          # (nspec, NGLL, NGLL) float32, [e, iz, ix]
+    mu: np.ndarray
+    rho: np.ndarray
+    abs_ispec: np.ndarray      # (nabs,) int32
+    abs_type: np.ndarray       # (nabs,) int32
+    nx: int = 0                # structured-box metadata (0 if unstructured)
+    nz: int = 0
+    extent: tuple = (0.0, 0.0, 0.0, 0.0)   # xmin, xmax, zmin, zmax
+    meta: dict = field(default_factory=dict)
+
+    @property
+    def nspec(self) -> int:
+        return int(self.knods.shape[0])
+
+    @property
+    def ngnod(self) -> int:
+        return int(self.knods.shape[1])
+
+    @property
+    def nnodes(self) -> int:
+        return int(self.coord.shape[0])
+
+    def expected_nglob(self) -> int:
+        if self.nx and self.nz:
+            return (4 * self.nx + 1) * (4 * self.nz + 1)
+        raise ValueError("nglob is only known in closed form for a box")
+
+    def validate(self):
+        ns = self.nspec
+        if self.ngnod not in (4, 9):
+            raise ValueError("ngnod must be 4 or 9")
+        if self.knods.min() < 0 or self.knods.max() >= self.nnodes:
+            raise ValueError("knods out of range")
+        for name in ("kappa", "mu", "rho"):
+            arr = getattr(self, name)
+            if arr.shape != (ns, NGLL, NGLL) or arr.dtype != np.float32:
+                raise ValueError(f"{name} must be float32 of shape {(ns, NGLL, NGLL)}")
+        if self.abs_ispec.shape != self.abs_type.shape:
+            raise ValueError("abs_ispec/abs_type size mismatch")
+        if len(self.abs_ispec) and (self.abs_ispec.min() < 0 or self.abs_ispec.max() >= ns):
+            raise ValueError("absorbing edge element out of range")
+        if len(self.abs_type) and not np.isin(self.abs_type, [1, 2, 3, 4]).all():
+            raise ValueError("absorbing edge type must be 1..4")
+        return self
+
+
+def gll_points():
+    s = np.sqrt(3.0 / 7.0)
+    return np.array([-1.0, -s, 0.0, s, 1.0])
+
+
+def box_mesh(xmin, xmax, zmin, zmax, nx, nz, *, ngnod=4, absorbing=("bottom", "right", "left"),
+             rho=2700.0, vp=3000.0, vs=1732.0) -> Mesh2D:
+    """Structured nx-by-nz box; elements row-major from the bottom-left
+    (ispec = iz*nx + ix).  ``rho``, ``vp``, ``vs`` are scalars or callables
+    f(ix_elem, iz_elem) -> value (per element)."""
+    if ngnod == 4:
+        nxn, nzn = nx + 1, nz + 1
+        xs = np.linspace(xmin, xmax, nxn)
+        zs = np.linspace(zmin, zmax, nzn)
+        X, Z = np.meshgrid(xs, zs)            # (nzn, nxn)
+        coord = np.stack([X.ravel(), Z.ravel()], axis=1)
+        iz, ix = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+        n0 = (iz * nxn + ix).ravel()
+        knods = np.stack([n0, n0 + 1, n0 + 1 + nxn, n0 + nxn], axis=1)
+    elif ngnod == 9:
+        nxn, nzn = 2 * nx + 1, 2 * nz + 1
+        xs = np.linspace(xmin, xmax, nxn)
+        zs = np.linspace(zmin, zmax, nzn)
+        X, Z = np.meshgrid(xs, zs)
+        coord = np.stack([X.ravel(), Z.ravel()], axis=1)
+        iz, ix = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+        b = ((2 * iz) * nxn + 2 * ix).ravel()
+        knods = np.stack([b, b + 2, b + 2 + 2 * nxn, b + 2 * nxn,      # corners
+                          b + 1, b + 2 + nxn, b + 1 + 2 * nxn, b + nxn,  # mid-sides
+                          b + 1 + nxn], axis=1)                         # centre
+    else:
+        raise ValueError("ngnod must be 4 or 9")
+    knods = knods.astype(np.int32)
+
+    def per_elem(val):
+        if callable(val):
+            out = np.empty((nz, nx), dtype=np.float64)
+            for jz in range(nz):
+                for jx in range(nx):
+                    out[jz, jx] = val(jx, jz)
+            return out.ravel()
+        return np.full(nx * nz, float(val))
+
+    r, p, s = per_elem(rho), per_elem(vp), per_elem(vs)
+    rr = np.repeat(r.astype(np.float32), NGLL * NGLL).reshape(-1, NGLL, NGLL)
+    pp = np.repeat(p.astype(np.float32), NGLL * NGLL).reshape(-1, NGLL, NGLL)
+    ss = np.repeat(s.astype(np.float32), NGLL * NGLL).reshape(-1, NGLL, NGLL)
+    kappa, mu, rho_pt = elastic_moduli(rr, pp, ss)
+
+    # Absorbing edges in element order; within an element bottom, right, top,
+    # left (so at a corner GLL point the later edge's normal is kept, as in the
+    # reference's last-edge-wins assembly).
+    want = {EDGE_TYPES[a] for a in absorbing}
+    ai, at = [], []
+    for e in range(nx * nz):
+        jz, jx = divmod(e, nx)
+        for t in (BOTTOM, RIGHT, TOP, LEFT):
+            if t not in want:
+                continue
+            if (t == BOTTOM and jz == 0) or (t == TOP and jz == nz - 1) or \
+               (t == LEFT and jx == 0) or (t == RIGHT and jx == nx - 1):
+                ai.append(e)
+                at.append(t)
+    return Mesh2D(coord=coord.astype(np.float64), knods=knods, kappa=kappa, mu=mu,
+                  rho=rho_pt, abs_ispec=np.array(ai, dtype=np.int32),
+                  abs_type=np.array(at, dtype=np.int32), nx=nx, nz=nz,
+                  extent=(xmin, xmax, zmin, zmax)).validate()
+
+
+def gll_coordinates(mesh: Mesh2D) -> np.ndarray:
+    """(nspec, NGLL, NGLL, 2) physical coordinates of every GLL point (float64,
+    only for set-up and plotting; the solver computes its own in FP32)."""
+    xi = gll_points()
+    c = mesh.coord[mesh.knods]  # (nspec, ngnod, 2)
+    XI, GA = np.meshgrid(xi, xi)  # [iz, ix]
+    if mesh.ngnod == 4:
+        N = np.stack([0.25 * (XI - 1) * (GA - 1), -0.25 * (XI + 1) * (GA - 1),
+                      0.25 * (XI + 1) * (GA + 1), -0.25 * (XI - 1) * (GA + 1)], axis=-1)
+    else:
+        N = np.stack([0.25 * (XI - 1) * XI * (GA - 1) * GA, 0.25 * (XI + 1) * XI * (GA - 1) * GA,
+                      0.25 * (XI + 1) * XI * (GA + 1) * GA, 0.25 * (XI - 1) * XI * (GA + 1) * GA,
+                      0.5 * (1 - XI * XI) * (GA - 1) * GA, 0.5 * (XI + 1) * XI * (1 - GA * GA),
+                      0.5 * (1 - XI * XI) * (GA + 1) * GA, 0.5 * (XI - 1) * XI * (1 - GA * GA),
+                      (1 - XI * XI) * (1 - GA * GA)], axis=-1)
+    return np.einsum("zxn,end->ezxd", N, c)
