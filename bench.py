@@ -1,0 +1,145 @@
+"""Benchmark: GLL point-updates/s of the 2D elastic P-SV forward step on one
+MI355X.
+
+Workload: 800x600 quads of 50 m (40 km x 30 km), NGLL=5 (480,000 elements,
+7,685,601 GLL points), six seeded horizontal layers (vp ~ U[2000, 6000] m/s,
+vs = vp/sqrt(3), rho = 310 vp^0.25), 8 Ricker sources (f0 = 5 Hz), 500 surface
+receivers sampled every step, dt = 4e-4 s, free top, Stacey elsewhere, FP32.
+
+A step is one full time step (sources, stiffness + Stacey + assembly,
+seismograms, divide/corrector/predictor).  Inputs are resident in HBM when the
+timed region starts.  value = nglob * steps / wall time.
+
+Prints one JSON line.  Also reports the stiffness kernel against the HBM
+roofline (its algorithmic bytes per launch / its mean launch duration measured
+with HIP events on the solver's stream) and the CPU oracle timed on a bounded
+sample of the same workload on this host.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def stiffness_bytes(nspec, nglob, nstacey):
+    """Algorithmic bytes of one stiffness launch: per local point an index,
+    5 geometry terms and (lambda+2mu, mu) (32 B x 25 = 800 B per element);
+    the displacement read once and the acceleration read and written once
+    (8 + 16 B per global point); Stacey elements also read 6 boundary terms
+    per point and the velocity (32 B x 25 per element)."""
+    return 800 * nspec + 24 * nglob + 800 * nstacey
+
+
+def step_bytes(nspec, nglob):
+    """Canonical bytes of one time step: u, v, a read + written and M^-1 read
+    once per global point (52 B) plus 800 B per element."""
+    return 52 * nglob + 800 * nspec
+
+
+def cpu_baseline(cfg, nsteps=8):
+    """Times the CPU oracle (FP32, 1 thread) on `nsteps` steps of the same
+    mesh, sources and medium (one receiver), excluding set-up."""
+    import oracle
+    from specfempp_amd.config import Receivers2D
+    oracle.build()
+    rec = Receivers2D(x=cfg.receivers.x[:1], z=cfg.receivers.z[:1])
+    o = oracle.Oracle2D(cfg.mesh, cfg.sources, rec, cfg.dt, nsteps)
+    t = time.perf_counter()
+    o.run(nstep=nsteps)
+    dt = time.perf_counter() - t
+    return {"value": o.nglob * nsteps / dt, "unit": "GLL point-updates/s", "cores": 1,
+            "kind": "port",
+            "sample": f"{nsteps} time steps of the full 800x600 layered workload "
+                      f"(480,000 elements, all 8 sources, 1 receiver), FP32 oracle "
+                      f"restatement, set-up excluded, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--nx", type=int, default=800)
+    ap.add_argument("--nz", type=int, default=600)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=40)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
+                                   "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import torch
+    from specfempp_amd.config import layered_box
+    from specfempp_amd.solver import Context
+
+    nstep = args.warmup + args.steps
+    cfg = layered_box(nx=args.nx, nz=args.nz, nstep=nstep)
+    ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, nstep,
+                  seismo_types=cfg.seismo_types)
+    ctx.initialize()
+    nstacey = len(set(cfg.mesh.abs_ispec.tolist()))
+
+    if args.warmup > 0:
+        ctx.run_forward(0, args.warmup)
+        ctx.synchronize()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.run_forward(args.warmup, args.steps)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ev_ms = ctx.last_run_ms()
+
+    seis = ctx.seismograms()
+    import numpy as np
+    finite = bool(np.isfinite(seis).all())
+
+    stiff_ms = ctx.time_stiffness(50)
+    sb = stiffness_bytes(ctx.nspec, ctx.nglob, nstacey)
+    achieved = sb / (stiff_ms * 1e-3) / 1e9
+
+    value = ctx.nglob * args.steps / wall
+    out = {
+        "metric": "GLL point-updates/s (FP32, 1 GPU)",
+        "value": value,
+        "unit": "GLL point-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"2D elastic P-SV forward, {args.nx}x{args.nz} quads of 50 m, "
+                               f"NGLL=5, 6 seeded layers, 8 Ricker sources, 500 receivers, "
+                               f"dt=4e-4",
+                   "nspec": ctx.nspec, "nglob": ctx.nglob, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "k_stiffness<float>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "algorithmic_bytes_per_launch": sb, "mean_launch_ms": stiff_ms},
+        "step_roofline": {"canonical_bytes_per_step": step_bytes(ctx.nspec, ctx.nglob),
+                          "achieved_GBs": step_bytes(ctx.nspec, ctx.nglob) * args.steps
+                          / wall / 1e9,
+                          "frac": step_bytes(ctx.nspec, ctx.nglob) * args.steps / wall / 1e9
+                          / HBM_PEAK_GBS,
+                          "event_ms": ev_ms},
+        "seismograms_finite": finite,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,186 @@
+/*
+ * specfempp_amd — MI355X (gfx950) implementation of SPECFEM++'s 2D elastic
+ * isotropic P-SV time-stepping path, exposed through a plain C ABI.
+ *
+ * Every entry point returns 0 on success and a negative status on error;
+ * smx_last_error() returns the message of the last failure on the calling
+ * thread.  A context owns its device buffers and one HIP stream; it is not
+ * thread-safe (use one context per host thread).  All host arrays passed in
+ * are copied; none is retained after the call returns.
+ *
+ * Citations are to the reference interface each entry point replaces
+ * (paths relative to the SPECFEM++ source tree).
+ */
+#ifndef SPECFEMPP_AMD_H
+#define SPECFEMPP_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SMX_OK 0
+#define SMX_ERR_ARG -1     /* invalid argument / inconsistent sizes */
+#define SMX_ERR_HIP -2     /* HIP runtime error */
+#define SMX_ERR_STATE -3   /* call not valid in the context's state */
+#define SMX_ERR_MESH -4    /* mesh / point location error */
+
+/* Wavefields (include/enumerations/wavefield.hpp simulation_field). */
+#define SMX_FIELD_FORWARD 0
+#define SMX_FIELD_ADJOINT 1
+#define SMX_FIELD_BACKWARD 2
+
+/* Seismogram types (specfem::wavefield::type). */
+#define SMX_SEIS_DISPLACEMENT 0
+#define SMX_SEIS_VELOCITY 1
+#define SMX_SEIS_ACCELERATION 2
+
+/* Field components selectable by smx_get_field / smx_set_field. */
+#define SMX_U 0
+#define SMX_V 1
+#define SMX_A 2
+#define SMX_MASS_INVERSE 3
+
+typedef struct smx_ctx smx_ctx;
+
+/* Mesh: what the reference reads from a mesh database
+ * (src/io/mesh/impl/fortran/dim2/mesh.cpp:26-215) plus the per-GLL-point
+ * material of assembly::properties (core/specfem/assembly/properties).
+ * Edge types follow include/enumerations/dim2/mesh_entities.hpp:29-38
+ * (1 bottom, 2 right, 3 top, 4 left). */
+typedef struct {
+  int nspec;            /* number of spectral elements */
+  int ngnod;            /* control nodes per element: 4 or 9 */
+  int nnodes;           /* number of control nodes */
+  const double *coord;  /* [nnodes][2] (x, z) */
+  const int *knods;     /* [nspec][ngnod] control-node ids */
+  const float *kappa;   /* [nspec][5][5] bulk modulus at GLL points (iz, ix) */
+  const float *mu;      /* [nspec][5][5] shear modulus */
+  const float *rho;     /* [nspec][5][5] density */
+  int nabs;             /* absorbing (Stacey) edges */
+  const int *abs_ispec; /* [nabs] element of each edge */
+  const int *abs_type;  /* [nabs] edge type 1..4 */
+} smx_mesh2d;
+
+/* Ricker point forces (core/specfem/source/dim2/vector_source/force_source.cpp,
+ * core/specfem/source_time_functions/ricker.cpp). */
+typedef struct {
+  int nsrc;
+  const double *x, *z;  /* [nsrc] location */
+  const float *angle;   /* [nsrc] degrees clockwise from +z */
+  const float *f0;      /* [nsrc] dominant frequency */
+  const float *tshift;  /* [nsrc] */
+  const float *factor;  /* [nsrc] amplitude */
+} smx_sources2d;
+
+/* Stations and the seismograms recorded at them
+ * (core/specfem/assembly/receivers/dim2/receivers.cpp:11-148). */
+typedef struct {
+  int nrec;
+  const double *x, *z;  /* [nrec] */
+  int ntypes;           /* seismogram types recorded */
+  const int *types;     /* [ntypes] SMX_SEIS_* */
+  int nstep_between_samples;
+} smx_receivers2d;
+
+/* Last error message of the calling thread (never NULL). */
+const char *smx_last_error(void);
+
+/* Library version string. */
+const char *smx_version(void);
+
+/* Builds the assembly on the host (GLL quadrature, Jacobian, global
+ * numbering, Stacey normals/weights, source arrays and time functions,
+ * receiver interpolants) and uploads it.  precision is 4 (FP32, the
+ * reference's type_real) or 8 (FP64 twin of the same kernels).
+ * Replaces specfem::assembly::assembly<dim2>::assembly
+ * (core/specfem/assembly/assembly/dim2/assembly.cpp:6-93) together with the
+ * domain_kernels / newmark constructors
+ * (include/kokkos_kernels/domain_kernels.hpp:58-59,
+ *  core/specfem/timescheme/newmark.hpp:131-135). */
+int smx_ctx_create2d(const smx_mesh2d *mesh, const smx_sources2d *sources,
+                     const smx_receivers2d *receivers, double dt, int nstep,
+                     int precision, smx_ctx **out);
+
+/* Frees every device buffer and the stream of the context. */
+int smx_ctx_destroy(smx_ctx *ctx);
+
+/* Sizes: nspec, nglob, number of seismogram samples, receivers. */
+int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
+                 int *nrec);
+
+/* Start time t0 of the simulation (src/io/sources/dim2/read_sources.cpp:142-172). */
+double smx_ctx_t0(const smx_ctx *ctx);
+
+/* Mass matrix + Stacey mass term, then its inverse.
+ * Replaces domain_kernels::initialize(dt)
+ * (include/kokkos_kernels/domain_kernels.hpp:151-179). */
+int smx_initialize(smx_ctx *ctx);
+
+/* Newmark predictor u += dt v + dt^2/2 a; v += dt/2 a; a = 0.  *dofs gets the
+ * number of degrees of freedom updated (nglob * 2).
+ * Replaces newmark::apply_predictor_phase_forward/backward
+ * (core/specfem/timescheme/newmark.tpp:152-174, 233-256). */
+int smx_predictor(smx_ctx *ctx, int field, int *dofs);
+
+/* Newmark corrector v += dt/2 a.
+ * Replaces newmark::apply_corrector_phase_forward/backward
+ * (core/specfem/timescheme/newmark.tpp:130-150, 189-210). */
+int smx_corrector(smx_ctx *ctx, int field, int *dofs);
+
+/* Source injection, stiffness (gradient, stress, divergence, Stacey) with
+ * assembly into the acceleration, then division by the mass matrix.
+ * *nelem gets the number of elements updated (nspec).
+ * Replaces domain_kernels::update_wavefields<elastic_psv>(istep)
+ * (include/kokkos_kernels/domain_kernels.hpp:72-141). */
+int smx_update_wavefields(smx_ctx *ctx, int field, int istep, int *nelem);
+
+/* Seismograms of the current fields into sample isig.
+ * Replaces domain_kernels::compute_seismograms(isig)
+ * (include/kokkos_kernels/domain_kernels.hpp:188-203). */
+int smx_compute_seismograms(smx_ctx *ctx, int field, int isig);
+
+/* nstep forward steps from istep0 on the fused path (stiffness + sources,
+ * fused divide/corrector/predictor, seismograms every
+ * nstep_between_samples), replayed from a HIP graph.  Returns after the work
+ * is enqueued; smx_synchronize waits for it.
+ * Replaces time_marching<forward>::run's loop
+ * (core/specfem/solver/time_marching.tpp:33-116). */
+int smx_run_forward(smx_ctx *ctx, int istep0, int nstep);
+
+/* Waits for all work queued on the context's stream. */
+int smx_synchronize(smx_ctx *ctx);
+
+/* Copies seismograms [nsig][ntypes][nrec][2] to host (element size 4 or 8
+ * bytes = the context's precision).  Replaces the receivers'
+ * seismogram_components host sync (newmark_tests.cpp:185). */
+int smx_get_seismograms(smx_ctx *ctx, void *host_out);
+
+/* Copies one field component array [nglob][2] of `field` to / from host
+ * (element size = the context's precision). */
+int smx_get_field(smx_ctx *ctx, int field, int which, void *host_out);
+int smx_set_field(smx_ctx *ctx, int field, int which, const void *host_in);
+
+/* Global numbering [nspec][5][5] (iz, ix) the context assembled
+ * (assembly::mesh index_mapping, core/specfem/assembly/mesh/dim2/impl/assemble.tpp). */
+int smx_get_ibool(smx_ctx *ctx, int *host_out);
+
+/* Device pointer of a field array [nglob][2] (for zero-copy interop). */
+int smx_field_device_ptr(smx_ctx *ctx, int field, int which, void **dptr);
+
+/* Elapsed milliseconds of the last smx_run_forward, measured with HIP events
+ * on the context's stream, and the number of stiffness launches it made. */
+int smx_last_run_stats(smx_ctx *ctx, double *ms, double *stiffness_ms,
+                       int *stiffness_launches);
+
+/* Launches the stiffness kernel alone nlaunch times on the context's stream
+ * (into a scratch acceleration, leaving the fields untouched) and returns the
+ * mean launch duration in ms measured with HIP events.  Measurement aid with
+ * no reference counterpart (the reference reports Kokkos-Tools region times
+ * for compute_stiffness_interaction.tpp:121,250). */
+int smx_time_stiffness(smx_ctx *ctx, int nlaunch, double *avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SPECFEMPP_AMD_H */

@@ -70,13 +70,13 @@ class Config2D:
 def homogeneous_box(nx=80, nz=60, h=50.0, nstep=2000, dt=5e-4, nrec=50) -> Config2D:
     """4000x3000 m box (80x60 quads of 50 m), rho=2700, vp=3000, vs=1732, free
     top, Stacey on the other three sides; vertical Ricker force (f0=10 Hz,
-    factor 1e10) at (2000 m, 2000 m); receivers at 100 m depth below the top,
+    factor 1e10) at (xmax/2, 2 zmax/3) = (2000 m, 2000 m); receivers 100 m below the top,
     x = 40 + 80 i (chosen spacing)."""
     xmax, zmax = nx * h, nz * h
     mesh = box_mesh(0.0, xmax, 0.0, zmax, nx, nz, ngnod=4,
                     absorbing=("bottom", "right", "left"),
                     rho=2700.0, vp=3000.0, vs=1732.0)
-    src = Sources2D.ricker([0.5 * xmax], [zmax - 1000.0], f0=10.0, factor=1e10)
+    src = Sources2D.ricker([0.5 * xmax], [zmax * 2.0 / 3.0], f0=10.0, factor=1e10)
     rx = 40.0 + (xmax - 80.0) / max(nrec - 1, 1) * np.arange(nrec)
     rec = Receivers2D(x=rx, z=np.full(nrec, zmax - 100.0))
     return Config2D("homogeneous_80x60", mesh, src, rec, dt, nstep,

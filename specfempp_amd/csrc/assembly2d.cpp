@@ -1,0 +1,663 @@
+// Host-side assembly of the 2D elastic P-SV problem (see assembly2d.hpp).
+//
+// All floating-point work here is FP32 with the promotions the reference's
+// host code uses; this translation unit is compiled with -ffp-contract=off so
+// that no multiply-add is fused (the reference's x86-64 host build has none).
+#include "assembly2d.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <limits>
+#include <map>
+#include <unordered_map>
+
+namespace smx {
+
+namespace {
+
+// Legendre value and derivative by recurrence
+// (src/quadrature/gll/gll_library.cpp:11-33, 53-93).
+float legendre(float z, int n) {
+  float p1 = 1.0, p2 = z, p3 = p2;
+  for (int k = 1; k < n; k++) {
+    const float fk = static_cast<float>(k);
+    p3 = ((2.0 * fk + 1.0) * z * p2 - fk * p1) / (fk + 1.0);
+    p1 = p2;
+    p2 = p3;
+  }
+  return p3;
+}
+
+float legendre_deriv(float z, int n) {
+  float p1 = 1.0, p2 = z, p1d = 0.0, p2d = 1.0, p3d = 1.0;
+  for (int k = 1; k < n; k++) {
+    const float fk = static_cast<float>(k);
+    const float p3 = ((2.0 * fk + 1.0) * z * p2 - fk * p1) / (fk + 1.0);
+    p3d = ((2.0 * fk + 1.0) * p2 + (2.0 * fk + 1.0) * z * p2d - fk * p1d) /
+          (fk + 1.0);
+    p1 = p2;
+    p2 = p3;
+    p1d = p2d;
+    p2d = p3d;
+  }
+  return p3d;
+}
+
+// Lagrange interpolants through the GLL nodes at xi
+// (src/quadrature/gll/lagrange_poly.cpp:11-43).
+void interpolants(float xi, const float *nodes, float *h) {
+  for (int d = 0; d < NGLL; d++) {
+    float num = 1.0, den = 1.0;
+    for (int i = 0; i < NGLL; i++) {
+      if (i == d)
+        continue;
+      num = num * (xi - nodes[i]);
+      den = den * (nodes[d] - nodes[i]);
+    }
+    const float inv = 1.0 / den;
+    h[d] = num * inv;
+  }
+}
+
+// Shape functions and derivatives of 4/9-node quads
+// (core/specfem/shape_function/dim2/shape_functions2d.cpp:7-32,
+//  shape_function_derivatives2d.cpp:7-75).  Each value is formed in double
+// from float inputs and rounded to float, as the reference's std::vector<float>
+// results are.
+void shape(float xi, float ga, int ngnod, float *N) {
+  if (ngnod == 4) {
+    N[0] = 0.25 * (xi - 1) * (ga - 1);
+    N[1] = -0.25 * (xi + 1) * (ga - 1);
+    N[2] = 0.25 * (xi + 1) * (ga + 1);
+    N[3] = -0.25 * (xi - 1) * (ga + 1);
+    return;
+  }
+  N[0] = 0.25 * (xi - 1) * xi * (ga - 1) * ga;
+  N[1] = 0.25 * (xi + 1) * xi * (ga - 1) * ga;
+  N[2] = 0.25 * (xi + 1) * xi * (ga + 1) * ga;
+  N[3] = 0.25 * (xi - 1) * xi * (ga + 1) * ga;
+  N[4] = 0.5 * (1 - xi * xi) * (ga - 1) * ga;
+  N[5] = 0.5 * (xi + 1) * xi * (1 - ga * ga);
+  N[6] = 0.5 * (1 - xi * xi) * (ga + 1) * ga;
+  N[7] = 0.5 * (xi - 1) * xi * (1 - ga * ga);
+  N[8] = (1 - xi * xi) * (1 - ga * ga);
+}
+
+void shape_deriv(float xi, float ga, int ngnod, float *dxi, float *dga) {
+  if (ngnod == 4) {
+    const float sp = xi + 1, sm = xi - 1, tp = ga + 1, tm = ga - 1;
+    dxi[0] = 0.25 * tm;
+    dxi[1] = -0.25 * tm;
+    dxi[2] = 0.25 * tp;
+    dxi[3] = -0.25 * tp;
+    dga[0] = 0.25 * sm;
+    dga[1] = -0.25 * sp;
+    dga[2] = 0.25 * sp;
+    dga[3] = -0.25 * sm;
+    return;
+  }
+  const float sp = xi + 1.0, sm = xi - 1.0, tp = ga + 1.0, tm = ga - 1.0;
+  const float s2 = xi * 2.0, t2 = ga * 2.0, ss = xi * xi, tt = ga * ga,
+              st = xi * ga;
+  dxi[0] = 0.25 * tm * ga * (s2 - 1.0);
+  dxi[1] = 0.25 * tm * ga * (s2 + 1.0);
+  dxi[2] = 0.25 * tp * ga * (s2 + 1.0);
+  dxi[3] = 0.25 * tp * ga * (s2 - 1.0);
+  dga[0] = 0.25 * sm * xi * (t2 - 1.0);
+  dga[1] = 0.25 * sp * xi * (t2 - 1.0);
+  dga[2] = 0.25 * sp * xi * (t2 + 1.0);
+  dga[3] = 0.25 * sm * xi * (t2 + 1.0);
+  dxi[4] = -1.0 * st * tm;
+  dxi[5] = 0.5 * (1.0 - tt) * (s2 + 1.0);
+  dxi[6] = -1.0 * st * tp;
+  dxi[7] = 0.5 * (1.0 - tt) * (s2 - 1.0);
+  dga[4] = 0.5 * (1.0 - ss) * (t2 - 1.0);
+  dga[5] = -1.0 * st * sp;
+  dga[6] = 0.5 * (1.0 - ss) * (t2 + 1.0);
+  dga[7] = -1.0 * st * sm;
+  dxi[8] = -1.0 * s2 * (1.0 - tt);
+  dga[8] = -1.0 * t2 * (1.0 - ss);
+}
+
+struct Geo {
+  float xix, xiz, gammax, gammaz, jac;
+};
+
+// core/specfem/jacobian/dim2/jacobian.cpp:25-51
+Geo jacobian_at(const float *cx, const float *cz, int ngnod, const float *dxi,
+                const float *dga) {
+  float xxi = 0.0, zxi = 0.0, xga = 0.0, zga = 0.0;
+  for (int n = 0; n < ngnod; n++) {
+    xxi += dxi[n] * cx[n];
+    zxi += dxi[n] * cz[n];
+    xga += dga[n] * cx[n];
+    zga += dga[n] * cz[n];
+  }
+  Geo g;
+  g.jac = xxi * zga - xga * zxi;
+  g.xix = zga / g.jac;
+  g.gammax = -zxi / g.jac;
+  g.xiz = -xga / g.jac;
+  g.gammaz = xxi / g.jac;
+  return g;
+}
+
+void location_at(const float *cx, const float *cz, int ngnod, float xi,
+                 float ga, float &x, float &z) {
+  float N[9];
+  shape(xi, ga, ngnod, N);
+  x = 0.0;
+  z = 0.0;
+  for (int n = 0; n < ngnod; n++) {
+    x += N[n] * cx[n];
+    z += N[n] * cz[n];
+  }
+}
+
+// Nearest-GLL-point search over a bucket grid.  Reproduces the reference's
+// brute-force scan (src/algorithms/dim2/locate_point_impl.cpp:13-45): the
+// smallest distance wins, ties go to the earliest (element, iz, ix) in scan
+// order.
+class PointGrid {
+public:
+  PointGrid(const std::vector<float> &px, const std::vector<float> &pz)
+      : px_(px), pz_(pz) {
+    const size_t n = px.size();
+    xmin_ = zmin_ = std::numeric_limits<float>::max();
+    float xmax = -xmin_, zmax = -zmin_;
+    for (size_t i = 0; i < n; ++i) {
+      xmin_ = std::min(xmin_, px[i]);
+      xmax = std::max(xmax, px[i]);
+      zmin_ = std::min(zmin_, pz[i]);
+      zmax = std::max(zmax, pz[i]);
+    }
+    const double area =
+        std::max(double(xmax - xmin_), 1e-6) * std::max(double(zmax - zmin_), 1e-6);
+    cell_ = static_cast<float>(std::sqrt(area / std::max<double>(n / 4.0, 1.0)));
+    if (!(cell_ > 0))
+      cell_ = 1.0f;
+    nxc_ = std::max(1, int((xmax - xmin_) / cell_) + 1);
+    nzc_ = std::max(1, int((zmax - zmin_) / cell_) + 1);
+    start_.assign(size_t(nxc_) * nzc_ + 1, 0);
+    std::vector<int> cellof(n);
+    for (size_t i = 0; i < n; ++i) {
+      cellof[i] = cell_index(px[i], pz[i]);
+      start_[cellof[i] + 1]++;
+    }
+    for (size_t c = 0; c < size_t(nxc_) * nzc_; ++c)
+      start_[c + 1] += start_[c];
+    items_.resize(n);
+    std::vector<int> fill(start_.begin(), start_.end() - 1);
+    for (size_t i = 0; i < n; ++i)
+      items_[fill[cellof[i]]++] = static_cast<int>(i);
+  }
+
+  // Returns the scan-order index of the nearest point.
+  long nearest(float x, float z) const {
+    const int cx = clampi(int((x - xmin_) / cell_), nxc_),
+              cz = clampi(int((z - zmin_) / cell_), nzc_);
+    float best = std::numeric_limits<float>::max();
+    long best_i = -1;
+    const int maxr = std::max(nxc_, nzc_);
+    for (int r = 0; r <= maxr; ++r) {
+      for (int jz = cz - r; jz <= cz + r; ++jz) {
+        if (jz < 0 || jz >= nzc_)
+          continue;
+        for (int jx = cx - r; jx <= cx + r; ++jx) {
+          if (jx < 0 || jx >= nxc_)
+            continue;
+          if (std::max(std::abs(jx - cx), std::abs(jz - cz)) != r)
+            continue;
+          const int c = jz * nxc_ + jx;
+          for (int k = start_[c]; k < start_[c + 1]; ++k) {
+            const int i = items_[k];
+            const float dx = x - px_[i], dz = z - pz_[i];
+            const float d = std::sqrt(dx * dx + dz * dz);
+            if (d < best || (d == best && i < best_i)) {
+              best = d;
+              best_i = i;
+            }
+          }
+        }
+      }
+      // Every point outside rings 0..r is farther than (r) cells from the
+      // query's cell boundary; stop once that bound exceeds the best distance
+      // with a safety margin.
+      if (best_i >= 0 && double(r) * cell_ > double(best) * 1.0001 + 1e-6)
+        break;
+    }
+    return best_i;
+  }
+
+private:
+  int cell_index(float x, float z) const {
+    return clampi(int((z - zmin_) / cell_), nzc_) * nxc_ +
+           clampi(int((x - xmin_) / cell_), nxc_);
+  }
+  static int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+  const std::vector<float> &px_, &pz_;
+  float xmin_, zmin_, cell_;
+  int nxc_, nzc_;
+  std::vector<int> start_, items_;
+};
+
+} // namespace
+
+Quadrature::Quadrature() {
+  // Lobatto nodes and weights of degree 4 (closed forms of the roots
+  // src/quadrature/gll/gll_library.cpp:102-147 computes).
+  const double s = std::sqrt(3.0 / 7.0);
+  const double xd[NGLL] = {-1.0, -s, 0.0, s, 1.0};
+  const double wd[NGLL] = {0.1, 49.0 / 90.0, 32.0 / 45.0, 49.0 / 90.0, 0.1};
+  for (int i = 0; i < NGLL; ++i) {
+    xi[i] = static_cast<float>(xd[i]);
+    w[i] = static_cast<float>(wd[i]);
+  }
+  // h'(i,j) = l'_j(xi_i) (src/quadrature/gll/lagrange_poly.cpp:161-190).
+  const int n = NGLL - 1;
+  const float fn = static_cast<float>(n);
+  for (int i = 0; i < NGLL; i++)
+    for (int j = 0; j < NGLL; j++) {
+      if (i == 0 && j == 0)
+        hprime[i][j] = -1.0 * fn * (fn + 1.0) * 0.25;
+      else if (i == n && j == n)
+        hprime[i][j] = 1.0 * fn * (fn + 1.0) * 0.25;
+      else if (i == j)
+        hprime[i][j] = 0.0;
+      else
+        hprime[i][j] = legendre(xi[i], n) /
+                           (legendre(xi[j], n) * (xi[i] - xi[j])) +
+                       (1.0 - xi[i] * xi[i]) * legendre_deriv(xi[i], n) /
+                           (fn * (fn + 1.0) * legendre(xi[j], n) *
+                            (xi[i] - xi[j]) * (xi[i] - xi[j]));
+    }
+}
+
+void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
+                const smx_receivers2d &rec, double dt_in, int nstep,
+                HostAssembly2D &A) {
+  if (m.nspec <= 0 || (m.ngnod != 4 && m.ngnod != 9) || m.nnodes <= 0 ||
+      !m.coord || !m.knods || !m.kappa || !m.mu || !m.rho)
+    throw SmxError(SMX_ERR_ARG, "invalid mesh description");
+  if (m.nabs < 0 || (m.nabs > 0 && (!m.abs_ispec || !m.abs_type)))
+    throw SmxError(SMX_ERR_ARG, "invalid absorbing edge list");
+  if (nstep <= 0 || !(dt_in > 0))
+    throw SmxError(SMX_ERR_ARG, "dt and nstep must be positive");
+  if (src.nsrc < 0 || rec.nrec < 0 || rec.ntypes < 0 ||
+      rec.nstep_between_samples <= 0)
+    throw SmxError(SMX_ERR_ARG, "invalid source/receiver description");
+
+  const int nspec = m.nspec, ngnod = m.ngnod;
+  A.nspec = nspec;
+  const Quadrature &q = A.quad;
+
+  // ---- control nodes (type_real coordinates)
+  std::vector<float> cx(size_t(nspec) * ngnod), cz(size_t(nspec) * ngnod);
+  for (int e = 0; e < nspec; ++e)
+    for (int n = 0; n < ngnod; ++n) {
+      const int id = m.knods[size_t(e) * ngnod + n];
+      if (id < 0 || id >= m.nnodes)
+        throw SmxError(SMX_ERR_ARG, "knods out of range");
+      cx[size_t(e) * ngnod + n] = static_cast<float>(m.coord[2 * size_t(id)]);
+      cz[size_t(e) * ngnod + n] = static_cast<float>(m.coord[2 * size_t(id) + 1]);
+    }
+
+  // ---- element colouring for the atomic-free scatter: two elements of one
+  // colour share no GLL point.  Conforming quads share points only through
+  // their corner nodes, so a greedy pass over the corner control nodes
+  // (per-node mask of the colours already touching it) is sufficient.
+  {
+    std::vector<uint32_t> node_mask(m.nnodes, 0u);
+    std::vector<int> color(nspec);
+    int ncolors = 0;
+    for (int e = 0; e < nspec; ++e) {
+      const int *kn = &m.knods[size_t(e) * ngnod];
+      uint32_t used = 0;
+      for (int c = 0; c < 4; ++c)
+        used |= node_mask[kn[c]];
+      int col = 0;
+      while (col < 32 && (used >> col) & 1u)
+        ++col;
+      if (col >= 32)
+        throw SmxError(SMX_ERR_MESH, "element colouring needs more than 32 colours");
+      color[e] = col;
+      ncolors = std::max(ncolors, col + 1);
+      for (int c = 0; c < 4; ++c)
+        node_mask[kn[c]] |= 1u << col;
+    }
+    // slots: colours one after another, each padded to whole wavefronts,
+    // elements of a colour in their original order
+    std::vector<int> count(ncolors, 0);
+    for (int e = 0; e < nspec; ++e)
+      count[color[e]]++;
+    A.color_offset.assign(ncolors + 1, 0);
+    A.color_count = count;
+    for (int c = 0; c < ncolors; ++c)
+      A.color_offset[c + 1] =
+          A.color_offset[c] + (count[c] + ELEM_TILE - 1) / ELEM_TILE * ELEM_TILE;
+    A.nspec_pad = A.color_offset[ncolors];
+    A.elem_slot.assign(nspec, 0);
+    std::vector<int> fill(A.color_offset.begin(), A.color_offset.end() - 1);
+    for (int e = 0; e < nspec; ++e)
+      A.elem_slot[e] = fill[color[e]]++;
+  }
+  const int ld = A.nspec_pad;
+  const std::vector<int> &slot_of = A.elem_slot;
+
+  // ---- global numbering from corner topology (first touch in element order)
+  A.ibool.assign(size_t(NPT) * ld, 0);
+  std::unordered_map<int, int> corner_ids;
+  std::unordered_map<uint64_t, int> edge_base; // sorted corner pair -> first id
+  corner_ids.reserve(size_t(nspec) * 2);
+  edge_base.reserve(size_t(nspec) * 3);
+  int next = 0;
+  auto corner = [&](int node) {
+    auto it = corner_ids.emplace(node, next);
+    if (it.second)
+      ++next;
+    return it.first->second;
+  };
+  // an edge owns NGLL-2 consecutive ids, ordered from its lower-id corner
+  auto edge = [&](int a, int b, int k) {
+    const uint64_t key = (uint64_t(uint32_t(std::min(a, b))) << 32) |
+                         uint32_t(std::max(a, b));
+    auto it = edge_base.emplace(key, next);
+    if (it.second)
+      next += NGLL - 2;
+    const int pos = (a < b) ? k : (NGLL - 1 - k);
+    return it.first->second + pos - 1;
+  };
+  for (int e = 0; e < nspec; ++e) {
+    const int *kn = &m.knods[size_t(e) * ngnod];
+    for (int iz = 0; iz < NGLL; ++iz)
+      for (int ix = 0; ix < NGLL; ++ix) {
+        const bool b = iz == 0, t = iz == NGLL - 1, l = ix == 0,
+                   r = ix == NGLL - 1;
+        int g;
+        if (b && l) g = corner(kn[0]);
+        else if (b && r) g = corner(kn[1]);
+        else if (t && r) g = corner(kn[2]);
+        else if (t && l) g = corner(kn[3]);
+        else if (b) g = edge(kn[0], kn[1], ix);
+        else if (r) g = edge(kn[1], kn[2], iz);
+        else if (t) g = edge(kn[3], kn[2], ix);
+        else if (l) g = edge(kn[0], kn[3], iz);
+        else g = next++;
+        A.ibool[size_t(iz * NGLL + ix) * ld + slot_of[e]] = g;
+      }
+  }
+  A.nglob = next;
+
+  // ---- geometry (core/specfem/assembly/jacobian_matrix/dim2/jacobian_matrix.cpp:28-93)
+  float dN[NPT][2][9];
+  for (int iz = 0; iz < NGLL; ++iz)
+    for (int ix = 0; ix < NGLL; ++ix)
+      shape_deriv(q.xi[ix], q.xi[iz], ngnod, dN[iz * NGLL + ix][0],
+                  dN[iz * NGLL + ix][1]);
+  A.geom.assign(size_t(5) * NPT * ld, 0.0f);
+  std::vector<float> px(size_t(nspec) * NPT), pz(size_t(nspec) * NPT);
+  for (int e = 0; e < nspec; ++e)
+    for (int p = 0; p < NPT; ++p) {
+      const Geo g = jacobian_at(&cx[size_t(e) * ngnod], &cz[size_t(e) * ngnod],
+                                ngnod, dN[p][0], dN[p][1]);
+      if (!(g.jac > 1e-10f))
+        throw SmxError(SMX_ERR_MESH, "non-positive Jacobian in element " +
+                                         std::to_string(e));
+      const size_t o = size_t(p) * ld + slot_of[e];
+      A.geom[0 * size_t(NPT) * ld + o] = g.xix;
+      A.geom[1 * size_t(NPT) * ld + o] = g.xiz;
+      A.geom[2 * size_t(NPT) * ld + o] = g.gammax;
+      A.geom[3 * size_t(NPT) * ld + o] = g.gammaz;
+      A.geom[4 * size_t(NPT) * ld + o] = g.jac;
+      location_at(&cx[size_t(e) * ngnod], &cz[size_t(e) * ngnod], ngnod,
+                  q.xi[p % NGLL], q.xi[p / NGLL], px[size_t(e) * NPT + p],
+                  pz[size_t(e) * NPT + p]);
+    }
+
+  // ---- material (include/medium/dim2/elastic/isotropic/properties.hpp:71-91)
+  A.lam2mu_mu.assign(size_t(2) * NPT * ld, 0.0f);
+  A.kappa_mu_rho.assign(size_t(3) * NPT * ld, 0.0f);
+  auto l2m_of = [](float kappa, float mu) {
+    return kappa + (static_cast<float>(4.0 / 3.0)) * mu;
+  };
+  for (int e = 0; e < nspec; ++e)
+    for (int p = 0; p < NPT; ++p) {
+      const size_t src_i = size_t(e) * NPT + p, o = size_t(p) * ld + slot_of[e];
+      const float kappa = m.kappa[src_i], mu = m.mu[src_i], rho = m.rho[src_i];
+      if (!(rho > 0) || !(mu >= 0) || !(kappa > 0))
+        throw SmxError(SMX_ERR_ARG, "non-physical material at element " +
+                                        std::to_string(e));
+      A.lam2mu_mu[o] = l2m_of(kappa, mu);
+      A.lam2mu_mu[size_t(NPT) * ld + o] = mu;
+      A.kappa_mu_rho[o] = kappa;
+      A.kappa_mu_rho[size_t(NPT) * ld + o] = mu;
+      A.kappa_mu_rho[2 * size_t(NPT) * ld + o] = rho;
+    }
+
+  // ---- Stacey (core/specfem/assembly/boundaries/dim2/impl/stacey.tpp:15-176):
+  // elements in ascending order, each element's edges in list order, the last
+  // edge covering a point sets its normal and weight.
+  std::map<int, std::vector<int>> edges_of;
+  for (int i = 0; i < m.nabs; ++i) {
+    const int e = m.abs_ispec[i], t = m.abs_type[i];
+    if (e < 0 || e >= nspec || t < 1 || t > 4)
+      throw SmxError(SMX_ERR_ARG, "invalid absorbing edge");
+    edges_of[e].push_back(t);
+  }
+  A.stacey_slot.assign(ld, -1);
+  A.nstacey = static_cast<int>(edges_of.size());
+  A.nstacey_pad = std::max(ELEM_TILE, (A.nstacey + ELEM_TILE - 1) / ELEM_TILE *
+                                          ELEM_TILE);
+  const int sld = A.nstacey_pad;
+  A.stacey.assign(size_t(6) * NPT * sld, 0.0f);
+  {
+    int slot = 0;
+    for (auto &kv : edges_of) {
+      const int e = kv.first;
+      A.stacey_slot[slot_of[e]] = slot;
+      // points not on an absorbing edge keep a unit normal and zero weight,
+      // which makes their traction contribution an exact -0.
+      for (int p = 0; p < NPT; ++p) {
+        A.stacey[(0 * size_t(NPT) + p) * sld + slot] = 1.0f;
+        A.stacey[(2 * size_t(NPT) + p) * sld + slot] = 1.0f;
+      }
+      for (int t : kv.second)
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const bool on = (t == 3 && iz == NGLL - 1) || (t == 1 && iz == 0) ||
+                            (t == 4 && ix == 0) || (t == 2 && ix == NGLL - 1);
+            if (!on)
+              continue;
+            const int p = iz * NGLL + ix;
+            const size_t o = size_t(p) * ld + slot_of[e];
+            const float xix = A.geom[o], xiz = A.geom[size_t(NPT) * ld + o],
+                        gax = A.geom[2 * size_t(NPT) * ld + o],
+                        gaz = A.geom[3 * size_t(NPT) * ld + o],
+                        jac = A.geom[4 * size_t(NPT) * ld + o];
+            float n0, n1, wgt;
+            // core/specfem/point/jacobian_matrix.hpp:424-449
+            if (t == 4) {
+              n0 = static_cast<float>(-1.0) * xix * jac;
+              n1 = static_cast<float>(-1.0) * xiz * jac;
+              wgt = q.w[iz];
+            } else if (t == 2) {
+              n0 = xix * jac;
+              n1 = xiz * jac;
+              wgt = q.w[iz];
+            } else if (t == 3) {
+              n0 = gax * jac;
+              n1 = gaz * jac;
+              wgt = q.w[ix];
+            } else {
+              n0 = static_cast<float>(-1.0) * gax * jac;
+              n1 = static_cast<float>(-1.0) * gaz * jac;
+              wgt = q.w[ix];
+            }
+            const float kappa = A.kappa_mu_rho[o],
+                        mu = A.kappa_mu_rho[size_t(NPT) * ld + o],
+                        rho = A.kappa_mu_rho[2 * size_t(NPT) * ld + o];
+            const float l2m = l2m_of(kappa, mu);
+            const float rvp = std::sqrt(l2m / rho) * rho;
+            const float rvs = std::sqrt(rho * mu);
+            const float nrm = std::sqrt(n0 * n0 + n1 * n1);
+            A.stacey[(0 * size_t(NPT) + p) * sld + slot] = n0;
+            A.stacey[(1 * size_t(NPT) + p) * sld + slot] = n1;
+            A.stacey[(2 * size_t(NPT) + p) * sld + slot] = nrm;
+            A.stacey[(3 * size_t(NPT) + p) * sld + slot] = wgt;
+            A.stacey[(4 * size_t(NPT) + p) * sld + slot] = rvp;
+            A.stacey[(5 * size_t(NPT) + p) * sld + slot] = rvs;
+          }
+      ++slot;
+    }
+  }
+
+  // ---- point location (src/algorithms/dim2/locate_point_impl.cpp:245-345)
+  PointGrid grid(px, pz);
+  // corner id -> elements touching it, ascending
+  std::unordered_map<int, std::vector<int>> elems_at_corner;
+  const int corner_p[4] = {0, (NGLL - 1) * NGLL, NGLL - 1, NPT - 1};
+  for (int e = 0; e < nspec; ++e)
+    for (int c = 0; c < 4; ++c)
+      elems_at_corner[A.ibool[size_t(corner_p[c]) * ld + slot_of[e]]].push_back(e);
+  auto locate = [&](double xd, double zd, int &ispec_out, float &xi_out,
+                    float &ga_out) {
+    const float x = static_cast<float>(xd), z = static_cast<float>(zd);
+    const long best = grid.nearest(x, z);
+    if (best < 0)
+      throw SmxError(SMX_ERR_MESH, "empty mesh");
+    const int guess = static_cast<int>(best / NPT);
+    std::vector<int> cand;
+    for (int c = 0; c < 4; ++c)
+      for (int e : elems_at_corner[A.ibool[size_t(corner_p[c]) * ld + slot_of[guess]]])
+        if (e != guess)
+          cand.push_back(e);
+    std::sort(cand.begin(), cand.end());
+    cand.erase(std::unique(cand.begin(), cand.end()), cand.end());
+    cand.insert(cand.begin(), guess);
+    float final_dist = std::numeric_limits<float>::max();
+    ispec_out = -1;
+    xi_out = ga_out = -9999.0f;
+    for (int e : cand) {
+      const float *ex = &cx[size_t(e) * ngnod], *ez = &cz[size_t(e) * ngnod];
+      float xi = 0.0f, ga = 0.0f;
+      for (int it = 0; it < 100; ++it) {
+        float lx, lz, dxs[9], dgs[9];
+        location_at(ex, ez, ngnod, xi, ga, lx, lz);
+        shape_deriv(xi, ga, ngnod, dxs, dgs);
+        const Geo g = jacobian_at(ex, ez, ngnod, dxs, dgs);
+        const float dx = -(lx - x), dz = -(lz - z);
+        const float dxi = g.xix * dx + g.xiz * dz;
+        const float dga = g.gammax * dx + g.gammaz * dz;
+        xi += dxi;
+        ga += dga;
+        xi = std::min(1.01f, std::max(-1.01f, xi));
+        ga = std::min(1.01f, std::max(-1.01f, ga));
+        if (std::abs(dxi) < 1e-12 && std::abs(dga) < 1e-12)
+          break;
+      }
+      float lx, lz;
+      location_at(ex, ez, ngnod, xi, ga, lx, lz);
+      const float d = std::sqrt((x - lx) * (x - lx) + (z - lz) * (z - lz));
+      if (d < final_dist) {
+        final_dist = d;
+        ispec_out = e;
+        xi_out = xi;
+        ga_out = ga;
+      }
+    }
+    if (ispec_out < 0 || std::fabs(std::fabs(xi_out) - 1.01f) < 1e-6 ||
+        std::fabs(std::fabs(ga_out) - 1.01f) < 1e-6)
+      throw SmxError(SMX_ERR_MESH, "point (" + std::to_string(xd) + ", " +
+                                       std::to_string(zd) +
+                                       ") lies outside the mesh");
+  };
+
+  // ---- sources: t0 rule (src/io/sources/dim2/read_sources.cpp:142-172),
+  // Ricker table (core/specfem/source_time_functions/ricker.cpp:55-66,
+  // impl/time_functions.hpp:77-83), array = h_xi h_gamma f
+  // (compute_source_array_from_vector.cpp:14-60, force_source.cpp:22-73).
+  const float dt = static_cast<float>(dt_in);
+  A.dt = dt;
+  A.dt_over2 = dt / 2.0;
+  A.dt2_over2 = dt * dt / 2.0;
+  A.nsrc = src.nsrc;
+  A.nstep = nstep;
+  A.src_ispec.assign(std::max(1, src.nsrc), 0);
+  A.src_elem.assign(std::max(1, src.nsrc), 0);
+  A.src_array.assign(size_t(std::max(1, src.nsrc)) * 2 * NPT, 0.0f);
+  A.stf.assign(size_t(nstep) * std::max(1, src.nsrc) * 2, 0.0f);
+  {
+    std::vector<float> t0s(src.nsrc);
+    float min_t0 = std::numeric_limits<float>::max();
+    for (int s = 0; s < src.nsrc; ++s) {
+      if (!(src.f0[s] > 0))
+        throw SmxError(SMX_ERR_ARG, "source f0 must be positive");
+      const float hdur = 1.0 / src.f0[s];
+      t0s[s] = -static_cast<float>(1.2) * hdur + src.tshift[s];
+      min_t0 = std::min(min_t0, t0s[s]);
+    }
+    A.t0 = src.nsrc ? min_t0 : 0.0;
+    const float t0 = src.nsrc ? min_t0 : 0.0f;
+    const float pi = static_cast<float>(M_PI);
+    for (int s = 0; s < src.nsrc; ++s) {
+      const float tshift = t0s[s] - min_t0;
+      const float f0 = src.f0[s];
+      for (int i = 0; i < nstep; ++i) {
+        const float t = t0 + i * dt;
+        const float tt = t - tshift;
+        const float a = pi * pi * f0 * f0;
+        const float d2 = (1.0 - 2.0 * a * tt * tt) * std::exp(-a * tt * tt);
+        const float val = src.factor[s] * d2;
+        A.stf[(size_t(i) * src.nsrc + s) * 2 + 0] = val;
+        A.stf[(size_t(i) * src.nsrc + s) * 2 + 1] = val;
+      }
+      int e;
+      float xi, ga;
+      locate(src.x[s], src.z[s], e, xi, ga);
+      A.src_ispec[s] = slot_of[e];
+      A.src_elem[s] = e;
+      float hx[NGLL], hg[NGLL];
+      interpolants(xi, q.xi, hx);
+      interpolants(ga, q.xi, hg);
+      const float ang = src.angle[s] * pi / static_cast<float>(180.0);
+      const float f[2] = {std::sin(ang), std::cos(ang)};
+      for (int iz = 0; iz < NGLL; ++iz)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const float hl = hx[ix] * hg[iz];
+          for (int c = 0; c < 2; ++c)
+            A.src_array[(size_t(s) * 2 + c) * NPT + iz * NGLL + ix] = hl * f[c];
+        }
+    }
+  }
+
+  // ---- receivers (core/specfem/assembly/receivers/dim2/receivers.cpp:63-104)
+  A.nrec = rec.nrec;
+  A.nsbs = rec.nstep_between_samples;
+  A.nsig = (nstep + A.nsbs - 1) / A.nsbs;
+  A.seis_types.assign(rec.types, rec.types + rec.ntypes);
+  for (int t : A.seis_types)
+    if (t < 0 || t > 2)
+      throw SmxError(SMX_ERR_ARG, "unsupported seismogram type");
+  A.rec_ispec.assign(std::max(1, rec.nrec), 0);
+  A.rec_elem.assign(std::max(1, rec.nrec), 0);
+  A.rec_xi.assign(std::max(1, rec.nrec), 0.0f);
+  A.rec_gamma.assign(std::max(1, rec.nrec), 0.0f);
+  A.rec_h.assign(size_t(std::max(1, rec.nrec)) * NPT, 0.0f);
+  for (int r = 0; r < rec.nrec; ++r) {
+    int e;
+    float xi, ga;
+    locate(rec.x[r], rec.z[r], e, xi, ga);
+    A.rec_ispec[r] = slot_of[e];
+    A.rec_elem[r] = e;
+    A.rec_xi[r] = xi;
+    A.rec_gamma[r] = ga;
+    float hx[NGLL], hg[NGLL];
+    interpolants(xi, q.xi, hx);
+    interpolants(ga, q.xi, hg);
+    for (int iz = 0; iz < NGLL; ++iz)
+      for (int ix = 0; ix < NGLL; ++ix)
+        A.rec_h[size_t(r) * NPT + iz * NGLL + ix] = hx[ix] * hg[iz];
+  }
+}
+
+} // namespace smx

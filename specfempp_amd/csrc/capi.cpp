@@ -1,0 +1,175 @@
+// C ABI of libspecfempp_amd (declared in include/specfempp_amd.h).  Every
+// entry point converts exceptions into a status code and a per-thread
+// message; none lets a C++ exception cross the boundary.
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+
+#include "solver2d.hpp"
+#include "specfempp_amd.h"
+
+struct smx_ctx {
+  std::unique_ptr<smx::Solver2DBase> solver;
+};
+
+namespace {
+
+thread_local std::string g_last_error;
+
+template <typename F> int guarded(F &&f) {
+  try {
+    f();
+    return SMX_OK;
+  } catch (const smx::SmxError &e) {
+    g_last_error = e.what();
+    return e.code;
+  } catch (const std::bad_alloc &) {
+    g_last_error = "host allocation failed";
+    return SMX_ERR_ARG;
+  } catch (const std::exception &e) {
+    g_last_error = e.what();
+    return SMX_ERR_ARG;
+  }
+}
+
+smx::Solver2DBase &S(smx_ctx *ctx) {
+  if (!ctx || !ctx->solver)
+    throw smx::SmxError(SMX_ERR_ARG, "null context");
+  return *ctx->solver;
+}
+
+} // namespace
+
+extern "C" {
+
+const char *smx_last_error(void) { return g_last_error.c_str(); }
+
+const char *smx_version(void) { return "specfempp_amd 0.1.0 (gfx950)"; }
+
+int smx_ctx_create2d(const smx_mesh2d *mesh, const smx_sources2d *sources,
+                     const smx_receivers2d *receivers, double dt, int nstep,
+                     int precision, smx_ctx **out) {
+  return guarded([&] {
+    if (!mesh || !sources || !receivers || !out)
+      throw smx::SmxError(SMX_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto a = std::make_unique<smx::HostAssembly2D>();
+    smx::assemble2d(*mesh, *sources, *receivers, dt, nstep, *a);
+    auto ctx = std::make_unique<smx_ctx>();
+    ctx->solver = smx::make_solver2d(std::move(a), precision);
+    *out = ctx.release();
+  });
+}
+
+int smx_ctx_destroy(smx_ctx *ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
+                 int *nrec) {
+  return guarded([&] {
+    const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
+    if (nspec) *nspec = A.nspec;
+    if (nglob) *nglob = A.nglob;
+    if (nsig) *nsig = A.nsig;
+    if (nrec) *nrec = A.nrec;
+  });
+}
+
+double smx_ctx_t0(const smx_ctx *ctx) {
+  if (!ctx || !ctx->solver)
+    return 0.0;
+  return ctx->solver->assembly().t0;
+}
+
+int smx_initialize(smx_ctx *ctx) {
+  return guarded([&] { S(ctx).initialize(); });
+}
+
+int smx_predictor(smx_ctx *ctx, int field, int *dofs) {
+  return guarded([&] {
+    const int n = S(ctx).predictor(field);
+    if (dofs) *dofs = n;
+  });
+}
+
+int smx_corrector(smx_ctx *ctx, int field, int *dofs) {
+  return guarded([&] {
+    const int n = S(ctx).corrector(field);
+    if (dofs) *dofs = n;
+  });
+}
+
+int smx_update_wavefields(smx_ctx *ctx, int field, int istep, int *nelem) {
+  return guarded([&] {
+    const int n = S(ctx).update_wavefields(field, istep);
+    if (nelem) *nelem = n;
+  });
+}
+
+int smx_compute_seismograms(smx_ctx *ctx, int field, int isig) {
+  return guarded([&] { S(ctx).compute_seismograms(field, isig); });
+}
+
+int smx_run_forward(smx_ctx *ctx, int istep0, int nstep) {
+  return guarded([&] { S(ctx).run_forward(istep0, nstep); });
+}
+
+int smx_synchronize(smx_ctx *ctx) {
+  return guarded([&] { S(ctx).synchronize(); });
+}
+
+int smx_get_seismograms(smx_ctx *ctx, void *host_out) {
+  return guarded([&] {
+    if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    S(ctx).get_seismograms(host_out);
+  });
+}
+
+int smx_get_field(smx_ctx *ctx, int field, int which, void *host_out) {
+  return guarded([&] {
+    if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    S(ctx).get_field(field, which, host_out);
+  });
+}
+
+int smx_set_field(smx_ctx *ctx, int field, int which, const void *host_in) {
+  return guarded([&] {
+    if (!host_in) throw smx::SmxError(SMX_ERR_ARG, "null input");
+    S(ctx).set_field(field, which, host_in);
+  });
+}
+
+int smx_get_ibool(smx_ctx *ctx, int *host_out) {
+  return guarded([&] {
+    if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    const auto &A = S(ctx).assembly();
+    for (int e = 0; e < A.nspec; ++e)
+      for (int p = 0; p < smx::NPT; ++p)
+        host_out[size_t(e) * smx::NPT + p] =
+            A.ibool[size_t(p) * A.nspec_pad + A.elem_slot[e]];
+  });
+}
+
+int smx_field_device_ptr(smx_ctx *ctx, int field, int which, void **dptr) {
+  return guarded([&] {
+    if (!dptr) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    *dptr = S(ctx).field_ptr(field, which);
+  });
+}
+
+int smx_last_run_stats(smx_ctx *ctx, double *ms, double *stiffness_ms,
+                       int *stiffness_launches) {
+  return guarded(
+      [&] { S(ctx).last_run_stats(ms, stiffness_ms, stiffness_launches); });
+}
+
+int smx_time_stiffness(smx_ctx *ctx, int nlaunch, double *avg_ms) {
+  return guarded([&] {
+    const double t = S(ctx).time_stiffness(nlaunch);
+    if (avg_ms) *avg_ms = t;
+  });
+}
+
+} // extern "C"

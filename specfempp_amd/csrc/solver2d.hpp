@@ -1,0 +1,40 @@
+// Device-resident 2D elastic P-SV solver: the object behind an smx_ctx.
+#pragma once
+
+#include <memory>
+
+#include "assembly2d.hpp"
+
+namespace smx {
+
+// Precision-independent interface; Solver2D<float> and Solver2D<double>
+// implement it in solver2d.hip.
+class Solver2DBase {
+public:
+  virtual ~Solver2DBase() = default;
+  virtual int precision() const = 0;
+  virtual const HostAssembly2D &assembly() const = 0;
+
+  virtual void initialize() = 0;
+  virtual int predictor(int field) = 0;
+  virtual int corrector(int field) = 0;
+  virtual int update_wavefields(int field, int istep) = 0;
+  virtual void compute_seismograms(int field, int isig) = 0;
+  virtual void run_forward(int istep0, int nstep) = 0;
+  virtual void synchronize() = 0;
+
+  virtual void get_seismograms(void *host) = 0;
+  virtual void get_field(int field, int which, void *host) = 0;
+  virtual void set_field(int field, int which, const void *host) = 0;
+  virtual void *field_ptr(int field, int which) = 0;
+  virtual void last_run_stats(double *ms, double *stiffness_ms,
+                              int *stiffness_launches) = 0;
+  // Launches only the stiffness kernel n times into a scratch acceleration
+  // and returns the mean duration in ms (HIP events on the context stream).
+  virtual double time_stiffness(int n) = 0;
+};
+
+std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
+                                            int precision);
+
+} // namespace smx

@@ -1,0 +1,231 @@
+"""Host-side mirror of the reference's time-stepping operator surface.
+
+Class and method names follow the C++ solver so a driver written against
+SPECFEM++ reads the same:
+
+* ``DomainKernels``  — specfem::kokkos_kernels::domain_kernels
+  (include/kokkos_kernels/domain_kernels.hpp:37-213)
+* ``Newmark``        — specfem::time_scheme::newmark<..., forward>
+  (core/specfem/timescheme/newmark.hpp:106-231, timescheme.hpp:16-201)
+* ``TimeMarching``   — specfem::solver::time_marching<forward, dim2, 5>
+  (core/specfem/solver/time_marching.tpp:8-131)
+
+Every operator runs on the GPU through libspecfempp_amd (C ABI); the host
+only sequences calls.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SMX_A, SMX_FIELD_FORWARD, SMX_MASS_INVERSE, SMX_U, SMX_V, check
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None
+
+
+class Context:
+    """An smx_ctx: the device-resident assembly, fields and seismograms."""
+
+    def __init__(self, mesh, sources, receivers, dt, nstep, *, nsbs=1,
+                 seismo_types=("displacement",), precision=4):
+        lib = _lib.load()
+        mesh.validate()
+        keep = []
+
+        def arr(x, dt_):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dt_))
+            keep.append(a)
+            return a
+
+        m = _lib.SmxMesh2D(mesh.nspec, mesh.ngnod, mesh.nnodes,
+                           _p(arr(mesh.coord, np.float64)), _p(arr(mesh.knods, np.int32)),
+                           _p(arr(mesh.kappa, np.float32)), _p(arr(mesh.mu, np.float32)),
+                           _p(arr(mesh.rho, np.float32)), len(mesh.abs_ispec),
+                           _p(arr(mesh.abs_ispec, np.int32)), _p(arr(mesh.abs_type, np.int32)))
+        s = _lib.SmxSources2D(len(sources), _p(arr(sources.x, np.float64)),
+                              _p(arr(sources.z, np.float64)), _p(arr(sources.angle, np.float32)),
+                              _p(arr(sources.f0, np.float32)), _p(arr(sources.tshift, np.float32)),
+                              _p(arr(sources.factor, np.float32)))
+        types = arr([_lib.SEIS_TYPES[t] for t in seismo_types], np.int32)
+        r = _lib.SmxReceivers2D(len(receivers), _p(arr(receivers.x, np.float64)),
+                                _p(arr(receivers.z, np.float64)), len(types), _p(types), int(nsbs))
+        h = ctypes.c_void_p()
+        check(lib.smx_ctx_create2d(ctypes.byref(m), ctypes.byref(s), ctypes.byref(r),
+                                   float(dt), int(nstep), int(precision), ctypes.byref(h)))
+        self.lib, self.h = lib, h
+        self.precision = precision
+        self.dtype = np.float32 if precision == 4 else np.float64
+        self.dt, self.nstep, self.nsbs = float(dt), int(nstep), int(nsbs)
+        self.seismo_types = tuple(seismo_types)
+        ns, ng, nsig, nr = (ctypes.c_int() for _ in range(4))
+        check(lib.smx_ctx_info(h, ctypes.byref(ns), ctypes.byref(ng), ctypes.byref(nsig),
+                               ctypes.byref(nr)))
+        self.nspec, self.nglob, self.nsig, self.nrec = ns.value, ng.value, nsig.value, nr.value
+        self.t0 = lib.smx_ctx_t0(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.smx_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---- raw operators --------------------------------------------------
+    def initialize(self):
+        check(self.lib.smx_initialize(self.h))
+
+    def synchronize(self):
+        check(self.lib.smx_synchronize(self.h))
+
+    def run_forward(self, istep0, nstep):
+        check(self.lib.smx_run_forward(self.h, int(istep0), int(nstep)))
+
+    def seismograms(self):
+        """(nsig, ntypes, nrec, 2) array of the recorded seismograms."""
+        out = np.empty((self.nsig, len(self.seismo_types), self.nrec, 2), dtype=self.dtype)
+        check(self.lib.smx_get_seismograms(self.h, _p(out)))
+        return out
+
+    def get_field(self, which, field=SMX_FIELD_FORWARD):
+        idx = {"u": SMX_U, "v": SMX_V, "a": SMX_A, "minv": SMX_MASS_INVERSE}[which]
+        out = np.empty((self.nglob, 2), dtype=self.dtype)
+        check(self.lib.smx_get_field(self.h, field, idx, _p(out)))
+        return out
+
+    def set_field(self, which, values, field=SMX_FIELD_FORWARD):
+        idx = {"u": SMX_U, "v": SMX_V, "a": SMX_A}[which]
+        v = np.ascontiguousarray(np.asarray(values, dtype=self.dtype).reshape(self.nglob, 2))
+        check(self.lib.smx_set_field(self.h, field, idx, _p(v)))
+
+    def ibool(self):
+        """(nspec, 5, 5) global numbering."""
+        out = np.empty((self.nspec, 5, 5), dtype=np.int32)
+        check(self.lib.smx_get_ibool(self.h, _p(out)))
+        return out
+
+    def last_run_ms(self):
+        ms, sms, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(self.lib.smx_last_run_stats(self.h, ctypes.byref(ms), ctypes.byref(sms),
+                                          ctypes.byref(n)))
+        return ms.value
+
+    def time_stiffness(self, nlaunch=20):
+        t = ctypes.c_double()
+        check(self.lib.smx_time_stiffness(self.h, int(nlaunch), ctypes.byref(t)))
+        return t.value
+
+
+class DomainKernels:
+    """specfem::kokkos_kernels::domain_kernels<forward, dim2, 5> for the
+    elastic P-SV medium."""
+
+    def __init__(self, ctx: Context, field=SMX_FIELD_FORWARD):
+        self.ctx, self.field = ctx, field
+
+    def initialize(self, dt=None):
+        # dt is fixed at assembly time (the Stacey mass term uses it)
+        self.ctx.initialize()
+
+    def update_wavefields(self, istep) -> int:
+        n = ctypes.c_int()
+        check(self.ctx.lib.smx_update_wavefields(self.ctx.h, self.field, int(istep),
+                                                 ctypes.byref(n)))
+        return n.value
+
+    def compute_seismograms(self, isig):
+        check(self.ctx.lib.smx_compute_seismograms(self.ctx.h, self.field, int(isig)))
+
+
+class Newmark:
+    """Forward Newmark time scheme (core/specfem/timescheme/newmark.hpp)."""
+
+    def __init__(self, ctx: Context, field=SMX_FIELD_FORWARD):
+        self.ctx, self.field = ctx, field
+        self.nstep, self.nsbs = ctx.nstep, ctx.nsbs
+        self.isig = 0
+
+    def iterate_forward(self):
+        for istep in range(self.nstep):
+            yield istep, self.ctx.dt
+
+    def apply_predictor_phase_forward(self) -> int:
+        n = ctypes.c_int()
+        check(self.ctx.lib.smx_predictor(self.ctx.h, self.field, ctypes.byref(n)))
+        return n.value
+
+    def apply_corrector_phase_forward(self) -> int:
+        n = ctypes.c_int()
+        check(self.ctx.lib.smx_corrector(self.ctx.h, self.field, ctypes.byref(n)))
+        return n.value
+
+    def compute_seismogram(self, istep) -> bool:
+        return istep % self.nsbs == 0
+
+    def get_seismogram_step(self) -> int:
+        return self.isig
+
+    def increment_seismogram_step(self):
+        self.isig += 1
+
+    def get_timestep(self) -> float:
+        return self.ctx.dt
+
+    def get_max_timestep(self) -> int:
+        return self.nstep
+
+    def get_max_seismogram_step(self) -> int:
+        return self.ctx.nsig
+
+
+class TimeMarching:
+    """specfem::solver::time_marching<forward, dim2, 5>.
+
+    ``run(fused=False)`` sequences the operators one by one exactly as the
+    reference loop does, with the same DOF / element-count checks;
+    ``run(fused=True)`` (default) enqueues the fused, graph-replayed step."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        self.kernels = DomainKernels(ctx)
+        self.time_scheme = Newmark(ctx)
+
+    def run(self, fused=True):
+        ctx = self.ctx
+        self.kernels.initialize(ctx.dt)
+        if fused:
+            ctx.run_forward(0, ctx.nstep)
+            ctx.synchronize()
+            return
+        total_dofs = 2 * (2 * ctx.nglob)
+        for istep, _ in self.time_scheme.iterate_forward():
+            dofs = self.time_scheme.apply_predictor_phase_forward()
+            nelem = self.kernels.update_wavefields(istep)
+            dofs += self.time_scheme.apply_corrector_phase_forward()
+            if self.time_scheme.compute_seismogram(istep):
+                self.kernels.compute_seismograms(self.time_scheme.get_seismogram_step())
+                self.time_scheme.increment_seismogram_step()
+            if dofs != total_dofs:
+                raise RuntimeError(f"The time loop has not updated all the degrees of "
+                                   f"freedom. Only {dofs} out of {total_dofs}.")
+            if nelem != ctx.nspec:
+                raise RuntimeError(f"The time loop has not updated all the elements. "
+                                   f"Only {nelem} out of {ctx.nspec}.")
+        ctx.synchronize()
+
+
+def simulate(config, *, precision=4, fused=True, nstep=None):
+    """Runs a forward simulation of a Config2D; returns (seismograms, ctx)."""
+    n = config.nstep if nstep is None else int(nstep)
+    ctx = Context(config.mesh, config.sources, config.receivers, config.dt, n,
+                  nsbs=config.nsbs, seismo_types=config.seismo_types, precision=precision)
+    TimeMarching(ctx).run(fused=fused)
+    return ctx.seismograms(), ctx

@@ -1,0 +1,89 @@
+"""CPU-side checks of the C ABI library and the host logic (no GPU calls)."""
+
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from specfempp_amd import _lib
+from specfempp_amd.config import homogeneous_box, layered_box
+from specfempp_amd.mesh import NGLL, box_mesh, elastic_moduli
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "specfempp_amd.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(smx_\w+)\s*\(", text)))
+
+
+def test_header_and_binding_agree():
+    assert declared_functions() == sorted(_lib.EXPORTS)
+
+
+def test_library_exports_every_declared_symbol():
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.skip("library not built (run __graft_entry__.build())")
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    for name in declared_functions():
+        assert hasattr(lib, name), name
+    # host-only calls that touch no device
+    lib.smx_version.restype = ctypes.c_char_p
+    assert lib.smx_version().startswith(b"specfempp_amd")
+    lib.smx_ctx_destroy.argtypes = [ctypes.c_void_p]
+    assert lib.smx_ctx_destroy(None) == 0
+    lib.smx_initialize.argtypes = [ctypes.c_void_p]
+    assert lib.smx_initialize(None) == -1  # null context is an argument error
+    lib.smx_last_error.restype = ctypes.c_char_p
+    assert b"null context" in lib.smx_last_error()
+
+
+def test_missing_library_fails_loudly(monkeypatch, tmp_path):
+    monkeypatch.setattr(_lib, "_lib", None)
+    monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
+    with pytest.raises(RuntimeError, match="no CPU fallback"):
+        _lib.load()
+
+
+def test_box_mesh_topology():
+    for ngnod in (4, 9):
+        m = box_mesh(0, 400, 0, 300, 8, 6, ngnod=ngnod, absorbing=("bottom", "left", "right"))
+        assert m.nspec == 48 and m.ngnod == ngnod
+        assert m.expected_nglob() == 33 * 25
+        # absorbing edges: 8 bottom + 6 left + 6 right
+        assert len(m.abs_ispec) == 20
+        assert set(m.abs_type.tolist()) == {1, 2, 4}
+        # corner nodes of element 0 are the box corner square
+        c = m.coord[m.knods[0, :4]]
+        np.testing.assert_allclose(c, [[0, 0], [50, 0], [50, 50], [0, 50]])
+
+
+def test_elastic_moduli_reference_promotions():
+    k, mu, rho = elastic_moduli(np.float32(2700.0), np.float32(3000.0), np.float32(1732.0))
+    assert mu == np.float32(np.float32(2700.0 * 1732.0) * np.float32(1732.0))
+    exact = 2700.0 * (np.float64(np.float32(9e6)) - (4.0 / 3.0) * 1732.0 * 1732.0)
+    assert k == np.float32(exact)
+
+
+def test_layered_config_is_seeded():
+    a = layered_box(nx=40, nz=30, nstep=10)
+    b = layered_box(nx=40, nz=30, nstep=10)
+    np.testing.assert_array_equal(a.mesh.kappa, b.mesh.kappa)
+    np.testing.assert_array_equal(a.sources.x, b.sources.x)
+    assert len(set(np.unique(a.mesh.mu).tolist())) == 6
+    rho = a.mesh.rho[:, 0, 0]
+    kappa, mu = a.mesh.kappa[:, 0, 0].astype(np.float64), a.mesh.mu[:, 0, 0].astype(np.float64)
+    vp = np.sqrt((kappa + 4 / 3 * mu) / rho)
+    np.testing.assert_allclose(rho, 310 * vp ** 0.25, rtol=1e-4)
+    assert vp.min() >= 2000 - 1 and vp.max() <= 6000 + 1
+
+
+def test_homogeneous_config():
+    c = homogeneous_box(nstep=5)
+    assert c.mesh.nspec == 4800 and c.mesh.expected_nglob() == 77361
+    assert len(c.receivers) == 50 and len(c.sources) == 1
+    assert c.mesh.kappa.shape == (4800, NGLL, NGLL)
