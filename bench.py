@@ -70,7 +70,22 @@ def main():
     ap.add_argument("--nz", type=int, default=600)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="only run 5 device-to-device copies of this many MiB (the "
+                         "known-size kernel that calibrates rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE) and exit")
     args = ap.parse_args()
+    if args.copy_probe_mb > 0:
+        import torch
+        n = args.copy_probe_mb * (1 << 20) // 4
+        x = torch.ones(n, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(5):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
+                          "launches": 5}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
                                    "design-only (see DESIGN.md)", "n_gpus": args.gpus}))

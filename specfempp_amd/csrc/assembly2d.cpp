@@ -6,6 +6,7 @@
 #include "assembly2d.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <limits>
 #include <map>
@@ -302,13 +303,14 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       cz[size_t(e) * ngnod + n] = static_cast<float>(m.coord[2 * size_t(id) + 1]);
     }
 
-  // ---- element colouring for the atomic-free scatter: two elements of one
-  // colour share no GLL point.  Conforming quads share points only through
-  // their corner nodes, so a greedy pass over the corner control nodes
-  // (per-node mask of the colours already touching it) is sufficient.
+  // ---- element colouring: two elements of one colour share no GLL point.
+  // Conforming quads share points only through their corner nodes, so a
+  // greedy pass over the corner control nodes (per-node mask of the colours
+  // already touching it) is sufficient.  The colours order the deterministic
+  // LDS assembly inside a tile and the mass-matrix scatter.
+  std::vector<int> ecolor_mesh(nspec);
   {
     std::vector<uint32_t> node_mask(m.nnodes, 0u);
-    std::vector<int> color(nspec);
     int ncolors = 0;
     for (int e = 0; e < nspec; ++e) {
       const int *kn = &m.knods[size_t(e) * ngnod];
@@ -316,77 +318,240 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       for (int c = 0; c < 4; ++c)
         used |= node_mask[kn[c]];
       int col = 0;
-      while (col < 32 && (used >> col) & 1u)
+      while (col < 32 && ((used >> col) & 1u))
         ++col;
       if (col >= 32)
         throw SmxError(SMX_ERR_MESH, "element colouring needs more than 32 colours");
-      color[e] = col;
+      ecolor_mesh[e] = col;
       ncolors = std::max(ncolors, col + 1);
       for (int c = 0; c < 4; ++c)
         node_mask[kn[c]] |= 1u << col;
     }
-    // slots: colours one after another, each padded to whole wavefronts,
-    // elements of a colour in their original order
-    std::vector<int> count(ncolors, 0);
-    for (int e = 0; e < nspec; ++e)
-      count[color[e]]++;
-    A.color_offset.assign(ncolors + 1, 0);
-    A.color_count = count;
-    for (int c = 0; c < ncolors; ++c)
-      A.color_offset[c + 1] =
-          A.color_offset[c] + (count[c] + ELEM_TILE - 1) / ELEM_TILE * ELEM_TILE;
-    A.nspec_pad = A.color_offset[ncolors];
-    A.elem_slot.assign(nspec, 0);
-    std::vector<int> fill(A.color_offset.begin(), A.color_offset.end() - 1);
-    for (int e = 0; e < nspec; ++e)
-      A.elem_slot[e] = fill[color[e]]++;
+    A.nelem_colors = ncolors;
   }
-  const int ld = A.nspec_pad;
-  const std::vector<int> &slot_of = A.elem_slot;
 
-  // ---- global numbering from corner topology (first touch in element order)
-  A.ibool.assign(size_t(NPT) * ld, 0);
-  std::unordered_map<int, int> corner_ids;
-  std::unordered_map<uint64_t, int> edge_base; // sorted corner pair -> first id
-  corner_ids.reserve(size_t(nspec) * 2);
-  edge_base.reserve(size_t(nspec) * 3);
-  int next = 0;
-  auto corner = [&](int node) {
-    auto it = corner_ids.emplace(node, next);
-    if (it.second)
-      ++next;
-    return it.first->second;
-  };
-  // an edge owns NGLL-2 consecutive ids, ordered from its lower-id corner
-  auto edge = [&](int a, int b, int k) {
-    const uint64_t key = (uint64_t(uint32_t(std::min(a, b))) << 32) |
-                         uint32_t(std::max(a, b));
-    auto it = edge_base.emplace(key, next);
-    if (it.second)
-      next += NGLL - 2;
-    const int pos = (a < b) ? k : (NGLL - 1 - k);
-    return it.first->second + pos - 1;
-  };
-  for (int e = 0; e < nspec; ++e) {
-    const int *kn = &m.knods[size_t(e) * ngnod];
-    for (int iz = 0; iz < NGLL; ++iz)
-      for (int ix = 0; ix < NGLL; ++ix) {
-        const bool b = iz == 0, t = iz == NGLL - 1, l = ix == 0,
-                   r = ix == NGLL - 1;
-        int g;
-        if (b && l) g = corner(kn[0]);
-        else if (b && r) g = corner(kn[1]);
-        else if (t && r) g = corner(kn[2]);
-        else if (t && l) g = corner(kn[3]);
-        else if (b) g = edge(kn[0], kn[1], ix);
-        else if (r) g = edge(kn[1], kn[2], iz);
-        else if (t) g = edge(kn[3], kn[2], ix);
-        else if (l) g = edge(kn[0], kn[3], iz);
-        else g = next++;
-        A.ibool[size_t(iz * NGLL + ix) * ld + slot_of[e]] = g;
-      }
+  // ---- provisional numbering from corner topology (first touch in element
+  // order); renumbered below once the tiles are known
+  std::vector<int> ib0(size_t(nspec) * NPT);
+  int nglob0 = 0;
+  {
+    std::unordered_map<int, int> corner_ids;
+    std::unordered_map<uint64_t, int> edge_base; // sorted corner pair -> first id
+    corner_ids.reserve(size_t(nspec) * 2);
+    edge_base.reserve(size_t(nspec) * 3);
+    int next = 0;
+    auto corner = [&](int node) {
+      auto it = corner_ids.emplace(node, next);
+      if (it.second)
+        ++next;
+      return it.first->second;
+    };
+    // an edge owns NGLL-2 consecutive ids, ordered from its lower-id corner
+    auto edge = [&](int a, int b, int k) {
+      const uint64_t key = (uint64_t(uint32_t(std::min(a, b))) << 32) |
+                           uint32_t(std::max(a, b));
+      auto it = edge_base.emplace(key, next);
+      if (it.second)
+        next += NGLL - 2;
+      const int pos = (a < b) ? k : (NGLL - 1 - k);
+      return it.first->second + pos - 1;
+    };
+    for (int e = 0; e < nspec; ++e) {
+      const int *kn = &m.knods[size_t(e) * ngnod];
+      for (int iz = 0; iz < NGLL; ++iz)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const bool b = iz == 0, t = iz == NGLL - 1, l = ix == 0,
+                     r = ix == NGLL - 1;
+          int g;
+          if (b && l) g = corner(kn[0]);
+          else if (b && r) g = corner(kn[1]);
+          else if (t && r) g = corner(kn[2]);
+          else if (t && l) g = corner(kn[3]);
+          else if (b) g = edge(kn[0], kn[1], ix);
+          else if (r) g = edge(kn[1], kn[2], iz);
+          else if (t) g = edge(kn[3], kn[2], ix);
+          else if (l) g = edge(kn[0], kn[3], iz);
+          else g = next++;
+          ib0[size_t(e) * NPT + iz * NGLL + ix] = g;
+        }
+    }
+    nglob0 = next;
   }
-  A.nglob = next;
+
+  // ---- tiles: elements sorted along a Morton (Z-order) curve of their
+  // centroids on a grid of the mean element size, cut into groups of
+  // ELEM_TILE.  On a structured box this yields 8x8-element squares.
+  std::vector<std::vector<int>> tile_elems;
+  {
+    std::vector<float> ccx(nspec), ccz(nspec);
+    float xmin = std::numeric_limits<float>::max(), zmin = xmin;
+    float xmax = -xmin, zmax = -zmin;
+    for (int e = 0; e < nspec; ++e) {
+      float sx = 0, sz = 0;
+      for (int c = 0; c < 4; ++c) {
+        sx += cx[size_t(e) * ngnod + c];
+        sz += cz[size_t(e) * ngnod + c];
+      }
+      ccx[e] = 0.25f * sx;
+      ccz[e] = 0.25f * sz;
+      xmin = std::min(xmin, ccx[e]);
+      xmax = std::max(xmax, ccx[e]);
+      zmin = std::min(zmin, ccz[e]);
+      zmax = std::max(zmax, ccz[e]);
+    }
+    // mean element size from the total element area
+    double area = 0.0;
+    for (int e = 0; e < nspec; ++e) {
+      const float *x = &cx[size_t(e) * ngnod], *z = &cz[size_t(e) * ngnod];
+      area += 0.5 * std::fabs(double(x[0]) * z[1] - double(x[1]) * z[0] +
+                              double(x[1]) * z[2] - double(x[2]) * z[1] +
+                              double(x[2]) * z[3] - double(x[3]) * z[2] +
+                              double(x[3]) * z[0] - double(x[0]) * z[3]);
+    }
+    const double h = std::sqrt(area / nspec);
+    auto spread = [](uint32_t v) {
+      uint64_t x = v & 0xffffffffu;
+      x = (x | (x << 16)) & 0x0000ffff0000ffffull;
+      x = (x | (x << 8)) & 0x00ff00ff00ff00ffull;
+      x = (x | (x << 4)) & 0x0f0f0f0f0f0f0f0full;
+      x = (x | (x << 2)) & 0x3333333333333333ull;
+      x = (x | (x << 1)) & 0x5555555555555555ull;
+      return x;
+    };
+    std::vector<std::pair<uint64_t, int>> key(nspec);
+    for (int e = 0; e < nspec; ++e) {
+      const uint32_t gx = uint32_t(std::max(0.0, std::floor((ccx[e] - xmin) / h + 0.5)));
+      const uint32_t gz = uint32_t(std::max(0.0, std::floor((ccz[e] - zmin) / h + 0.5)));
+      key[e] = {spread(gx) | (spread(gz) << 1), e};
+    }
+    std::sort(key.begin(), key.end());
+    for (int i = 0; i < nspec; i += ELEM_TILE) {
+      std::vector<int> t;
+      for (int k = i; k < std::min(nspec, i + ELEM_TILE); ++k)
+        t.push_back(key[k].second);
+      tile_elems.push_back(std::move(t));
+    }
+  }
+  const int ntiles = static_cast<int>(tile_elems.size());
+
+  // ---- node ownership: a node touched by one tile only is exclusive to it.
+  // Final numbering: each tile's exclusive nodes contiguously, tile after
+  // tile (Morton order), then every shared node.
+  std::vector<int> new_id(nglob0, -1);
+  std::vector<std::vector<int>> tile_shared(ntiles); // provisional ids
+  {
+    std::vector<int> touch(nglob0, 0), last(nglob0, -1);
+    for (int t = 0; t < ntiles; ++t)
+      for (int e : tile_elems[t])
+        for (int p = 0; p < NPT; ++p) {
+          const int g = ib0[size_t(e) * NPT + p];
+          if (last[g] != t) {
+            last[g] = t;
+            touch[g]++;
+          }
+        }
+    int next = 0;
+    std::vector<int> nex(ntiles, 0);
+    A.tiles.assign(ntiles, TileInfo{});
+    for (int t = 0; t < ntiles; ++t) {
+      A.tiles[t].ex_off = next;
+      for (int e : tile_elems[t])
+        for (int p = 0; p < NPT; ++p) {
+          const int g = ib0[size_t(e) * NPT + p];
+          if (touch[g] == 1 && new_id[g] < 0)
+            new_id[g] = next++;
+        }
+      A.tiles[t].nex = next - A.tiles[t].ex_off;
+    }
+    A.nex_total = next;
+    std::fill(last.begin(), last.end(), -1);
+    for (int t = 0; t < ntiles; ++t)
+      for (int e : tile_elems[t])
+        for (int p = 0; p < NPT; ++p) {
+          const int g = ib0[size_t(e) * NPT + p];
+          if (touch[g] > 1) {
+            if (new_id[g] < 0)
+              new_id[g] = next++;
+            if (last[g] != t) {
+              last[g] = t;
+              tile_shared[t].push_back(g);
+            }
+          }
+        }
+    A.nglob = next;
+    A.nshared = next - A.nex_total;
+    if (A.nglob != nglob0)
+      throw SmxError(SMX_ERR_MESH, "numbering lost nodes");
+  }
+
+  // ---- launch order: Morton order (one launch covers every tile; points
+  // shared between tiles are summed afterwards from per-tile partials)
+  std::vector<int> tile_order(ntiles);
+  for (int t = 0; t < ntiles; ++t)
+    tile_order[t] = t;
+
+  // ---- slots: launch-order tile index * ELEM_TILE + position in the tile
+  A.ntiles = ntiles;
+  A.nspec_pad = ntiles * ELEM_TILE;
+  const int ld = A.nspec_pad;
+  A.elem_slot.assign(nspec, 0);
+  A.ibool.assign(size_t(NPT) * ld, 0);
+  A.lidx.assign(size_t(NPT) * ld, 0);
+  A.ecolor.assign(ld, 0xff);
+  {
+    std::vector<TileInfo> ordered(ntiles);
+    std::vector<int> local(A.nglob, -1);
+    for (int k = 0; k < ntiles; ++k) {
+      const int t = tile_order[k];
+      TileInfo info = A.tiles[t];
+      info.ecount = static_cast<int>(tile_elems[t].size());
+      info.sh_off = static_cast<int>(A.sh_list.size());
+      info.nsh = static_cast<int>(tile_shared[t].size());
+      for (int i = 0; i < info.nsh; ++i) {
+        const int g = new_id[tile_shared[t][i]];
+        A.sh_list.push_back(g);
+        local[g] = info.nex + i;
+      }
+      if (info.nex + info.nsh > NLOC_MAX)
+        throw SmxError(SMX_ERR_MESH, "tile has too many GLL points");
+      for (int j = 0; j < info.ecount; ++j) {
+        const int e = tile_elems[t][j];
+        const int s = k * ELEM_TILE + j;
+        A.elem_slot[e] = s;
+        A.ecolor[s] = static_cast<uint8_t>(ecolor_mesh[e]);
+        for (int p = 0; p < NPT; ++p) {
+          const int g = new_id[ib0[size_t(e) * NPT + p]];
+          A.ibool[size_t(p) * ld + s] = g;
+          const int li = g < A.nex_total ? g - info.ex_off : local[g];
+          if (li < 0 || li >= info.nex + info.nsh)
+            throw SmxError(SMX_ERR_MESH, "tile-local numbering failed");
+          A.lidx[size_t(p) * ld + s] = static_cast<uint16_t>(li);
+        }
+      }
+      for (int i = 0; i < info.nsh; ++i)
+        local[A.sh_list[info.sh_off + i]] = -1;
+      ordered[k] = info;
+    }
+    A.tiles = std::move(ordered);
+  }
+  // shared point j -> its partial slots in sh_list order (tile order), the
+  // fixed summation order of the shared pass
+  {
+    A.shp_off.assign(A.nshared + 1, 0);
+    for (int g : A.sh_list)
+      A.shp_off[g - A.nex_total + 1]++;
+    for (int j = 0; j < A.nshared; ++j)
+      A.shp_off[j + 1] += A.shp_off[j];
+    A.shp_pos.assign(A.sh_list.size(), 0);
+    std::vector<int> fill(A.shp_off.begin(), A.shp_off.end() - 1);
+    for (size_t k = 0; k < A.sh_list.size(); ++k)
+      A.shp_pos[fill[A.sh_list[k] - A.nex_total]++] = static_cast<int>(k);
+    int nmax = 0;
+    for (const auto &t : A.tiles)
+      nmax = std::max(nmax, t.nex + t.nsh);
+    A.nloc_max = nmax;
+  }
+  const std::vector<int> &slot_of = A.elem_slot;
 
   // ---- geometry (core/specfem/assembly/jacobian_matrix/dim2/jacobian_matrix.cpp:28-93)
   float dN[NPT][2][9];
@@ -657,6 +822,69 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
     for (int iz = 0; iz < NGLL; ++iz)
       for (int ix = 0; ix < NGLL; ++ix)
         A.rec_h[size_t(r) * NPT + iz * NGLL + ix] = hx[ix] * hg[iz];
+  }
+
+  // ---- per-tile source entries: every point of a source's element lives in
+  // that element's tile, which injects the term into its LDS accumulator
+  {
+    std::vector<std::array<int, 3>> ent; // (tile, source, point)
+    for (int s = 0; s < src.nsrc; ++s)
+      for (int p = 0; p < NPT; ++p)
+        ent.push_back({A.src_ispec[s] / ELEM_TILE, s, p});
+    std::stable_sort(ent.begin(), ent.end(),
+                     [](const auto &a, const auto &b) { return a[0] < b[0]; });
+    for (auto &t : A.tiles) {
+      t.src_off = 0;
+      t.nsrc = 0;
+    }
+    for (size_t i = 0; i < ent.size(); ++i) {
+      const int k = ent[i][0], s = ent[i][1], p = ent[i][2];
+      TileInfo &t = A.tiles[k];
+      if (t.nsrc == 0)
+        t.src_off = static_cast<int>(i);
+      t.nsrc++;
+      A.src_li.push_back(A.lidx[size_t(p) * ld + A.src_ispec[s]]);
+      A.src_idx.push_back(s);
+      A.src_L.push_back(A.src_array[(size_t(s) * 2 + 0) * NPT + p]);
+      A.src_L.push_back(A.src_array[(size_t(s) * 2 + 1) * NPT + p]);
+    }
+  }
+
+  // ---- receiver entries (receiver r, point p) -> entry r*NPT + p.  Entries
+  // on a tile's exclusive node are evaluated by that tile; entries on shared
+  // nodes by the seismogram pass that runs before the shared-node update.
+  {
+    const int nent = A.nrec * NPT;
+    A.rent_g.assign(std::max(1, nent), 0);
+    A.rent_shared.assign(std::max(1, nent), -1);
+    A.rent_h.assign(std::max(1, nent), 0.0f);
+    std::vector<std::array<int, 3>> ent; // (tile, local node, entry)
+    for (int r = 0; r < A.nrec; ++r)
+      for (int p = 0; p < NPT; ++p) {
+        const int s = A.rec_ispec[r];
+        const int g = A.ibool[size_t(p) * ld + s];
+        const int id = r * NPT + p;
+        A.rent_g[id] = g;
+        A.rent_h[id] = A.rec_h[size_t(r) * NPT + p];
+        if (g >= A.nex_total)
+          A.rent_shared[id] = g - A.nex_total;
+        else
+          ent.push_back({s / ELEM_TILE, A.lidx[size_t(p) * ld + s], id});
+      }
+    std::stable_sort(ent.begin(), ent.end(),
+                     [](const auto &a, const auto &b) { return a[0] < b[0]; });
+    for (auto &t : A.tiles) {
+      t.rec_off = 0;
+      t.nrec = 0;
+    }
+    for (size_t i = 0; i < ent.size(); ++i) {
+      TileInfo &t = A.tiles[ent[i][0]];
+      if (t.nrec == 0)
+        t.rec_off = static_cast<int>(i);
+      t.nrec++;
+      A.trec_li.push_back(ent[i][1]);
+      A.trec_entry.push_back(ent[i][2]);
+    }
   }
 }
 

@@ -4,10 +4,16 @@
 // dim2/assembly.cpp:6-93) for one medium (elastic P-SV, isotropic): GLL
 // quadrature, Jacobian from the control nodes, global numbering, Stacey
 // normals/weights, source arrays + source time functions and receiver
-// interpolants, all in FP32 with the reference's operation order.  The output
-// is laid out for the gfx950 kernels: element-fastest structure-of-arrays
-// ([point][element], element stride padded to a multiple of 64) so that one
-// wavefront's 64 lanes read 64 consecutive elements.
+// interpolants, all in FP32 with the reference's operation order.
+//
+// Device layout.  Elements are grouped into tiles of up to ELEM_TILE
+// elements (Morton order of the element centroids; 8x8 squares on a
+// structured box); one workgroup owns one tile.  A GLL point touched by a
+// single tile is "exclusive" to it; the final numbering stores each tile's
+// exclusive points contiguously, tile after tile, followed by all points
+// shared between tiles.  Per-element arrays are
+// structure-of-arrays [point][slot] with slot = tile * ELEM_TILE + position,
+// so one wavefront's 64 lanes read 64 consecutive elements.
 #pragma once
 
 #include <cstdint>
@@ -20,8 +26,9 @@
 namespace smx {
 
 constexpr int NGLL = 5;
-constexpr int NPT = NGLL * NGLL; // GLL points per element
-constexpr int ELEM_TILE = 64;    // elements per wavefront
+constexpr int NPT = NGLL * NGLL;  // GLL points per element
+constexpr int ELEM_TILE = 64;     // elements per tile (= lanes of a wavefront)
+constexpr int NLOC_MAX = 1600;    // GLL points of a tile (64 * 25 at most)
 
 struct SmxError : std::runtime_error {
   int code;
@@ -36,27 +43,49 @@ struct Quadrature {
   Quadrature();
 };
 
+// One tile (layout shared with the device code).
+struct TileInfo {
+  int32_t ecount;  // elements in the tile
+  int32_t ex_off;  // first exclusive point (global id)
+  int32_t nex;     // exclusive points (local ids 0..nex-1)
+  int32_t sh_off;  // first entry in sh_list
+  int32_t nsh;     // shared points (local ids nex..nex+nsh-1)
+  int32_t src_off; // source entries
+  int32_t nsrc;
+  int32_t rec_off; // receiver entries on exclusive points
+  int32_t nrec;
+};
+
 struct HostAssembly2D {
   int nspec = 0, nspec_pad = 0, nglob = 0;
+  int nex_total = 0, nshared = 0; // exclusive points are ids [0, nex_total)
   Quadrature quad;
 
-  // Element slots: elements are grouped by colour (no two elements of a
-  // colour share a GLL point), each colour padded to whole wavefronts.
-  // Colour c occupies slots [color_offset[c], color_offset[c] +
-  // color_count[c]); elem_slot maps the mesh's element index to its slot.
-  std::vector<int> color_offset, color_count, elem_slot;
+  // Tiles in launch order (one workgroup each, all in one launch).
+  int ntiles = 0, nloc_max = 0;
+  std::vector<TileInfo> tiles;
+  // Global ids of each tile's shared points; a tile writes its partial sum
+  // for sh_list[k] to slot k of the partials buffer.
+  std::vector<int32_t> sh_list;
+  // Shared point j (global id nex_total + j) sums the partial slots
+  // shp_pos[shp_off[j] .. shp_off[j+1]) in that (tile) order.
+  std::vector<int32_t> shp_off, shp_pos;
 
-  // [NPT][nspec_pad] (point p = iz*NGLL + ix), indexed by slot
-  std::vector<int32_t> ibool;
+  // Element colouring (no two elements of a colour share a point).
+  int nelem_colors = 0;
+  std::vector<uint8_t> ecolor;  // [nspec_pad], 0xff for padding slots
+  std::vector<int> elem_slot;   // mesh element -> slot
+
+  std::vector<int32_t> ibool;   // [NPT][nspec_pad] global ids
+  std::vector<uint16_t> lidx;   // [NPT][nspec_pad] tile-local ids
   // [5][NPT][nspec_pad]: xix, xiz, gammax, gammaz, jacobian
   std::vector<float> geom;
   // [2][NPT][nspec_pad]: lambda+2mu, mu
   std::vector<float> lam2mu_mu;
-  // [3][NPT][nspec_pad]: kappa, mu, rho (model parameters, kept for kernels
-  // that need them outside the stiffness operator)
+  // [3][NPT][nspec_pad]: kappa, mu, rho (model parameters)
   std::vector<float> kappa_mu_rho;
 
-  // Stacey: element -> boundary slot (-1 if none); per slot [6][NPT][nst_pad]
+  // Stacey: slot -> boundary slot (-1 if none); per slot [6][NPT][nst_pad]
   // (n0, n1, |n|, edge weight (0 if the point is not on an absorbing edge),
   // rho*vp, rho*vs).
   std::vector<int32_t> stacey_slot;
@@ -64,18 +93,26 @@ struct HostAssembly2D {
   std::vector<float> stacey;
 
   // Sources: element slot (src_ispec) and mesh element (src_elem), array
-  // [nsrc][2][NPT], time function [nstep][nsrc][2].
+  // [nsrc][2][NPT], time function [nstep][nsrc][2]; per-tile entries
+  // (tile-local point, source, L_x, L_z).
   int nsrc = 0, nstep = 0;
   std::vector<int32_t> src_ispec, src_elem;
   std::vector<float> src_array;
   std::vector<float> stf;
+  std::vector<int32_t> src_li, src_idx;
+  std::vector<float> src_L;
   double t0 = 0.0;
 
-  // Receivers: element, xi, gamma, interpolant [nrec][NPT].
+  // Receivers: slot, mesh element, xi, gamma, interpolant [nrec][NPT].
   int nrec = 0;
-  std::vector<int32_t> rec_ispec, rec_elem; // slot, mesh element
+  std::vector<int32_t> rec_ispec, rec_elem;
   std::vector<float> rec_xi, rec_gamma;
   std::vector<float> rec_h;
+  // Receiver entries id = r*NPT + p: global point, shared index (-1 if the
+  // point is exclusive), weight; per-tile entries on exclusive points.
+  std::vector<int32_t> rent_g, rent_shared;
+  std::vector<float> rent_h;
+  std::vector<int32_t> trec_li, trec_entry;
   std::vector<int32_t> seis_types;
   int nsbs = 1, nsig = 0;
 

@@ -2,28 +2,34 @@
 // P-SV time step.
 //
 // Kernels (each cites the reference dispatch it replaces):
-//   k_stiffness   compute_stiffness_interaction (include/kokkos_kernels/impl/
-//                 compute_stiffness_interaction.tpp:140-247): gather, GLL
-//                 gradient, stress, J*sigma*dxi/dx, weighted divergence,
-//                 Stacey traction, scatter-add into the acceleration.
-//   k_sources     compute_source_interaction (compute_source_interaction.tpp:17-104)
-//   k_predictor   newmark predictor (core/specfem/timescheme/newmark.tpp:67-128)
-//   k_divide      divide_mass_matrix (include/kokkos_kernels/impl/divide_mass_matrix.tpp:13-65)
-//   k_corrector   newmark corrector (newmark.tpp:13-62)
-//   k_step_node   fused divide + corrector + next-step predictor (no
-//                 reference counterpart: the three range kernels in one pass)
-//   k_seismo      compute_seismograms (include/kokkos_kernels/impl/compute_seismogram.tpp:19-171)
+//   k_tile      compute_source_interaction + compute_stiffness_interaction
+//               (include/kokkos_kernels/impl/compute_source_interaction.tpp:17-104,
+//                compute_stiffness_interaction.tpp:140-247) for one tile of
+//               elements: gather, GLL gradient, stress, J*sigma*dxi/dx,
+//               weighted divergence, Stacey traction, assembly in LDS; then,
+//               for the tile's exclusive points, divide_mass_matrix
+//               (divide_mass_matrix.tpp:13-65), the Newmark corrector and the
+//               next step's predictor (core/specfem/timescheme/newmark.tpp:13-128)
+//               in the same pass.  For points shared with other tiles the
+//               tile stores its partial sum in a private slot.
+//   k_shared    sums the partials of each shared point in fixed order, then
+//               the same divide / corrector / predictor.
+//   k_seis_fused, k_seismo   compute_seismograms
+//               (include/kokkos_kernels/impl/compute_seismogram.tpp:19-171).
+//   k_predictor, k_corrector newmark predictor / corrector (operator path).
 //   k_mass/k_invert  compute_mass_matrix + invert_mass_matrix
-//                 (compute_mass_matrix.tpp:23-122, invert_mass_matrix.tpp:13-55)
+//               (compute_mass_matrix.tpp:23-122, invert_mass_matrix.tpp:13-55).
 //
-// Stiffness launch geometry: a 320-thread block (5 wavefronts) owns 64
-// consecutive elements; wavefront w handles GLL column ix = w of all 64
-// elements, lane l handles element tile*64 + l.  Element arrays are stored
-// [point][element] so every per-point load of a wavefront is one contiguous
-// 256 B (FP32) segment.  The element's displacement and the xi-direction
-// stress integrand pass between the five column wavefronts through LDS
-// ([component][point][lane], conflict-free); the gamma-direction terms stay
-// in registers of the owning column.
+// k_tile launch geometry: one 320-thread workgroup (5 wavefronts) per tile of
+// up to 64 elements; wavefront w handles GLL column ix = w of every element,
+// lane l handles element l of the tile.  The tile's points live in LDS
+// (displacement and accumulator); per-element arrays are [point][slot] so
+// each per-point load of a wavefront is one contiguous segment.  Assembly in
+// LDS runs colour by colour (elements of a colour share no point) and the
+// shared points' partials are summed in tile order, so every sum has a fixed
+// order, no atomics are used and runs are bitwise reproducible.  One launch
+// covers all tiles: a tile writes only its exclusive points and its private
+// partial slots, and reads shared points that only k_shared updates.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -65,157 +71,23 @@ template <> struct Vec16<double> {
   static constexpr int W = 2;
 };
 
-constexpr int STIFF_THREADS = NGLL * 64;
+constexpr int TILE_THREADS = NGLL * ELEM_TILE;
 constexpr int NODE_THREADS = 256;
 
-// ------------------------------------------------------------------------
-// Stiffness + Stacey + assembly.  Arithmetic follows the reference order:
-// gradient (include/algorithms/gradient.hpp:40-82), stress
-// (include/medium/dim2/elastic/isotropic/stress.hpp:38-78), F = J sigma
-// dxi/dx (core/specfem/point/stress.hpp:166-183), divergence
-// (include/algorithms/divergence.hpp:40-74), negation, Stacey
-// (include/boundary_conditions/stacey/stacey.tpp:46-80), atomic add.
-// One launch covers one element colour (slots [ebase, ebase + ecount)); no
-// two elements of a colour share a GLL point, so the scatter into A is a
-// plain read-modify-write: no atomics, and a fixed, run-to-run identical
-// summation order (colour by colour).
-template <typename real>
-__global__ void __launch_bounds__(STIFF_THREADS)
-    k_stiffness(const int *__restrict__ ibool, const real *__restrict__ geom,
-                const real *__restrict__ mat, const int *__restrict__ sslot,
-                const real *__restrict__ stacey, const real *__restrict__ U,
-                const real *__restrict__ V, real *__restrict__ A, int ebase,
-                int ecount, int ld, int sld, QuadArgs<real> q) {
-  using V2 = typename Vec2<real>::T;
-  __shared__ real su[2][NPT][64];
-  __shared__ real sf[2][NPT][64];
-  const int lane = threadIdx.x & 63;
-  const int ix = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-  const int local = blockIdx.x * 64 + lane;
-  const int e = ebase + local;
-  const bool valid = local < ecount;
-  const size_t pld = size_t(NPT) * ld;
-  const V2 *__restrict__ U2 = reinterpret_cast<const V2 *>(U);
-  const V2 *__restrict__ V2p = reinterpret_cast<const V2 *>(V);
-  V2 *__restrict__ A2 = reinterpret_cast<V2 *>(A);
+// What a tile launch does with the assembled points.
+enum TileMode : int {
+  MODE_STEP = 0,    // divide + corrector + next predictor; store u, v
+  MODE_FINAL = 1,   // divide + corrector; store v, a (end of a run)
+  MODE_ASSEMBLE = 2 // a := (a + sources + stiffness) / M (operator path)
+};
 
-  real hx[NGLL];  // l'_l(xi_ix), used by the xi-gradient
-  real hxt[NGLL]; // l'_ix(xi_l), used by the xi-divergence
-#pragma unroll
-  for (int l = 0; l < NGLL; ++l) {
-    hx[l] = q.hp[ix][l];
-    hxt[l] = q.hp[l][ix];
-  }
-  const real wix = q.w[ix];
-
-  int g[NGLL];
-  real ux[NGLL], uz[NGLL];
-#pragma unroll
-  for (int iz = 0; iz < NGLL; ++iz) {
-    const int p = iz * NGLL + ix;
-    g[iz] = ibool[size_t(p) * ld + e];
-    const V2 uu = U2[g[iz]];
-    ux[iz] = uu.x;
-    uz[iz] = uu.y;
-    su[0][p][lane] = ux[iz];
-    su[1][p][lane] = uz[iz];
-  }
-  __syncthreads();
-
-  real f1x[NGLL], f1z[NGLL];
-#pragma unroll
-  for (int iz = 0; iz < NGLL; ++iz) {
-    const int p = iz * NGLL + ix;
-    real dxi_x = 0, dxi_z = 0, dga_x = 0, dga_z = 0;
-#pragma unroll
-    for (int l = 0; l < NGLL; ++l) {
-      dxi_x += hx[l] * su[0][iz * NGLL + l][lane];
-      dga_x += q.hp[iz][l] * ux[l];
-      dxi_z += hx[l] * su[1][iz * NGLL + l][lane];
-      dga_z += q.hp[iz][l] * uz[l];
-    }
-    const size_t o = size_t(p) * ld + e;
-    const real xix = geom[o], xiz = geom[pld + o], gax = geom[2 * pld + o],
-               gaz = geom[3 * pld + o], jac = geom[4 * pld + o];
-    const real l2m = mat[o], mu = mat[pld + o];
-    const real lam = l2m - real(2.0) * mu;
-    const real du_xx = xix * dxi_x + gax * dga_x;
-    const real du_xz = xiz * dxi_x + gaz * dga_x;
-    const real du_zx = xix * dxi_z + gax * dga_z;
-    const real du_zz = xiz * dxi_z + gaz * dga_z;
-    const real sxx = l2m * du_xx + lam * du_zz;
-    const real szz = l2m * du_zz + lam * du_xx;
-    const real sxz = mu * (du_xz + du_zx);
-    sf[0][p][lane] = jac * (sxx * xix + sxz * xiz);
-    sf[1][p][lane] = jac * (sxz * xix + szz * xiz);
-    f1x[iz] = jac * (sxx * gax + sxz * gaz);
-    f1z[iz] = jac * (sxz * gax + szz * gaz);
-  }
-  __syncthreads();
-
-  const int slot = valid ? sslot[e] : -1;
-#pragma unroll
-  for (int iz = 0; iz < NGLL; ++iz) {
-    real t1x = 0, t1z = 0, t2x = 0, t2z = 0;
-#pragma unroll
-    for (int l = 0; l < NGLL; ++l) {
-      t1x += sf[0][iz * NGLL + l][lane] * hxt[l] * q.w[l];
-      t1z += sf[1][iz * NGLL + l][lane] * hxt[l] * q.w[l];
-      t2x += f1x[l] * q.hp[l][iz] * q.w[l];
-      t2z += f1z[l] * q.hp[l][iz] * q.w[l];
-    }
-    real ax = q.w[iz] * t1x + wix * t2x;
-    real az = q.w[iz] * t1z + wix * t2z;
-    ax *= real(-1.0);
-    az *= real(-1.0);
-    if (slot >= 0) {
-      const int p = iz * NGLL + ix;
-      const size_t so = size_t(p) * sld + slot;
-      const size_t sp = size_t(NPT) * sld;
-      const real n0 = stacey[so], n1 = stacey[sp + so],
-                 nrm = stacey[2 * sp + so], wgt = stacey[3 * sp + so],
-                 rvp = stacey[4 * sp + so], rvs = stacey[5 * sp + so];
-      const V2 vv = V2p[g[iz]];
-      const real vx = vv.x, vz = vv.y;
-      const real vn = vx * n0 + vz * n1;
-      const real fx = ((vn * n0 / (nrm * nrm)) * (rvp - rvs)) + vx * rvs;
-      const real fz = ((vn * n1 / (nrm * nrm)) * (rvp - rvs)) + vz * rvs;
-      ax += real(-1.0) * fx * nrm * wgt;
-      az += real(-1.0) * fz * nrm * wgt;
-    }
-    if (valid) {
-      V2 acc = A2[g[iz]];
-      acc.x += ax;
-      acc.y += az;
-      A2[g[iz]] = acc;
-    }
-  }
-}
+struct SeisTypes {
+  int t[4];
+};
 
 // ------------------------------------------------------------------------
-// Point sources: A[ibool] += stf(step) * h_xi h_gamma f
-// (include/medium/dim2/elastic/isotropic/source.hpp:36-59).
-template <typename real>
-__global__ void k_sources(real *__restrict__ A, const real *__restrict__ stf,
-                          const real *__restrict__ src_array,
-                          const int *__restrict__ src_ispec,
-                          const int *__restrict__ ibool, int ld, int nsrc,
-                          const int *__restrict__ step_ptr, int istep) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int s = t / NPT, p = t % NPT;
-  if (s >= nsrc)
-    return;
-  const int step = step_ptr ? *step_ptr : istep;
-  const int g = ibool[size_t(p) * ld + src_ispec[s]];
-#pragma unroll
-  for (int c = 0; c < 2; ++c)
-    unsafeAtomicAdd(&A[2 * size_t(g) + c],
-                    stf[(size_t(step) * nsrc + s) * 2 + c] *
-                        src_array[(size_t(s) * 2 + c) * NPT + p]);
-}
-
-// ------------------------------------------------------------------------
-// Newmark pieces on [nglob][2] arrays, 16 B per thread per array.
+// Newmark arithmetic shared by every kernel (same expressions everywhere, so
+// the fused and operator paths round identically).
 template <typename real>
 __device__ __forceinline__ void predict(real &u, real &v, real &a, real dt,
                                         real dto2, real dt2o2) {
@@ -231,6 +103,421 @@ __device__ __forceinline__ void divide_correct(real &a, real &v, real minv,
   v += dto2 * a;
 }
 
+// Sum of the tile partials of shared point j, in tile order.
+template <typename V2>
+__device__ __forceinline__ V2 shared_sum(const V2 *__restrict__ part,
+                                         const int *__restrict__ shp_off,
+                                         const int *__restrict__ shp_pos,
+                                         int j) {
+  V2 acc;
+  acc.x = 0;
+  acc.y = 0;
+  for (int k = shp_off[j]; k < shp_off[j + 1]; ++k) {
+    const V2 p = part[shp_pos[k]];
+    acc.x += p.x;
+    acc.y += p.y;
+  }
+  return acc;
+}
+
+// ------------------------------------------------------------------------
+// Dynamic LDS of k_tile: s_u[nloc_max], s_a[nloc_max] (point displacement and
+// accumulator), then a region reused as the staged element displacement, the
+// xi stress integrand and the corrected velocity of the tile's points.
+template <typename real>
+__host__ __device__ constexpr size_t tile_lds_bytes(int nloc_max) {
+  return 2 * size_t(nloc_max) * 2 * sizeof(real) +
+         (size_t(2 * NPT * ELEM_TILE) > size_t(nloc_max) * 2
+              ? size_t(2 * NPT * ELEM_TILE)
+              : size_t(nloc_max) * 2) *
+             sizeof(real);
+}
+
+// The FP32 time-step variant is held to 5 waves per SIMD (4 workgroups per
+// CU) so that enough tiles are in flight to cover memory latency.
+template <typename real, int MODE>
+__global__ void __launch_bounds__(TILE_THREADS) __attribute__((
+    amdgpu_waves_per_eu(sizeof(real) == 4 && MODE == MODE_STEP ? 5 : 1, 8)))
+    k_tile(const TileInfo *__restrict__ tiles, int nloc_max,
+           const int *__restrict__ sh_list, const uint16_t *__restrict__ lidx,
+           const uint8_t *__restrict__ ecolor, int ncolors,
+           const real *__restrict__ geom, const real *__restrict__ mat,
+           const int *__restrict__ sslot, const real *__restrict__ stacey,
+           real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
+           const real *__restrict__ Minv, real *__restrict__ part,
+           const int *__restrict__ src_li, const int *__restrict__ src_idx,
+           const real *__restrict__ src_L, const real *__restrict__ stf,
+           int nsrc_total, const int *__restrict__ trec_li,
+           const int *__restrict__ trec_entry, const real *__restrict__ rent_h,
+           real *__restrict__ seis_terms, int nentries, SeisTypes types,
+           int ntypes, int nsbs, const int *__restrict__ step_ptr, int istep,
+           int ld, int sld, real dt, real dto2, real dt2o2, QuadArgs<real> q) {
+  using V2 = typename Vec2<real>::T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  V2 *s_u = reinterpret_cast<V2 *>(lds_raw);
+  V2 *s_a = s_u + nloc_max;
+  real *xr = reinterpret_cast<real *>(s_a + nloc_max);
+  // xr as [2][NPT][ELEM_TILE]: staged displacement, then xi integrand
+  auto X = [&](int c, int p, int l) -> real & {
+    return xr[(c * NPT + p) * ELEM_TILE + l];
+  };
+  V2 *s_v = reinterpret_cast<V2 *>(xr);
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int ix = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tile = blockIdx.x;
+  const TileInfo T = tiles[tile];
+  const int nloc = T.nex + T.nsh;
+  const int slot = tile * ELEM_TILE + lane;
+  const bool valid = lane < T.ecount;
+  const size_t pld = size_t(NPT) * ld;
+  V2 *__restrict__ U2 = reinterpret_cast<V2 *>(U);
+  V2 *__restrict__ V2p = reinterpret_cast<V2 *>(V);
+  V2 *__restrict__ A2 = reinterpret_cast<V2 *>(A);
+  V2 *__restrict__ P2 = reinterpret_cast<V2 *>(part);
+  const V2 *__restrict__ M2 = reinterpret_cast<const V2 *>(Minv);
+  const int step = step_ptr ? *step_ptr : istep;
+
+  // ---- per-element data of this thread's column, issued first so its
+  // latency overlaps the gather of the tile's points
+  int li[NGLL];
+  real gxix[NGLL], gxiz[NGLL], ggax[NGLL], ggaz[NGLL], gjac[NGLL], gl2m[NGLL],
+      gmu[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    const size_t o = size_t(iz * NGLL + ix) * ld + slot;
+    li[iz] = valid ? int(lidx[o]) : 0;
+    gxix[iz] = geom[o];
+    gxiz[iz] = geom[pld + o];
+    ggax[iz] = geom[2 * pld + o];
+    ggaz[iz] = geom[3 * pld + o];
+    gjac[iz] = geom[4 * pld + o];
+    gl2m[iz] = mat[o];
+    gmu[iz] = mat[pld + o];
+  }
+
+  // ---- tile points: displacement in, accumulator cleared
+  for (int i = tid; i < nloc; i += TILE_THREADS) {
+    const int g = i < T.nex ? T.ex_off + i : sh_list[T.sh_off + i - T.nex];
+    s_u[i] = U2[g];
+    V2 z;
+    z.x = 0;
+    z.y = 0;
+    if (MODE == MODE_ASSEMBLE && i < T.nex)
+      z = A2[g];
+    s_a[i] = z;
+  }
+  __syncthreads();
+  // point sources first, in list order (one thread: fixed order)
+  if (tid == 0)
+    for (int k = T.src_off; k < T.src_off + T.nsrc; ++k) {
+      const int l = src_li[k], s = src_idx[k];
+      V2 acc = s_a[l];
+      acc.x += stf[(size_t(step) * nsrc_total + s) * 2 + 0] * src_L[2 * k];
+      acc.y += stf[(size_t(step) * nsrc_total + s) * 2 + 1] * src_L[2 * k + 1];
+      s_a[l] = acc;
+    }
+
+  // ---- element displacement (this thread: column ix, rows 0..4)
+  real ux[NGLL], uz[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    const int p = iz * NGLL + ix;
+    const V2 uu = s_u[li[iz]];
+    ux[iz] = uu.x;
+    uz[iz] = uu.y;
+    X(0, p, lane) = ux[iz];
+    X(1, p, lane) = uz[iz];
+  }
+  __syncthreads();
+
+  // ---- gradient (include/algorithms/gradient.hpp:40-82)
+  real dxi_x[NGLL], dxi_z[NGLL], dga_x[NGLL], dga_z[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    real a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      a0 += q.hp[ix][l] * X(0, iz * NGLL + l, lane);
+      b0 += q.hp[iz][l] * ux[l];
+      a1 += q.hp[ix][l] * X(1, iz * NGLL + l, lane);
+      b1 += q.hp[iz][l] * uz[l];
+    }
+    dxi_x[iz] = a0;
+    dxi_z[iz] = a1;
+    dga_x[iz] = b0;
+    dga_z[iz] = b1;
+  }
+  __syncthreads(); // the staging area is reused for the xi integrand
+
+  // ---- stress (stress.hpp:38-78) and F = J sigma dxi/dx (point/stress.hpp:166-183)
+  real f1x[NGLL], f1z[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    const int p = iz * NGLL + ix;
+    const real xix = gxix[iz], xiz = gxiz[iz], gax = ggax[iz], gaz = ggaz[iz],
+               jac = gjac[iz], l2m = gl2m[iz], mu = gmu[iz];
+    const real lam = l2m - real(2.0) * mu;
+    const real du_xx = xix * dxi_x[iz] + gax * dga_x[iz];
+    const real du_xz = xiz * dxi_x[iz] + gaz * dga_x[iz];
+    const real du_zx = xix * dxi_z[iz] + gax * dga_z[iz];
+    const real du_zz = xiz * dxi_z[iz] + gaz * dga_z[iz];
+    const real sxx = l2m * du_xx + lam * du_zz;
+    const real szz = l2m * du_zz + lam * du_xx;
+    const real sxz = mu * (du_xz + du_zx);
+    X(0, p, lane) = jac * (sxx * xix + sxz * xiz);
+    X(1, p, lane) = jac * (sxz * xix + szz * xiz);
+    f1x[iz] = jac * (sxx * gax + sxz * gaz);
+    f1z[iz] = jac * (sxz * gax + szz * gaz);
+  }
+  __syncthreads();
+
+  // ---- divergence (divergence.hpp:40-74), negation, Stacey (stacey.tpp:46-80)
+  const int bslot = valid ? sslot[slot] : -1;
+  real rx[NGLL], rz[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    real t1x = 0, t1z = 0, t2x = 0, t2z = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      t1x += X(0, iz * NGLL + l, lane) * q.hp[l][ix] * q.w[l];
+      t1z += X(1, iz * NGLL + l, lane) * q.hp[l][ix] * q.w[l];
+      t2x += f1x[l] * q.hp[l][iz] * q.w[l];
+      t2z += f1z[l] * q.hp[l][iz] * q.w[l];
+    }
+    real ax = q.w[iz] * t1x + q.w[ix] * t2x;
+    real az = q.w[iz] * t1z + q.w[ix] * t2z;
+    ax *= real(-1.0);
+    az *= real(-1.0);
+    if (bslot >= 0) {
+      const int p = iz * NGLL + ix;
+      const size_t so = size_t(p) * sld + bslot;
+      const size_t sp = size_t(NPT) * sld;
+      const real n0 = stacey[so], n1 = stacey[sp + so],
+                 nrm = stacey[2 * sp + so], wgt = stacey[3 * sp + so],
+                 rvp = stacey[4 * sp + so], rvs = stacey[5 * sp + so];
+      const int g = li[iz] < T.nex ? T.ex_off + li[iz]
+                                   : sh_list[T.sh_off + li[iz] - T.nex];
+      const V2 vv = V2p[g];
+      const real vx = vv.x, vz = vv.y;
+      const real vn = vx * n0 + vz * n1;
+      const real fx = ((vn * n0 / (nrm * nrm)) * (rvp - rvs)) + vx * rvs;
+      const real fz = ((vn * n1 / (nrm * nrm)) * (rvp - rvs)) + vz * rvs;
+      ax += real(-1.0) * fx * nrm * wgt;
+      az += real(-1.0) * fz * nrm * wgt;
+    }
+    rx[iz] = ax;
+    rz[iz] = az;
+  }
+
+  // ---- assembly into the tile accumulator, colour by colour
+  const int ecol = valid ? int(ecolor[slot]) : -1;
+  for (int c = 0; c < ncolors; ++c) {
+    if (ecol == c) {
+#pragma unroll
+      for (int iz = 0; iz < NGLL; ++iz) {
+        V2 acc = s_a[li[iz]];
+        acc.x += rx[iz];
+        acc.y += rz[iz];
+        s_a[li[iz]] = acc;
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- exclusive points: finish the step; shared points: store partials
+  const bool sample = seis_terms != nullptr && (step % nsbs) == 0;
+  for (int i = tid; i < T.nex; i += TILE_THREADS) {
+    const int g = T.ex_off + i;
+    const V2 acc = s_a[i];
+    const V2 m = M2[g];
+    if (MODE == MODE_ASSEMBLE) {
+      V2 a;
+      a.x = acc.x * m.x;
+      a.y = acc.y * m.y;
+      A2[g] = a;
+      continue;
+    }
+    V2 v = V2p[g];
+    V2 a = acc;
+    divide_correct(a.x, v.x, m.x, dto2);
+    divide_correct(a.y, v.y, m.y, dto2);
+    if (sample) {
+      s_a[i] = a;
+      s_v[i] = v;
+    }
+    if (MODE == MODE_STEP) {
+      V2 u = s_u[i];
+      predict(u.x, v.x, a.x, dt, dto2, dt2o2);
+      predict(u.y, v.y, a.y, dt, dto2, dt2o2);
+      U2[g] = u;
+      V2p[g] = v;
+    } else {
+      V2p[g] = v;
+      A2[g] = a;
+    }
+  }
+  for (int i = T.nex + tid; i < nloc; i += TILE_THREADS)
+    P2[T.sh_off + i - T.nex] = s_a[i];
+
+  // ---- receiver terms on exclusive points: h * field at step n
+  if (MODE != MODE_ASSEMBLE && sample && T.nrec > 0) {
+    __syncthreads();
+    for (int k = T.rec_off + tid; k < T.rec_off + T.nrec; k += TILE_THREADS) {
+      const int l = trec_li[k], id = trec_entry[k];
+      const real h = rent_h[id];
+      for (int it = 0; it < ntypes; ++it) {
+        const V2 f = types.t[it] == 0 ? s_u[l] : (types.t[it] == 1 ? s_v[l] : s_a[l]);
+        real *o = seis_terms + (size_t(it) * nentries + id) * 2;
+        o[0] = h * f.x;
+        o[1] = h * f.y;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------
+// Points shared between tiles: sum the tiles' partials in fixed order and
+// finish the step as k_tile does for exclusive points.  Thread 0 advances
+// the device step counter (fused modes).
+template <typename real, int MODE>
+__global__ void __launch_bounds__(NODE_THREADS)
+    k_shared(real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
+             const real *__restrict__ Minv, const real *__restrict__ part,
+             const int *__restrict__ shp_off, const int *__restrict__ shp_pos,
+             int nshared, int nex_total, real dt, real dto2, real dt2o2,
+             int *__restrict__ step_ptr) {
+  using V2 = typename Vec2<real>::T;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j == 0 && step_ptr && MODE != MODE_ASSEMBLE)
+    *step_ptr += 1;
+  if (j >= nshared)
+    return;
+  V2 *__restrict__ U2 = reinterpret_cast<V2 *>(U);
+  V2 *__restrict__ V2p = reinterpret_cast<V2 *>(V);
+  V2 *__restrict__ A2 = reinterpret_cast<V2 *>(A);
+  const V2 *__restrict__ M2 = reinterpret_cast<const V2 *>(Minv);
+  const int g = nex_total + j;
+  V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off, shp_pos, j);
+  const V2 m = M2[g];
+  if (MODE == MODE_ASSEMBLE) {
+    const V2 a0 = A2[g];
+    V2 a;
+    a.x = (a0.x + acc.x) * m.x;
+    a.y = (a0.y + acc.y) * m.y;
+    A2[g] = a;
+    return;
+  }
+  V2 v = V2p[g];
+  divide_correct(acc.x, v.x, m.x, dto2);
+  divide_correct(acc.y, v.y, m.y, dto2);
+  if (MODE == MODE_STEP) {
+    V2 u = U2[g];
+    predict(u.x, v.x, acc.x, dt, dto2, dt2o2);
+    predict(u.y, v.y, acc.y, dt, dto2, dt2o2);
+    U2[g] = u;
+    V2p[g] = v;
+  } else {
+    V2p[g] = v;
+    A2[g] = acc;
+  }
+}
+
+// ------------------------------------------------------------------------
+// Fused-path seismograms, between the tile launch and k_shared: entries on
+// exclusive points were evaluated by their tile; entries on shared points
+// are evaluated here from the tile partials with the same divide / corrector
+// arithmetic.  Sum over the 25 points in fixed order.
+template <typename real>
+__global__ void k_seis_fused(const real *__restrict__ U,
+                             const real *__restrict__ V,
+                             const real *__restrict__ Minv,
+                             const real *__restrict__ part,
+                             const int *__restrict__ shp_off,
+                             const int *__restrict__ shp_pos,
+                             const int *__restrict__ rent_g,
+                             const int *__restrict__ rent_shared,
+                             const real *__restrict__ rent_h,
+                             const real *__restrict__ seis_terms, int nrec,
+                             SeisTypes types, int ntypes, int nsbs,
+                             const int *__restrict__ step_ptr, real dto2,
+                             real *__restrict__ out) {
+  using V2 = typename Vec2<real>::T;
+  const int step = *step_ptr;
+  if (step % nsbs != 0)
+    return;
+  const int isig = step / nsbs;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = t % nrec, it = t / nrec;
+  if (it >= ntypes)
+    return;
+  const int type = types.t[it];
+  const int nentries = nrec * NPT;
+  real sx = 0, sz = 0;
+  for (int p = 0; p < NPT; ++p) {
+    const int id = r * NPT + p;
+    const int j = rent_shared[id];
+    real tx, tz;
+    if (j < 0) {
+      tx = seis_terms[(size_t(it) * nentries + id) * 2];
+      tz = seis_terms[(size_t(it) * nentries + id) * 2 + 1];
+    } else {
+      const int g = rent_g[id];
+      real fx, fz;
+      if (type == 0) {
+        fx = U[2 * size_t(g)];
+        fz = U[2 * size_t(g) + 1];
+      } else {
+        V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off,
+                            shp_pos, j);
+        real vx = V[2 * size_t(g)], vz = V[2 * size_t(g) + 1];
+        divide_correct(acc.x, vx, Minv[2 * size_t(g)], dto2);
+        divide_correct(acc.y, vz, Minv[2 * size_t(g) + 1], dto2);
+        fx = type == 1 ? vx : acc.x;
+        fz = type == 1 ? vz : acc.y;
+      }
+      const real h = rent_h[id];
+      tx = h * fx;
+      tz = h * fz;
+    }
+    sx += tx;
+    sz += tz;
+  }
+  const size_t o = ((size_t(isig) * ntypes + it) * nrec + r) * 2;
+  out[o] = sx;
+  out[o + 1] = sz;
+}
+
+// Operator-path seismograms from the finished fields: s = sum_p h_p F[ibool].
+template <typename real>
+__global__ void k_seismo(const real *__restrict__ U, const real *__restrict__ V,
+                         const real *__restrict__ A,
+                         const int *__restrict__ rent_g,
+                         const real *__restrict__ rent_h, int nrec,
+                         SeisTypes types, int ntypes, int isig,
+                         real *__restrict__ out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = t % nrec, it = t / nrec;
+  if (it >= ntypes)
+    return;
+  const int type = types.t[it];
+  const real *F = type == 0 ? U : (type == 1 ? V : A);
+  real sx = 0, sz = 0;
+  for (int p = 0; p < NPT; ++p) {
+    const int id = r * NPT + p;
+    const int g = rent_g[id];
+    const real h = rent_h[id];
+    sx += h * F[2 * size_t(g)];
+    sz += h * F[2 * size_t(g) + 1];
+  }
+  const size_t o = ((size_t(isig) * ntypes + it) * nrec + r) * 2;
+  out[o] = sx;
+  out[o + 1] = sz;
+}
+
+// ------------------------------------------------------------------------
+// Newmark pieces on [nglob][2] arrays, 16 B per thread per array.
 template <typename real>
 __global__ void __launch_bounds__(NODE_THREADS)
     k_predictor(real *__restrict__ U, real *__restrict__ V,
@@ -254,23 +541,6 @@ __global__ void __launch_bounds__(NODE_THREADS)
 
 template <typename real>
 __global__ void __launch_bounds__(NODE_THREADS)
-    k_divide(real *__restrict__ A, const real *__restrict__ Minv, long n2) {
-  using VT = typename Vec16<real>::T;
-  constexpr int W = Vec16<real>::W;
-  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i * W >= n2)
-    return;
-  real a[W], m[W];
-  *reinterpret_cast<VT *>(a) = reinterpret_cast<const VT *>(A)[i];
-  *reinterpret_cast<VT *>(m) = reinterpret_cast<const VT *>(Minv)[i];
-#pragma unroll
-  for (int k = 0; k < W; ++k)
-    a[k] = a[k] * m[k];
-  reinterpret_cast<VT *>(A)[i] = *reinterpret_cast<VT *>(a);
-}
-
-template <typename real>
-__global__ void __launch_bounds__(NODE_THREADS)
     k_corrector(real *__restrict__ V, const real *__restrict__ A, long n2,
                 real dto2) {
   using VT = typename Vec16<real>::T;
@@ -287,125 +557,39 @@ __global__ void __launch_bounds__(NODE_THREADS)
   reinterpret_cast<VT *>(V)[i] = *reinterpret_cast<VT *>(v);
 }
 
-// Divide + corrector of step n, then (if do_predict) the predictor of step
-// n+1, in one pass; thread 0 advances the device step counter.
-template <typename real>
-__global__ void __launch_bounds__(NODE_THREADS)
-    k_step_node(real *__restrict__ U, real *__restrict__ V,
-                real *__restrict__ A, const real *__restrict__ Minv, long n2,
-                real dt, real dto2, real dt2o2, int do_predict,
-                int *__restrict__ step_ptr) {
-  using VT = typename Vec16<real>::T;
-  constexpr int W = Vec16<real>::W;
-  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i == 0 && step_ptr)
-    *step_ptr += 1;
-  if (i * W >= n2)
-    return;
-  real u[W], v[W], a[W], m[W];
-  *reinterpret_cast<VT *>(v) = reinterpret_cast<const VT *>(V)[i];
-  *reinterpret_cast<VT *>(a) = reinterpret_cast<const VT *>(A)[i];
-  *reinterpret_cast<VT *>(m) = reinterpret_cast<const VT *>(Minv)[i];
-#pragma unroll
-  for (int k = 0; k < W; ++k)
-    divide_correct(a[k], v[k], m[k], dto2);
-  if (do_predict) {
-    *reinterpret_cast<VT *>(u) = reinterpret_cast<const VT *>(U)[i];
-#pragma unroll
-    for (int k = 0; k < W; ++k)
-      predict(u[k], v[k], a[k], dt, dto2, dt2o2);
-    reinterpret_cast<VT *>(U)[i] = *reinterpret_cast<VT *>(u);
-  }
-  reinterpret_cast<VT *>(V)[i] = *reinterpret_cast<VT *>(v);
-  reinterpret_cast<VT *>(A)[i] = *reinterpret_cast<VT *>(a);
-}
-
-// ------------------------------------------------------------------------
-// Seismograms: s = sum_{iz,ix} h(iz,ix) F[ibool] (interpolate_function,
-// include/algorithms/interpolate.hpp).  raw != 0 reads the acceleration
-// before division by the mass matrix and the velocity before the corrector
-// (the fused step records between stiffness and the node pass) and applies
-// the same divide/correct arithmetic per point.
-struct SeisTypes {
-  int t[4];
-};
-
-template <typename real>
-__global__ void k_seismo(const real *__restrict__ U, const real *__restrict__ V,
-                         const real *__restrict__ A,
-                         const real *__restrict__ Minv, int raw, real dto2,
-                         const int *__restrict__ rec_ispec,
-                         const real *__restrict__ rec_h,
-                         const int *__restrict__ ibool, int ld, int nrec,
-                         SeisTypes types, int ntypes, int nsbs,
-                         const int *__restrict__ step_ptr, int isig_direct,
-                         real *__restrict__ out) {
-  int isig = isig_direct;
-  if (step_ptr) {
-    const int step = *step_ptr;
-    if (step % nsbs != 0)
-      return;
-    isig = step / nsbs;
-  }
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = t % nrec, it = t / nrec;
-  if (it >= ntypes)
-    return;
-  const int type = types.t[it];
-  const int e = rec_ispec[r];
-  real sx = 0, sz = 0;
-  for (int p = 0; p < NPT; ++p) {
-    const int g = ibool[size_t(p) * ld + e];
-    const real h = rec_h[size_t(r) * NPT + p];
-    real fx, fz;
-    if (type == 0) {
-      fx = U[2 * size_t(g)];
-      fz = U[2 * size_t(g) + 1];
-    } else if (!raw) {
-      const real *F = type == 1 ? V : A;
-      fx = F[2 * size_t(g)];
-      fz = F[2 * size_t(g) + 1];
-    } else {
-      real ax = A[2 * size_t(g)], az = A[2 * size_t(g) + 1];
-      real vx = V[2 * size_t(g)], vz = V[2 * size_t(g) + 1];
-      divide_correct(ax, vx, Minv[2 * size_t(g)], dto2);
-      divide_correct(az, vz, Minv[2 * size_t(g) + 1], dto2);
-      fx = type == 1 ? vx : ax;
-      fz = type == 1 ? vz : az;
-    }
-    sx += h * fx;
-    sz += h * fz;
-  }
-  const size_t o = ((size_t(isig) * ntypes + it) * nrec + r) * 2;
-  out[o] = sx;
-  out[o + 1] = sz;
+__global__ void k_set_step(int *__restrict__ step_ptr, int value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *step_ptr = value;
 }
 
 // ------------------------------------------------------------------------
 // Mass matrix: M[ibool] += rho (w_z w_x J) + Stacey term with the velocity
 // set to (-dt/2, -dt/2) (include/boundary_conditions/stacey/stacey.tpp:817-853).
-// One launch per element colour (plain read-modify-write, as k_stiffness).
+// One launch per element colour (elements of a colour share no point, so
+// the scatter is a plain read-modify-write in a fixed order).
 template <typename real>
 __global__ void k_mass(real *__restrict__ M, const int *__restrict__ ibool,
+                       const uint8_t *__restrict__ ecolor, int color,
                        const real *__restrict__ geom,
                        const real *__restrict__ kmr,
                        const int *__restrict__ sslot,
-                       const real *__restrict__ stacey, int ebase, int ecount,
-                       int ld, int sld, QuadArgs<real> q, real dt) {
+                       const real *__restrict__ stacey, int ld, int sld,
+                       QuadArgs<real> q, real dt) {
   const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (t >= long(NPT) * ecount)
+  if (t >= long(NPT) * ld)
     return;
-  const int p = int(t / ecount), e = ebase + int(t % ecount);
+  const int p = int(t / ld), e = int(t % ld);
+  if (int(ecolor[e]) != color)
+    return;
   const size_t pld = size_t(NPT) * ld;
-  const size_t o = size_t(p) * ld + e;
   const int iz = p / NGLL, ix = p % NGLL;
-  const real rho = kmr[2 * pld + o];
-  const real jac = geom[4 * pld + o];
+  const real rho = kmr[2 * pld + t];
+  const real jac = geom[4 * pld + t];
   const real f = q.w[iz] * q.w[ix] * jac;
   real m0 = rho * f, m1 = rho * f;
-  const int slot = sslot[e];
-  if (slot >= 0) {
-    const size_t so = size_t(p) * sld + slot, sp = size_t(NPT) * sld;
+  const int bslot = sslot[e];
+  if (bslot >= 0) {
+    const size_t so = size_t(p) * sld + bslot, sp = size_t(NPT) * sld;
     const real n0 = stacey[so], n1 = stacey[sp + so], nrm = stacey[2 * sp + so],
                wgt = stacey[3 * sp + so], rvp = stacey[4 * sp + so],
                rvs = stacey[5 * sp + so];
@@ -416,14 +600,9 @@ __global__ void k_mass(real *__restrict__ M, const int *__restrict__ ibool,
     m0 += real(-1.0) * fx * nrm * wgt;
     m1 += real(-1.0) * fz * nrm * wgt;
   }
-  const int g = ibool[o];
+  const int g = ibool[t];
   M[2 * size_t(g)] += m0;
   M[2 * size_t(g) + 1] += m1;
-}
-
-__global__ void k_set_step(int *__restrict__ step_ptr, int value) {
-  if (threadIdx.x == 0 && blockIdx.x == 0)
-    *step_ptr = value;
 }
 
 template <typename real>
@@ -444,32 +623,55 @@ public:
     SMX_HIP(hipEventCreate(&ev1_));
     nglob_pad_ = (A.nglob + 255) / 256 * 256;
     n2_ = 2L * nglob_pad_;
-    ibool_ = upload(A.ibool);
+    tiles_ = upload_raw<TileInfo>(A.tiles);
+    sh_list_ = upload_raw<int>(A.sh_list);
+    shp_off_ = upload_raw<int>(A.shp_off);
+    shp_pos_ = upload_raw<int>(A.shp_pos);
+    lidx_ = upload_raw<uint16_t>(A.lidx);
+    ecolor_ = upload_raw<uint8_t>(A.ecolor);
+    ibool_ = upload_raw<int>(A.ibool);
     geom_ = upload_real(A.geom);
     mat_ = upload_real(A.lam2mu_mu);
     kmr_ = upload_real(A.kappa_mu_rho);
-    sslot_ = upload(A.stacey_slot);
+    sslot_ = upload_raw<int>(A.stacey_slot);
     stacey_ = upload_real(A.stacey);
-    src_ispec_ = upload(A.src_ispec);
-    src_array_ = upload_real(A.src_array);
+    src_li_ = upload_raw<int>(A.src_li);
+    src_idx_ = upload_raw<int>(A.src_idx);
+    src_L_ = upload_real(A.src_L);
     stf_ = upload_real(A.stf);
-    rec_ispec_ = upload(A.rec_ispec);
-    rec_h_ = upload_real(A.rec_h);
+    trec_li_ = upload_raw<int>(A.trec_li);
+    trec_entry_ = upload_raw<int>(A.trec_entry);
+    rent_g_ = upload_raw<int>(A.rent_g);
+    rent_shared_ = upload_raw<int>(A.rent_shared);
+    rent_h_ = upload_real(A.rent_h);
     minv_ = alloc<real>(n2_);
-    SMX_HIP(hipMalloc(&step_, sizeof(int)));
-    allocs_.push_back(step_);
-    const size_t nseis = size_t(std::max(1, A.nsig)) * A.seis_types.size() *
-                         std::max(1, A.nrec) * 2;
+    part_ = alloc<real>(2 * std::max<size_t>(1, A.sh_list.size()));
+    lds_bytes_ = tile_lds_bytes<real>(std::max(1, A.nloc_max));
+    SMX_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_tile<real, MODE_STEP>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes_)));
+    SMX_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_tile<real, MODE_FINAL>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes_)));
+    SMX_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_tile<real, MODE_ASSEMBLE>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes_)));
+    step_ = alloc<int>(1);
+    const int ntypes = int(A.seis_types.size());
+    const size_t nseis =
+        size_t(std::max(1, A.nsig)) * ntypes * std::max(1, A.nrec) * 2;
     seis_ = alloc<real>(std::max<size_t>(nseis, 2));
+    seis_terms_ = alloc<real>(
+        std::max<size_t>(size_t(ntypes) * A.nrec * NPT * 2, 2));
     for (int i = 0; i < NGLL; ++i) {
       quad_.w[i] = A.quad.w[i];
       for (int j = 0; j < NGLL; ++j)
         quad_.hp[i][j] = A.quad.hprime[i][j];
     }
-    for (int k = 0; k < 4; ++k)
-      stypes_.t[k] = k < int(A.seis_types.size()) ? A.seis_types[k] : 0;
-    if (A.seis_types.size() > 4)
+    if (ntypes > 4)
       throw SmxError(SMX_ERR_ARG, "at most 4 seismogram types");
+    for (int k = 0; k < 4; ++k)
+      stypes_.t[k] = k < ntypes ? A.seis_types[k] : 0;
     dt_ = A.dt;
     dto2_ = A.dt_over2;
     dt2o2_ = A.dt2_over2;
@@ -497,14 +699,11 @@ public:
     const HostAssembly2D &A = *asm_;
     real *M = minv_;
     SMX_HIP(hipMemsetAsync(M, 0, n2_ * sizeof(real), stream_));
-    for (size_t c = 0; c < A.color_count.size(); ++c) {
-      const int cnt = A.color_count[c];
-      if (cnt == 0)
-        continue;
-      const long n = long(NPT) * cnt;
+    const long n = long(NPT) * A.nspec_pad;
+    for (int c = 0; c < A.nelem_colors; ++c) {
       k_mass<real><<<dim3((n + 255) / 256), dim3(256), 0, stream_>>>(
-          M, ibool_, geom_, kmr_, sslot_, stacey_, A.color_offset[c], cnt,
-          A.nspec_pad, A.nstacey_pad, quad_, real(A.dt));
+          M, ibool_, ecolor_, c, geom_, kmr_, sslot_, stacey_, A.nspec_pad,
+          A.nstacey_pad, quad_, real(A.dt));
       SMX_HIP(hipGetLastError());
     }
     const long m = 2L * A.nglob;
@@ -516,7 +715,7 @@ public:
 
   int predictor(int f) override {
     Field &F = field(f);
-    launch_predictor(F, dt_, dto2_, dt2o2_);
+    launch_predictor(F);
     return 2 * asm_->nglob;
   }
 
@@ -534,11 +733,8 @@ public:
     if (istep < 0 || istep >= A.nstep)
       throw SmxError(SMX_ERR_ARG, "istep outside [0, nstep)");
     Field &F = field(f);
-    launch_sources(F, nullptr, istep);
-    launch_stiffness(F.u, F.v, F.a);
-    k_divide<real><<<node_grid(), dim3(NODE_THREADS), 0, stream_>>>(
-        F.a, minv_, n2_);
-    SMX_HIP(hipGetLastError());
+    launch_tiles<MODE_ASSEMBLE>(F.u, F.v, F.a, nullptr, istep, nullptr);
+    launch_shared<MODE_ASSEMBLE>(F.u, F.v, F.a, nullptr);
     return A.nspec;
   }
 
@@ -546,8 +742,14 @@ public:
     const HostAssembly2D &A = *asm_;
     if (isig < 0 || isig >= A.nsig)
       throw SmxError(SMX_ERR_ARG, "seismogram step outside [0, nsig)");
+    const int ntypes = int(A.seis_types.size());
+    if (A.nrec == 0 || ntypes == 0)
+      return;
     Field &F = field(f);
-    launch_seismo(F, /*raw=*/0, nullptr, isig);
+    const int n = A.nrec * ntypes;
+    k_seismo<real><<<dim3((n + 127) / 128), dim3(128), 0, stream_>>>(
+        F.u, F.v, F.a, rent_g_, rent_h_, A.nrec, stypes_, ntypes, isig, seis_);
+    SMX_HIP(hipGetLastError());
   }
 
   void run_forward(int istep0, int nstep) override {
@@ -559,7 +761,7 @@ public:
     SMX_HIP(hipEventRecord(ev0_, stream_));
     k_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_, istep0);
     SMX_HIP(hipGetLastError());
-    launch_predictor(F, dt_, dto2_, dt2o2_);
+    launch_predictor(F);
     const int body = nstep - 1; // steps followed by a fused predictor
     const int reps = body / kGraphSteps;
     if (reps > 0) {
@@ -568,8 +770,8 @@ public:
         SMX_HIP(hipGraphLaunch(graph_exec_, stream_));
     }
     for (int s = reps * kGraphSteps; s < body; ++s)
-      fused_step(F, /*do_predict=*/1);
-    fused_step(F, /*do_predict=*/0);
+      fused_step<MODE_STEP>(F);
+    fused_step<MODE_FINAL>(F);
     SMX_HIP(hipEventRecord(ev1_, stream_));
     last_launches_ = nstep;
     have_stats_ = true;
@@ -615,16 +817,28 @@ public:
       *launches = last_launches_;
   }
 
+  // Mean duration of one step's tile launches (every tile colour, MODE_STEP,
+  // no seismograms) on scratch copies of u and v.
   double time_stiffness(int n) override {
     if (n < 1)
       throw SmxError(SMX_ERR_ARG, "n must be positive");
+    require_init();
     Field &F = field(SMX_FIELD_FORWARD);
-    if (!scratch_a_)
-      scratch_a_ = alloc<real>(n2_);
-    launch_stiffness(F.u, F.v, scratch_a_); // warm-up
+    if (!scratch_u_) {
+      scratch_u_ = alloc<real>(n2_);
+      scratch_v_ = alloc<real>(n2_);
+      scratch_part_ = alloc<real>(2 * std::max<size_t>(1, asm_->sh_list.size()));
+    }
+    SMX_HIP(hipMemcpyAsync(scratch_u_, F.u, n2_ * sizeof(real),
+                           hipMemcpyDeviceToDevice, stream_));
+    SMX_HIP(hipMemcpyAsync(scratch_v_, F.v, n2_ * sizeof(real),
+                           hipMemcpyDeviceToDevice, stream_));
+    launch_tiles<MODE_STEP>(scratch_u_, scratch_v_, nullptr, scratch_part_, 0,
+                            nullptr); // warm-up
     SMX_HIP(hipEventRecord(ev0_, stream_));
     for (int i = 0; i < n; ++i)
-      launch_stiffness(F.u, F.v, scratch_a_);
+      launch_tiles<MODE_STEP>(scratch_u_, scratch_v_, nullptr, scratch_part_, 0,
+                              nullptr);
     SMX_HIP(hipEventRecord(ev1_, stream_));
     SMX_HIP(hipEventSynchronize(ev1_));
     float t = 0;
@@ -647,19 +861,24 @@ private:
     SMX_HIP(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), stream_));
     return static_cast<T *>(p);
   }
-  int *upload(const std::vector<int32_t> &h) {
-    int *d = alloc<int>(h.size());
-    SMX_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(int),
-                           hipMemcpyHostToDevice, stream_));
-    SMX_HIP(hipStreamSynchronize(stream_));
+  template <typename T, typename H> T *upload_raw(const std::vector<H> &h) {
+    static_assert(sizeof(T) == sizeof(H), "layout mismatch");
+    T *d = alloc<T>(h.size());
+    if (!h.empty()) {
+      SMX_HIP(hipMemcpyAsync(d, h.data(), h.size() * sizeof(T),
+                             hipMemcpyHostToDevice, stream_));
+      SMX_HIP(hipStreamSynchronize(stream_));
+    }
     return d;
   }
   real *upload_real(const std::vector<float> &h) {
     real *d = alloc<real>(h.size());
-    std::vector<real> tmp(h.begin(), h.end());
-    SMX_HIP(hipMemcpyAsync(d, tmp.data(), tmp.size() * sizeof(real),
-                           hipMemcpyHostToDevice, stream_));
-    SMX_HIP(hipStreamSynchronize(stream_));
+    if (!h.empty()) {
+      std::vector<real> tmp(h.begin(), h.end());
+      SMX_HIP(hipMemcpyAsync(d, tmp.data(), tmp.size() * sizeof(real),
+                             hipMemcpyHostToDevice, stream_));
+      SMX_HIP(hipStreamSynchronize(stream_));
+    }
     return d;
   }
 
@@ -693,54 +912,55 @@ private:
     return dim3(unsigned((n2_ / W + NODE_THREADS - 1) / NODE_THREADS));
   }
 
-  void launch_predictor(Field &F, real dt, real dto2, real dt2o2) {
+  void launch_predictor(Field &F) {
     k_predictor<real><<<node_grid(), dim3(NODE_THREADS), 0, stream_>>>(
-        F.u, F.v, F.a, n2_, dt, dto2, dt2o2);
+        F.u, F.v, F.a, n2_, dt_, dto2_, dt2o2_);
     SMX_HIP(hipGetLastError());
   }
-  void launch_sources(Field &F, const int *step_ptr, int istep) {
-    const HostAssembly2D &A = *asm_;
-    if (A.nsrc == 0)
-      return;
-    const int n = A.nsrc * NPT;
-    k_sources<real><<<dim3((n + 127) / 128), dim3(128), 0, stream_>>>(
-        F.a, stf_, src_array_, src_ispec_, ibool_, A.nspec_pad, A.nsrc,
-        step_ptr, istep);
-    SMX_HIP(hipGetLastError());
-  }
-  void launch_stiffness(const real *U, const real *V, real *Aout) {
-    const HostAssembly2D &A = *asm_;
-    for (size_t c = 0; c < A.color_count.size(); ++c) {
-      const int cnt = A.color_count[c];
-      if (cnt == 0)
-        continue;
-      k_stiffness<real><<<dim3((cnt + 63) / 64), dim3(STIFF_THREADS), 0,
-                           stream_>>>(ibool_, geom_, mat_, sslot_, stacey_, U,
-                                      V, Aout, A.color_offset[c], cnt,
-                                      A.nspec_pad, A.nstacey_pad, quad_);
-      SMX_HIP(hipGetLastError());
-    }
-  }
-  void launch_seismo(Field &F, int raw, const int *step_ptr, int isig) {
+
+  // One launch over every tile.  `part` defaults to the context's partials
+  // buffer; seismogram terms are written only when `terms` is given.
+  template <int MODE>
+  void launch_tiles(real *U, real *V, real *Aout, real *part, int istep,
+                    const int *step_ptr, real *terms = nullptr) {
     const HostAssembly2D &A = *asm_;
     const int ntypes = int(A.seis_types.size());
-    if (A.nrec == 0 || ntypes == 0)
-      return;
-    const int n = A.nrec * ntypes;
-    k_seismo<real><<<dim3((n + 127) / 128), dim3(128), 0, stream_>>>(
-        F.u, F.v, F.a, minv_, raw, dto2_, rec_ispec_, rec_h_, ibool_,
-        A.nspec_pad, A.nrec, stypes_, ntypes, A.nsbs, step_ptr, isig, seis_);
+    k_tile<real, MODE><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_,
+                         stream_>>>(
+        tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
+        mat_, sslot_, stacey_, U, V, Aout, minv_, part ? part : part_, src_li_,
+        src_idx_, src_L_, stf_, std::max(1, A.nsrc), trec_li_, trec_entry_,
+        rent_h_, terms, A.nrec * NPT, stypes_, ntypes, A.nsbs, step_ptr, istep,
+        A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_);
     SMX_HIP(hipGetLastError());
   }
-  // One fused step: sources + stiffness, seismograms from the raw
-  // acceleration, node pass (divide + corrector [+ next predictor]).
-  void fused_step(Field &F, int do_predict) {
-    launch_sources(F, step_, 0);
-    launch_stiffness(F.u, F.v, F.a);
-    launch_seismo(F, /*raw=*/1, step_, 0);
-    k_step_node<real><<<node_grid(), dim3(NODE_THREADS), 0, stream_>>>(
-        F.u, F.v, F.a, minv_, n2_, dt_, dto2_, dt2o2_, do_predict, step_);
+  template <int MODE>
+  void launch_shared(real *U, real *V, real *Aout, int *step_ptr) {
+    const HostAssembly2D &A = *asm_;
+    const int n = std::max(1, A.nshared);
+    k_shared<real, MODE><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS),
+                           dim3(NODE_THREADS), 0, stream_>>>(
+        U, V, Aout, minv_, part_, shp_off_, shp_pos_, A.nshared, A.nex_total,
+        dt_, dto2_, dt2o2_, step_ptr);
     SMX_HIP(hipGetLastError());
+  }
+  // One fused step: tiles (sources + stiffness + assembly + finish of the
+  // exclusive points), seismograms, shared points.
+  template <int MODE> void fused_step(Field &F) {
+    const HostAssembly2D &A = *asm_;
+    const int ntypes = int(A.seis_types.size());
+    const bool seis = A.nrec > 0 && ntypes > 0;
+    launch_tiles<MODE>(F.u, F.v, F.a, nullptr, 0, step_,
+                       seis ? seis_terms_ : nullptr);
+    if (seis) {
+      const int n = A.nrec * ntypes;
+      k_seis_fused<real><<<dim3((n + 127) / 128), dim3(128), 0, stream_>>>(
+          F.u, F.v, minv_, part_, shp_off_, shp_pos_, rent_g_, rent_shared_,
+          rent_h_, seis_terms_, A.nrec, stypes_, ntypes, A.nsbs, step_, dto2_,
+          seis_);
+      SMX_HIP(hipGetLastError());
+    }
+    launch_shared<MODE>(F.u, F.v, F.a, step_);
   }
   void build_graph(Field &F) {
     if (graph_exec_)
@@ -748,7 +968,7 @@ private:
     SMX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
       for (int s = 0; s < kGraphSteps; ++s)
-        fused_step(F, 1);
+        fused_step<MODE_STEP>(F);
     } catch (...) {
       hipGraph_t g;
       (void)hipStreamEndCapture(stream_, &g);
@@ -766,11 +986,20 @@ private:
   hipGraphExec_t graph_exec_ = nullptr;
   std::vector<void *> allocs_;
   long nglob_pad_ = 0, n2_ = 0;
-  int *ibool_ = nullptr, *sslot_ = nullptr, *src_ispec_ = nullptr,
-      *rec_ispec_ = nullptr, *step_ = nullptr;
+  TileInfo *tiles_ = nullptr;
+  int *sh_list_ = nullptr, *ibool_ = nullptr, *sslot_ = nullptr,
+      *src_li_ = nullptr, *src_idx_ = nullptr, *trec_li_ = nullptr,
+      *trec_entry_ = nullptr, *rent_g_ = nullptr, *rent_shared_ = nullptr,
+      *step_ = nullptr;
+  uint16_t *lidx_ = nullptr;
+  uint8_t *ecolor_ = nullptr;
   real *geom_ = nullptr, *mat_ = nullptr, *kmr_ = nullptr, *stacey_ = nullptr,
-       *src_array_ = nullptr, *stf_ = nullptr, *rec_h_ = nullptr,
-       *minv_ = nullptr, *seis_ = nullptr, *scratch_a_ = nullptr;
+       *src_L_ = nullptr, *stf_ = nullptr, *rent_h_ = nullptr,
+       *minv_ = nullptr, *part_ = nullptr, *seis_ = nullptr,
+       *seis_terms_ = nullptr, *scratch_u_ = nullptr, *scratch_v_ = nullptr,
+       *scratch_part_ = nullptr;
+  int *shp_off_ = nullptr, *shp_pos_ = nullptr;
+  size_t lds_bytes_ = 0;
   Field fields_[3];
   QuadArgs<real> quad_;
   SeisTypes stypes_;
