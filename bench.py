@@ -28,13 +28,28 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 
 
-def stiffness_bytes(nspec, nglob, nstacey):
-    """Algorithmic bytes of one stiffness launch: per local point an index,
-    5 geometry terms and (lambda+2mu, mu) (32 B x 25 = 800 B per element);
-    the displacement read once and the acceleration read and written once
-    (8 + 16 B per global point); Stacey elements also read 6 boundary terms
-    per point and the velocity (32 B x 25 per element)."""
-    return 800 * nspec + 24 * nglob + 800 * nstacey
+def tile_kernel_bytes(nspec, layout, nstacey):
+    """Algorithmic bytes of one k_tile launch (one time step, FP32):
+    * per GLL point of an element: a 2-byte tile-local index, 5 geometry terms
+      and (lambda+2mu, mu): 30 B x 25 = 750 B per element;
+    * per point exclusive to a tile: u, v, 1/M read, u, v written: 40 B;
+    * per point shared between tiles: u read once (8 B); per partial slot
+      (a tile's share of a shared point): 8 B written;
+    * Stacey elements: 6 boundary terms and the velocity per point,
+      32 B x 25 = 800 B per element."""
+    return (750 * nspec + 40 * layout["nexclusive"] + 8 * layout["nshared"]
+            + 8 * layout["npartials"] + 800 * nstacey)
+
+
+def measured_traffic(path=os.path.join(ROOT, "profiles", "k_tile_layered_800x600.md")):
+    """HBM bytes per k_tile launch on the 800x600 workload as recorded from
+    the rocprofv3 FETCH_SIZE / WRITE_SIZE passes in the profile summary."""
+    try:
+        import re
+        m = re.search(r"traffic_bytes_per_launch = (\d+)", open(path).read())
+        return int(m.group(1)) if m else None
+    except OSError:
+        return None
 
 
 def step_bytes(nspec, nglob):
@@ -70,6 +85,9 @@ def main():
     ap.add_argument("--nz", type=int, default=600)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=40)
+    ap.add_argument("--traffic", type=float, default=None,
+                    help="HBM bytes per k_tile launch measured with rocprofv3 --pmc "
+                         "(FETCH_SIZE x 2 + WRITE_SIZE, copy-kernel calibrated), if known")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="only run 5 device-to-device copies of this many MiB (the "
                          "known-size kernel that calibrates rocprofv3 FETCH_SIZE / "
@@ -118,7 +136,7 @@ def main():
     finite = bool(np.isfinite(seis).all())
 
     stiff_ms = ctx.time_stiffness(50)
-    sb = stiffness_bytes(ctx.nspec, ctx.nglob, nstacey)
+    sb = tile_kernel_bytes(ctx.nspec, ctx.layout, nstacey)
     achieved = sb / (stiff_ms * 1e-3) / 1e9
 
     value = ctx.nglob * args.steps / wall
@@ -139,10 +157,13 @@ def main():
                                f"NGLL=5, 6 seeded layers, 8 Ricker sources, 500 receivers, "
                                f"dt=4e-4",
                    "nspec": ctx.nspec, "nglob": ctx.nglob, "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "k_stiffness<float>",
+        "roofline": {"bound": "hbm", "kernel": "k_tile<float, MODE_STEP>",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "algorithmic_bytes_per_launch": sb, "mean_launch_ms": stiff_ms},
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": args.traffic if args.traffic is not None else (
+                         measured_traffic() if (args.nx, args.nz) == (800, 600) else None),
+                     "algorithmic_bytes_per_launch": sb, "mean_launch_ms": stiff_ms,
+                     "layout": ctx.layout},
         "step_roofline": {"canonical_bytes_per_step": step_bytes(ctx.nspec, ctx.nglob),
                           "achieved_GBs": step_bytes(ctx.nspec, ctx.nglob) * args.steps
                           / wall / 1e9,

@@ -19,7 +19,7 @@ SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
 # Every symbol include/specfempp_amd.h declares.
 EXPORTS = [
     "smx_last_error", "smx_version", "smx_ctx_create2d", "smx_ctx_destroy",
-    "smx_ctx_info", "smx_ctx_t0", "smx_initialize", "smx_predictor",
+    "smx_ctx_info", "smx_ctx_layout", "smx_ctx_t0", "smx_initialize", "smx_predictor",
     "smx_corrector", "smx_update_wavefields", "smx_compute_seismograms",
     "smx_run_forward", "smx_synchronize", "smx_get_seismograms", "smx_get_field",
     "smx_set_field", "smx_get_ibool", "smx_field_device_ptr", "smx_last_run_stats",
@@ -76,6 +76,7 @@ def load():
                                  ctypes.POINTER(SmxReceivers2D), d, i, i, ctypes.POINTER(P)]),
         "smx_ctx_destroy": (i, [P]),
         "smx_ctx_info": (i, [P, IP, IP, IP, IP]),
+        "smx_ctx_layout": (i, [P, IP, IP, IP, IP]),
         "smx_ctx_t0": (d, [P]),
         "smx_initialize": (i, [P]),
         "smx_predictor": (i, [P, i, IP]),

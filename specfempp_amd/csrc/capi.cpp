@@ -77,6 +77,17 @@ int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
   });
 }
 
+int smx_ctx_layout(const smx_ctx *ctx, int *ntiles, int *nexclusive,
+                   int *nshared, int *npartials) {
+  return guarded([&] {
+    const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
+    if (ntiles) *ntiles = A.ntiles;
+    if (nexclusive) *nexclusive = A.nex_total;
+    if (nshared) *nshared = A.nshared;
+    if (npartials) *npartials = static_cast<int>(A.sh_list.size());
+  });
+}
+
 double smx_ctx_t0(const smx_ctx *ctx) {
   if (!ctx || !ctx->solver)
     return 0.0;

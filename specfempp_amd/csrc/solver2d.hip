@@ -73,6 +73,10 @@ template <> struct Vec16<double> {
 
 constexpr int TILE_THREADS = NGLL * ELEM_TILE;
 constexpr int NODE_THREADS = 256;
+constexpr int SEIS_GROUPS = 8; // (type, receiver) pairs per k_seis_fused block
+// Exclusive points per k_tile thread whose node-pass inputs are requested
+// before the LDS assembly (an interior 8x8-element tile has 961 of them).
+constexpr int kNodePrefetch = 4;
 
 // What a tile launch does with the assembled points.
 enum TileMode : int {
@@ -133,11 +137,12 @@ __host__ __device__ constexpr size_t tile_lds_bytes(int nloc_max) {
              sizeof(real);
 }
 
-// The FP32 time-step variant is held to 5 waves per SIMD (4 workgroups per
-// CU) so that enough tiles are in flight to cover memory latency.
+// The FP32 time-step variant is held to 7 waves per SIMD (at most 73
+// VGPRs) so that register use does not cut below the 5 tiles per CU the LDS
+// footprint allows.
 template <typename real, int MODE>
 __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
-    amdgpu_waves_per_eu(sizeof(real) == 4 && MODE == MODE_STEP ? 5 : 1, 8)))
+    amdgpu_waves_per_eu(sizeof(real) == 4 && MODE == MODE_STEP ? 7 : 1, 8)))
     k_tile(const TileInfo *__restrict__ tiles, int nloc_max,
            const int *__restrict__ sh_list, const uint16_t *__restrict__ lidx,
            const uint8_t *__restrict__ ecolor, int ncolors,
@@ -179,23 +184,11 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
   const V2 *__restrict__ M2 = reinterpret_cast<const V2 *>(Minv);
   const int step = step_ptr ? *step_ptr : istep;
 
-  // ---- per-element data of this thread's column, issued first so its
-  // latency overlaps the gather of the tile's points
+  // tile-local ids of this thread's column (iz = 0..4, ix)
   int li[NGLL];
-  real gxix[NGLL], gxiz[NGLL], ggax[NGLL], ggaz[NGLL], gjac[NGLL], gl2m[NGLL],
-      gmu[NGLL];
 #pragma unroll
-  for (int iz = 0; iz < NGLL; ++iz) {
-    const size_t o = size_t(iz * NGLL + ix) * ld + slot;
-    li[iz] = valid ? int(lidx[o]) : 0;
-    gxix[iz] = geom[o];
-    gxiz[iz] = geom[pld + o];
-    ggax[iz] = geom[2 * pld + o];
-    ggaz[iz] = geom[3 * pld + o];
-    gjac[iz] = geom[4 * pld + o];
-    gl2m[iz] = mat[o];
-    gmu[iz] = mat[pld + o];
-  }
+  for (int iz = 0; iz < NGLL; ++iz)
+    li[iz] = valid ? int(lidx[size_t(iz * NGLL + ix) * ld + slot]) : 0;
 
   // ---- tile points: displacement in, accumulator cleared
   for (int i = tid; i < nloc; i += TILE_THREADS) {
@@ -256,8 +249,10 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
 #pragma unroll
   for (int iz = 0; iz < NGLL; ++iz) {
     const int p = iz * NGLL + ix;
-    const real xix = gxix[iz], xiz = gxiz[iz], gax = ggax[iz], gaz = ggaz[iz],
-               jac = gjac[iz], l2m = gl2m[iz], mu = gmu[iz];
+    const size_t o = size_t(p) * ld + slot;
+    const real xix = geom[o], xiz = geom[pld + o], gax = geom[2 * pld + o],
+               gaz = geom[3 * pld + o], jac = geom[4 * pld + o];
+    const real l2m = mat[o], mu = mat[pld + o];
     const real lam = l2m - real(2.0) * mu;
     const real du_xx = xix * dxi_x[iz] + gax * dga_x[iz];
     const real du_xz = xiz * dxi_x[iz] + gaz * dga_x[iz];
@@ -310,6 +305,19 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
     rx[iz] = ax;
     rz[iz] = az;
   }
+  // mass inverse and velocity of this thread's first exclusive points,
+  // requested now so their latency overlaps the colour-ordered assembly
+  // (only this thread touches these points until it finishes them)
+  V2 pre_m[kNodePrefetch], pre_v[kNodePrefetch];
+#pragma unroll
+  for (int k = 0; k < kNodePrefetch; ++k) {
+    const int i = tid + k * TILE_THREADS;
+    if (i < T.nex) {
+      pre_m[k] = M2[T.ex_off + i];
+      if (MODE != MODE_ASSEMBLE)
+        pre_v[k] = V2p[T.ex_off + i];
+    }
+  }
 
   // ---- assembly into the tile accumulator, colour by colour
   const int ecol = valid ? int(ecolor[slot]) : -1;
@@ -328,10 +336,18 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
 
   // ---- exclusive points: finish the step; shared points: store partials
   const bool sample = seis_terms != nullptr && (step % nsbs) == 0;
-  for (int i = tid; i < T.nex; i += TILE_THREADS) {
+  for (int i = tid, k = 0; i < T.nex; i += TILE_THREADS, ++k) {
     const int g = T.ex_off + i;
     const V2 acc = s_a[i];
-    const V2 m = M2[g];
+    V2 m, v;
+    if (k < kNodePrefetch) {
+      m = pre_m[k];
+      v = pre_v[k];
+    } else {
+      m = M2[g];
+      if (MODE != MODE_ASSEMBLE)
+        v = V2p[g];
+    }
     if (MODE == MODE_ASSEMBLE) {
       V2 a;
       a.x = acc.x * m.x;
@@ -339,7 +355,6 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
       A2[g] = a;
       continue;
     }
-    V2 v = V2p[g];
     V2 a = acc;
     divide_correct(a.x, v.x, m.x, dto2);
     divide_correct(a.y, v.y, m.y, dto2);
@@ -444,18 +459,20 @@ __global__ void k_seis_fused(const real *__restrict__ U,
                              const int *__restrict__ step_ptr, real dto2,
                              real *__restrict__ out) {
   using V2 = typename Vec2<real>::T;
+  // 32 threads per (type, receiver): lane p < 25 evaluates point p, lane 0
+  // then sums the 25 terms in point order.
+  __shared__ real s_t[SEIS_GROUPS][NPT][2];
   const int step = *step_ptr;
   if (step % nsbs != 0)
     return;
   const int isig = step / nsbs;
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = t % nrec, it = t / nrec;
-  if (it >= ntypes)
-    return;
-  const int type = types.t[it];
-  const int nentries = nrec * NPT;
-  real sx = 0, sz = 0;
-  for (int p = 0; p < NPT; ++p) {
+  const int grp = threadIdx.x >> 5, p = threadIdx.x & 31;
+  const int pair = blockIdx.x * SEIS_GROUPS + grp;
+  const int r = pair % nrec, it = pair / nrec;
+  const bool active = it < ntypes && p < NPT;
+  if (active) {
+    const int type = types.t[it];
+    const int nentries = nrec * NPT;
     const int id = r * NPT + p;
     const int j = rent_shared[id];
     real tx, tz;
@@ -481,12 +498,20 @@ __global__ void k_seis_fused(const real *__restrict__ U,
       tx = h * fx;
       tz = h * fz;
     }
-    sx += tx;
-    sz += tz;
+    s_t[grp][p][0] = tx;
+    s_t[grp][p][1] = tz;
   }
-  const size_t o = ((size_t(isig) * ntypes + it) * nrec + r) * 2;
-  out[o] = sx;
-  out[o + 1] = sz;
+  __syncthreads();
+  if (active && p == 0) {
+    real sx = 0, sz = 0;
+    for (int k = 0; k < NPT; ++k) {
+      sx += s_t[grp][k][0];
+      sz += s_t[grp][k][1];
+    }
+    const size_t o = ((size_t(isig) * ntypes + it) * nrec + r) * 2;
+    out[o] = sx;
+    out[o + 1] = sz;
+  }
 }
 
 // Operator-path seismograms from the finished fields: s = sum_p h_p F[ibool].
@@ -954,7 +979,8 @@ private:
                        seis ? seis_terms_ : nullptr);
     if (seis) {
       const int n = A.nrec * ntypes;
-      k_seis_fused<real><<<dim3((n + 127) / 128), dim3(128), 0, stream_>>>(
+      k_seis_fused<real><<<dim3((n + SEIS_GROUPS - 1) / SEIS_GROUPS),
+                           dim3(32 * SEIS_GROUPS), 0, stream_>>>(
           F.u, F.v, minv_, part_, shp_off_, shp_pos_, rent_g_, rent_shared_,
           rent_h_, seis_terms_, A.nrec, stypes_, ntypes, A.nsbs, step_, dto2_,
           seis_);

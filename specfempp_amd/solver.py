@@ -67,6 +67,11 @@ class Context:
                                ctypes.byref(nr)))
         self.nspec, self.nglob, self.nsig, self.nrec = ns.value, ng.value, nsig.value, nr.value
         self.t0 = lib.smx_ctx_t0(h)
+        nt, nx, nsh, npart = (ctypes.c_int() for _ in range(4))
+        check(lib.smx_ctx_layout(h, ctypes.byref(nt), ctypes.byref(nx), ctypes.byref(nsh),
+                                 ctypes.byref(npart)))
+        self.layout = {"ntiles": nt.value, "nexclusive": nx.value, "nshared": nsh.value,
+                       "npartials": npart.value}
 
     def close(self):
         if getattr(self, "h", None):

@@ -42,6 +42,22 @@ def test_library_exports_every_declared_symbol():
     assert b"null context" in lib.smx_last_error()
 
 
+def test_cpp_shim_compiles(tmp_path):
+    """The reference-named C++ classes compile against the C header."""
+    import shutil
+    import subprocess
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    src = tmp_path / "use_shim.cpp"
+    src.write_text('#include "specfempp_amd_shim.hpp"\n'
+                   "int main() { specfempp_amd::domain_kernels k(nullptr);\n"
+                   "  specfempp_amd::newmark n(nullptr, 10, 1, 1e-3f);\n"
+                   "  return n.compute_seismogram(0) ? 0 : 1; }\n")
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
+                       capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(_lib, "_lib", None)
     monkeypatch.setattr(_lib, "LIB_PATH", str(tmp_path / "missing.so"))
