@@ -49,6 +49,8 @@ def _load():
     lib.oracle2d_get.restype = ctypes.c_long
     lib.oracle2d_set_field.argtypes = [P, i, P]
     lib.oracle2d_apply_stiffness.argtypes = [P, i]
+    lib.oracle2d_forward_store.argtypes = [P, i, P]
+    lib.oracle2d_adjoint.argtypes = [P, i, P, P]
     lib.oracle2d_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -88,6 +90,7 @@ class Oracle2D:
         rx, rz = arr(receivers.x, np.float64), arr(receivers.z, np.float64)
         self.nrec = len(rx)
         self.nstep = int(nstep)
+        self.nloc = mesh.nspec * 25
         self.h = lib.oracle2d_create(precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
                                      _ptr(coord), _ptr(knods), _ptr(kappa), _ptr(mu), _ptr(rho),
                                      len(ai), _ptr(ai), _ptr(at), float(dt), int(nstep),
@@ -124,6 +127,27 @@ class Oracle2D:
     def apply_stiffness(self, with_stacey=True):
         self.lib.oracle2d_apply_stiffness(self.h, int(with_stacey))
         return self.get("a")
+
+    def forward_store(self, nstep=None):
+        """Forward run from rest keeping every state for the adjoint;
+        returns displacement seismograms (nstep, nrec, 2)."""
+        n = self.nstep if nstep is None else int(nstep)
+        out = np.zeros((n, self.nrec, 2), dtype=np.float64)
+        if self.lib.oracle2d_forward_store(self.h, n, _ptr(out)) != 0:
+            raise RuntimeError(self.lib.oracle2d_last_error().decode())
+        self._stored = n
+        return out
+
+    def adjoint(self, gsrc):
+        """Discrete-adjoint gradient (3, nspec, 5, 5) = d chi / d (kappa, mu,
+        rho) for dchi/ds = gsrc (nstep, nrec, 2), after forward_store."""
+        g = np.ascontiguousarray(np.asarray(gsrc, dtype=np.float64))
+        n = g.shape[0]
+        nloc = self.nloc
+        out = np.zeros(3 * nloc, dtype=np.float64)
+        if self.lib.oracle2d_adjoint(self.h, n, _ptr(g), _ptr(out)) != 0:
+            raise RuntimeError(self.lib.oracle2d_last_error().decode())
+        return out.reshape(3, -1, 5, 5)
 
     def run(self, nstep=None, nsbs=1, types=("displacement",)):
         """Advances nstep steps; returns seismograms (nsig, ntypes, nrec, 2)."""

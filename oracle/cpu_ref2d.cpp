@@ -829,6 +829,146 @@ template <typename real> struct Oracle2D {
     divide_mass(a);
     corrector(v, a, deltatover2);
   }
+
+  // ------------------------------------------- discrete adjoint (checker) --
+  // Exact adjoint of the time step above (not a reference routine: the
+  // reference reconstructs the forward field backwards and accumulates the
+  // Tromp et al. (2005) kernels; see ref_kernels below for that formula).
+  // Forward step k maps (u, v, a)_k to (u, v, a)_{k+1}:
+  //   u~ = u + dt v + dt^2/2 a,  v~ = v + dt/2 a,
+  //   a' = W (f_k - K u~ - B v~)  with W = 1/M' (M' = M + dt/2 rowsum B),
+  //   u' = u~,  v' = v~ + dt/2 a'.
+  // Stored per step: u_{k+1}, v~_k (before the corrector), a_{k+1}.
+  std::vector<std::vector<real>> st_u, st_v, st_a;
+
+  void forward_store(int nsteps, real *seis /* [nsteps][nrec][2] */) {
+    std::fill(u.begin(), u.end(), real(0));
+    std::fill(v.begin(), v.end(), real(0));
+    std::fill(a.begin(), a.end(), real(0));
+    st_u.assign(nsteps, {});
+    st_v.assign(nsteps, {});
+    st_a.assign(nsteps, {});
+    const int types[1] = {0};
+    for (int k = 0; k < nsteps; ++k) {
+      predictor(u, v, a, deltat, deltatover2, deltasquareover2);
+      add_sources(k, a);
+      stiffness(u, v, a);
+      divide_mass(a);
+      st_u[k] = u;
+      st_v[k] = v;
+      st_a[k] = a;
+      corrector(v, a, deltatover2);
+      seismograms(u, v, a, 1, types, seis + size_t(k) * nrec * 2);
+    }
+  }
+
+  // y += B x at the absorbing points (B v is minus the Stacey traction).
+  void apply_B(const std::vector<real> &x, std::vector<real> &y) const {
+    for (int e : order) {
+      if (!elem_stacey[e])
+        continue;
+      for (int p = 0; p < NGLL * NGLL; ++p) {
+        const int i = e * NGLL * NGLL + p;
+        if (!pt_stacey[i])
+          continue;
+        const int g = ibool[i];
+        real t[2] = {0, 0};
+        const real xv[2] = {x[2 * g], x[2 * g + 1]};
+        stacey_traction(i, xv, t);
+        y[2 * g] -= t[0];
+        y[2 * g + 1] -= t[1];
+      }
+    }
+  }
+
+  // Gradient of sum_k chi_k(u_{k+1}) w.r.t. (kappa, mu, rho) at every local
+  // point, given dchi/ds_k = gsrc[k][r][c] for the displacement seismograms.
+  void adjoint(int nsteps, const double *gsrc, double *grad /* [3][nloc] */) {
+    const size_t n2 = size_t(nglob) * 2, nloc = size_t(nspec) * NGLL * NGLL;
+    std::vector<real> ub(n2, 0), vb(n2, 0), ab(n2, 0), rb(n2, 0), Kr(n2),
+        Br(n2), zero(n2, 0);
+    std::vector<double> gk(nloc, 0), gm(nloc, 0), gr(nloc, 0);
+    const real c = deltatover2, h = deltasquareover2, dt = deltat;
+    for (int k = nsteps - 1; k >= 0; --k) {
+      // misfit source: ub += H^T g_k
+      for (int r = 0; r < nrec; ++r)
+        for (int p = 0; p < NGLL * NGLL; ++p) {
+          const int g = ibool[rec_ispec[r] * NGLL * NGLL + p];
+          const real hw = rec_h[size_t(r) * NGLL * NGLL + p];
+          for (int cc = 0; cc < 2; ++cc)
+            ub[2 * g + cc] += hw * real(gsrc[(size_t(k) * nrec + r) * 2 + cc]);
+        }
+      for (size_t i = 0; i < n2; ++i)
+        rb[i] = minv[i] * (ab[i] + c * vb[i]);
+      // K rb (no boundary) and B rb
+      std::fill(Kr.begin(), Kr.end(), real(0));
+      stiffness(rb, zero, Kr, false);
+      for (auto &x : Kr)
+        x = -x;
+      std::fill(Br.begin(), Br.end(), real(0));
+      apply_B(rb, Br);
+      // parameter gradient of step k
+      const std::vector<real> &U1 = st_u[k], &V0 = st_v[k], &A1 = st_a[k];
+      for (int e = 0; e < nspec; ++e) {
+        real dr[NGLL][NGLL][2][2], du[NGLL][NGLL][2][2];
+        element_gradient(e, rb, dr);
+        element_gradient(e, U1, du);
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const int i = idx(e, iz, ix);
+            const int g = ibool[i];
+            const double wj = double(gll.w[iz]) * gll.w[ix] * jac[i].jacobian;
+            const double divr = double(dr[iz][ix][0][0]) + dr[iz][ix][1][1];
+            const double divu = double(du[iz][ix][0][0]) + du[iz][ix][1][1];
+            const double exz_r = 0.5 * (double(dr[iz][ix][0][1]) + dr[iz][ix][1][0]);
+            const double exz_u = 0.5 * (double(du[iz][ix][0][1]) + du[iz][ix][1][0]);
+            const double ee = double(dr[iz][ix][0][0]) * du[iz][ix][0][0] +
+                              double(dr[iz][ix][1][1]) * du[iz][ix][1][1] +
+                              2.0 * exz_r * exz_u;
+            gk[i] -= wj * divr * divu;
+            gm[i] -= wj * (2.0 * ee - 2.0 / 3.0 * divr * divu);
+            gr[i] -= wj * (double(rb[2 * g]) * A1[2 * g] +
+                           double(rb[2 * g + 1]) * A1[2 * g + 1]);
+            if (elem_stacey[e] && pt_stacey[i]) {
+              const double n0 = pt_n0[i], n1 = pt_n1[i];
+              const double nrm = std::sqrt(n0 * n0 + n1 * n1);
+              const double nh0 = n0 / nrm, nh1 = n1 / nrm, wt = pt_w[i];
+              const double r0 = rb[2 * g], r1 = rb[2 * g + 1];
+              const double v0 = V0[2 * g], v1 = V0[2 * g + 1];
+              const double a0 = A1[2 * g], a1 = A1[2 * g + 1];
+              const double rn = r0 * nh0 + r1 * nh1, vn = v0 * nh0 + v1 * nh1;
+              const double s1 = nh0 + nh1;
+              const double dP = -(nrm * wt * rn * vn +
+                                  0.5 * dt * nrm * wt *
+                                      (r0 * a0 * nh0 * s1 + r1 * a1 * nh1 * s1));
+              const double dS =
+                  -(nrm * wt * (r0 * v0 + r1 * v1 - rn * vn) +
+                    0.5 * dt * nrm * wt *
+                        (r0 * a0 * (1.0 - nh0 * s1) + r1 * a1 * (1.0 - nh1 * s1)));
+              const double l2m = double(kappa[i]) + 4.0 / 3.0 * double(mu[i]);
+              const double rh = rho[i], m = mu[i];
+              const double P = std::sqrt(rh * l2m), S = std::sqrt(rh * m);
+              gk[i] += dP * rh / (2.0 * P);
+              gm[i] += dP * (4.0 / 3.0) * rh / (2.0 * P) + dS * rh / (2.0 * S);
+              gr[i] += dP * l2m / (2.0 * P) + dS * m / (2.0 * S);
+            }
+          }
+      }
+      // adjoint state update
+      for (size_t i = 0; i < n2; ++i) {
+        const real uu = ub[i] - Kr[i];
+        const real vv = vb[i] - Br[i];
+        ub[i] = uu;
+        vb[i] = vv + dt * uu;
+        ab[i] = c * vv + h * uu;
+      }
+    }
+    for (size_t i = 0; i < nloc; ++i) {
+      grad[i] = gk[i];
+      grad[nloc + i] = gm[i];
+      grad[2 * nloc + i] = gr[i];
+    }
+  }
 };
 
 // ===================================================== C interface ======
@@ -986,6 +1126,43 @@ int oracle2d_set_field(void *hv, int what, const double *in) {
     return 0;
   };
   return h->f ? set(h->f) : set(h->d);
+}
+
+// Forward run from rest storing every step's state for the adjoint;
+// displacement seismograms into seis[nsteps][nrec][2].
+int oracle2d_forward_store(void *hv, int nsteps, double *seis) {
+  auto *h = static_cast<OracleHandle *>(hv);
+  auto run = [&](auto *o) {
+    using real = std::remove_reference_t<decltype(o->deltat)>;
+    std::vector<real> buf(size_t(nsteps) * o->nrec * 2);
+    o->forward_store(nsteps, buf.data());
+    for (size_t k = 0; k < buf.size(); ++k)
+      seis[k] = buf[k];
+    return 0;
+  };
+  try {
+    return h->f ? run(h->f) : run(h->d);
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Discrete-adjoint gradient of a seismogram functional with
+// dchi/ds = gsrc[nsteps][nrec][2] (after oracle2d_forward_store); grad is
+// [3][nspec*25] for (kappa, mu, rho).
+int oracle2d_adjoint(void *hv, int nsteps, const double *gsrc, double *grad) {
+  auto *h = static_cast<OracleHandle *>(hv);
+  try {
+    if (h->f)
+      h->f->adjoint(nsteps, gsrc, grad);
+    else
+      h->d->adjoint(nsteps, gsrc, grad);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
 }
 
 // One stiffness evaluation: a <- -K u (+ Stacey with v) on the current
