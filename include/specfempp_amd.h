@@ -186,6 +186,46 @@ int smx_last_run_stats(smx_ctx *ctx, double *ms, double *stiffness_ms,
  * for compute_stiffness_interaction.tpp:121,250). */
 int smx_time_stiffness(smx_ctx *ctx, int nlaunch, double *avg_ms);
 
+/* ---- adjoint / Frechet path ------------------------------------------ */
+
+/* Replaces the material at every GLL point (kappa, mu, rho, each
+ * [nspec][5][5] in mesh element order), refreshes every derived array
+ * (lambda+2mu, the Stacey impedances) and rebuilds the mass matrix.
+ * Checkpoints of an earlier run are dropped.  Replaces reassembling
+ * specfem::assembly::properties (core/specfem/assembly/properties) for a new
+ * model. */
+int smx_set_model(smx_ctx *ctx, const float *kappa, const float *mu,
+                  const float *rho);
+
+/* Forward run of all nstep steps from rest, with seismograms as
+ * smx_run_forward, keeping (u, v) in device memory every `every` steps.
+ * Replaces the forward half of a combined simulation, which writes the last
+ * wavefield and the Stacey boundary history instead
+ * (include/io/wavefield/writer.tpp:114-200). */
+int smx_run_forward_checkpointed(smx_ctx *ctx, int every);
+
+/* Discrete-adjoint gradient of a functional chi of the displacement
+ * seismograms, given dchi/ds = gsrc[nstep][nrec][2] (element size = the
+ * context's precision; row k is the seismogram after step k).  Recomputes
+ * each checkpoint segment (newest first), recording its states, then runs
+ * the adjoint steps of the segment backwards, accumulating the kappa, mu and
+ * rho gradients.  Needs a preceding smx_run_forward_checkpointed for the
+ * current model; returns after the run.  Replaces the combined loop
+ * time_marching<combined>::run (core/specfem/solver/time_marching.tpp:133-264)
+ * with frechet_kernels::compute_derivatives
+ * (include/kokkos_kernels/frechet_kernels.hpp:39-53). */
+int smx_run_adjoint(smx_ctx *ctx, const void *gsrc);
+
+/* Gradient [3][nspec][5][5] (d chi / d kappa, d mu, d rho at every GLL
+ * point, mesh element order) of the last smx_run_adjoint, as double.
+ * Replaces reading assembly::kernels (core/specfem/assembly/kernels). */
+int smx_get_gradient(smx_ctx *ctx, double *host_out);
+
+/* Elapsed milliseconds (HIP events) of the last checkpointed forward run and
+ * of the last adjoint run (recompute included). */
+int smx_last_adjoint_stats(smx_ctx *ctx, double *forward_ms,
+                           double *adjoint_ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -23,7 +23,8 @@ EXPORTS = [
     "smx_corrector", "smx_update_wavefields", "smx_compute_seismograms",
     "smx_run_forward", "smx_synchronize", "smx_get_seismograms", "smx_get_field",
     "smx_set_field", "smx_get_ibool", "smx_field_device_ptr", "smx_last_run_stats",
-    "smx_time_stiffness",
+    "smx_time_stiffness", "smx_set_model", "smx_run_forward_checkpointed",
+    "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
 ]
 
 
@@ -92,6 +93,11 @@ def load():
         "smx_field_device_ptr": (i, [P, i, i, ctypes.POINTER(P)]),
         "smx_last_run_stats": (i, [P, DP, DP, IP]),
         "smx_time_stiffness": (i, [P, i, DP]),
+        "smx_set_model": (i, [P, P, P, P]),
+        "smx_run_forward_checkpointed": (i, [P, i]),
+        "smx_run_adjoint": (i, [P, P]),
+        "smx_get_gradient": (i, [P, P]),
+        "smx_last_adjoint_stats": (i, [P, DP, DP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -1,0 +1,104 @@
+"""Adjoint (Frechet) path: seismograms as a differentiable function of the
+material, and the L2 waveform misfit.
+
+The reference computes material kernels with a combined simulation that
+re-runs the forward field backwards in time next to the adjoint field
+(core/specfem/solver/time_marching.tpp:133-264) and accumulates them in
+frechet_kernels::compute_derivatives
+(include/kokkos_kernels/frechet_kernels.hpp:39-53,
+include/kokkos_kernels/impl/compute_material_derivatives.tpp:14-153).  Here the
+gradient is the exact discrete adjoint of the GPU time step: the forward run
+keeps its state every ``every`` steps, the adjoint run recomputes each
+segment (newest first) and steps the adjoint state backwards through it, all
+in device memory.  The gradient is d chi / d (kappa, mu, rho) at every GLL
+point, i.e. the reference's kernels already integrated against the
+quadrature weights and Jacobian.
+
+``Seismograms`` wraps this as a ``torch.autograd.Function``: its forward
+returns the displacement seismograms, its backward turns the incoming
+d loss / d seismogram into the adjoint source.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+from .solver import Context
+
+
+def l2_misfit(s, d, dt):
+    """chi = 1/2 dt sum (s - d)^2 over steps, receivers and components."""
+    r = np.asarray(s, dtype=np.float64) - np.asarray(d, dtype=np.float64)
+    return 0.5 * dt * float(np.sum(r * r))
+
+
+def _displacement_index(sim: Context) -> int:
+    if "displacement" not in sim.seismo_types:
+        raise ValueError("the context must record displacement seismograms")
+    return sim.seismo_types.index("displacement")
+
+
+def forward_seismograms(sim: Context, kappa, mu, rho, every=100):
+    """Sets the model, runs the checkpointed forward and returns the
+    displacement seismograms (nsig, nrec, 2)."""
+    sim.set_model(kappa, mu, rho)
+    sim.run_forward_checkpointed(every)
+    sim.synchronize()
+    sim._model_version = getattr(sim, "_model_version", 0) + 1
+    return sim.seismograms()[:, _displacement_index(sim)]
+
+
+def adjoint_gradient(sim: Context, dchi_ds):
+    """Gradient (3, nspec, 5, 5) for dchi/ds = dchi_ds (nsig, nrec, 2), after
+    forward_seismograms on the same context.  Seismogram i was recorded after
+    step i * nsbs; the other steps get no adjoint source."""
+    g = np.asarray(dchi_ds, dtype=np.float64)
+    if g.shape != (sim.nsig, sim.nrec, 2):
+        raise ValueError(f"dchi/ds must have shape {(sim.nsig, sim.nrec, 2)}, got {g.shape}")
+    gsrc = np.zeros((sim.nstep, sim.nrec, 2), dtype=np.float64)
+    gsrc[::sim.nsbs][:sim.nsig] = g
+    sim.run_adjoint(gsrc)
+    return sim.gradient()
+
+
+def misfit_and_gradient(sim: Context, model, observed, every=100):
+    """L2 waveform misfit against ``observed`` (nsig, nrec, 2) and its
+    gradient with respect to ``model`` = (kappa, mu, rho)."""
+    s = forward_seismograms(sim, *model, every=every)
+    dt_rec = sim.dt * sim.nsbs
+    chi = l2_misfit(s, observed, dt_rec)
+    grad = adjoint_gradient(sim, dt_rec * (s.astype(np.float64) - observed))
+    return chi, grad
+
+
+try:  # torch is plumbing for the autograd interface, not a compute path
+    import torch
+
+    class Seismograms(torch.autograd.Function):
+        """s = Seismograms.apply(kappa, mu, rho, sim, every): displacement
+        seismograms (nsig, nrec, 2) of context ``sim`` for the material
+        tensors (nspec, 5, 5); differentiable in kappa, mu and rho."""
+
+        @staticmethod
+        def forward(ctx, kappa, mu, rho, sim, every=100):
+            host = [t.detach().to("cpu", torch.float32).numpy() for t in (kappa, mu, rho)]
+            s = forward_seismograms(sim, *host, every=every)
+            ctx.sim = sim
+            ctx.version = sim._model_version
+            ctx.like = kappa
+            return torch.from_numpy(np.array(s)).to(kappa.device)
+
+        @staticmethod
+        def backward(ctx, grad_s):
+            sim = ctx.sim
+            if sim._model_version != ctx.version:
+                raise RuntimeError("the context ran another model since this forward; "
+                                   "its checkpoints are gone")
+            g = adjoint_gradient(sim, grad_s.detach().to("cpu", torch.float64).numpy())
+            like = ctx.like
+            out = [torch.from_numpy(g[i]).to(device=like.device, dtype=like.dtype)
+                   for i in range(3)]
+            return out[0], out[1], out[2], None, None
+
+except ImportError:  # pragma: no cover - torch is part of the image
+    Seismograms = None

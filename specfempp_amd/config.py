@@ -15,7 +15,7 @@ from dataclasses import dataclass
 
 import numpy as np
 
-from .mesh import Mesh2D, box_mesh
+from .mesh import Mesh2D, box_mesh, elastic_moduli, gll_coordinates
 
 
 @dataclass
@@ -110,3 +110,27 @@ def layered_box(nx=800, nz=600, h=50.0, nstep=5000, dt=4e-4, nsrc=8, nrec=500,
     return Config2D(f"layered_{nx}x{nz}", mesh, src, rec, dt, nstep,
                     meta={"nx": nx, "nz": nz, "h": h, "vp_layers": vps.tolist(),
                           "seed": seed})
+
+
+def gaussian_vs_anomaly(mesh: Mesh2D, x0=2000.0, z0=1500.0, sigma=300.0, amp=0.05,
+                        rho=2700.0, vp=3000.0, vs=1732.0):
+    """(kappa, mu, rho) at every GLL point of a homogeneous medium whose vs
+    carries a Gaussian anomaly vs (1 + amp exp(-r^2 / (2 sigma^2)))."""
+    xz = gll_coordinates(mesh)
+    r2 = (xz[..., 0] - x0) ** 2 + (xz[..., 1] - z0) ** 2
+    vs_pt = vs * (1.0 + amp * np.exp(-r2 / (2.0 * sigma * sigma)))
+    shape = vs_pt.shape
+    return elastic_moduli(np.full(shape, rho, np.float32), np.full(shape, vp, np.float32),
+                          vs_pt.astype(np.float32))
+
+
+def adjoint_case(nx=80, nz=60, nstep=2000, dt=5e-4, nrec=50):
+    """Adjoint-gradient workload: the 80x60 homogeneous box, its source and
+    50 receivers; the true model adds a 5% Gaussian vs anomaly (sigma 300 m)
+    at (2000 m, 1500 m).  Returns (config with the homogeneous starting
+    model, true (kappa, mu, rho))."""
+    cfg = homogeneous_box(nx=nx, nz=nz, nstep=nstep, dt=dt, nrec=nrec)
+    h = cfg.meta["h"]
+    true = gaussian_vs_anomaly(cfg.mesh, x0=0.5 * nx * h, z0=0.5 * nz * h)
+    cfg.name = f"adjoint_{nx}x{nz}"
+    return cfg, true

@@ -274,6 +274,8 @@ Quadrature::Quadrature() {
     }
 }
 
+void refresh_boundary_matrices(HostAssembly2D &A);
+
 void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
                 const smx_receivers2d &rec, double dt_in, int nstep,
                 HostAssembly2D &A) {
@@ -886,6 +888,113 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       A.trec_entry.push_back(ent[i][2]);
     }
   }
+
+  // ---- receiver entries grouped by node (adjoint source injection in a
+  // fixed order)
+  {
+    std::map<int, std::vector<int>> by_node;
+    for (int id = 0; id < A.nrec * NPT; ++id)
+      by_node[A.rent_g[id]].push_back(id);
+    A.rin_off.assign(1, 0);
+    for (auto &kv : by_node) {
+      A.rin_node.push_back(kv.first);
+      for (int id : kv.second)
+        A.rin_ent.push_back(id);
+      A.rin_off.push_back(static_cast<int>(A.rin_ent.size()));
+    }
+    A.sh_rin.assign(std::max(1, A.nshared), -1);
+    for (size_t j = 0; j < A.rin_node.size(); ++j) {
+      const int g = A.rin_node[j];
+      if (g >= A.nex_total)
+        A.sh_rin[g - A.nex_total] = static_cast<int>(j);
+      else
+        A.rin_excl.push_back(static_cast<int>(j));
+    }
+  }
+
+  // ---- absorbing points and nodes
+  {
+    std::map<int, int> node_index;
+    for (int s = 0; s < ld; ++s) {
+      const int bs = A.stacey_slot[s];
+      if (bs < 0)
+        continue;
+      for (int p = 0; p < NPT; ++p) {
+        if (A.stacey[(3 * size_t(NPT) + p) * sld + bs] == 0.0f)
+          continue; // not on an absorbing edge
+        const int g = A.ibool[size_t(p) * ld + s];
+        A.bpt_slot.push_back(s);
+        A.bpt_p.push_back(p);
+        A.bpt_bslot.push_back(bs);
+        A.bpt_g.push_back(g);
+        if (node_index.emplace(g, static_cast<int>(A.bnode_g.size())).second)
+          A.bnode_g.push_back(g);
+      }
+    }
+    refresh_boundary_matrices(A);
+  }
+}
+
+void refresh_boundary_matrices(HostAssembly2D &A) {
+  const int sld = A.nstacey_pad;
+  std::unordered_map<int, int> index;
+  for (size_t j = 0; j < A.bnode_g.size(); ++j)
+    index[A.bnode_g[j]] = static_cast<int>(j);
+  A.bsum.assign(3 * std::max<size_t>(1, A.bnode_g.size()), 0.0);
+  std::vector<double> &acc = A.bsum;
+  for (size_t k = 0; k < A.bpt_g.size(); ++k) {
+    const int p = A.bpt_p[k], bs = A.bpt_bslot[k];
+    const size_t sp = size_t(NPT) * sld;
+    const double n0 = A.stacey[size_t(p) * sld + bs],
+                 n1 = A.stacey[sp + size_t(p) * sld + bs],
+                 nrm = A.stacey[2 * sp + size_t(p) * sld + bs],
+                 w = A.stacey[3 * sp + size_t(p) * sld + bs],
+                 P = A.stacey[4 * sp + size_t(p) * sld + bs],
+                 S = A.stacey[5 * sp + size_t(p) * sld + bs];
+    const double h0 = n0 / nrm, h1 = n1 / nrm, f = nrm * w;
+    const int j = index[A.bpt_g[k]];
+    acc[3 * j + 0] += f * ((P - S) * h0 * h0 + S);
+    acc[3 * j + 1] += f * ((P - S) * h0 * h1);
+    acc[3 * j + 2] += f * ((P - S) * h1 * h1 + S);
+  }
+}
+
+void set_model2d(HostAssembly2D &A, const float *kappa, const float *mu,
+                 const float *rho) {
+  const int ld = A.nspec_pad, sld = A.nstacey_pad;
+  const size_t pld = size_t(NPT) * ld;
+  auto l2m_of = [](float k, float m) {
+    return k + (static_cast<float>(4.0 / 3.0)) * m;
+  };
+  for (int e = 0; e < A.nspec; ++e) // validate before changing anything
+    for (int p = 0; p < NPT; ++p) {
+      const size_t i = size_t(e) * NPT + p;
+      if (!(rho[i] > 0) || !(mu[i] >= 0) || !(kappa[i] > 0))
+        throw SmxError(SMX_ERR_ARG, "non-physical material at element " +
+                                        std::to_string(e));
+    }
+  for (int e = 0; e < A.nspec; ++e)
+    for (int p = 0; p < NPT; ++p) {
+      const size_t i = size_t(e) * NPT + p;
+      const size_t o = size_t(p) * ld + A.elem_slot[e];
+      A.lam2mu_mu[o] = l2m_of(kappa[i], mu[i]);
+      A.lam2mu_mu[pld + o] = mu[i];
+      A.kappa_mu_rho[o] = kappa[i];
+      A.kappa_mu_rho[pld + o] = mu[i];
+      A.kappa_mu_rho[2 * pld + o] = rho[i];
+    }
+  // Stacey impedances (include/medium/dim2/elastic/isotropic/properties.hpp:77-87)
+  const size_t sp = size_t(NPT) * sld;
+  for (size_t k = 0; k < A.bpt_g.size(); ++k) {
+    const int p = A.bpt_p[k], bs = A.bpt_bslot[k], s = A.bpt_slot[k];
+    const size_t o = size_t(p) * ld + s;
+    const float kk = A.kappa_mu_rho[o], mm = A.kappa_mu_rho[pld + o],
+                rr = A.kappa_mu_rho[2 * pld + o];
+    const float l2m = l2m_of(kk, mm);
+    A.stacey[4 * sp + size_t(p) * sld + bs] = std::sqrt(l2m / rr) * rr;
+    A.stacey[5 * sp + size_t(p) * sld + bs] = std::sqrt(rr * mm);
+  }
+  refresh_boundary_matrices(A);
 }
 
 } // namespace smx

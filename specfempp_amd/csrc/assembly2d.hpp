@@ -116,6 +116,20 @@ struct HostAssembly2D {
   std::vector<int32_t> seis_types;
   int nsbs = 1, nsig = 0;
 
+  // Adjoint support.  Receiver points as a node list: node rin_node[j]
+  // collects the entries rin_ent[rin_off[j] .. rin_off[j+1]) (ascending).
+  std::vector<int32_t> rin_node, rin_off, rin_ent;
+  // Receiver node of each shared point (index into rin_node, -1 if none) and
+  // the receiver nodes that are exclusive points.
+  std::vector<int32_t> sh_rin, rin_excl;
+  // Absorbing GLL points (slot, point, boundary slot, global node), and the
+  // absorbing nodes with the 2x2 boundary matrix summed over their points
+  // (bsum: xx, xz, zz, in double from the FP32 Stacey arrays), so that B v at
+  // node g is bsum(g) v(g).
+  std::vector<int32_t> bpt_slot, bpt_p, bpt_bslot, bpt_g;
+  std::vector<int32_t> bnode_g;
+  std::vector<double> bsum;
+
   // Time scheme constants as the reference forms them (newmark.hpp:131-135).
   float dt = 0, dt_over2 = 0, dt2_over2 = 0;
 };
@@ -124,5 +138,11 @@ struct HostAssembly2D {
 void assemble2d(const smx_mesh2d &mesh, const smx_sources2d &sources,
                 const smx_receivers2d &receivers, double dt, int nstep,
                 HostAssembly2D &out);
+
+// Replaces the material at every GLL point (arrays [nspec][5][5] in mesh
+// element order) and refreshes every quantity derived from it (lambda+2mu,
+// mu, the Stacey impedances and boundary matrices).
+void set_model2d(HostAssembly2D &A, const float *kappa, const float *mu,
+                 const float *rho);
 
 } // namespace smx

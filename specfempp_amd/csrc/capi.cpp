@@ -183,4 +183,36 @@ int smx_time_stiffness(smx_ctx *ctx, int nlaunch, double *avg_ms) {
   });
 }
 
+int smx_set_model(smx_ctx *ctx, const float *kappa, const float *mu,
+                  const float *rho) {
+  return guarded([&] {
+    if (!kappa || !mu || !rho) throw smx::SmxError(SMX_ERR_ARG, "null model");
+    S(ctx).set_model(kappa, mu, rho);
+  });
+}
+
+int smx_run_forward_checkpointed(smx_ctx *ctx, int every) {
+  return guarded([&] { S(ctx).run_forward_checkpointed(every); });
+}
+
+int smx_run_adjoint(smx_ctx *ctx, const void *gsrc) {
+  return guarded([&] {
+    if (!gsrc && S(ctx).assembly().nrec > 0)
+      throw smx::SmxError(SMX_ERR_ARG, "null adjoint source");
+    S(ctx).run_adjoint(gsrc);
+  });
+}
+
+int smx_get_gradient(smx_ctx *ctx, double *host_out) {
+  return guarded([&] {
+    if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    S(ctx).get_gradient(host_out);
+  });
+}
+
+int smx_last_adjoint_stats(smx_ctx *ctx, double *forward_ms,
+                           double *adjoint_ms) {
+  return guarded([&] { S(ctx).last_adjoint_stats(forward_ms, adjoint_ms); });
+}
+
 } // extern "C"

@@ -82,8 +82,19 @@ constexpr int kNodePrefetch = 4;
 enum TileMode : int {
   MODE_STEP = 0,    // divide + corrector + next predictor; store u, v
   MODE_FINAL = 1,   // divide + corrector; store v, a (end of a run)
-  MODE_ASSEMBLE = 2 // a := (a + sources + stiffness) / M (operator path)
+  MODE_ASSEMBLE = 2, // a := (a + sources + stiffness) / M (operator path)
+  MODE_RECORD = 3    // MODE_STEP that also keeps (u~, v~, a') for the adjoint
 };
+
+// Forward-state record of step k, slot j = k mod every, in a buffer of
+// every+1 slots of three [nglob_pad][2] arrays: the predicted displacement
+// and velocity entering step k and the acceleration it produces.
+template <typename V2>
+__device__ __forceinline__ V2 *record_ptr(void *rec, long stride, int j,
+                                          int c) {
+  return reinterpret_cast<V2 *>(reinterpret_cast<char *>(rec) +
+                                (size_t(j) * 3 + c) * size_t(stride));
+}
 
 struct SeisTypes {
   int t[4];
@@ -155,8 +166,9 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
            int nsrc_total, const int *__restrict__ trec_li,
            const int *__restrict__ trec_entry, const real *__restrict__ rent_h,
            real *__restrict__ seis_terms, int nentries, SeisTypes types,
-           int ntypes, int nsbs, const int *__restrict__ step_ptr, int istep,
-           int ld, int sld, real dt, real dto2, real dt2o2, QuadArgs<real> q) {
+           int ntypes, int nsbs, int *__restrict__ step_ptr, int istep,
+           int ld, int sld, real dt, real dto2, real dt2o2, QuadArgs<real> q,
+           void *__restrict__ rec, long rec_stride, int every) {
   using V2 = typename Vec2<real>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V2 *s_u = reinterpret_cast<V2 *>(lds_raw);
@@ -183,6 +195,15 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
   V2 *__restrict__ P2 = reinterpret_cast<V2 *>(part);
   const V2 *__restrict__ M2 = reinterpret_cast<const V2 *>(Minv);
   const int step = step_ptr ? *step_ptr : istep;
+  V2 *rec_a = nullptr, *rec_u = nullptr, *rec_v = nullptr;
+  if (MODE == MODE_RECORD) {
+    const int j = step % every;
+    rec_a = record_ptr<V2>(rec, rec_stride, j, 2);
+    rec_u = record_ptr<V2>(rec, rec_stride, j + 1, 0);
+    rec_v = record_ptr<V2>(rec, rec_stride, j + 1, 1);
+    if (tile == 0 && tid == 0)
+      step_ptr[1] = step; // k_shared reads the step from here
+  }
 
   // tile-local ids of this thread's column (iz = 0..4, ix)
   int li[NGLL];
@@ -362,12 +383,18 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
       s_a[i] = a;
       s_v[i] = v;
     }
-    if (MODE == MODE_STEP) {
+    if (MODE == MODE_STEP || MODE == MODE_RECORD) {
+      if (MODE == MODE_RECORD)
+        rec_a[g] = a;
       V2 u = s_u[i];
       predict(u.x, v.x, a.x, dt, dto2, dt2o2);
       predict(u.y, v.y, a.y, dt, dto2, dt2o2);
       U2[g] = u;
       V2p[g] = v;
+      if (MODE == MODE_RECORD) {
+        rec_u[g] = u;
+        rec_v[g] = v;
+      }
     } else {
       V2p[g] = v;
       A2[g] = a;
@@ -402,11 +429,18 @@ __global__ void __launch_bounds__(NODE_THREADS)
              const real *__restrict__ Minv, const real *__restrict__ part,
              const int *__restrict__ shp_off, const int *__restrict__ shp_pos,
              int nshared, int nex_total, real dt, real dto2, real dt2o2,
-             int *__restrict__ step_ptr) {
+             int *__restrict__ step_ptr, void *__restrict__ rec,
+             long rec_stride, int every) {
   using V2 = typename Vec2<real>::T;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j == 0 && step_ptr && MODE != MODE_ASSEMBLE)
+  int step = 0;
+  if (MODE == MODE_RECORD) {
+    step = step_ptr[1];
+    if (j == 0)
+      step_ptr[0] = step + 1;
+  } else if (j == 0 && step_ptr && MODE != MODE_ASSEMBLE) {
     *step_ptr += 1;
+  }
   if (j >= nshared)
     return;
   V2 *__restrict__ U2 = reinterpret_cast<V2 *>(U);
@@ -427,12 +461,19 @@ __global__ void __launch_bounds__(NODE_THREADS)
   V2 v = V2p[g];
   divide_correct(acc.x, v.x, m.x, dto2);
   divide_correct(acc.y, v.y, m.y, dto2);
-  if (MODE == MODE_STEP) {
+  if (MODE == MODE_STEP || MODE == MODE_RECORD) {
+    const int jr = MODE == MODE_RECORD ? step % every : 0;
+    if (MODE == MODE_RECORD)
+      record_ptr<V2>(rec, rec_stride, jr, 2)[g] = acc;
     V2 u = U2[g];
     predict(u.x, v.x, acc.x, dt, dto2, dt2o2);
     predict(u.y, v.y, acc.y, dt, dto2, dt2o2);
     U2[g] = u;
     V2p[g] = v;
+    if (MODE == MODE_RECORD) {
+      record_ptr<V2>(rec, rec_stride, jr + 1, 0)[g] = u;
+      record_ptr<V2>(rec, rec_stride, jr + 1, 1)[g] = v;
+    }
   } else {
     V2p[g] = v;
     A2[g] = acc;
@@ -638,6 +679,393 @@ __global__ void k_invert(real *__restrict__ M, long n) {
 }
 
 // ========================================================================
+// Discrete adjoint of the fused time step.
+//
+// Forward step k maps (u, v, a)_k to (u, v, a)_{k+1} through
+//   u~ = u + dt v + dt^2/2 a,  v~ = v + dt/2 a,
+//   a' = W (f_k - K u~ - B v~),  W = M'^-1 (mass with the Stacey term),
+//   u' = u~,  v' = v~ + dt/2 a'.
+// Its transpose, with adjoint state (ub, vb, ab) and rb = W (ab + dt/2 vb),
+// runs k = nstep-1 .. 0:
+//   ub += H^T g_k                       (misfit source of seismogram k)
+//   uu = ub - K rb,  vv = vb - B rb
+//   ub = uu,  vb = vv + dt uu,  ab = dt/2 vv + dt^2/2 uu
+// and the parameter gradient of step k is -rb^T (dK u~ + dB v~ + dM' a').
+// The state kept between steps is (ub, vb, rb); ab only enters rb.  rb
+// alternates between two buffers by step parity, so that a step reads rb_k
+// everywhere while writing rb_{k-1}.
+//
+// Per-step gradient accumulators (final scaling by the quadrature weight
+// and Jacobian, and the chain rule from the Stacey impedances to
+// kappa/mu/rho, happen once on the host):
+//   Gd[p][slot] += div(rb) div(u~),  Ge[p][slot] += eps(rb):eps(u~),
+//   R[g] += rb(g) . a'(g),
+//   SP, SS[p][bslot] += -rb^T dB/dP v~ - rb^T dM'/dP a' (and for S), with
+//   P = rho vp and S = rho vs at absorbing points.
+
+// Node update shared by k_adj_tile (exclusive points) and k_adj_shared.
+template <typename real, typename V2>
+__device__ __forceinline__ void adj_finish(const V2 acc, const V2 r, V2 &ub,
+                                           V2 &vb, const V2 m, int b,
+                                           const real *__restrict__ bsum,
+                                           real dt, real dto2, real dt2o2,
+                                           V2 &rbn) {
+  // acc = -K rb (the element kernel's sign), so ub - K rb = ub + acc
+  const real uux = ub.x + acc.x, uuz = ub.y + acc.y;
+  real vvx = vb.x, vvz = vb.y;
+  if (b >= 0) {
+    const real bxx = bsum[3 * b], bxz = bsum[3 * b + 1], bzz = bsum[3 * b + 2];
+    vvx = vvx - (bxx * r.x + bxz * r.y);
+    vvz = vvz - (bxz * r.x + bzz * r.y);
+  }
+  ub.x = uux;
+  ub.y = uuz;
+  vb.x = vvx + dt * uux;
+  vb.y = vvz + dt * uuz;
+  const real abx = dto2 * vvx + dt2o2 * uux, abz = dto2 * vvz + dt2o2 * uuz;
+  rbn.x = m.x * (abx + dto2 * vb.x);
+  rbn.y = m.y * (abz + dto2 * vb.y);
+}
+
+// ub(node) += sum over its receiver entries of h * g_step (entry order).
+template <typename real, typename V2>
+__device__ __forceinline__ void adj_inject(V2 &ub, int j,
+                                           const int *__restrict__ rin_off,
+                                           const int *__restrict__ rin_ent,
+                                           const real *__restrict__ rent_h,
+                                           const real *__restrict__ gsrc,
+                                           int nrec, int step) {
+  for (int k = rin_off[j]; k < rin_off[j + 1]; ++k) {
+    const int id = rin_ent[k];
+    const int r = id / NPT;
+    const real h = rent_h[id];
+    const real *gs = gsrc + (size_t(step) * nrec + r) * 2;
+    ub.x += h * gs[0];
+    ub.y += h * gs[1];
+  }
+}
+
+// Adjoint element pass of step k (= step_ptr[0]) over one tile: K rb
+// assembled in LDS as in k_tile, gradient-density accumulation from rb and
+// the recorded u~, the Stacey impedance terms, and the node update of the
+// tile's exclusive points; shared points get their partial of -K rb.
+template <typename real>
+__global__ void __launch_bounds__(TILE_THREADS)
+    k_adj_tile(const TileInfo *__restrict__ tiles, int nloc_max,
+               const int *__restrict__ sh_list,
+               const uint16_t *__restrict__ lidx,
+               const uint8_t *__restrict__ ecolor, int ncolors,
+               const real *__restrict__ geom, const real *__restrict__ mat,
+               const int *__restrict__ sslot, const real *__restrict__ stacey,
+               real *__restrict__ RB0, real *__restrict__ RB1,
+               real *__restrict__ UB, real *__restrict__ VB,
+               const real *__restrict__ Minv, real *__restrict__ part,
+               void *__restrict__ rec, long rec_stride, int every,
+               const int *__restrict__ bidx, const real *__restrict__ bsum,
+               real *__restrict__ Gd, real *__restrict__ Ge,
+               real *__restrict__ R, real *__restrict__ SP,
+               real *__restrict__ SS, int *__restrict__ step_ptr, int ld,
+               int sld, real dt, real dto2, real dt2o2, QuadArgs<real> q) {
+  using V2 = typename Vec2<real>::T;
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  V2 *s_r = reinterpret_cast<V2 *>(lds_raw);  // rb at the tile points
+  V2 *s_a = s_r + nloc_max;                     // u~, then the accumulator
+  real *xr = reinterpret_cast<real *>(s_a + nloc_max);
+  auto X = [&](int c, int p, int l) -> real & {
+    return xr[(c * NPT + p) * ELEM_TILE + l];
+  };
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int ix = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int tile = blockIdx.x;
+  const TileInfo T = tiles[tile];
+  const int nloc = T.nex + T.nsh;
+  const int slot = tile * ELEM_TILE + lane;
+  const bool valid = lane < T.ecount;
+  const size_t pld = size_t(NPT) * ld;
+  const int step = step_ptr[0];
+  if (tile == 0 && tid == 0)
+    step_ptr[1] = step; // k_adj_shared reads the step from here
+  const V2 *__restrict__ RBc =
+      reinterpret_cast<const V2 *>((step & 1) ? RB1 : RB0);
+  V2 *__restrict__ RBn = reinterpret_cast<V2 *>((step & 1) ? RB0 : RB1);
+  V2 *__restrict__ UB2 = reinterpret_cast<V2 *>(UB);
+  V2 *__restrict__ VB2 = reinterpret_cast<V2 *>(VB);
+  V2 *__restrict__ P2 = reinterpret_cast<V2 *>(part);
+  const V2 *__restrict__ M2 = reinterpret_cast<const V2 *>(Minv);
+  const int jr = step % every;
+  const V2 *__restrict__ Ut = record_ptr<V2>(rec, rec_stride, jr, 0);
+  const V2 *__restrict__ Vt = record_ptr<V2>(rec, rec_stride, jr, 1);
+  const V2 *__restrict__ At = record_ptr<V2>(rec, rec_stride, jr, 2);
+
+  int li[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz)
+    li[iz] = valid ? int(lidx[size_t(iz * NGLL + ix) * ld + slot]) : 0;
+
+  for (int i = tid; i < nloc; i += TILE_THREADS) {
+    const int g = i < T.nex ? T.ex_off + i : sh_list[T.sh_off + i - T.nex];
+    s_r[i] = RBc[g];
+    s_a[i] = Ut[g];
+  }
+  __syncthreads();
+
+  // ---- reference-coordinate derivatives of u~, then of rb
+  real wx[NGLL], wz[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    const V2 w = s_a[li[iz]];
+    wx[iz] = w.x;
+    wz[iz] = w.y;
+    X(0, iz * NGLL + ix, lane) = w.x;
+    X(1, iz * NGLL + ix, lane) = w.y;
+  }
+  __syncthreads();
+  real uxi_x[NGLL], uxi_z[NGLL], uga_x[NGLL], uga_z[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    real a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      a0 += q.hp[ix][l] * X(0, iz * NGLL + l, lane);
+      b0 += q.hp[iz][l] * wx[l];
+      a1 += q.hp[ix][l] * X(1, iz * NGLL + l, lane);
+      b1 += q.hp[iz][l] * wz[l];
+    }
+    uxi_x[iz] = a0;
+    uxi_z[iz] = a1;
+    uga_x[iz] = b0;
+    uga_z[iz] = b1;
+  }
+  __syncthreads(); // staging reused; s_a holds no u~ reads from here on
+  for (int i = tid; i < nloc; i += TILE_THREADS) {
+    V2 z;
+    z.x = 0;
+    z.y = 0;
+    s_a[i] = z;
+  }
+  real ux[NGLL], uz[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    const V2 r = s_r[li[iz]];
+    ux[iz] = r.x;
+    uz[iz] = r.y;
+    X(0, iz * NGLL + ix, lane) = r.x;
+    X(1, iz * NGLL + ix, lane) = r.y;
+  }
+  __syncthreads();
+  real dxi_x[NGLL], dxi_z[NGLL], dga_x[NGLL], dga_z[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    real a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      a0 += q.hp[ix][l] * X(0, iz * NGLL + l, lane);
+      b0 += q.hp[iz][l] * ux[l];
+      a1 += q.hp[ix][l] * X(1, iz * NGLL + l, lane);
+      b1 += q.hp[iz][l] * uz[l];
+    }
+    dxi_x[iz] = a0;
+    dxi_z[iz] = a1;
+    dga_x[iz] = b0;
+    dga_z[iz] = b1;
+  }
+  __syncthreads();
+
+  // ---- gradient densities, stress of rb and the xi integrand
+  real f1x[NGLL], f1z[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    const int p = iz * NGLL + ix;
+    const size_t o = size_t(p) * ld + slot;
+    const real xix = geom[o], xiz = geom[pld + o], gax = geom[2 * pld + o],
+               gaz = geom[3 * pld + o], jac = geom[4 * pld + o];
+    const real l2m = mat[o], mu = mat[pld + o];
+    const real lam = l2m - real(2.0) * mu;
+    const real du_xx = xix * dxi_x[iz] + gax * dga_x[iz];
+    const real du_xz = xiz * dxi_x[iz] + gaz * dga_x[iz];
+    const real du_zx = xix * dxi_z[iz] + gax * dga_z[iz];
+    const real du_zz = xiz * dxi_z[iz] + gaz * dga_z[iz];
+    const real dw_xx = xix * uxi_x[iz] + gax * uga_x[iz];
+    const real dw_xz = xiz * uxi_x[iz] + gaz * uga_x[iz];
+    const real dw_zx = xix * uxi_z[iz] + gax * uga_z[iz];
+    const real dw_zz = xiz * uxi_z[iz] + gaz * uga_z[iz];
+    if (valid) {
+      const real divr = du_xx + du_zz, divw = dw_xx + dw_zz;
+      const real er = real(0.5) * (du_xz + du_zx);
+      const real ew = real(0.5) * (dw_xz + dw_zx);
+      Gd[o] += divr * divw;
+      Ge[o] += du_xx * dw_xx + du_zz * dw_zz + real(2.0) * er * ew;
+    }
+    const real sxx = l2m * du_xx + lam * du_zz;
+    const real szz = l2m * du_zz + lam * du_xx;
+    const real sxz = mu * (du_xz + du_zx);
+    X(0, p, lane) = jac * (sxx * xix + sxz * xiz);
+    X(1, p, lane) = jac * (sxz * xix + szz * xiz);
+    f1x[iz] = jac * (sxx * gax + sxz * gaz);
+    f1z[iz] = jac * (sxz * gax + szz * gaz);
+  }
+  __syncthreads();
+
+  // ---- divergence and sign (the element kernel's -K rb)
+  real rx[NGLL], rz[NGLL];
+#pragma unroll
+  for (int iz = 0; iz < NGLL; ++iz) {
+    real t1x = 0, t1z = 0, t2x = 0, t2z = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      t1x += X(0, iz * NGLL + l, lane) * q.hp[l][ix] * q.w[l];
+      t1z += X(1, iz * NGLL + l, lane) * q.hp[l][ix] * q.w[l];
+      t2x += f1x[l] * q.hp[l][iz] * q.w[l];
+      t2z += f1z[l] * q.hp[l][iz] * q.w[l];
+    }
+    real ax = q.w[iz] * t1x + q.w[ix] * t2x;
+    real az = q.w[iz] * t1z + q.w[ix] * t2z;
+    rx[iz] = real(-1.0) * ax;
+    rz[iz] = real(-1.0) * az;
+  }
+
+  // ---- Stacey impedance terms at this column's absorbing points
+  const int bslot = valid ? sslot[slot] : -1;
+  if (bslot >= 0) {
+    const size_t sp = size_t(NPT) * sld;
+#pragma unroll
+    for (int iz = 0; iz < NGLL; ++iz) {
+      const int p = iz * NGLL + ix;
+      const size_t so = size_t(p) * sld + bslot;
+      const real wgt = stacey[3 * sp + so];
+      if (wgt != real(0)) {
+        const real n0 = stacey[so], n1 = stacey[sp + so],
+                   nrm = stacey[2 * sp + so];
+        const int g = li[iz] < T.nex ? T.ex_off + li[iz]
+                                     : sh_list[T.sh_off + li[iz] - T.nex];
+        const V2 r = s_r[li[iz]], v = Vt[g], a = At[g];
+        const real h0 = n0 / nrm, h1 = n1 / nrm, f = nrm * wgt;
+        const real rn = r.x * h0 + r.y * h1, vn = v.x * h0 + v.y * h1;
+        const real s1 = h0 + h1;
+        const real dP = -(f * rn * vn + real(0.5) * dt * f *
+                                            (r.x * a.x * h0 * s1 +
+                                             r.y * a.y * h1 * s1));
+        const real dS =
+            -(f * (r.x * v.x + r.y * v.y - rn * vn) +
+              real(0.5) * dt * f *
+                  (r.x * a.x * (real(1.0) - h0 * s1) +
+                   r.y * a.y * (real(1.0) - h1 * s1)));
+        SP[so] += dP;
+        SS[so] += dS;
+      }
+    }
+  }
+
+  // ---- assembly into the tile accumulator, colour by colour
+  const int ecol = valid ? int(ecolor[slot]) : -1;
+  for (int c = 0; c < ncolors; ++c) {
+    if (ecol == c) {
+#pragma unroll
+      for (int iz = 0; iz < NGLL; ++iz) {
+        V2 acc = s_a[li[iz]];
+        acc.x += rx[iz];
+        acc.y += rz[iz];
+        s_a[li[iz]] = acc;
+      }
+    }
+    __syncthreads();
+  }
+
+  // ---- exclusive points: adjoint node update; shared points: partials
+  for (int i = tid; i < T.nex; i += TILE_THREADS) {
+    const int g = T.ex_off + i;
+    const V2 r = s_r[i], a = At[g];
+    R[g] += r.x * a.x + r.y * a.y;
+    V2 ub = UB2[g], vb = VB2[g], rbn;
+    adj_finish<real>(s_a[i], r, ub, vb, M2[g], bidx[g], bsum, dt, dto2, dt2o2,
+                     rbn);
+    UB2[g] = ub;
+    VB2[g] = vb;
+    RBn[g] = rbn;
+  }
+  for (int i = T.nex + tid; i < nloc; i += TILE_THREADS)
+    P2[T.sh_off + i - T.nex] = s_a[i];
+}
+
+// Shared points of adjoint step k: partial sums in tile order and the node
+// update; then the misfit source of step k-1 at every receiver node (shared
+// ones by their updating thread, exclusive ones by the threads after
+// nshared).  Thread 0 moves the step counter to k-1.
+template <typename real>
+__global__ void __launch_bounds__(NODE_THREADS)
+    k_adj_shared(real *__restrict__ RB0, real *__restrict__ RB1,
+                 real *__restrict__ UB, real *__restrict__ VB,
+                 const real *__restrict__ Minv, const real *__restrict__ part,
+                 const int *__restrict__ shp_off,
+                 const int *__restrict__ shp_pos, int nshared, int nex_total,
+                 void *__restrict__ rec, long rec_stride, int every,
+                 const int *__restrict__ bidx, const real *__restrict__ bsum,
+                 real *__restrict__ R, const int *__restrict__ sh_rin,
+                 const int *__restrict__ rin_excl, int nrin_excl,
+                 const int *__restrict__ rin_node,
+                 const int *__restrict__ rin_off,
+                 const int *__restrict__ rin_ent,
+                 const real *__restrict__ rent_h,
+                 const real *__restrict__ gsrc, int nrec,
+                 int *__restrict__ step_ptr, real dt, real dto2, real dt2o2) {
+  using V2 = typename Vec2<real>::T;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  const int step = step_ptr[1];
+  if (t == 0)
+    step_ptr[0] = step - 1;
+  V2 *__restrict__ UB2 = reinterpret_cast<V2 *>(UB);
+  if (t < nshared) {
+    const int g = nex_total + t;
+    const V2 *__restrict__ RBc =
+        reinterpret_cast<const V2 *>((step & 1) ? RB1 : RB0);
+    V2 *__restrict__ RBn = reinterpret_cast<V2 *>((step & 1) ? RB0 : RB1);
+    V2 *__restrict__ VB2 = reinterpret_cast<V2 *>(VB);
+    const V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off,
+                              shp_pos, t);
+    const V2 r = RBc[g];
+    const V2 a = record_ptr<V2>(rec, rec_stride, step % every, 2)[g];
+    R[g] += r.x * a.x + r.y * a.y;
+    V2 ub = UB2[g], vb = VB2[g], rbn;
+    adj_finish<real>(acc, r, ub, vb, reinterpret_cast<const V2 *>(Minv)[g],
+                     bidx[g], bsum, dt, dto2, dt2o2, rbn);
+    const int j = sh_rin[t];
+    if (j >= 0 && step > 0)
+      adj_inject<real>(ub, j, rin_off, rin_ent, rent_h, gsrc, nrec, step - 1);
+    UB2[g] = ub;
+    VB2[g] = vb;
+    RBn[g] = rbn;
+  } else if (t < nshared + nrin_excl && step > 0) {
+    const int j = rin_excl[t - nshared];
+    const int g = rin_node[j];
+    V2 ub = UB2[g];
+    adj_inject<real>(ub, j, rin_off, rin_ent, rent_h, gsrc, nrec, step - 1);
+    UB2[g] = ub;
+  }
+}
+
+// Misfit source of one step at every receiver node (the first adjoint step).
+template <typename real>
+__global__ void k_adj_inject(real *__restrict__ UB,
+                             const int *__restrict__ rin_node,
+                             const int *__restrict__ rin_off,
+                             const int *__restrict__ rin_ent,
+                             const real *__restrict__ rent_h,
+                             const real *__restrict__ gsrc, int nrec,
+                             int nnodes, int step) {
+  using V2 = typename Vec2<real>::T;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nnodes)
+    return;
+  V2 *__restrict__ UB2 = reinterpret_cast<V2 *>(UB);
+  const int g = rin_node[j];
+  V2 ub = UB2[g];
+  adj_inject<real>(ub, j, rin_off, rin_ent, rent_h, gsrc, nrec, step);
+  UB2[g] = ub;
+}
+
+// ========================================================================
 template <typename real> class Solver2D final : public Solver2DBase {
 public:
   Solver2D(std::unique_ptr<HostAssembly2D> a) : asm_(std::move(a)) {
@@ -681,7 +1109,15 @@ public:
     SMX_HIP(hipFuncSetAttribute(
         reinterpret_cast<const void *>(&k_tile<real, MODE_ASSEMBLE>),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes_)));
-    step_ = alloc<int>(1);
+    SMX_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_tile<real, MODE_RECORD>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes_)));
+    SMX_HIP(hipFuncSetAttribute(
+        reinterpret_cast<const void *>(&k_adj_tile<real>),
+        hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes_)));
+    // [0] step of the forward / recompute, [1] its copy for k_shared,
+    // [2] step of the adjoint, [3] its copy for k_adj_shared
+    step_ = alloc<int>(4);
     const int ntypes = int(A.seis_types.size());
     const size_t nseis =
         size_t(std::max(1, A.nsig)) * ntypes * std::max(1, A.nrec) * 2;
@@ -705,15 +1141,17 @@ public:
   }
 
   ~Solver2D() override {
-    if (graph_exec_)
-      (void)hipGraphExecDestroy(graph_exec_);
-    if (graph_)
-      (void)hipGraphDestroy(graph_);
+    drop_graph(graph_, graph_exec_);
+    drop_graph(rec_graph_, rec_graph_exec_);
+    drop_graph(adj_graph_, adj_graph_exec_);
     (void)hipStreamSynchronize(stream_);
     for (void *p : allocs_)
       (void)hipFree(p);
     (void)hipEventDestroy(ev0_);
     (void)hipEventDestroy(ev1_);
+    for (hipEvent_t e : adj_ev_)
+      if (e)
+        (void)hipEventDestroy(e);
     (void)hipStreamDestroy(stream_);
   }
 
@@ -873,6 +1311,200 @@ public:
     return last_stiff_ms_;
   }
 
+  // ---------------------------------------------------------------- adjoint
+  void set_model(const float *kappa, const float *mu,
+                 const float *rho) override {
+    HostAssembly2D &A = *asm_;
+    set_model2d(A, kappa, mu, rho);
+    copy_to(mat_, A.lam2mu_mu);
+    copy_to(kmr_, A.kappa_mu_rho);
+    copy_to(stacey_, A.stacey);
+    if (bsum_)
+      copy_to(bsum_, A.bsum);
+    ckpt_every_ = 0;
+    initialize();
+  }
+
+  void run_forward_checkpointed(int every) override {
+    require_init();
+    const HostAssembly2D &A = *asm_;
+    if (every < 1)
+      throw SmxError(SMX_ERR_ARG, "checkpoint interval must be positive");
+    every = std::min(every, A.nstep);
+    const int nseg = (A.nstep + every - 1) / every;
+    if (nseg > ckpt_cap_) {
+      release(ckpt_);
+      ckpt_ = alloc<real>(size_t(nseg) * 2 * n2_);
+      ckpt_cap_ = nseg;
+    }
+    if (every != rec_every_) {
+      release(rec_);
+      rec_ = alloc<real>(size_t(every + 1) * 3 * n2_);
+      rec_every_ = every;
+      drop_graph(rec_graph_, rec_graph_exec_);
+      drop_graph(adj_graph_, adj_graph_exec_);
+    }
+    ensure_adjoint_events();
+    Field &F = field(SMX_FIELD_FORWARD);
+    SMX_HIP(hipEventRecord(adj_ev_[0], stream_));
+    for (real *p : {F.u, F.v, F.a}) // from rest
+      SMX_HIP(hipMemsetAsync(p, 0, n2_ * sizeof(real), stream_));
+    k_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_, 0);
+    SMX_HIP(hipGetLastError());
+    launch_predictor(F);
+    for (int sgm = 0; sgm < nseg; ++sgm) {
+      const int k0 = sgm * every, len = std::min(every, A.nstep - k0);
+      real *ck = ckpt_ + size_t(sgm) * 2 * n2_;
+      SMX_HIP(hipMemcpyAsync(ck, F.u, n2_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+      SMX_HIP(hipMemcpyAsync(ck + n2_, F.v, n2_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+      const bool last = sgm == nseg - 1;
+      const int body = last ? len - 1 : len;
+      const int reps = body / kGraphSteps;
+      if (reps > 0) {
+        build_graph(F);
+        for (int r = 0; r < reps; ++r)
+          SMX_HIP(hipGraphLaunch(graph_exec_, stream_));
+      }
+      for (int st = reps * kGraphSteps; st < body; ++st)
+        fused_step<MODE_STEP>(F);
+      if (last)
+        fused_step<MODE_FINAL>(F);
+    }
+    SMX_HIP(hipEventRecord(adj_ev_[1], stream_));
+    ckpt_every_ = every;
+    have_adj_stats_ = false;
+  }
+
+  void run_adjoint(const void *gsrc_host) override {
+    const HostAssembly2D &A = *asm_;
+    if (!ckpt_every_)
+      throw SmxError(SMX_ERR_STATE,
+                     "no checkpointed forward run for the current model");
+    ensure_adjoint();
+    const size_t ng = size_t(A.nstep) * A.nrec * 2;
+    if (ng)
+      SMX_HIP(hipMemcpyAsync(gsrc_, gsrc_host, ng * sizeof(real),
+                             hipMemcpyHostToDevice, stream_));
+    Field &W = field(SMX_FIELD_BACKWARD);
+    Field &B = field(SMX_FIELD_ADJOINT);
+    const int every = ckpt_every_;
+    const int nseg = (A.nstep + every - 1) / every;
+    SMX_HIP(hipEventRecord(adj_ev_[2], stream_));
+    for (real *p : {B.u, B.v, rb0_, rb1_, racc_})
+      SMX_HIP(hipMemsetAsync(p, 0, n2_ * sizeof(real), stream_));
+    for (real *p : {gd_, ge_})
+      SMX_HIP(hipMemsetAsync(p, 0, size_t(NPT) * A.nspec_pad * sizeof(real),
+                             stream_));
+    for (real *p : {sp_, ss_})
+      SMX_HIP(hipMemsetAsync(p, 0, size_t(NPT) * A.nstacey_pad * sizeof(real),
+                             stream_));
+    const int nrin = int(A.rin_node.size());
+    if (nrin > 0) {
+      k_adj_inject<real><<<dim3((nrin + 255) / 256), dim3(256), 0, stream_>>>(
+          B.u, rin_node_, rin_off_, rin_ent_, rent_h_, gsrc_, A.nrec, nrin,
+          A.nstep - 1);
+      SMX_HIP(hipGetLastError());
+    }
+    for (int sgm = nseg - 1; sgm >= 0; --sgm) {
+      const int k0 = sgm * every, len = std::min(every, A.nstep - k0);
+      const real *ck = ckpt_ + size_t(sgm) * 2 * n2_;
+      // recompute the segment from its checkpoint, recording every step
+      SMX_HIP(hipMemcpyAsync(W.u, ck, n2_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+      SMX_HIP(hipMemcpyAsync(W.v, ck + n2_, n2_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+      SMX_HIP(hipMemcpyAsync(rec_, ck, 2 * n2_ * sizeof(real),
+                             hipMemcpyDeviceToDevice, stream_));
+      k_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_, k0);
+      SMX_HIP(hipGetLastError());
+      const int rreps = len / kGraphSteps;
+      if (rreps > 0) {
+        build_rec_graph(W);
+        for (int r = 0; r < rreps; ++r)
+          SMX_HIP(hipGraphLaunch(rec_graph_exec_, stream_));
+      }
+      for (int st = rreps * kGraphSteps; st < len; ++st)
+        record_step(W);
+      // adjoint steps k0+len-1 .. k0
+      k_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_ + 2, k0 + len - 1);
+      SMX_HIP(hipGetLastError());
+      if (rreps > 0) {
+        build_adj_graph(B);
+        for (int r = 0; r < rreps; ++r)
+          SMX_HIP(hipGraphLaunch(adj_graph_exec_, stream_));
+      }
+      for (int st = rreps * kGraphSteps; st < len; ++st)
+        adjoint_step(B);
+    }
+    SMX_HIP(hipEventRecord(adj_ev_[3], stream_));
+    SMX_HIP(hipStreamSynchronize(stream_));
+    have_adj_stats_ = true;
+    have_gradient_ = true;
+  }
+
+  void get_gradient(double *out) override {
+    if (!have_gradient_)
+      throw SmxError(SMX_ERR_STATE, "no adjoint run");
+    const HostAssembly2D &A = *asm_;
+    const int ld = A.nspec_pad, sld = A.nstacey_pad;
+    const size_t pld = size_t(NPT) * ld, sp = size_t(NPT) * sld;
+    std::vector<real> gd(pld), ge(pld), rr(n2_ / 2), sP(sp), sS(sp);
+    download(gd.data(), gd_, pld);
+    download(ge.data(), ge_, pld);
+    download(rr.data(), racc_, n2_ / 2);
+    download(sP.data(), sp_, sp);
+    download(sS.data(), ss_, sp);
+    const size_t nloc = size_t(A.nspec) * NPT;
+    for (int e = 0; e < A.nspec; ++e) {
+      const int slot = A.elem_slot[e], bs = A.stacey_slot[slot];
+      for (int p = 0; p < NPT; ++p) {
+        const size_t o = size_t(p) * ld + slot;
+        const int iz = p / NGLL, ix = p % NGLL;
+        const double wj =
+            double(A.quad.w[iz]) * A.quad.w[ix] * A.geom[4 * pld + o];
+        const double Gd = gd[o], Ge = ge[o];
+        const int g = A.ibool[o];
+        double gk = -wj * Gd;
+        double gm = -wj * (2.0 * Ge - 2.0 / 3.0 * Gd);
+        double gr = -wj * double(rr[g]);
+        if (bs >= 0 && A.stacey[3 * sp + size_t(p) * sld + bs] != 0.0f) {
+          const double k = A.kappa_mu_rho[o], m = A.kappa_mu_rho[pld + o],
+                       rh = A.kappa_mu_rho[2 * pld + o];
+          const double l2m = k + 4.0 / 3.0 * m;
+          const double P = std::sqrt(rh * l2m), S = std::sqrt(rh * m);
+          const double dP = sP[size_t(p) * sld + bs],
+                       dS = sS[size_t(p) * sld + bs];
+          gk += dP * rh / (2.0 * P);
+          gm += dP * (4.0 / 3.0) * rh / (2.0 * P);
+          gr += dP * l2m / (2.0 * P);
+          if (S > 0) {
+            gm += dS * rh / (2.0 * S);
+            gr += dS * m / (2.0 * S);
+          }
+        }
+        const size_t i = size_t(e) * NPT + p;
+        out[i] = gk;
+        out[nloc + i] = gm;
+        out[2 * nloc + i] = gr;
+      }
+    }
+  }
+
+  void last_adjoint_stats(double *fwd_ms, double *adj_ms) override {
+    float tf = 0, ta = 0;
+    if (have_adj_stats_) {
+      SMX_HIP(hipEventSynchronize(adj_ev_[3]));
+      SMX_HIP(hipEventElapsedTime(&tf, adj_ev_[0], adj_ev_[1]));
+      SMX_HIP(hipEventElapsedTime(&ta, adj_ev_[2], adj_ev_[3]));
+    }
+    if (fwd_ms)
+      *fwd_ms = tf;
+    if (adj_ms)
+      *adj_ms = ta;
+  }
+
 private:
   struct Field {
     real *u = nullptr, *v = nullptr, *a = nullptr;
@@ -947,7 +1579,7 @@ private:
   // buffer; seismogram terms are written only when `terms` is given.
   template <int MODE>
   void launch_tiles(real *U, real *V, real *Aout, real *part, int istep,
-                    const int *step_ptr, real *terms = nullptr) {
+                    int *step_ptr, real *terms = nullptr) {
     const HostAssembly2D &A = *asm_;
     const int ntypes = int(A.seis_types.size());
     k_tile<real, MODE><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_,
@@ -956,7 +1588,8 @@ private:
         mat_, sslot_, stacey_, U, V, Aout, minv_, part ? part : part_, src_li_,
         src_idx_, src_L_, stf_, std::max(1, A.nsrc), trec_li_, trec_entry_,
         rent_h_, terms, A.nrec * NPT, stypes_, ntypes, A.nsbs, step_ptr, istep,
-        A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_);
+        A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, rec_,
+        rec_stride(), std::max(1, rec_every_));
     SMX_HIP(hipGetLastError());
   }
   template <int MODE>
@@ -966,8 +1599,125 @@ private:
     k_shared<real, MODE><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS),
                            dim3(NODE_THREADS), 0, stream_>>>(
         U, V, Aout, minv_, part_, shp_off_, shp_pos_, A.nshared, A.nex_total,
-        dt_, dto2_, dt2o2_, step_ptr);
+        dt_, dto2_, dt2o2_, step_ptr, rec_, rec_stride(),
+        std::max(1, rec_every_));
     SMX_HIP(hipGetLastError());
+  }
+  long rec_stride() const { return long(n2_ * sizeof(real)); }
+
+  // Recompute step: the fused step that also records (u~, v~, a').
+  void record_step(Field &W) {
+    launch_tiles<MODE_RECORD>(W.u, W.v, W.a, nullptr, 0, step_);
+    launch_shared<MODE_RECORD>(W.u, W.v, W.a, step_);
+  }
+  // One adjoint step (device step counter step_[2]).
+  void adjoint_step(Field &B) {
+    const HostAssembly2D &A = *asm_;
+    k_adj_tile<real><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_,
+                       stream_>>>(
+        tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
+        mat_, sslot_, stacey_, rb0_, rb1_, B.u, B.v, minv_, part_, rec_,
+        rec_stride(), rec_every_, bidx_, bsum_, gd_, ge_, racc_, sp_, ss_,
+        step_ + 2, A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_);
+    SMX_HIP(hipGetLastError());
+    const int nex_rin = int(A.rin_excl.size());
+    const int n = std::max(1, A.nshared + nex_rin);
+    k_adj_shared<real><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS),
+                         dim3(NODE_THREADS), 0, stream_>>>(
+        rb0_, rb1_, B.u, B.v, minv_, part_, shp_off_, shp_pos_, A.nshared,
+        A.nex_total, rec_, rec_stride(), rec_every_, bidx_, bsum_, racc_,
+        sh_rin_, rin_excl_, nex_rin, rin_node_, rin_off_, rin_ent_, rent_h_,
+        gsrc_, A.nrec, step_ + 2, dt_, dto2_, dt2o2_);
+    SMX_HIP(hipGetLastError());
+  }
+  template <typename F> void capture(hipGraph_t &g, hipGraphExec_t &ge, F body) {
+    if (ge)
+      return;
+    SMX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      body();
+    } catch (...) {
+      hipGraph_t tmp;
+      (void)hipStreamEndCapture(stream_, &tmp);
+      if (tmp)
+        (void)hipGraphDestroy(tmp);
+      throw;
+    }
+    SMX_HIP(hipStreamEndCapture(stream_, &g));
+    SMX_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  }
+  void build_rec_graph(Field &W) {
+    capture(rec_graph_, rec_graph_exec_, [&] {
+      for (int s = 0; s < kGraphSteps; ++s)
+        record_step(W);
+    });
+  }
+  void build_adj_graph(Field &B) {
+    capture(adj_graph_, adj_graph_exec_, [&] {
+      for (int s = 0; s < kGraphSteps; ++s)
+        adjoint_step(B);
+    });
+  }
+  static void drop_graph(hipGraph_t &g, hipGraphExec_t &ge) {
+    if (ge)
+      (void)hipGraphExecDestroy(ge);
+    if (g)
+      (void)hipGraphDestroy(g);
+    g = nullptr;
+    ge = nullptr;
+  }
+  template <typename T> void release(T *&p) {
+    if (!p)
+      return;
+    SMX_HIP(hipStreamSynchronize(stream_));
+    for (size_t i = 0; i < allocs_.size(); ++i)
+      if (allocs_[i] == static_cast<void *>(p)) {
+        allocs_.erase(allocs_.begin() + i);
+        break;
+      }
+    SMX_HIP(hipFree(p));
+    p = nullptr;
+  }
+  template <typename H> void copy_to(real *d, const std::vector<H> &h) {
+    std::vector<real> tmp(h.begin(), h.end());
+    SMX_HIP(hipMemcpyAsync(d, tmp.data(), tmp.size() * sizeof(real),
+                           hipMemcpyHostToDevice, stream_));
+    SMX_HIP(hipStreamSynchronize(stream_));
+  }
+  template <typename T> void download(T *h, const T *d, size_t n) {
+    SMX_HIP(hipMemcpyAsync(h, d, n * sizeof(T), hipMemcpyDeviceToHost,
+                           stream_));
+    SMX_HIP(hipStreamSynchronize(stream_));
+  }
+  void ensure_adjoint_events() {
+    for (hipEvent_t &e : adj_ev_)
+      if (!e)
+        SMX_HIP(hipEventCreate(&e));
+  }
+  // Device arrays of the adjoint, allocated on the first adjoint run.
+  void ensure_adjoint() {
+    if (rb0_)
+      return;
+    const HostAssembly2D &A = *asm_;
+    std::vector<int32_t> bidx(size_t(nglob_pad_), -1);
+    for (size_t j = 0; j < A.bnode_g.size(); ++j)
+      bidx[A.bnode_g[j]] = int32_t(j);
+    bidx_ = upload_raw<int>(bidx);
+    bsum_ = alloc<real>(A.bsum.size());
+    copy_to(bsum_, A.bsum);
+    sh_rin_ = upload_raw<int>(A.sh_rin);
+    rin_excl_ = upload_raw<int>(A.rin_excl);
+    rin_node_ = upload_raw<int>(A.rin_node);
+    rin_off_ = upload_raw<int>(A.rin_off);
+    rin_ent_ = upload_raw<int>(A.rin_ent);
+    gsrc_ = alloc<real>(std::max<size_t>(2, size_t(A.nstep) * A.nrec * 2));
+    rb0_ = alloc<real>(n2_);
+    rb1_ = alloc<real>(n2_);
+    racc_ = alloc<real>(n2_);
+    gd_ = alloc<real>(size_t(NPT) * A.nspec_pad);
+    ge_ = alloc<real>(size_t(NPT) * A.nspec_pad);
+    sp_ = alloc<real>(size_t(NPT) * A.nstacey_pad);
+    ss_ = alloc<real>(size_t(NPT) * A.nstacey_pad);
   }
   // One fused step: tiles (sources + stiffness + assembly + finish of the
   // exclusive points), seismograms, shared points.
@@ -989,27 +1739,19 @@ private:
     launch_shared<MODE>(F.u, F.v, F.a, step_);
   }
   void build_graph(Field &F) {
-    if (graph_exec_)
-      return;
-    SMX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
-    try {
+    capture(graph_, graph_exec_, [&] {
       for (int s = 0; s < kGraphSteps; ++s)
         fused_step<MODE_STEP>(F);
-    } catch (...) {
-      hipGraph_t g;
-      (void)hipStreamEndCapture(stream_, &g);
-      throw;
-    }
-    SMX_HIP(hipStreamEndCapture(stream_, &graph_));
-    SMX_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+    });
   }
 
   std::unique_ptr<HostAssembly2D> asm_;
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  hipGraph_t graph_ = nullptr;
-  hipGraphExec_t graph_exec_ = nullptr;
+  hipGraph_t graph_ = nullptr, rec_graph_ = nullptr, adj_graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr, rec_graph_exec_ = nullptr,
+                 adj_graph_exec_ = nullptr;
   std::vector<void *> allocs_;
   long nglob_pad_ = 0, n2_ = 0;
   TileInfo *tiles_ = nullptr;
@@ -1033,6 +1775,16 @@ private:
   bool initialized_ = false, have_stats_ = false;
   double last_stiff_ms_ = 0;
   int last_launches_ = 0;
+  // adjoint: checkpoints [nseg][u, v], the recompute record (every+1 slots
+  // of u~, v~, a'), adjoint rb in two parity buffers, gradient accumulators
+  real *ckpt_ = nullptr, *rec_ = nullptr, *rb0_ = nullptr, *rb1_ = nullptr,
+       *racc_ = nullptr, *gd_ = nullptr, *ge_ = nullptr, *sp_ = nullptr,
+       *ss_ = nullptr, *bsum_ = nullptr, *gsrc_ = nullptr;
+  int *bidx_ = nullptr, *sh_rin_ = nullptr, *rin_excl_ = nullptr,
+      *rin_node_ = nullptr, *rin_off_ = nullptr, *rin_ent_ = nullptr;
+  int ckpt_cap_ = 0, rec_every_ = 0, ckpt_every_ = 0;
+  hipEvent_t adj_ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  bool have_adj_stats_ = false, have_gradient_ = false;
 };
 
 std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,

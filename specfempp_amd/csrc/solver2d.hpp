@@ -32,6 +32,21 @@ public:
   // Launches only the stiffness kernel n times into a scratch acceleration
   // and returns the mean duration in ms (HIP events on the context stream).
   virtual double time_stiffness(int n) = 0;
+
+  // Replaces the material ([nspec][5][5] per parameter, mesh element order)
+  // and rebuilds the mass matrix; earlier checkpoints are dropped.
+  virtual void set_model(const float *kappa, const float *mu,
+                         const float *rho) = 0;
+  // Forward run of all nstep steps from rest (seismograms as run_forward),
+  // keeping the state every `every` steps for the adjoint.
+  virtual void run_forward_checkpointed(int every) = 0;
+  // Discrete-adjoint gradient of a functional of the displacement
+  // seismograms with dchi/ds = gsrc[nstep][nrec][2] (host, context
+  // precision), using the checkpoints of the last checkpointed run.
+  virtual void run_adjoint(const void *gsrc) = 0;
+  // Gradient [3][nspec][5][5] (kappa, mu, rho) as double.
+  virtual void get_gradient(double *host) = 0;
+  virtual void last_adjoint_stats(double *forward_ms, double *adjoint_ms) = 0;
 };
 
 std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,

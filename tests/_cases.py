@@ -64,3 +64,29 @@ def small_homogeneous(nstep=400):
 def small_layered(nstep=300, nx=120, nz=90):
     """A reduced layered box (same generator as the 800x600 workload)."""
     return layered_box(nx=nx, nz=nz, nstep=nstep, nsrc=3, nrec=40, seed=1001)
+
+
+def adjoint_small(nx=12, nz=9, nstep=300, dt=2.5e-4, nrec=6, seed=5):
+    """Small heterogeneous box for adjoint checks: free top, Stacey on the
+    other sides, one Ricker force (f0 = 25 Hz), receivers 25 m below the top;
+    kappa, mu, rho carry smooth seeded +-3% perturbations per GLL point."""
+    from specfempp_amd.mesh import gll_coordinates
+    h = 50.0
+    mesh = box_mesh(0.0, nx * h, 0.0, nz * h, nx, nz, ngnod=4,
+                    absorbing=("bottom", "right", "left"), rho=2700.0, vp=3000.0, vs=1732.0)
+    rng = np.random.default_rng(seed)
+    xz = gll_coordinates(mesh)
+    for name in ("kappa", "mu", "rho"):
+        kx, kz, ph = rng.uniform(1.0, 3.0), rng.uniform(1.0, 3.0), rng.uniform(0, 2 * np.pi)
+        f = np.sin(kx * np.pi * xz[..., 0] / (nx * h) + ph) * np.cos(kz * np.pi * xz[..., 1] / (nz * h))
+        setattr(mesh, name, (getattr(mesh, name) * (1.0 + 0.03 * f)).astype(np.float32))
+    src = Sources2D.ricker([nx * h / 2], [nz * h * 0.6], f0=25.0, factor=1e10)
+    rec = Receivers2D(x=np.linspace(40.0, nx * h - 40.0, nrec), z=np.full(nrec, nz * h - 25.0))
+    return Config2D(f"adjoint_small_{nx}x{nz}", mesh, src, rec, dt, nstep)
+
+
+def perturbed_model(cfg, scale=(1.02, 0.97, 1.01)):
+    """(kappa, mu, rho) of cfg scaled per parameter (the 'true' model)."""
+    m = cfg.mesh
+    return tuple((x.astype(np.float64) * s).astype(np.float32)
+                 for x, s in zip((m.kappa, m.mu, m.rho), scale))

@@ -1,0 +1,192 @@
+"""GPU adjoint path (through the C ABI) against the oracle's discrete adjoint
+and against finite differences of the GPU's own FP64 misfit.
+
+Tolerances (relative L2 over each of the kappa / mu / rho gradients):
+* FP64 GPU vs FP64 oracle: 5e-5.  The GPU assembles its coefficients
+  (Jacobian, moduli, impedances) in FP32 as the reference does, so even in
+  FP64 arithmetic it solves a problem whose coefficients differ from the FP64
+  oracle's by ~1e-7 relative (its seismograms differ by ~2e-6).
+* FP32 GPU vs FP64 oracle: 2e-4 (FP32 forward states and accumulation).
+* Finite differences (central, FP64 GPU misfit, relative step 1e-3):
+  5e-4 for the FP64 gradient, 1e-3 for the FP32 gradient.  The material is
+  FP32 and the moduli and impedances derived from it are formed in FP32 (as
+  in the reference), which puts ~6e-8 relative noise on a 1e-3 step, and the
+  central difference's own truncation error is ~2e-5 .. 7e-5 here; the exact
+  pin of the adjoint is the oracle's (FP64 coefficients, relative step 1e-4,
+  error < 3e-6) together with the GPU-vs-oracle gradient match above.
+"""
+
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from specfempp_amd import _lib
+from specfempp_amd.adjoint import Seismograms, l2_misfit, misfit_and_gradient
+from specfempp_amd.solver import Context
+from tests._cases import adjoint_small, perturbed_model, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _ctx(cfg, precision):
+    c = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep, precision=precision)
+    c.initialize()
+    return c
+
+
+def _model(cfg):
+    return (cfg.mesh.kappa, cfg.mesh.mu, cfg.mesh.rho)
+
+
+def _gsrc(cfg, seed=7):
+    rng = np.random.default_rng(seed)
+    t = np.arange(cfg.nstep) * cfg.dt
+    w = np.sin(2 * np.pi * 40.0 * t)[:, None, None]
+    return rng.standard_normal((cfg.nstep, len(cfg.receivers), 2)) * w * 1e-9
+
+
+@pytest.fixture(scope="module")
+def oracle_grad(oracle_lib):
+    cfg = adjoint_small(nstep=300)
+    o = oracle_lib.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep, precision=8)
+    s = o.forward_store()
+    gs = _gsrc(cfg)
+    return cfg, s, gs, o.adjoint(gs)
+
+
+@pytest.mark.parametrize("precision,tol", [(8, 5e-5), (4, 2e-4)])
+def test_adjoint_gradient_matches_oracle(oracle_grad, precision, tol):
+    cfg, s_ref, gs, g_ref = oracle_grad
+    _say(f"adjoint vs oracle: 12x9 box, 300 steps, precision {precision}")
+    c = _ctx(cfg, precision)
+    c.run_forward_checkpointed(every=64)
+    c.synchronize()
+    e_s = rel_l2(c.seismograms()[:, 0], s_ref)
+    _say(f"  seismogram rel L2 vs oracle: {e_s:.3e}")
+    assert e_s < 1e-5
+    c.run_adjoint(gs.astype(c.dtype))
+    g = c.gradient()
+    errs = [rel_l2(g[i], g_ref[i]) for i in range(3)]
+    _say(f"  gradient rel L2 (kappa, mu, rho): {errs}")
+    assert max(errs) < tol, errs
+
+
+def test_adjoint_independent_of_checkpoint_interval(oracle_grad):
+    cfg, _, gs, _ = oracle_grad
+    _say("adjoint: checkpoint intervals 1, 7, 64, 300 (FP32)")
+    c = _ctx(cfg, 4)
+    grads = []
+    for every in (64, 1, 7, 300):
+        c.run_forward_checkpointed(every=every)
+        c.run_adjoint(gs.astype(np.float32))
+        grads.append(c.gradient())
+    for g in grads[1:]:
+        for i in range(3):
+            assert rel_l2(g[i], grads[0][i]) < 1e-6
+
+
+def test_checkpointed_forward_seismograms_are_the_forward_ones():
+    cfg = adjoint_small(nstep=300)
+    _say("checkpointed forward vs plain forward (FP32, bitwise)")
+    c = _ctx(cfg, 4)
+    c.run_forward(0, cfg.nstep)
+    c.synchronize()
+    s0 = c.seismograms().copy()
+    c.run_forward_checkpointed(every=50)
+    c.synchronize()
+    assert np.array_equal(c.seismograms(), s0)
+
+
+def test_adjoint_zero_and_linear(oracle_grad):
+    cfg, _, gs, _ = oracle_grad
+    _say("adjoint: zero source, linearity (FP64)")
+    c = _ctx(cfg, 8)
+    c.run_forward_checkpointed(every=100)
+    c.run_adjoint(np.zeros_like(gs))
+    assert np.all(c.gradient() == 0.0)
+    c.run_adjoint(gs)
+    g1 = c.gradient()
+    c.run_adjoint(2.0 * gs)
+    g2 = c.gradient()
+    np.testing.assert_allclose(g2, 2.0 * g1, rtol=1e-12, atol=1e-12 * np.abs(g1).max())
+
+
+def _misfit(c, model, obs):
+    c.set_model(*model)
+    c.run_forward_checkpointed(every=100)
+    c.synchronize()
+    return l2_misfit(c.seismograms()[:, 0], obs, c.dt)
+
+
+def test_adjoint_finite_differences():
+    cfg = adjoint_small(nstep=300)
+    _say("adjoint vs finite differences of the FP64 GPU misfit, 6 directions")
+    c64, c32 = _ctx(cfg, 8), _ctx(cfg, 4)
+    c64.set_model(*perturbed_model(cfg))
+    c64.run_forward_checkpointed(every=100)
+    c64.synchronize()
+    obs = c64.seismograms()[:, 0].copy()
+    m0 = _model(cfg)
+    _, g64 = misfit_and_gradient(c64, m0, obs, every=100)
+    _, g32 = misfit_and_gradient(c32, m0, obs.astype(np.float32), every=100)
+    rng = np.random.default_rng(21)
+    base = [x.astype(np.float64) for x in m0]
+    for k, which in enumerate((0, 1, 2, None, None, None)):
+        dirs = [np.zeros_like(x) for x in base]
+        for i in ([which] if which is not None else [0, 1, 2]):
+            dirs[i] = rng.standard_normal(base[i].shape) * base[i] * 1e-3
+        mp = [(x + d).astype(np.float32) for x, d in zip(base, dirs)]
+        mm = [(x - d).astype(np.float32) for x, d in zip(base, dirs)]
+        fd = _misfit(c64, mp, obs) - _misfit(c64, mm, obs)
+        step = [p.astype(np.float64) - m.astype(np.float64) for p, m in zip(mp, mm)]
+        a64 = sum(float(np.sum(g * d)) for g, d in zip(g64, step))
+        a32 = sum(float(np.sum(g * d)) for g, d in zip(g32, step))
+        e64, e32 = abs(a64 - fd) / abs(fd), abs(a32 - fd) / abs(fd)
+        _say(f"  direction {k} ({which}): FD {fd:.6e}  fp64 err {e64:.2e}  fp32 err {e32:.2e}")
+        assert e64 < 5e-4
+        assert e32 < 1e-3
+
+
+def test_autograd_function_matches_direct_gradient():
+    torch = pytest.importorskip("torch")
+    cfg = adjoint_small(nstep=200)
+    _say("torch.autograd.Function: L2 misfit backward (FP32)")
+    c = _ctx(cfg, 4)
+    c.set_model(*perturbed_model(cfg))
+    c.run_forward_checkpointed(every=50)
+    c.synchronize()
+    obs = c.seismograms()[:, 0].copy()
+    m0 = _model(cfg)
+    chi, g = misfit_and_gradient(c, m0, obs, every=50)
+    k, m, r = (torch.tensor(x, requires_grad=True) for x in m0)
+    s = Seismograms.apply(k, m, r, c, 50)
+    loss = 0.5 * cfg.dt * ((s.double() - torch.from_numpy(obs).double()) ** 2).sum()
+    loss.backward()
+    assert abs(loss.item() - chi) <= 1e-12 * abs(chi) + 1e-30
+    for t, ref in zip((k, m, r), g):
+        assert rel_l2(t.grad.numpy(), ref) < 1e-6
+
+
+def test_adjoint_error_paths():
+    cfg = adjoint_small(nstep=50)
+    c = _ctx(cfg, 4)
+    with pytest.raises(RuntimeError, match="checkpointed"):
+        c.run_adjoint(np.zeros((cfg.nstep, len(cfg.receivers), 2)))
+    c.run_forward_checkpointed(every=10)
+    c.set_model(*_model(cfg))  # a new model drops the checkpoints
+    with pytest.raises(RuntimeError, match="checkpointed"):
+        c.run_adjoint(np.zeros((cfg.nstep, len(cfg.receivers), 2)))
+    with pytest.raises(RuntimeError, match="positive"):
+        c.run_forward_checkpointed(every=0)
+    bad = [x.copy() for x in _model(cfg)]
+    bad[2][0, 0, 0] = -1.0
+    with pytest.raises(RuntimeError, match="non-physical"):
+        c.set_model(*bad)
+    assert _lib.load().smx_run_adjoint(None, None) != 0

@@ -1,0 +1,76 @@
+"""The oracle's discrete adjoint, pinned by mathematics rather than by data
+(the reference ships no kernel values for this path):
+
+* central finite differences of the L2 waveform misfit in FP64 (relative
+  step 1e-4, whose truncation error is below 1e-6 here), in random
+  directions of kappa, mu and rho separately and jointly (a dropped Stacey or
+  mass term, a wrong sign or a transposed index fails these);
+* zero adjoint source gives a zero gradient, and the gradient is linear in the
+  adjoint source.
+"""
+
+import dataclasses
+
+import numpy as np
+import pytest
+
+from specfempp_amd.adjoint import l2_misfit
+from tests._cases import adjoint_small, perturbed_model
+
+
+def _oracle(oracle_lib, cfg, model=None, precision=8):
+    mesh = cfg.mesh
+    if model is not None:
+        mesh = dataclasses.replace(mesh, kappa=model[0], mu=model[1], rho=model[2])
+    return oracle_lib.Oracle2D(mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep,
+                               precision=precision)
+
+
+def _seis(oracle_lib, cfg, model):
+    return _oracle(oracle_lib, cfg, model).forward_store()
+
+
+@pytest.fixture(scope="module")
+def case(oracle_lib):
+    cfg = adjoint_small(nstep=240)
+    obs = _seis(oracle_lib, cfg, perturbed_model(cfg))
+    o = _oracle(oracle_lib, cfg)
+    s = o.forward_store()
+    grad = o.adjoint(cfg.dt * (s - obs))
+    return cfg, obs, grad
+
+
+def test_oracle_adjoint_finite_differences(oracle_lib, case):
+    cfg, obs, grad = case
+    m0 = [x.astype(np.float64) for x in (cfg.mesh.kappa, cfg.mesh.mu, cfg.mesh.rho)]
+    rng = np.random.default_rng(11)
+    worst = 0.0
+    for which in (0, 1, 2, None):
+        dirs = [np.zeros_like(x) for x in m0]
+        for i in ([which] if which is not None else [0, 1, 2]):
+            dirs[i] = rng.standard_normal(m0[i].shape) * m0[i] * 1e-4
+        # the model is FP32: use the exactly representable +- perturbations
+        mp = [(x + d).astype(np.float32) for x, d in zip(m0, dirs)]
+        mm = [(x - d).astype(np.float32) for x, d in zip(m0, dirs)]
+        chip = l2_misfit(_seis(oracle_lib, cfg, mp), obs, cfg.dt)
+        chim = l2_misfit(_seis(oracle_lib, cfg, mm), obs, cfg.dt)
+        fd = chip - chim
+        an = sum(float(np.sum(g * (p.astype(np.float64) - m.astype(np.float64))))
+                 for g, p, m in zip(grad, mp, mm))
+        err = abs(fd - an) / abs(fd)
+        worst = max(worst, err)
+        # central differences: truncation ~ step^2 (1.7e-7 .. 7e-7 here)
+        assert err < 3e-6, (which, fd, an, err)
+    print("oracle adjoint vs FD: worst relative error", worst)
+
+
+def test_oracle_adjoint_zero_and_linear(oracle_lib, case):
+    cfg, _, _ = case
+    o = _oracle(oracle_lib, cfg)
+    o.forward_store()
+    rng = np.random.default_rng(3)
+    gsrc = rng.standard_normal((cfg.nstep, len(cfg.receivers), 2)) * 1e-12
+    g1 = o.adjoint(gsrc)
+    g2 = o.adjoint(2.0 * gsrc)
+    assert np.all(o.adjoint(np.zeros_like(gsrc)) == 0.0)
+    np.testing.assert_allclose(g2, 2.0 * g1, rtol=1e-12, atol=1e-30 + 1e-12 * np.abs(g1).max())
