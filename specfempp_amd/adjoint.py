@@ -102,3 +102,20 @@ try:  # torch is plumbing for the autograd interface, not a compute path
 
 except ImportError:  # pragma: no cover - torch is part of the image
     Seismograms = None
+
+
+def exact_direction(model32, direction, steps=(1, -1, 2, -2)):
+    """Quantizes `direction` (float64, same shape as the FP32 array
+    `model32`) to multiples of the FP32 spacing of each value, so that the
+    model points model + t * direction (t in `steps`) are exactly
+    representable in FP32 and lie on one line: finite differences of a
+    function of the FP32 model then see no rounding of the step."""
+    m32 = np.asarray(model32, dtype=np.float32)
+    m = m32.astype(np.float64)
+    u = np.spacing(np.abs(m32)).astype(np.float64)
+    dq = np.round(np.asarray(direction, dtype=np.float64) / u) * u
+    for t in steps:
+        p = m + t * dq
+        if not np.array_equal(p.astype(np.float32).astype(np.float64), p):
+            raise ValueError("the step leaves the FP32 binade of some values; use a smaller step")
+    return dq

@@ -134,3 +134,20 @@ def adjoint_case(nx=80, nz=60, nstep=2000, dt=5e-4, nrec=50):
     true = gaussian_vs_anomaly(cfg.mesh, x0=0.5 * nx * h, z0=0.5 * nz * h)
     cfg.name = f"adjoint_{nx}x{nz}"
     return cfg, true
+
+
+def smooth_perturbation(mesh: Mesh2D, rng, nmodes=6, kmax=4):
+    """A seeded smooth field at every GLL point (nspec, 5, 5) with unit RMS:
+    a random combination of ``nmodes`` products sin(p pi x / Lx + a)
+    cos(q pi z / Lz + b), 1 <= p, q <= kmax (finite-difference directions)."""
+    xz = gll_coordinates(mesh)
+    x0, x1 = float(xz[..., 0].min()), float(xz[..., 0].max())
+    z0, z1 = float(xz[..., 1].min()), float(xz[..., 1].max())
+    X = (xz[..., 0] - x0) / max(x1 - x0, 1e-30)
+    Z = (xz[..., 1] - z0) / max(z1 - z0, 1e-30)
+    f = np.zeros(X.shape)
+    for _ in range(nmodes):
+        p, q = rng.integers(1, kmax + 1, size=2)
+        a, b = rng.uniform(0, 2 * np.pi, size=2)
+        f += rng.standard_normal() * np.sin(p * np.pi * X + a) * np.cos(q * np.pi * Z + b)
+    return f / np.sqrt(np.mean(f * f))

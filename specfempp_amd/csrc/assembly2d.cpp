@@ -935,27 +935,51 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
   }
 }
 
+// Model-derived coefficients in double for FP64 contexts: lambda+2mu at every
+// point and the Stacey impedances at absorbing points, from the FP32 model.
+static void refresh_fp64(HostAssembly2D &A) {
+  const int ld = A.nspec_pad, sld = A.nstacey_pad;
+  const size_t pld = size_t(NPT) * ld, sp = size_t(NPT) * sld;
+  A.lam2mu64.assign(pld, 0.0);
+  for (size_t o = 0; o < pld; ++o)
+    A.lam2mu64[o] = double(A.kappa_mu_rho[o]) +
+                    4.0 / 3.0 * double(A.kappa_mu_rho[pld + o]);
+  A.imp64.assign(2 * sp, 0.0);
+  for (size_t k = 0; k < A.bpt_g.size(); ++k) {
+    const size_t o = size_t(A.bpt_p[k]) * ld + A.bpt_slot[k];
+    const size_t so = size_t(A.bpt_p[k]) * sld + A.bpt_bslot[k];
+    const double mu = A.kappa_mu_rho[pld + o], rho = A.kappa_mu_rho[2 * pld + o];
+    A.imp64[so] = std::sqrt(A.lam2mu64[o] / rho) * rho;
+    A.imp64[sp + so] = std::sqrt(rho * mu);
+  }
+}
+
 void refresh_boundary_matrices(HostAssembly2D &A) {
+  refresh_fp64(A);
   const int sld = A.nstacey_pad;
   std::unordered_map<int, int> index;
   for (size_t j = 0; j < A.bnode_g.size(); ++j)
     index[A.bnode_g[j]] = static_cast<int>(j);
   A.bsum.assign(3 * std::max<size_t>(1, A.bnode_g.size()), 0.0);
-  std::vector<double> &acc = A.bsum;
+  A.bsum64.assign(A.bsum.size(), 0.0);
+  const size_t sp = size_t(NPT) * sld;
   for (size_t k = 0; k < A.bpt_g.size(); ++k) {
     const int p = A.bpt_p[k], bs = A.bpt_bslot[k];
-    const size_t sp = size_t(NPT) * sld;
-    const double n0 = A.stacey[size_t(p) * sld + bs],
-                 n1 = A.stacey[sp + size_t(p) * sld + bs],
-                 nrm = A.stacey[2 * sp + size_t(p) * sld + bs],
-                 w = A.stacey[3 * sp + size_t(p) * sld + bs],
-                 P = A.stacey[4 * sp + size_t(p) * sld + bs],
-                 S = A.stacey[5 * sp + size_t(p) * sld + bs];
+    const size_t so = size_t(p) * sld + bs;
+    const double n0 = A.stacey[so], n1 = A.stacey[sp + so],
+                 nrm = A.stacey[2 * sp + so], w = A.stacey[3 * sp + so];
     const double h0 = n0 / nrm, h1 = n1 / nrm, f = nrm * w;
     const int j = index[A.bpt_g[k]];
-    acc[3 * j + 0] += f * ((P - S) * h0 * h0 + S);
-    acc[3 * j + 1] += f * ((P - S) * h0 * h1);
-    acc[3 * j + 2] += f * ((P - S) * h1 * h1 + S);
+    const double imp[2][2] = {{A.stacey[4 * sp + so], A.stacey[5 * sp + so]},
+                              {A.imp64[so], A.imp64[sp + so]}};
+    std::vector<double> *out[2] = {&A.bsum, &A.bsum64};
+    for (int v = 0; v < 2; ++v) {
+      const double P = imp[v][0], S = imp[v][1];
+      std::vector<double> &acc = *out[v];
+      acc[3 * j + 0] += f * ((P - S) * h0 * h0 + S);
+      acc[3 * j + 1] += f * ((P - S) * h0 * h1);
+      acc[3 * j + 2] += f * ((P - S) * h1 * h1 + S);
+    }
   }
 }
 

@@ -129,6 +129,10 @@ struct HostAssembly2D {
   std::vector<int32_t> bpt_slot, bpt_p, bpt_bslot, bpt_g;
   std::vector<int32_t> bnode_g;
   std::vector<double> bsum;
+  // The model-derived coefficients formed in double (for FP64 contexts):
+  // lambda+2mu [NPT][nspec_pad], Stacey impedances [2][NPT][nstacey_pad]
+  // (rho vp, rho vs) and the boundary matrices built from them.
+  std::vector<double> lam2mu64, imp64, bsum64;
 
   // Time scheme constants as the reference forms them (newmark.hpp:131-135).
   float dt = 0, dt_over2 = 0, dt2_over2 = 0;

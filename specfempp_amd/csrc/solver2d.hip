@@ -32,6 +32,7 @@
 // partial slots, and reads shared points that only k_shared updates.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1137,14 +1138,19 @@ public:
     dto2_ = A.dt_over2;
     dt2o2_ = A.dt2_over2;
     field(SMX_FIELD_FORWARD);
+    upload_material();
     SMX_HIP(hipStreamSynchronize(stream_));
   }
 
   ~Solver2D() override {
     drop_graph(graph_, graph_exec_);
-    drop_graph(rec_graph_, rec_graph_exec_);
-    drop_graph(adj_graph_, adj_graph_exec_);
+    for (int b = 0; b < 2; ++b) {
+      drop_graph(rec_graph_[b], rec_graph_exec_[b]);
+      drop_graph(adj_graph_[b], adj_graph_exec_[b]);
+    }
     (void)hipStreamSynchronize(stream_);
+    if (adj_stream_)
+      (void)hipStreamSynchronize(adj_stream_);
     for (void *p : allocs_)
       (void)hipFree(p);
     (void)hipEventDestroy(ev0_);
@@ -1152,6 +1158,12 @@ public:
     for (hipEvent_t e : adj_ev_)
       if (e)
         (void)hipEventDestroy(e);
+    for (hipEvent_t e : {rec_done_[0], rec_done_[1], adj_done_[0],
+                         adj_done_[1], sync_ev_[0], sync_ev_[1]})
+      if (e)
+        (void)hipEventDestroy(e);
+    if (adj_stream_)
+      (void)hipStreamDestroy(adj_stream_);
     (void)hipStreamDestroy(stream_);
   }
 
@@ -1316,11 +1328,7 @@ public:
                  const float *rho) override {
     HostAssembly2D &A = *asm_;
     set_model2d(A, kappa, mu, rho);
-    copy_to(mat_, A.lam2mu_mu);
-    copy_to(kmr_, A.kappa_mu_rho);
-    copy_to(stacey_, A.stacey);
-    if (bsum_)
-      copy_to(bsum_, A.bsum);
+    upload_material();
     ckpt_every_ = 0;
     initialize();
   }
@@ -1338,11 +1346,13 @@ public:
       ckpt_cap_ = nseg;
     }
     if (every != rec_every_) {
-      release(rec_);
-      rec_ = alloc<real>(size_t(every + 1) * 3 * n2_);
+      for (int b = 0; b < 2; ++b) {
+        release(rec_[b]);
+        rec_[b] = alloc<real>(size_t(every + 1) * 3 * n2_);
+        drop_graph(rec_graph_[b], rec_graph_exec_[b]);
+        drop_graph(adj_graph_[b], adj_graph_exec_[b]);
+      }
       rec_every_ = every;
-      drop_graph(rec_graph_, rec_graph_exec_);
-      drop_graph(adj_graph_, adj_graph_exec_);
     }
     ensure_adjoint_events();
     Field &F = field(SMX_FIELD_FORWARD);
@@ -1377,6 +1387,11 @@ public:
     have_adj_stats_ = false;
   }
 
+  // Segments are processed newest first.  The recompute of segment s-1
+  // (context stream, record buffer (s-1)%2) runs concurrently with the
+  // adjoint steps of segment s (adjoint stream, record buffer s%2); events
+  // order each recompute after the adjoint that last read its buffer and
+  // each adjoint after the recompute of its segment.
   void run_adjoint(const void *gsrc_host) override {
     const HostAssembly2D &A = *asm_;
     if (!ckpt_every_)
@@ -1400,44 +1415,59 @@ public:
     for (real *p : {sp_, ss_})
       SMX_HIP(hipMemsetAsync(p, 0, size_t(NPT) * A.nstacey_pad * sizeof(real),
                              stream_));
+    // fork: the adjoint stream starts after everything queued so far
+    SMX_HIP(hipEventRecord(sync_ev_[0], stream_));
+    SMX_HIP(hipStreamWaitEvent(adj_stream_, sync_ev_[0], 0));
     const int nrin = int(A.rin_node.size());
     if (nrin > 0) {
-      k_adj_inject<real><<<dim3((nrin + 255) / 256), dim3(256), 0, stream_>>>(
-          B.u, rin_node_, rin_off_, rin_ent_, rent_h_, gsrc_, A.nrec, nrin,
-          A.nstep - 1);
+      k_adj_inject<real><<<dim3((nrin + 255) / 256), dim3(256), 0,
+                           adj_stream_>>>(B.u, rin_node_, rin_off_, rin_ent_,
+                                          rent_h_, gsrc_, A.nrec, nrin,
+                                          A.nstep - 1);
       SMX_HIP(hipGetLastError());
     }
     for (int sgm = nseg - 1; sgm >= 0; --sgm) {
+      const int b = sgm & 1;
       const int k0 = sgm * every, len = std::min(every, A.nstep - k0);
+      const int reps = len / kGraphSteps;
       const real *ck = ckpt_ + size_t(sgm) * 2 * n2_;
-      // recompute the segment from its checkpoint, recording every step
+      // recompute the segment from its checkpoint into record buffer b,
+      // once the adjoint of segment sgm+2 (its last reader) is done
+      if (sgm + 2 < nseg)
+        SMX_HIP(hipStreamWaitEvent(stream_, adj_done_[b], 0));
       SMX_HIP(hipMemcpyAsync(W.u, ck, n2_ * sizeof(real),
                              hipMemcpyDeviceToDevice, stream_));
       SMX_HIP(hipMemcpyAsync(W.v, ck + n2_, n2_ * sizeof(real),
                              hipMemcpyDeviceToDevice, stream_));
-      SMX_HIP(hipMemcpyAsync(rec_, ck, 2 * n2_ * sizeof(real),
+      SMX_HIP(hipMemcpyAsync(rec_[b], ck, 2 * n2_ * sizeof(real),
                              hipMemcpyDeviceToDevice, stream_));
       k_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_, k0);
       SMX_HIP(hipGetLastError());
-      const int rreps = len / kGraphSteps;
-      if (rreps > 0) {
-        build_rec_graph(W);
-        for (int r = 0; r < rreps; ++r)
-          SMX_HIP(hipGraphLaunch(rec_graph_exec_, stream_));
+      if (reps > 0) {
+        build_rec_graph(W, b);
+        for (int r = 0; r < reps; ++r)
+          SMX_HIP(hipGraphLaunch(rec_graph_exec_[b], stream_));
       }
-      for (int st = rreps * kGraphSteps; st < len; ++st)
-        record_step(W);
-      // adjoint steps k0+len-1 .. k0
-      k_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_ + 2, k0 + len - 1);
+      for (int st = reps * kGraphSteps; st < len; ++st)
+        record_step(W, rec_[b]);
+      SMX_HIP(hipEventRecord(rec_done_[b], stream_));
+      // adjoint steps k0+len-1 .. k0 on the adjoint stream
+      SMX_HIP(hipStreamWaitEvent(adj_stream_, rec_done_[b], 0));
+      k_set_step<<<dim3(1), dim3(64), 0, adj_stream_>>>(step_ + 2,
+                                                         k0 + len - 1);
       SMX_HIP(hipGetLastError());
-      if (rreps > 0) {
-        build_adj_graph(B);
-        for (int r = 0; r < rreps; ++r)
-          SMX_HIP(hipGraphLaunch(adj_graph_exec_, stream_));
+      if (reps > 0) {
+        build_adj_graph(B, b);
+        for (int r = 0; r < reps; ++r)
+          SMX_HIP(hipGraphLaunch(adj_graph_exec_[b], adj_stream_));
       }
-      for (int st = rreps * kGraphSteps; st < len; ++st)
-        adjoint_step(B);
+      for (int st = reps * kGraphSteps; st < len; ++st)
+        adjoint_step(B, rec_[b], adj_stream_);
+      SMX_HIP(hipEventRecord(adj_done_[b], adj_stream_));
     }
+    // join
+    SMX_HIP(hipEventRecord(sync_ev_[1], adj_stream_));
+    SMX_HIP(hipStreamWaitEvent(stream_, sync_ev_[1], 0));
     SMX_HIP(hipEventRecord(adj_ev_[3], stream_));
     SMX_HIP(hipStreamSynchronize(stream_));
     have_adj_stats_ = true;
@@ -1579,7 +1609,8 @@ private:
   // buffer; seismogram terms are written only when `terms` is given.
   template <int MODE>
   void launch_tiles(real *U, real *V, real *Aout, real *part, int istep,
-                    int *step_ptr, real *terms = nullptr) {
+                    int *step_ptr, real *terms = nullptr,
+                    real *rec = nullptr) {
     const HostAssembly2D &A = *asm_;
     const int ntypes = int(A.seis_types.size());
     k_tile<real, MODE><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_,
@@ -1588,74 +1619,97 @@ private:
         mat_, sslot_, stacey_, U, V, Aout, minv_, part ? part : part_, src_li_,
         src_idx_, src_L_, stf_, std::max(1, A.nsrc), trec_li_, trec_entry_,
         rent_h_, terms, A.nrec * NPT, stypes_, ntypes, A.nsbs, step_ptr, istep,
-        A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, rec_,
+        A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, rec,
         rec_stride(), std::max(1, rec_every_));
     SMX_HIP(hipGetLastError());
   }
   template <int MODE>
-  void launch_shared(real *U, real *V, real *Aout, int *step_ptr) {
+  void launch_shared(real *U, real *V, real *Aout, int *step_ptr,
+                     real *rec = nullptr) {
     const HostAssembly2D &A = *asm_;
     const int n = std::max(1, A.nshared);
     k_shared<real, MODE><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS),
                            dim3(NODE_THREADS), 0, stream_>>>(
         U, V, Aout, minv_, part_, shp_off_, shp_pos_, A.nshared, A.nex_total,
-        dt_, dto2_, dt2o2_, step_ptr, rec_, rec_stride(),
+        dt_, dto2_, dt2o2_, step_ptr, rec, rec_stride(),
         std::max(1, rec_every_));
     SMX_HIP(hipGetLastError());
   }
   long rec_stride() const { return long(n2_ * sizeof(real)); }
 
-  // Recompute step: the fused step that also records (u~, v~, a').
-  void record_step(Field &W) {
-    launch_tiles<MODE_RECORD>(W.u, W.v, W.a, nullptr, 0, step_);
-    launch_shared<MODE_RECORD>(W.u, W.v, W.a, step_);
-  }
-  // One adjoint step (device step counter step_[2]).
-  void adjoint_step(Field &B) {
+  // Material arrays on the device.  An FP32 context uses the FP32 values
+  // the host formed (the reference's rounding); an FP64 context takes
+  // lambda+2mu and the Stacey impedances formed in double from the same
+  // FP32 model, so that it is an FP64 solve of that model.
+  void upload_material() {
     const HostAssembly2D &A = *asm_;
-    k_adj_tile<real><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_,
-                       stream_>>>(
+    std::vector<double> m(A.lam2mu_mu.begin(), A.lam2mu_mu.end());
+    std::vector<double> st(A.stacey.begin(), A.stacey.end());
+    if (sizeof(real) == 8) {
+      const size_t sp = A.imp64.size() / 2; // NPT * nstacey_pad
+      std::copy(A.lam2mu64.begin(), A.lam2mu64.end(), m.begin());
+      std::copy(A.imp64.begin(), A.imp64.end(), st.begin() + 4 * sp);
+    }
+    copy_to(mat_, m);
+    copy_to(stacey_, st);
+    copy_to(kmr_, A.kappa_mu_rho);
+    if (bsum_)
+      copy_to(bsum_, sizeof(real) == 8 ? A.bsum64 : A.bsum);
+  }
+
+  // Recompute step: the fused step that also records (u~, v~, a') into
+  // `rec` (context stream).
+  void record_step(Field &W, real *rec) {
+    launch_tiles<MODE_RECORD>(W.u, W.v, W.a, nullptr, 0, step_, nullptr, rec);
+    launch_shared<MODE_RECORD>(W.u, W.v, W.a, step_, rec);
+  }
+  // One adjoint step reading the record `rec` (device step counter
+  // step_[2]; own partials buffer, so that it may overlap a recompute).
+  void adjoint_step(Field &B, real *rec, hipStream_t st) {
+    const HostAssembly2D &A = *asm_;
+    k_adj_tile<real><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_, st>>>(
         tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
-        mat_, sslot_, stacey_, rb0_, rb1_, B.u, B.v, minv_, part_, rec_,
+        mat_, sslot_, stacey_, rb0_, rb1_, B.u, B.v, minv_, part_adj_, rec,
         rec_stride(), rec_every_, bidx_, bsum_, gd_, ge_, racc_, sp_, ss_,
         step_ + 2, A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_);
     SMX_HIP(hipGetLastError());
     const int nex_rin = int(A.rin_excl.size());
     const int n = std::max(1, A.nshared + nex_rin);
     k_adj_shared<real><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS),
-                         dim3(NODE_THREADS), 0, stream_>>>(
-        rb0_, rb1_, B.u, B.v, minv_, part_, shp_off_, shp_pos_, A.nshared,
-        A.nex_total, rec_, rec_stride(), rec_every_, bidx_, bsum_, racc_,
+                         dim3(NODE_THREADS), 0, st>>>(
+        rb0_, rb1_, B.u, B.v, minv_, part_adj_, shp_off_, shp_pos_, A.nshared,
+        A.nex_total, rec, rec_stride(), rec_every_, bidx_, bsum_, racc_,
         sh_rin_, rin_excl_, nex_rin, rin_node_, rin_off_, rin_ent_, rent_h_,
         gsrc_, A.nrec, step_ + 2, dt_, dto2_, dt2o2_);
     SMX_HIP(hipGetLastError());
   }
-  template <typename F> void capture(hipGraph_t &g, hipGraphExec_t &ge, F body) {
+  template <typename F>
+  void capture(hipGraph_t &g, hipGraphExec_t &ge, hipStream_t st, F body) {
     if (ge)
       return;
-    SMX_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    SMX_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
     try {
       body();
     } catch (...) {
-      hipGraph_t tmp;
-      (void)hipStreamEndCapture(stream_, &tmp);
+      hipGraph_t tmp = nullptr;
+      (void)hipStreamEndCapture(st, &tmp);
       if (tmp)
         (void)hipGraphDestroy(tmp);
       throw;
     }
-    SMX_HIP(hipStreamEndCapture(stream_, &g));
+    SMX_HIP(hipStreamEndCapture(st, &g));
     SMX_HIP(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
   }
-  void build_rec_graph(Field &W) {
-    capture(rec_graph_, rec_graph_exec_, [&] {
+  void build_rec_graph(Field &W, int b) {
+    capture(rec_graph_[b], rec_graph_exec_[b], stream_, [&] {
       for (int s = 0; s < kGraphSteps; ++s)
-        record_step(W);
+        record_step(W, rec_[b]);
     });
   }
-  void build_adj_graph(Field &B) {
-    capture(adj_graph_, adj_graph_exec_, [&] {
+  void build_adj_graph(Field &B, int b) {
+    capture(adj_graph_[b], adj_graph_exec_[b], adj_stream_, [&] {
       for (int s = 0; s < kGraphSteps; ++s)
-        adjoint_step(B);
+        adjoint_step(B, rec_[b], adj_stream_);
     });
   }
   static void drop_graph(hipGraph_t &g, hipGraphExec_t &ge) {
@@ -1699,12 +1753,17 @@ private:
     if (rb0_)
       return;
     const HostAssembly2D &A = *asm_;
+    SMX_HIP(hipStreamCreateWithFlags(&adj_stream_, hipStreamNonBlocking));
+    for (hipEvent_t *e : {&rec_done_[0], &rec_done_[1], &adj_done_[0],
+                          &adj_done_[1], &sync_ev_[0], &sync_ev_[1]})
+      SMX_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+    part_adj_ = alloc<real>(2 * std::max<size_t>(1, A.sh_list.size()));
     std::vector<int32_t> bidx(size_t(nglob_pad_), -1);
     for (size_t j = 0; j < A.bnode_g.size(); ++j)
       bidx[A.bnode_g[j]] = int32_t(j);
     bidx_ = upload_raw<int>(bidx);
     bsum_ = alloc<real>(A.bsum.size());
-    copy_to(bsum_, A.bsum);
+    copy_to(bsum_, sizeof(real) == 8 ? A.bsum64 : A.bsum);
     sh_rin_ = upload_raw<int>(A.sh_rin);
     rin_excl_ = upload_raw<int>(A.rin_excl);
     rin_node_ = upload_raw<int>(A.rin_node);
@@ -1739,7 +1798,7 @@ private:
     launch_shared<MODE>(F.u, F.v, F.a, step_);
   }
   void build_graph(Field &F) {
-    capture(graph_, graph_exec_, [&] {
+    capture(graph_, graph_exec_, stream_, [&] {
       for (int s = 0; s < kGraphSteps; ++s)
         fused_step<MODE_STEP>(F);
     });
@@ -1749,9 +1808,9 @@ private:
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-  hipGraph_t graph_ = nullptr, rec_graph_ = nullptr, adj_graph_ = nullptr;
-  hipGraphExec_t graph_exec_ = nullptr, rec_graph_exec_ = nullptr,
-                 adj_graph_exec_ = nullptr;
+  hipGraph_t graph_ = nullptr, rec_graph_[2] = {}, adj_graph_[2] = {};
+  hipGraphExec_t graph_exec_ = nullptr, rec_graph_exec_[2] = {},
+                 adj_graph_exec_[2] = {};
   std::vector<void *> allocs_;
   long nglob_pad_ = 0, n2_ = 0;
   TileInfo *tiles_ = nullptr;
@@ -1777,13 +1836,17 @@ private:
   int last_launches_ = 0;
   // adjoint: checkpoints [nseg][u, v], the recompute record (every+1 slots
   // of u~, v~, a'), adjoint rb in two parity buffers, gradient accumulators
-  real *ckpt_ = nullptr, *rec_ = nullptr, *rb0_ = nullptr, *rb1_ = nullptr,
+  real *ckpt_ = nullptr, *rec_[2] = {}, *part_adj_ = nullptr, *rb0_ = nullptr,
+       *rb1_ = nullptr,
        *racc_ = nullptr, *gd_ = nullptr, *ge_ = nullptr, *sp_ = nullptr,
        *ss_ = nullptr, *bsum_ = nullptr, *gsrc_ = nullptr;
   int *bidx_ = nullptr, *sh_rin_ = nullptr, *rin_excl_ = nullptr,
       *rin_node_ = nullptr, *rin_off_ = nullptr, *rin_ent_ = nullptr;
   int ckpt_cap_ = 0, rec_every_ = 0, ckpt_every_ = 0;
   hipEvent_t adj_ev_[4] = {nullptr, nullptr, nullptr, nullptr};
+  // adjoint stream (default priority) and its ordering events
+  hipStream_t adj_stream_ = nullptr;
+  hipEvent_t rec_done_[2] = {}, adj_done_[2] = {}, sync_ev_[2] = {};
   bool have_adj_stats_ = false, have_gradient_ = false;
 };
 

@@ -7,13 +7,10 @@ Tolerances (relative L2 over each of the kappa / mu / rho gradients):
   FP64 arithmetic it solves a problem whose coefficients differ from the FP64
   oracle's by ~1e-7 relative (its seismograms differ by ~2e-6).
 * FP32 GPU vs FP64 oracle: 2e-4 (FP32 forward states and accumulation).
-* Finite differences (central, FP64 GPU misfit, relative step 1e-3):
-  5e-4 for the FP64 gradient, 1e-3 for the FP32 gradient.  The material is
-  FP32 and the moduli and impedances derived from it are formed in FP32 (as
-  in the reference), which puts ~6e-8 relative noise on a 1e-3 step, and the
-  central difference's own truncation error is ~2e-5 .. 7e-5 here; the exact
-  pin of the adjoint is the oracle's (FP64 coefficients, relative step 1e-4,
-  error < 3e-6) together with the GPU-vs-oracle gradient match above.
+* Finite differences (fourth-order central stencil on the FP64 GPU misfit,
+  relative step 1e-3, directions quantized so that every stencil point is an
+  exact FP32 model): 1e-5 for the FP64 gradient, 1e-3 for the FP32
+  gradient.
 """
 
 import dataclasses
@@ -23,7 +20,8 @@ import numpy as np
 import pytest
 
 from specfempp_amd import _lib
-from specfempp_amd.adjoint import Seismograms, l2_misfit, misfit_and_gradient
+from specfempp_amd.adjoint import Seismograms, exact_direction, l2_misfit, misfit_and_gradient
+from specfempp_amd.config import smooth_perturbation
 from specfempp_amd.solver import Context
 from tests._cases import adjoint_small, perturbed_model, rel_l2
 
@@ -138,19 +136,22 @@ def test_adjoint_finite_differences():
     _, g32 = misfit_and_gradient(c32, m0, obs.astype(np.float32), every=100)
     rng = np.random.default_rng(21)
     base = [x.astype(np.float64) for x in m0]
-    for k, which in enumerate((0, 1, 2, None, None, None)):
+    for k, (which, smooth) in enumerate(((0, False), (1, False), (2, False), (None, False),
+                                         (None, True), (None, True))):
         dirs = [np.zeros_like(x) for x in base]
         for i in ([which] if which is not None else [0, 1, 2]):
-            dirs[i] = rng.standard_normal(base[i].shape) * base[i] * 1e-3
-        mp = [(x + d).astype(np.float32) for x, d in zip(base, dirs)]
-        mm = [(x - d).astype(np.float32) for x, d in zip(base, dirs)]
-        fd = _misfit(c64, mp, obs) - _misfit(c64, mm, obs)
-        step = [p.astype(np.float64) - m.astype(np.float64) for p, m in zip(mp, mm)]
-        a64 = sum(float(np.sum(g * d)) for g, d in zip(g64, step))
-        a32 = sum(float(np.sum(g * d)) for g, d in zip(g32, step))
+            f = smooth_perturbation(cfg.mesh, rng) if smooth else rng.standard_normal(base[i].shape)
+            dirs[i] = exact_direction(m0[i], f * base[i] * 1e-3)
+
+        def chi(t):
+            return _misfit(c64, [(x + t * d).astype(np.float32) for x, d in zip(base, dirs)],
+                           obs)
+        fd = (8.0 * (chi(1) - chi(-1)) - (chi(2) - chi(-2))) / 12.0
+        a64 = sum(float(np.sum(g * d)) for g, d in zip(g64, dirs))
+        a32 = sum(float(np.sum(g * d)) for g, d in zip(g32, dirs))
         e64, e32 = abs(a64 - fd) / abs(fd), abs(a32 - fd) / abs(fd)
-        _say(f"  direction {k} ({which}): FD {fd:.6e}  fp64 err {e64:.2e}  fp32 err {e32:.2e}")
-        assert e64 < 5e-4
+        _say(f"  direction {k} ({which}, smooth={smooth}): FD {fd:.6e}  fp64 err {e64:.2e}  fp32 err {e32:.2e}")
+        assert e64 < 1e-5
         assert e32 < 1e-3
 
 
