@@ -51,6 +51,7 @@ def _load():
     lib.oracle2d_apply_stiffness.argtypes = [P, i]
     lib.oracle2d_forward_store.argtypes = [P, i, P]
     lib.oracle2d_adjoint.argtypes = [P, i, P, P]
+    lib.oracle2d_ref_kernels.argtypes = [P, i, P, P, P]
     lib.oracle2d_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -148,6 +149,18 @@ class Oracle2D:
         if self.lib.oracle2d_adjoint(self.h, n, _ptr(g), _ptr(out)) != 0:
             raise RuntimeError(self.lib.oracle2d_last_error().decode())
         return out.reshape(3, -1, 5, 5)
+
+    def ref_kernels(self, gsrc):
+        """(gradient (3, nspec, 5, 5), kernels (6, nspec, 5, 5)): the
+        discrete-adjoint gradient and the reference's Frechet-kernel formula
+        on the same fields -- [0:3] with the reference's quirks (grad(s+ + s),
+        kappa + mu/3), [3:6] corrected -- after forward_store."""
+        g = np.ascontiguousarray(np.asarray(gsrc, dtype=np.float64))
+        grad = np.zeros(3 * self.nloc, dtype=np.float64)
+        k = np.zeros(6 * self.nloc, dtype=np.float64)
+        if self.lib.oracle2d_ref_kernels(self.h, g.shape[0], _ptr(g), _ptr(grad), _ptr(k)) != 0:
+            raise RuntimeError(self.lib.oracle2d_last_error().decode())
+        return grad.reshape(3, -1, 5, 5), k.reshape(6, -1, 5, 5)
 
     def run(self, nstep=None, nsbs=1, types=("displacement",)):
         """Advances nstep steps; returns seismograms (nsig, ntypes, nrec, 2)."""

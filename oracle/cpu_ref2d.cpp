@@ -883,7 +883,20 @@ template <typename real> struct Oracle2D {
 
   // Gradient of sum_k chi_k(u_{k+1}) w.r.t. (kappa, mu, rho) at every local
   // point, given dchi/ds_k = gsrc[k][r][c] for the displacement seismograms.
-  void adjoint(int nsteps, const double *gsrc, double *grad /* [3][nloc] */) {
+  // refk (optional, [6][nloc]): the reference's kernel formula evaluated on
+  // this adjoint's fields with the adjoint displacement s+ = rb / dt and the
+  // forward displacement s = u~ (compute_frechet_derivatives,
+  // include/medium/dim2/elastic/isotropic/frechet_derivative.hpp:98-178, as
+  // called by compute_material_derivatives.tpp:112-151):
+  //   [0..2] kappa, mu, rho kernels with the reference's two quirks -- the
+  //          "backward" gradient is grad(s+ + s) (the two-field gradient
+  //          reuses its accumulators, include/algorithms/gradient.hpp:302-316)
+  //          and kappa' = (lambda+2mu) - mu = kappa + mu/3 (line 77);
+  //   [3..5] the same formula with grad(s) and kappa.
+  // The density term uses s+ . a' (forward acceleration), the time-by-parts
+  // form of the reference's adjoint-acceleration . backward-displacement.
+  void adjoint(int nsteps, const double *gsrc, double *grad /* [3][nloc] */,
+               double *refk = nullptr) {
     const size_t n2 = size_t(nglob) * 2, nloc = size_t(nspec) * NGLL * NGLL;
     std::vector<real> ub(n2, 0), vb(n2, 0), ab(n2, 0), rb(n2, 0), Kr(n2),
         Br(n2), zero(n2, 0);
@@ -929,6 +942,37 @@ template <typename real> struct Oracle2D {
             gm[i] -= wj * (2.0 * ee - 2.0 / 3.0 * divr * divu);
             gr[i] -= wj * (double(rb[2 * g]) * A1[2 * g] +
                            double(rb[2 * g + 1]) * A1[2 * g + 1]);
+            if (refk) {
+              double ad[2][2], bq[2][2], bc[2][2];
+              for (int c = 0; c < 2; ++c)
+                for (int d = 0; d < 2; ++d) {
+                  ad[c][d] = double(dr[iz][ix][c][d]) / double(dt);
+                  bc[c][d] = du[iz][ix][c][d];
+                  bq[c][d] = ad[c][d] + bc[c][d];
+                }
+              auto point_kernels = [&](const double (&b)[2][2], double kap,
+                                       double *out3) {
+                const double ad_xz = 0.5 * (ad[1][0] + ad[0][1]);
+                const double b_xz = 0.5 * (b[1][0] + b[0][1]);
+                const double k_kl = (ad[0][0] + ad[1][1]) * (b[0][0] + b[1][1]);
+                const double m_kl = ad[0][0] * b[0][0] + ad[1][1] * b[1][1] +
+                                    2.0 * ad_xz * b_xz - 1.0 / 3.0 * k_kl;
+                const double r_kl = (double(rb[2 * g]) * A1[2 * g] +
+                                     double(rb[2 * g + 1]) * A1[2 * g + 1]) /
+                                    double(dt);
+                out3[0] = -kap * double(dt) * k_kl;
+                out3[1] = -2.0 * double(mu[i]) * double(dt) * m_kl;
+                out3[2] = -double(rho[i]) * double(dt) * r_kl;
+              };
+              const double l2m = double(kappa[i]) + 4.0 / 3.0 * double(mu[i]);
+              double q[3], c3[3];
+              point_kernels(bq, l2m - double(mu[i]), q);
+              point_kernels(bc, double(kappa[i]), c3);
+              for (int t = 0; t < 3; ++t) {
+                refk[t * nloc + i] += q[t];
+                refk[(3 + t) * nloc + i] += c3[t];
+              }
+            }
             if (elem_stacey[e] && pt_stacey[i]) {
               const double n0 = pt_n0[i], n1 = pt_n1[i];
               const double nrm = std::sqrt(n0 * n0 + n1 * n1);
@@ -1158,6 +1202,26 @@ int oracle2d_adjoint(void *hv, int nsteps, const double *gsrc, double *grad) {
       h->f->adjoint(nsteps, gsrc, grad);
     else
       h->d->adjoint(nsteps, gsrc, grad);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// The reference's kernel formula on the adjoint's fields (see
+// Oracle2D::adjoint): refk [6][nspec*25] = quirky (kappa, mu, rho) and
+// corrected (kappa, mu, rho) kernels; grad as oracle2d_adjoint.
+int oracle2d_ref_kernels(void *hv, int nsteps, const double *gsrc,
+                         double *grad, double *refk) {
+  auto *h = static_cast<OracleHandle *>(hv);
+  try {
+    const size_t nloc = size_t(h->f ? h->f->nspec : h->d->nspec) * 25;
+    std::fill(refk, refk + 6 * nloc, 0.0);
+    if (h->f)
+      h->f->adjoint(nsteps, gsrc, grad, refk);
+    else
+      h->d->adjoint(nsteps, gsrc, grad, refk);
     return 0;
   } catch (const std::exception &e) {
     g_err = e.what();

@@ -111,8 +111,9 @@ def test_fp64_twin(oracle_lib):
     e64 = rel_l2(s64, ref64)
     e32 = rel_l2(s32, s64)
     _say(f"  fp64 GPU vs fp64 oracle = {e64:.3e}; fp32 GPU vs fp64 GPU = {e32:.3e}")
-    # the FP64 GPU run is fed the FP32-rounded assembly (geometry, moduli, time
-    # function); the FP64 oracle assembles in FP64 -> ~1e-7 input difference
+    # the FP64 GPU run is fed the FP32-rounded geometry and time function
+    # (it forms the moduli and impedances in FP64 from the FP32 model); the
+    # FP64 oracle assembles everything in FP64 -> ~1e-7 input difference
     assert e64 <= 1e-5
     assert e32 <= 1e-5
 
