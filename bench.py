@@ -157,6 +157,67 @@ def adjoint_leg(nfd=20, every=100):
                           "median_rel_err": float(np.median(errs))}}
 
 
+def batch_leg(nshots=64, nstep=3000, baseline_steps=300):
+    """Batched multi-shot forward: 64 independent shots, nx ~ U[40, 400] quads
+    of 50 m, nz = round(0.75 nx), six seeded layers each, one Ricker source
+    (f0 ~ U[5, 15] Hz) and 100 surface receivers per shot, dt = 4e-4 s, 3000
+    steps, FP32.  The layout (tile size, graph length) comes from the
+    autotuner.  All shots advance together, one launch per kernel per step.
+    The plain per-shot GPU path (one context per shot, run one after the
+    other) is timed on `baseline_steps` steps of every shot and scaled to
+    `nstep`."""
+    from specfempp_amd.autotune import autotune
+    from specfempp_amd.config import batched_shots
+    from specfempp_amd.solver import Context
+
+    t = time.perf_counter()
+    shots = batched_shots(nshots=nshots, nstep=nstep)
+    gen_s = time.perf_counter() - t
+    t = time.perf_counter()
+    best, timings = autotune(shots, steps=30, warmup=5, use_cache=False)
+    tune_s = time.perf_counter() - t
+    t = time.perf_counter()
+    ctx = Context.batch(shots, options=best)
+    ctx.initialize()
+    setup_s = time.perf_counter() - t
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.run_forward(0, nstep)
+    ctx.synchronize()
+    wall = time.perf_counter() - t
+    ev_ms = ctx.last_run_ms()
+    stiff_ms = ctx.time_stiffness(20)
+    nstacey = sum(len(set(c.mesh.abs_ispec.tolist())) for c in shots)
+    sb = tile_kernel_bytes(ctx.nspec, ctx.layout, nstacey)
+    nglob, nspec, layout = ctx.nglob, ctx.nspec, dict(ctx.layout)
+    ctx.close()
+    # plain per-shot path: default layout, one context per shot, in turn
+    per_shot_ms = 0.0
+    for c in shots:
+        cc = Context(c.mesh, c.sources, c.receivers, c.dt, baseline_steps)
+        cc.initialize()
+        cc.run_forward(0, baseline_steps)
+        cc.synchronize()
+        per_shot_ms += cc.last_run_ms() / baseline_steps
+        cc.close()
+    return {"workload": f"{nshots} shots, nx ~ U[40, 400], nz = 0.75 nx, 50 m quads, 6 seeded "
+                        f"layers, 1 Ricker source and 100 receivers per shot, dt=4e-4, "
+                        f"{nstep} steps",
+            "nspec": nspec, "nglob": nglob, "layout": layout, "options": best,
+            "autotune_ms_per_step": {k: v for k, v in timings.items()},
+            "shots_per_s": nshots / wall, "wall_s": wall, "event_ms": ev_ms,
+            "ms_per_step": wall * 1e3 / nstep,
+            "point_updates_per_s": nglob * nstep / wall,
+            "k_tile": {"mean_launch_ms": stiff_ms, "algorithmic_bytes_per_launch": sb,
+                       "achieved_GBs": sb / (stiff_ms * 1e-3) / 1e9,
+                       "frac": sb / (stiff_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+            "per_shot_path": {"ms_per_step_sum": per_shot_ms,
+                              "shots_per_s": nshots / (per_shot_ms * nstep * 1e-3),
+                              "timed_steps_per_shot": baseline_steps},
+            "speedup_vs_per_shot": per_shot_ms * nstep * 1e-3 / wall,
+            "host_s": {"generate": gen_s, "autotune": tune_s, "assemble": setup_s}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -167,6 +228,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint-gradient leg (80x60 box, 2000 steps)")
+    ap.add_argument("--no-batch", action="store_true",
+                    help="skip the batched-shots leg (64 shots, 3000 steps)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per k_tile launch measured with rocprofv3 --pmc "
@@ -258,6 +321,8 @@ def main():
     del ctx
     if not args.no_adjoint:
         out["adjoint"] = adjoint_leg()
+    if not args.no_batch:
+        out["batch"] = batch_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     print(json.dumps(out), flush=True)

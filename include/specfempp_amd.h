@@ -101,6 +101,37 @@ int smx_ctx_create2d(const smx_mesh2d *mesh, const smx_sources2d *sources,
                      const smx_receivers2d *receivers, double dt, int nstep,
                      int precision, smx_ctx **out);
 
+/* Layout choices (results keep their meaning; a zero field takes the
+ * default).  Chosen per mesh by specfempp_amd.autotune. */
+typedef struct smx_options {
+  int tile_elems;  /* elements per tile, 1..64 (default 64) */
+  int graph_steps; /* time steps per replayed HIP graph, 1..256 (default 8) */
+} smx_options;
+
+/* smx_ctx_create2d with layout options (opts may be NULL). */
+int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
+                          const smx_receivers2d *receivers, double dt,
+                          int nstep, int precision, const smx_options *opts,
+                          smx_ctx **out);
+
+/* One context for nshots independent shots (meshes of any sizes, each with
+ * its own sources, receivers and medium; one dt, nstep and seismogram
+ * set-up).  Every operator and smx_run_forward then advance all shots with
+ * one launch per kernel per step; seismograms hold the shots' receivers in
+ * shot order.  Each shot's results equal those of its own context (same
+ * options) bit for bit.  No reference counterpart: the reference runs one
+ * simulation per process (core/specfem/solver/time_marching.tpp:8-131). */
+int smx_ctx_create2d_batch(int nshots, const smx_mesh2d *meshes,
+                           const smx_sources2d *sources,
+                           const smx_receivers2d *receivers, double dt,
+                           int nstep, int precision, const smx_options *opts,
+                           smx_ctx **out);
+
+/* Shot boundaries of a context: element, receiver and tile offsets (arrays
+ * of nshots+1, each may be NULL). */
+int smx_ctx_shots(const smx_ctx *ctx, int *nshots, int *elem_off, int *rec_off,
+                  int *tile_off);
+
 /* Frees every device buffer and the stream of the context. */
 int smx_ctx_destroy(smx_ctx *ctx);
 

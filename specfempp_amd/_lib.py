@@ -25,6 +25,7 @@ EXPORTS = [
     "smx_set_field", "smx_get_ibool", "smx_field_device_ptr", "smx_last_run_stats",
     "smx_time_stiffness", "smx_set_model", "smx_run_forward_checkpointed",
     "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
+    "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots",
 ]
 
 
@@ -46,6 +47,10 @@ class SmxReceivers2D(ctypes.Structure):
     _fields_ = [("nrec", ctypes.c_int), ("x", ctypes.c_void_p), ("z", ctypes.c_void_p),
                 ("ntypes", ctypes.c_int), ("types", ctypes.c_void_p),
                 ("nstep_between_samples", ctypes.c_int)]
+
+
+class SmxOptions(ctypes.Structure):
+    _fields_ = [("tile_elems", ctypes.c_int), ("graph_steps", ctypes.c_int)]
 
 
 class SmxError(RuntimeError):
@@ -98,6 +103,13 @@ def load():
         "smx_run_adjoint": (i, [P, P]),
         "smx_get_gradient": (i, [P, P]),
         "smx_last_adjoint_stats": (i, [P, DP, DP]),
+        "smx_ctx_create2d_opts": (i, [ctypes.POINTER(SmxMesh2D), ctypes.POINTER(SmxSources2D),
+                                      ctypes.POINTER(SmxReceivers2D), d, i, i,
+                                      ctypes.POINTER(SmxOptions), ctypes.POINTER(P)]),
+        "smx_ctx_create2d_batch": (i, [i, ctypes.POINTER(SmxMesh2D), ctypes.POINTER(SmxSources2D),
+                                       ctypes.POINTER(SmxReceivers2D), d, i, i,
+                                       ctypes.POINTER(SmxOptions), ctypes.POINTER(P)]),
+        "smx_ctx_shots": (i, [P, IP, P, P, P]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

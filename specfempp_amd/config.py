@@ -151,3 +151,23 @@ def smooth_perturbation(mesh: Mesh2D, rng, nmodes=6, kmax=4):
         a, b = rng.uniform(0, 2 * np.pi, size=2)
         f += rng.standard_normal() * np.sin(p * np.pi * X + a) * np.cos(q * np.pi * Z + b)
     return f / np.sqrt(np.mean(f * f))
+
+
+def batched_shots(nshots=64, seed=2025, nstep=3000, dt=4e-4, nrec=100, nmin=40, nmax=400,
+                  h=50.0):
+    """Independent shots on meshes of uneven size: shot k has nx ~ U[nmin, nmax]
+    quads, nz = round(0.75 nx), six seeded layers (as layered_box), one Ricker
+    source with f0 ~ U[5, 15] Hz at a seeded interior position and ``nrec``
+    surface receivers; free top, Stacey elsewhere."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(nshots):
+        nx = int(rng.integers(nmin, nmax + 1))
+        nz = int(round(0.75 * nx))
+        f0 = float(rng.uniform(5.0, 15.0))
+        cfg = layered_box(nx=nx, nz=nz, h=h, nstep=nstep, dt=dt, nsrc=1, nrec=nrec,
+                          seed=int(rng.integers(1 << 30)), f0=f0)
+        cfg.name = f"shot{k}_{nx}x{nz}"
+        cfg.meta["f0"] = f0
+        out.append(cfg)
+    return out

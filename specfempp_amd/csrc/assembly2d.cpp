@@ -275,10 +275,15 @@ Quadrature::Quadrature() {
 }
 
 void refresh_boundary_matrices(HostAssembly2D &A);
+static void build_shared_csr(HostAssembly2D &A);
+static void finish_assembly(HostAssembly2D &A);
 
 void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
                 const smx_receivers2d &rec, double dt_in, int nstep,
-                HostAssembly2D &A) {
+                HostAssembly2D &A, const AssemblyOptions &opt) {
+  const int tile_size = opt.tile_elems;
+  if (tile_size < 1 || tile_size > ELEM_TILE)
+    throw SmxError(SMX_ERR_ARG, "tile_elems must lie in [1, 64]");
   if (m.nspec <= 0 || (m.ngnod != 4 && m.ngnod != 9) || m.nnodes <= 0 ||
       !m.coord || !m.knods || !m.kappa || !m.mu || !m.rho)
     throw SmxError(SMX_ERR_ARG, "invalid mesh description");
@@ -382,7 +387,8 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
 
   // ---- tiles: elements sorted along a Morton (Z-order) curve of their
   // centroids on a grid of the mean element size, cut into groups of
-  // ELEM_TILE.  On a structured box this yields 8x8-element squares.
+  // tile_size (<= ELEM_TILE).  On a structured box 64 yields 8x8-element
+  // squares, 32 yields 8x4 and 16 yields 4x4.
   std::vector<std::vector<int>> tile_elems;
   {
     std::vector<float> ccx(nspec), ccz(nspec);
@@ -427,9 +433,9 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       key[e] = {spread(gx) | (spread(gz) << 1), e};
     }
     std::sort(key.begin(), key.end());
-    for (int i = 0; i < nspec; i += ELEM_TILE) {
+    for (int i = 0; i < nspec; i += tile_size) {
       std::vector<int> t;
-      for (int k = i; k < std::min(nspec, i + ELEM_TILE); ++k)
+      for (int k = i; k < std::min(nspec, i + tile_size); ++k)
         t.push_back(key[k].second);
       tile_elems.push_back(std::move(t));
     }
@@ -536,23 +542,7 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
     }
     A.tiles = std::move(ordered);
   }
-  // shared point j -> its partial slots in sh_list order (tile order), the
-  // fixed summation order of the shared pass
-  {
-    A.shp_off.assign(A.nshared + 1, 0);
-    for (int g : A.sh_list)
-      A.shp_off[g - A.nex_total + 1]++;
-    for (int j = 0; j < A.nshared; ++j)
-      A.shp_off[j + 1] += A.shp_off[j];
-    A.shp_pos.assign(A.sh_list.size(), 0);
-    std::vector<int> fill(A.shp_off.begin(), A.shp_off.end() - 1);
-    for (size_t k = 0; k < A.sh_list.size(); ++k)
-      A.shp_pos[fill[A.sh_list[k] - A.nex_total]++] = static_cast<int>(k);
-    int nmax = 0;
-    for (const auto &t : A.tiles)
-      nmax = std::max(nmax, t.nex + t.nsh);
-    A.nloc_max = nmax;
-  }
+  build_shared_csr(A);
   const std::vector<int> &slot_of = A.elem_slot;
 
   // ---- geometry (core/specfem/assembly/jacobian_matrix/dim2/jacobian_matrix.cpp:28-93)
@@ -889,6 +879,46 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
     }
   }
 
+  A.shot_elem_off = {0, A.nspec};
+  A.shot_src_off = {0, A.nsrc};
+  A.shot_rec_off = {0, A.nrec};
+  A.shot_tile_off = {0, A.ntiles};
+  A.shot_t0 = {A.t0};
+  finish_assembly(A);
+}
+
+
+// Shared point j -> its partial slots in sh_list order (tile order), the
+// fixed summation order of the shared pass; and the largest tile.
+static void build_shared_csr(HostAssembly2D &A) {
+  A.shp_off.assign(A.nshared + 1, 0);
+  for (int g : A.sh_list)
+    A.shp_off[g - A.nex_total + 1]++;
+  for (int j = 0; j < A.nshared; ++j)
+    A.shp_off[j + 1] += A.shp_off[j];
+  A.shp_pos.assign(A.sh_list.size(), 0);
+  std::vector<int> fill(A.shp_off.begin(), A.shp_off.end() - 1);
+  for (size_t k = 0; k < A.sh_list.size(); ++k)
+    A.shp_pos[fill[A.sh_list[k] - A.nex_total]++] = static_cast<int>(k);
+  int nmax = 0;
+  for (const auto &t : A.tiles)
+    nmax = std::max(nmax, t.nex + t.nsh);
+  A.nloc_max = nmax;
+}
+
+// Lists derived from the numbering, receivers and Stacey arrays: receiver
+// nodes for the adjoint source, absorbing points and nodes.
+static void finish_assembly(HostAssembly2D &A) {
+  const int ld = A.nspec_pad, sld = A.nstacey_pad;
+  A.rin_node.clear();
+  A.rin_off.clear();
+  A.rin_ent.clear();
+  A.rin_excl.clear();
+  A.bpt_slot.clear();
+  A.bpt_p.clear();
+  A.bpt_bslot.clear();
+  A.bpt_g.clear();
+  A.bnode_g.clear();
   // ---- receiver entries grouped by node (adjoint source injection in a
   // fixed order)
   {
@@ -1019,6 +1049,165 @@ void set_model2d(HostAssembly2D &A, const float *kappa, const float *mu,
     A.stacey[5 * sp + size_t(p) * sld + bs] = std::sqrt(rr * mm);
   }
   refresh_boundary_matrices(A);
+}
+
+void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
+  if (shots.empty())
+    throw SmxError(SMX_ERR_ARG, "no shots");
+  const HostAssembly2D &S0 = shots[0];
+  for (const auto &S : shots)
+    if (S.nstep != S0.nstep || S.dt != S0.dt || S.nsbs != S0.nsbs ||
+        S.seis_types != S0.seis_types)
+      throw SmxError(SMX_ERR_ARG,
+                     "shots must share dt, nstep and the seismogram set-up");
+  M = HostAssembly2D{};
+  M.quad = S0.quad;
+  M.nstep = S0.nstep;
+  M.dt = S0.dt;
+  M.dt_over2 = S0.dt_over2;
+  M.dt2_over2 = S0.dt2_over2;
+  M.nsbs = S0.nsbs;
+  M.nsig = S0.nsig;
+  M.seis_types = S0.seis_types;
+  M.t0 = S0.t0;
+  // totals and per-shot bases
+  const size_t ns = shots.size();
+  std::vector<int> tile_b(ns), ex_b(ns), sh_b(ns), shl_b(ns), el_b(ns),
+      src_b(ns), rec_b(ns), st_b(ns), sent_b(ns), rent_b(ns);
+  int ntiles = 0, nex = 0, nsh = 0, nshl = 0, nel = 0, nsrc = 0, nrec = 0,
+      nst = 0, nsent = 0, nrent = 0;
+  for (size_t k = 0; k < ns; ++k) {
+    const HostAssembly2D &S = shots[k];
+    tile_b[k] = ntiles, ex_b[k] = nex, sh_b[k] = nsh, shl_b[k] = nshl;
+    el_b[k] = nel, src_b[k] = nsrc, rec_b[k] = nrec, st_b[k] = nst;
+    sent_b[k] = nsent, rent_b[k] = nrent;
+    ntiles += S.ntiles, nex += S.nex_total, nsh += S.nshared;
+    nshl += int(S.sh_list.size()), nel += S.nspec, nsrc += S.nsrc;
+    nrec += S.nrec, nst += S.nstacey, nsent += int(S.src_li.size());
+    nrent += int(S.trec_li.size());
+  }
+  M.nspec = nel;
+  M.ntiles = ntiles;
+  M.nspec_pad = ntiles * ELEM_TILE;
+  M.nex_total = nex;
+  M.nshared = nsh;
+  M.nglob = nex + nsh;
+  M.nsrc = nsrc;
+  M.nrec = nrec;
+  M.nstacey = nst;
+  M.nstacey_pad = std::max(ELEM_TILE, (nst + ELEM_TILE - 1) / ELEM_TILE * ELEM_TILE);
+  const int ld = M.nspec_pad, sld = M.nstacey_pad;
+  const size_t pld = size_t(NPT) * ld, sp = size_t(NPT) * sld;
+  M.ecolor.assign(ld, 0xff);
+  M.elem_slot.assign(nel, 0);
+  M.ibool.assign(pld, 0);
+  M.lidx.assign(pld, 0);
+  M.geom.assign(5 * pld, 0.0f);
+  M.lam2mu_mu.assign(2 * pld, 0.0f);
+  M.kappa_mu_rho.assign(3 * pld, 0.0f);
+  M.stacey_slot.assign(ld, -1);
+  M.stacey.assign(6 * sp, 0.0f);
+  M.src_ispec.assign(std::max(1, nsrc), 0);
+  M.src_elem.assign(std::max(1, nsrc), 0);
+  M.src_array.assign(size_t(std::max(1, nsrc)) * 2 * NPT, 0.0f);
+  M.stf.assign(size_t(M.nstep) * std::max(1, nsrc) * 2, 0.0f);
+  M.rec_ispec.assign(std::max(1, nrec), 0);
+  M.rec_elem.assign(std::max(1, nrec), 0);
+  M.rec_xi.assign(std::max(1, nrec), 0.0f);
+  M.rec_gamma.assign(std::max(1, nrec), 0.0f);
+  M.rec_h.assign(size_t(std::max(1, nrec)) * NPT, 0.0f);
+  M.rent_g.assign(std::max(1, nrec * NPT), 0);
+  M.rent_shared.assign(std::max(1, nrec * NPT), -1);
+  M.rent_h.assign(std::max(1, nrec * NPT), 0.0f);
+  M.shot_elem_off.assign(1, 0);
+  M.shot_src_off.assign(1, 0);
+  M.shot_rec_off.assign(1, 0);
+  M.shot_tile_off.assign(1, 0);
+  for (size_t k = 0; k < ns; ++k) {
+    const HostAssembly2D &S = shots[k];
+    const int sld0 = S.nspec_pad, ssl = S.nstacey_pad;
+    const int slot_b = tile_b[k] * ELEM_TILE;
+    auto remap = [&](int g) {
+      return g < S.nex_total ? ex_b[k] + g : nex + sh_b[k] + (g - S.nex_total);
+    };
+    M.nloc_max = std::max(M.nloc_max, S.nloc_max);
+    M.nelem_colors = std::max(M.nelem_colors, S.nelem_colors);
+    for (TileInfo t : S.tiles) {
+      t.ex_off += ex_b[k];
+      t.sh_off += shl_b[k];
+      t.src_off += sent_b[k];
+      t.rec_off += rent_b[k];
+      M.tiles.push_back(t);
+    }
+    for (int g : S.sh_list)
+      M.sh_list.push_back(remap(g));
+    for (int e = 0; e < S.nspec; ++e)
+      M.elem_slot[el_b[k] + e] = slot_b + S.elem_slot[e];
+    for (int sl = 0; sl < sld0; ++sl) {
+      const int d = slot_b + sl;
+      M.ecolor[d] = S.ecolor[sl];
+      M.stacey_slot[d] = S.stacey_slot[sl] >= 0 ? st_b[k] + S.stacey_slot[sl] : -1;
+      for (int p = 0; p < NPT; ++p) {
+        const size_t so = size_t(p) * sld0 + sl, dO = size_t(p) * ld + d;
+        M.ibool[dO] = S.ecolor[sl] == 0xff ? 0 : remap(S.ibool[so]);
+        M.lidx[dO] = S.lidx[so];
+        for (int c = 0; c < 5; ++c)
+          M.geom[c * pld + dO] = S.geom[c * size_t(NPT) * sld0 + so];
+        for (int c = 0; c < 2; ++c)
+          M.lam2mu_mu[c * pld + dO] = S.lam2mu_mu[c * size_t(NPT) * sld0 + so];
+        for (int c = 0; c < 3; ++c)
+          M.kappa_mu_rho[c * pld + dO] = S.kappa_mu_rho[c * size_t(NPT) * sld0 + so];
+      }
+    }
+    for (int b = 0; b < S.nstacey; ++b)
+      for (int c = 0; c < 6; ++c)
+        for (int p = 0; p < NPT; ++p)
+          M.stacey[(c * size_t(NPT) + p) * sld + st_b[k] + b] =
+              S.stacey[(c * size_t(NPT) + p) * ssl + b];
+    for (int s = 0; s < S.nsrc; ++s) {
+      const int d = src_b[k] + s;
+      M.src_ispec[d] = slot_b + S.src_ispec[s];
+      M.src_elem[d] = el_b[k] + S.src_elem[s];
+      for (int i = 0; i < 2 * NPT; ++i)
+        M.src_array[size_t(d) * 2 * NPT + i] = S.src_array[size_t(s) * 2 * NPT + i];
+      for (int it = 0; it < M.nstep; ++it)
+        for (int c = 0; c < 2; ++c)
+          M.stf[(size_t(it) * nsrc + d) * 2 + c] =
+              S.stf[(size_t(it) * S.nsrc + s) * 2 + c];
+    }
+    for (size_t i = 0; i < S.src_li.size(); ++i) {
+      M.src_li.push_back(S.src_li[i]);
+      M.src_idx.push_back(src_b[k] + S.src_idx[i]);
+      M.src_L.push_back(S.src_L[2 * i]);
+      M.src_L.push_back(S.src_L[2 * i + 1]);
+    }
+    for (int r = 0; r < S.nrec; ++r) {
+      const int d = rec_b[k] + r;
+      M.rec_ispec[d] = slot_b + S.rec_ispec[r];
+      M.rec_elem[d] = el_b[k] + S.rec_elem[r];
+      M.rec_xi[d] = S.rec_xi[r];
+      M.rec_gamma[d] = S.rec_gamma[r];
+      for (int p = 0; p < NPT; ++p) {
+        const int id = d * NPT + p, sid = r * NPT + p;
+        M.rec_h[id] = S.rec_h[sid];
+        const int g = remap(S.rent_g[sid]);
+        M.rent_g[id] = g;
+        M.rent_shared[id] = g >= nex ? g - nex : -1;
+        M.rent_h[id] = S.rent_h[sid];
+      }
+    }
+    for (size_t i = 0; i < S.trec_li.size(); ++i) {
+      M.trec_li.push_back(S.trec_li[i]);
+      M.trec_entry.push_back(rec_b[k] * NPT + S.trec_entry[i]);
+    }
+    M.shot_elem_off.push_back(el_b[k] + S.nspec);
+    M.shot_src_off.push_back(src_b[k] + S.nsrc);
+    M.shot_rec_off.push_back(rec_b[k] + S.nrec);
+    M.shot_tile_off.push_back(tile_b[k] + S.ntiles);
+    M.shot_t0.push_back(S.t0);
+  }
+  build_shared_csr(M);
+  finish_assembly(M);
 }
 
 } // namespace smx

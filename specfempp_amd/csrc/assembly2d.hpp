@@ -136,12 +136,30 @@ struct HostAssembly2D {
 
   // Time scheme constants as the reference forms them (newmark.hpp:131-135).
   float dt = 0, dt_over2 = 0, dt2_over2 = 0;
+
+  // Batches of independent shots: shot s owns mesh elements
+  // [shot_elem_off[s], shot_elem_off[s+1]), sources, receivers and tiles
+  // likewise; shot_t0[s] is its time origin.  A single mesh is one shot.
+  std::vector<int32_t> shot_elem_off, shot_src_off, shot_rec_off, shot_tile_off;
+  std::vector<double> shot_t0;
+};
+
+// Assembly choices that change the layout, not the results' meaning.
+struct AssemblyOptions {
+  int tile_elems = ELEM_TILE; // elements per tile (1..64)
 };
 
 // Builds the assembly; throws SmxError on invalid input.
 void assemble2d(const smx_mesh2d &mesh, const smx_sources2d &sources,
                 const smx_receivers2d &receivers, double dt, int nstep,
-                HostAssembly2D &out);
+                HostAssembly2D &out, const AssemblyOptions &opt = {});
+
+// Concatenates independently assembled shots (same dt, nstep, seismogram
+// types and sampling) into one assembly whose tiles are the shots' tiles in
+// shot order: one launch then advances every shot.  Every shot keeps its own
+// numbering order, tiling and summation order, so its results are those of
+// its own context bit for bit.
+void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &out);
 
 // Replaces the material at every GLL point (arrays [nspec][5][5] in mesh
 // element order) and refreshes every quantity derived from it (lambda+2mu,

@@ -5,6 +5,8 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "solver2d.hpp"
 #include "specfempp_amd.h"
@@ -59,6 +61,95 @@ int smx_ctx_create2d(const smx_mesh2d *mesh, const smx_sources2d *sources,
     auto ctx = std::make_unique<smx_ctx>();
     ctx->solver = smx::make_solver2d(std::move(a), precision);
     *out = ctx.release();
+  });
+}
+
+namespace {
+smx::AssemblyOptions assembly_options(const smx_options *o) {
+  smx::AssemblyOptions a;
+  if (o && o->tile_elems)
+    a.tile_elems = o->tile_elems;
+  return a;
+}
+int graph_steps(const smx_options *o) {
+  return (o && o->graph_steps) ? o->graph_steps : 8;
+}
+} // namespace
+
+int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
+                          const smx_receivers2d *receivers, double dt,
+                          int nstep, int precision, const smx_options *opts,
+                          smx_ctx **out) {
+  return guarded([&] {
+    if (!mesh || !sources || !receivers || !out)
+      throw smx::SmxError(SMX_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto a = std::make_unique<smx::HostAssembly2D>();
+    smx::assemble2d(*mesh, *sources, *receivers, dt, nstep, *a,
+                    assembly_options(opts));
+    auto ctx = std::make_unique<smx_ctx>();
+    ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts));
+    *out = ctx.release();
+  });
+}
+
+int smx_ctx_create2d_batch(int nshots, const smx_mesh2d *meshes,
+                           const smx_sources2d *sources,
+                           const smx_receivers2d *receivers, double dt,
+                           int nstep, int precision, const smx_options *opts,
+                           smx_ctx **out) {
+  return guarded([&] {
+    if (nshots < 1 || !meshes || !sources || !receivers || !out)
+      throw smx::SmxError(SMX_ERR_ARG, "null argument or no shots");
+    *out = nullptr;
+    // shots are assembled independently on host threads, then merged
+    std::vector<smx::HostAssembly2D> shots(nshots);
+    std::vector<std::string> errors(nshots);
+    std::vector<int> codes(nshots, 0);
+    const smx::AssemblyOptions aopt = assembly_options(opts);
+    const int nthreads = std::max(
+        1, std::min<int>(nshots, std::min(16u, std::thread::hardware_concurrency())));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nthreads; ++t)
+      pool.emplace_back([&, t] {
+        for (int k = t; k < nshots; k += nthreads) {
+          try {
+            smx::assemble2d(meshes[k], sources[k], receivers[k], dt, nstep,
+                            shots[k], aopt);
+          } catch (const smx::SmxError &e) {
+            codes[k] = e.code;
+            errors[k] = e.what();
+          } catch (const std::exception &e) {
+            codes[k] = SMX_ERR_ARG;
+            errors[k] = e.what();
+          }
+        }
+      });
+    for (auto &th : pool)
+      th.join();
+    for (int k = 0; k < nshots; ++k)
+      if (codes[k])
+        throw smx::SmxError(codes[k], "shot " + std::to_string(k) + ": " + errors[k]);
+    auto a = std::make_unique<smx::HostAssembly2D>();
+    smx::merge_assemblies(shots, *a);
+    shots.clear();
+    auto ctx = std::make_unique<smx_ctx>();
+    ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts));
+    *out = ctx.release();
+  });
+}
+
+int smx_ctx_shots(const smx_ctx *ctx, int *nshots, int *elem_off, int *rec_off,
+                  int *tile_off) {
+  return guarded([&] {
+    const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
+    const int n = static_cast<int>(A.shot_elem_off.size()) - 1;
+    if (nshots) *nshots = n;
+    for (int k = 0; k <= n; ++k) {
+      if (elem_off) elem_off[k] = A.shot_elem_off[k];
+      if (rec_off) rec_off[k] = A.shot_rec_off[k];
+      if (tile_off) tile_off[k] = A.shot_tile_off[k];
+    }
   });
 }
 

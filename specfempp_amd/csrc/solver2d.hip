@@ -1069,7 +1069,9 @@ __global__ void k_adj_inject(real *__restrict__ UB,
 // ========================================================================
 template <typename real> class Solver2D final : public Solver2DBase {
 public:
-  Solver2D(std::unique_ptr<HostAssembly2D> a) : asm_(std::move(a)) {
+  Solver2D(std::unique_ptr<HostAssembly2D> a, int graph_steps)
+      : asm_(std::move(a)) {
+    gsteps_ = graph_steps;
     const HostAssembly2D &A = *asm_;
     SMX_HIP(hipGetDevice(&device_));
     SMX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -1238,13 +1240,13 @@ public:
     SMX_HIP(hipGetLastError());
     launch_predictor(F);
     const int body = nstep - 1; // steps followed by a fused predictor
-    const int reps = body / kGraphSteps;
+    const int reps = body / gsteps_;
     if (reps > 0) {
       build_graph(F);
       for (int r = 0; r < reps; ++r)
         SMX_HIP(hipGraphLaunch(graph_exec_, stream_));
     }
-    for (int s = reps * kGraphSteps; s < body; ++s)
+    for (int s = reps * gsteps_; s < body; ++s)
       fused_step<MODE_STEP>(F);
     fused_step<MODE_FINAL>(F);
     SMX_HIP(hipEventRecord(ev1_, stream_));
@@ -1371,13 +1373,13 @@ public:
                              hipMemcpyDeviceToDevice, stream_));
       const bool last = sgm == nseg - 1;
       const int body = last ? len - 1 : len;
-      const int reps = body / kGraphSteps;
+      const int reps = body / gsteps_;
       if (reps > 0) {
         build_graph(F);
         for (int r = 0; r < reps; ++r)
           SMX_HIP(hipGraphLaunch(graph_exec_, stream_));
       }
-      for (int st = reps * kGraphSteps; st < body; ++st)
+      for (int st = reps * gsteps_; st < body; ++st)
         fused_step<MODE_STEP>(F);
       if (last)
         fused_step<MODE_FINAL>(F);
@@ -1429,7 +1431,7 @@ public:
     for (int sgm = nseg - 1; sgm >= 0; --sgm) {
       const int b = sgm & 1;
       const int k0 = sgm * every, len = std::min(every, A.nstep - k0);
-      const int reps = len / kGraphSteps;
+      const int reps = len / gsteps_;
       const real *ck = ckpt_ + size_t(sgm) * 2 * n2_;
       // recompute the segment from its checkpoint into record buffer b,
       // once the adjoint of segment sgm+2 (its last reader) is done
@@ -1448,7 +1450,7 @@ public:
         for (int r = 0; r < reps; ++r)
           SMX_HIP(hipGraphLaunch(rec_graph_exec_[b], stream_));
       }
-      for (int st = reps * kGraphSteps; st < len; ++st)
+      for (int st = reps * gsteps_; st < len; ++st)
         record_step(W, rec_[b]);
       SMX_HIP(hipEventRecord(rec_done_[b], stream_));
       // adjoint steps k0+len-1 .. k0 on the adjoint stream
@@ -1461,7 +1463,7 @@ public:
         for (int r = 0; r < reps; ++r)
           SMX_HIP(hipGraphLaunch(adj_graph_exec_[b], adj_stream_));
       }
-      for (int st = reps * kGraphSteps; st < len; ++st)
+      for (int st = reps * gsteps_; st < len; ++st)
         adjoint_step(B, rec_[b], adj_stream_);
       SMX_HIP(hipEventRecord(adj_done_[b], adj_stream_));
     }
@@ -1539,7 +1541,7 @@ private:
   struct Field {
     real *u = nullptr, *v = nullptr, *a = nullptr;
   };
-  static constexpr int kGraphSteps = 8;
+  int gsteps_ = 8; // time steps per replayed graph
 
   template <typename T> T *alloc(size_t n) {
     void *p = nullptr;
@@ -1702,13 +1704,13 @@ private:
   }
   void build_rec_graph(Field &W, int b) {
     capture(rec_graph_[b], rec_graph_exec_[b], stream_, [&] {
-      for (int s = 0; s < kGraphSteps; ++s)
+      for (int s = 0; s < gsteps_; ++s)
         record_step(W, rec_[b]);
     });
   }
   void build_adj_graph(Field &B, int b) {
     capture(adj_graph_[b], adj_graph_exec_[b], adj_stream_, [&] {
-      for (int s = 0; s < kGraphSteps; ++s)
+      for (int s = 0; s < gsteps_; ++s)
         adjoint_step(B, rec_[b], adj_stream_);
     });
   }
@@ -1799,7 +1801,7 @@ private:
   }
   void build_graph(Field &F) {
     capture(graph_, graph_exec_, stream_, [&] {
-      for (int s = 0; s < kGraphSteps; ++s)
+      for (int s = 0; s < gsteps_; ++s)
         fused_step<MODE_STEP>(F);
     });
   }
@@ -1851,11 +1853,13 @@ private:
 };
 
 std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
-                                            int precision) {
+                                            int precision, int graph_steps) {
+  if (graph_steps < 1 || graph_steps > 256)
+    throw SmxError(SMX_ERR_ARG, "graph_steps must lie in [1, 256]");
   if (precision == 4)
-    return std::make_unique<Solver2D<float>>(std::move(a));
+    return std::make_unique<Solver2D<float>>(std::move(a), graph_steps);
   if (precision == 8)
-    return std::make_unique<Solver2D<double>>(std::move(a));
+    return std::make_unique<Solver2D<double>>(std::move(a), graph_steps);
   throw SmxError(SMX_ERR_ARG, "precision must be 4 or 8");
 }
 

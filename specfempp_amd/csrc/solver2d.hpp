@@ -49,7 +49,8 @@ public:
   virtual void last_adjoint_stats(double *forward_ms, double *adjoint_ms) = 0;
 };
 
+// graph_steps: time steps per replayed HIP graph.
 std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
-                                            int precision);
+                                            int precision, int graph_steps = 8);
 
 } // namespace smx

@@ -28,37 +28,69 @@ def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None
 
 
+def _structs(mesh, sources, receivers, seismo_types, nsbs, keep):
+    """ctypes descriptions of one mesh / source set / receiver set; the
+    arrays they point to are appended to ``keep``."""
+    mesh.validate()
+
+    def arr(x, dt_):
+        a = np.ascontiguousarray(np.asarray(x, dtype=dt_))
+        keep.append(a)
+        return a
+
+    m = _lib.SmxMesh2D(mesh.nspec, mesh.ngnod, mesh.nnodes,
+                       _p(arr(mesh.coord, np.float64)), _p(arr(mesh.knods, np.int32)),
+                       _p(arr(mesh.kappa, np.float32)), _p(arr(mesh.mu, np.float32)),
+                       _p(arr(mesh.rho, np.float32)), len(mesh.abs_ispec),
+                       _p(arr(mesh.abs_ispec, np.int32)), _p(arr(mesh.abs_type, np.int32)))
+    s = _lib.SmxSources2D(len(sources), _p(arr(sources.x, np.float64)),
+                          _p(arr(sources.z, np.float64)), _p(arr(sources.angle, np.float32)),
+                          _p(arr(sources.f0, np.float32)), _p(arr(sources.tshift, np.float32)),
+                          _p(arr(sources.factor, np.float32)))
+    types = arr([_lib.SEIS_TYPES[t] for t in seismo_types], np.int32)
+    r = _lib.SmxReceivers2D(len(receivers), _p(arr(receivers.x, np.float64)),
+                            _p(arr(receivers.z, np.float64)), len(types), _p(types), int(nsbs))
+    return m, s, r
+
+
+def _options(options):
+    if options is None:
+        return None
+    o = _lib.SmxOptions(int(options.get("tile_elems", 0)), int(options.get("graph_steps", 0)))
+    return ctypes.byref(o)
+
+
 class Context:
-    """An smx_ctx: the device-resident assembly, fields and seismograms."""
+    """An smx_ctx: the device-resident assembly, fields and seismograms.
+
+    ``options`` (dict, optional): ``tile_elems`` (elements per tile, 1..64)
+    and ``graph_steps`` (time steps per replayed graph), as chosen by
+    ``specfempp_amd.autotune``.  ``Context.batch`` builds one context over
+    many independent shots."""
 
     def __init__(self, mesh, sources, receivers, dt, nstep, *, nsbs=1,
-                 seismo_types=("displacement",), precision=4):
+                 seismo_types=("displacement",), precision=4, options=None, _batch=None):
         lib = _lib.load()
-        mesh.validate()
         keep = []
-
-        def arr(x, dt_):
-            a = np.ascontiguousarray(np.asarray(x, dtype=dt_))
-            keep.append(a)
-            return a
-
-        m = _lib.SmxMesh2D(mesh.nspec, mesh.ngnod, mesh.nnodes,
-                           _p(arr(mesh.coord, np.float64)), _p(arr(mesh.knods, np.int32)),
-                           _p(arr(mesh.kappa, np.float32)), _p(arr(mesh.mu, np.float32)),
-                           _p(arr(mesh.rho, np.float32)), len(mesh.abs_ispec),
-                           _p(arr(mesh.abs_ispec, np.int32)), _p(arr(mesh.abs_type, np.int32)))
-        s = _lib.SmxSources2D(len(sources), _p(arr(sources.x, np.float64)),
-                              _p(arr(sources.z, np.float64)), _p(arr(sources.angle, np.float32)),
-                              _p(arr(sources.f0, np.float32)), _p(arr(sources.tshift, np.float32)),
-                              _p(arr(sources.factor, np.float32)))
-        types = arr([_lib.SEIS_TYPES[t] for t in seismo_types], np.int32)
-        r = _lib.SmxReceivers2D(len(receivers), _p(arr(receivers.x, np.float64)),
-                                _p(arr(receivers.z, np.float64)), len(types), _p(types), int(nsbs))
         h = ctypes.c_void_p()
-        check(lib.smx_ctx_create2d(ctypes.byref(m), ctypes.byref(s), ctypes.byref(r),
-                                   float(dt), int(nstep), int(precision), ctypes.byref(h)))
+        if _batch is None:
+            m, s, r = _structs(mesh, sources, receivers, seismo_types, nsbs, keep)
+            check(lib.smx_ctx_create2d_opts(ctypes.byref(m), ctypes.byref(s), ctypes.byref(r),
+                                            float(dt), int(nstep), int(precision),
+                                            _options(options), ctypes.byref(h)))
+        else:
+            parts = [_structs(c.mesh, c.sources, c.receivers, seismo_types, nsbs, keep)
+                     for c in _batch]
+            n = len(parts)
+            M = (_lib.SmxMesh2D * n)(*[p[0] for p in parts])
+            S = (_lib.SmxSources2D * n)(*[p[1] for p in parts])
+            R = (_lib.SmxReceivers2D * n)(*[p[2] for p in parts])
+            check(lib.smx_ctx_create2d_batch(n, M, S, R, float(dt), int(nstep), int(precision),
+                                             _options(options), ctypes.byref(h)))
+        del keep
         self.lib, self.h = lib, h
         self.precision = precision
+        self.options = dict(options or {})
         self.dtype = np.float32 if precision == 4 else np.float64
         self.dt, self.nstep, self.nsbs = float(dt), int(nstep), int(nsbs)
         self.seismo_types = tuple(seismo_types)
@@ -72,6 +104,33 @@ class Context:
                                  ctypes.byref(npart)))
         self.layout = {"ntiles": nt.value, "nexclusive": nx.value, "nshared": nsh.value,
                        "npartials": npart.value}
+        nsh_ = ctypes.c_int()
+        check(lib.smx_ctx_shots(h, ctypes.byref(nsh_), None, None, None))
+        k = nsh_.value
+        eo, ro, to = (np.zeros(k + 1, np.int32) for _ in range(3))
+        check(lib.smx_ctx_shots(h, ctypes.byref(nsh_), _p(eo), _p(ro), _p(to)))
+        self.nshots = k
+        self.shot_elem_off, self.shot_rec_off, self.shot_tile_off = eo, ro, to
+
+    @classmethod
+    def batch(cls, configs, *, precision=4, options=None):
+        """One context advancing every Config2D of ``configs`` (same dt,
+        nstep, seismogram types and sampling) with one launch per kernel per
+        step."""
+        c0 = configs[0]
+        for c in configs:
+            if (c.dt, c.nstep, c.nsbs, tuple(c.seismo_types)) != \
+                    (c0.dt, c0.nstep, c0.nsbs, tuple(c0.seismo_types)):
+                raise ValueError("batched shots must share dt, nstep and the seismogram set-up")
+        return cls(None, None, None, c0.dt, c0.nstep, nsbs=c0.nsbs,
+                   seismo_types=c0.seismo_types, precision=precision, options=options,
+                   _batch=list(configs))
+
+    def shot_seismograms(self, seis=None):
+        """Per-shot views (nsig, ntypes, nrec_shot, 2) of the seismograms."""
+        seis = self.seismograms() if seis is None else seis
+        o = self.shot_rec_off
+        return [seis[:, :, o[k]:o[k + 1]] for k in range(self.nshots)]
 
     def close(self):
         if getattr(self, "h", None):
@@ -256,10 +315,21 @@ class TimeMarching:
         ctx.synchronize()
 
 
-def simulate(config, *, precision=4, fused=True, nstep=None):
+def simulate(config, *, precision=4, fused=True, nstep=None, options=None):
     """Runs a forward simulation of a Config2D; returns (seismograms, ctx)."""
     n = config.nstep if nstep is None else int(nstep)
     ctx = Context(config.mesh, config.sources, config.receivers, config.dt, n,
-                  nsbs=config.nsbs, seismo_types=config.seismo_types, precision=precision)
+                  nsbs=config.nsbs, seismo_types=config.seismo_types, precision=precision,
+                  options=options)
     TimeMarching(ctx).run(fused=fused)
     return ctx.seismograms(), ctx
+
+
+def simulate_batch(configs, *, precision=4, options=None):
+    """Runs every Config2D of ``configs`` in one batched context; returns
+    (list of per-shot seismograms (nsig, ntypes, nrec_shot, 2), ctx)."""
+    ctx = Context.batch(configs, precision=precision, options=options)
+    ctx.initialize()
+    ctx.run_forward(0, ctx.nstep)
+    ctx.synchronize()
+    return ctx.shot_seismograms(), ctx
