@@ -30,14 +30,20 @@ HBM_PEAK_GBS = 8000.0
 
 def tile_kernel_bytes(nspec, layout, nstacey):
     """Algorithmic bytes of one k_tile launch (one time step, FP32):
-    * per GLL point of an element: a 2-byte tile-local index, 5 geometry terms
-      and (lambda+2mu, mu): 30 B x 25 = 750 B per element;
+    * per GLL point of an element a 2-byte tile-local index (50 B/element);
+    * geometry: 5 terms per GLL point (500 B/element), or 5 per element
+      (20 B) in tiles of affine elements;
+    * moduli (lambda+2mu, mu): per GLL point (200 B/element), or per element
+      (8 B) in tiles of element-constant material;
     * per point exclusive to a tile: u, v, 1/M read, u, v written: 40 B;
     * per point shared between tiles: u read once (8 B); per partial slot
       (a tile's share of a shared point): 8 B written;
     * Stacey elements: 6 boundary terms and the velocity per point,
       32 B x 25 = 800 B per element."""
-    return (750 * nspec + 40 * layout["nexclusive"] + 8 * layout["nshared"]
+    ge = layout.get("geom_elem_elements", 0)
+    me = layout.get("mat_elem_elements", 0)
+    per_elem = 50 * nspec + 500 * (nspec - ge) + 20 * ge + 200 * (nspec - me) + 8 * me
+    return (per_elem + 40 * layout["nexclusive"] + 8 * layout["nshared"]
             + 8 * layout["npartials"] + 800 * nstacey)
 
 

@@ -24,7 +24,8 @@ SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
 
 def build(force: bool = False) -> str:
     """Compiles the oracle with make (g++, no GPU needed)."""
-    if force or not os.path.exists(_LIB_PATH):
+    if force or not os.path.exists(_LIB_PATH) or not os.path.exists(
+            os.path.join(_HERE, "build", "liboracle3d.so")):
         subprocess.run(["make", "-s", "-C", _HERE, "-B" if force else "all"], check=True)
     return _LIB_PATH
 
@@ -173,3 +174,103 @@ class Oracle2D:
             raise RuntimeError(self.lib.oracle2d_last_error().decode())
         self.istep += nstep
         return out[:got]
+
+
+# ---------------------------------------------------------------- 3D oracle
+_LIB3_PATH = os.path.join(_HERE, "build", "liboracle3d.so")
+_lib3 = None
+
+
+def _load3():
+    global _lib3
+    if _lib3 is not None:
+        return _lib3
+    if not os.path.exists(_LIB3_PATH):
+        build(force=True)
+    lib = ctypes.CDLL(_LIB3_PATH)
+    P = ctypes.c_void_p
+    i, f, d = ctypes.c_int, ctypes.c_float, ctypes.c_double
+    lib.oracle3d_create.restype = P
+    lib.oracle3d_create.argtypes = [i, i, i, P, P, P, P, P, i, P, P, f, i, i, P, P, P, P, P,
+                                    P, P, P, i, P, P, P]
+    lib.oracle3d_destroy.argtypes = [P]
+    lib.oracle3d_nglob.argtypes = [P]
+    lib.oracle3d_t0.argtypes = [P]
+    lib.oracle3d_t0.restype = d
+    lib.oracle3d_run.argtypes = [P, i, i, P, P]
+    lib.oracle3d_get.argtypes = [P, i, P]
+    lib.oracle3d_get.restype = ctypes.c_long
+    lib.oracle3d_apply_stiffness.argtypes = [P, P, P, i]
+    lib.oracle3d_last_error.restype = ctypes.c_char_p
+    _lib3 = lib
+    return lib
+
+
+GET3 = {"minv": 0, "ibool": 1, "jacobian": 2, "source_array": 3, "receiver_h": 4,
+        "u": 5, "v": 6, "a": 7}
+
+
+class Oracle3D:
+    """One 3D forward simulation on the CPU (precision 4 = FP32, 8 = FP64)."""
+
+    def __init__(self, mesh, sources, receivers, dt, nstep, precision=4):
+        lib = _load3()
+        self._keep = []
+
+        def arr(x, dt_):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dt_))
+            self._keep.append(a)
+            return a
+
+        self.nrec = len(receivers.x)
+        self.nstep = int(nstep)
+        self.h = lib.oracle3d_create(
+            precision, mesh.nspec, mesh.nnodes, _ptr(arr(mesh.coord, np.float64)),
+            _ptr(arr(mesh.knods, np.int32)), _ptr(arr(mesh.kappa, np.float32)),
+            _ptr(arr(mesh.mu, np.float32)), _ptr(arr(mesh.rho, np.float32)),
+            len(mesh.abs_ispec), _ptr(arr(mesh.abs_ispec, np.int32)),
+            _ptr(arr(mesh.abs_face, np.int32)), float(dt), int(nstep), len(sources.x),
+            _ptr(arr(sources.x, np.float64)), _ptr(arr(sources.y, np.float64)),
+            _ptr(arr(sources.z, np.float64)), _ptr(arr(sources.kind, np.int32)),
+            _ptr(arr(sources.comp, np.float32)), _ptr(arr(sources.f0, np.float32)),
+            _ptr(arr(sources.tshift, np.float32)), _ptr(arr(sources.factor, np.float32)),
+            self.nrec, _ptr(arr(receivers.x, np.float64)), _ptr(arr(receivers.y, np.float64)),
+            _ptr(arr(receivers.z, np.float64)))
+        if not self.h:
+            raise RuntimeError("oracle3d_create failed: " + lib.oracle3d_last_error().decode())
+        self.lib = lib
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.oracle3d_destroy(self.h)
+            self.h = None
+
+    @property
+    def nglob(self):
+        return self.lib.oracle3d_nglob(self.h)
+
+    @property
+    def t0(self):
+        return self.lib.oracle3d_t0(self.h)
+
+    def get(self, what):
+        n = self.lib.oracle3d_get(self.h, GET3[what], None)
+        out = np.empty(n, dtype=np.float64)
+        self.lib.oracle3d_get(self.h, GET3[what], _ptr(out))
+        return out
+
+    def apply_stiffness(self, u, v=None, with_stacey=True):
+        u = np.ascontiguousarray(np.asarray(u, dtype=np.float64).ravel())
+        v = np.zeros_like(u) if v is None else np.ascontiguousarray(
+            np.asarray(v, dtype=np.float64).ravel())
+        self.lib.oracle3d_apply_stiffness(self.h, _ptr(u), _ptr(v), int(with_stacey))
+        return self.get("a").reshape(-1, 3)
+
+    def run(self, nstep=None, types=("displacement",)):
+        """Seismograms (nstep, ntypes, nrec, 3) from rest."""
+        n = self.nstep if nstep is None else int(nstep)
+        t = np.array([SEIS_TYPES[s] for s in types], dtype=np.int32)
+        out = np.zeros((n, len(t), self.nrec, 3), dtype=np.float64)
+        if self.lib.oracle3d_run(self.h, n, len(t), _ptr(t), _ptr(out)) != 0:
+            raise RuntimeError(self.lib.oracle3d_last_error().decode())
+        return out

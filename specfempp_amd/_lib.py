@@ -25,7 +25,7 @@ EXPORTS = [
     "smx_set_field", "smx_get_ibool", "smx_field_device_ptr", "smx_last_run_stats",
     "smx_time_stiffness", "smx_set_model", "smx_run_forward_checkpointed",
     "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
-    "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots",
+    "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots", "smx_ctx_compact",
 ]
 
 
@@ -110,6 +110,7 @@ def load():
                                        ctypes.POINTER(SmxReceivers2D), d, i, i,
                                        ctypes.POINTER(SmxOptions), ctypes.POINTER(P)]),
         "smx_ctx_shots": (i, [P, IP, P, P, P]),
+        "smx_ctx_compact": (i, [P, IP, IP]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

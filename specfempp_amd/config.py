@@ -94,8 +94,8 @@ def layered_box(nx=800, nz=600, h=50.0, nstep=5000, dt=4e-4, nsrc=8, nrec=500,
     xmax, zmax = nx * h, nz * h
     per = max(nz // nlayers, 1)
 
-    def layer(jz):
-        return min((nz - 1 - jz) // per, nlayers - 1)  # layer 0 at the top
+    def layer(jz):  # layer 0 at the top; jz may be an index array
+        return np.minimum((nz - 1 - np.asarray(jz)) // per, nlayers - 1)
 
     mesh = box_mesh(0.0, xmax, 0.0, zmax, nx, nz, ngnod=4,
                     absorbing=("bottom", "right", "left"),

@@ -571,6 +571,34 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
                   pz[size_t(e) * NPT + p]);
     }
 
+  // ---- element-centre geometry of affine elements (the five terms agree at
+  // every GLL point up to FP32 rounding: relative 4e-7 of the element's
+  // largest term)
+  A.geom_e.assign(size_t(5) * ld, 0.0f);
+  A.elem_affine.assign(ld, 0);
+  {
+    float c0[9], c1[9];
+    shape_deriv(0.0f, 0.0f, ngnod, c0, c1);
+    for (int e = 0; e < nspec; ++e) {
+      const int sl = slot_of[e];
+      const Geo gc = jacobian_at(&cx[size_t(e) * ngnod], &cz[size_t(e) * ngnod],
+                                 ngnod, c0, c1);
+      const float cv[5] = {gc.xix, gc.xiz, gc.gammax, gc.gammaz, gc.jac};
+      float scale_d = std::max(std::max(std::fabs(cv[0]), std::fabs(cv[1])),
+                               std::max(std::fabs(cv[2]), std::fabs(cv[3])));
+      bool ok = true;
+      for (int p = 0; p < NPT && ok; ++p)
+        for (int c = 0; c < 5 && ok; ++c) {
+          const float v = A.geom[(c * size_t(NPT) + p) * ld + sl];
+          const float scale = c < 4 ? scale_d : std::fabs(cv[4]);
+          ok = std::fabs(v - cv[c]) <= 4e-7f * scale;
+        }
+      A.elem_affine[sl] = ok ? 1 : 0;
+      for (int c = 0; c < 5; ++c)
+        A.geom_e[c * size_t(ld) + sl] = cv[c];
+    }
+  }
+
   // ---- material (include/medium/dim2/elastic/isotropic/properties.hpp:71-91)
   A.lam2mu_mu.assign(size_t(2) * NPT * ld, 0.0f);
   A.kappa_mu_rho.assign(size_t(3) * NPT * ld, 0.0f);
@@ -879,6 +907,7 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
     }
   }
 
+  refresh_compact(A);
   A.shot_elem_off = {0, A.nspec};
   A.shot_src_off = {0, A.nsrc};
   A.shot_rec_off = {0, A.nrec};
@@ -1049,6 +1078,35 @@ void set_model2d(HostAssembly2D &A, const float *kappa, const float *mu,
     A.stacey[5 * sp + size_t(p) * sld + bs] = std::sqrt(rr * mm);
   }
   refresh_boundary_matrices(A);
+  refresh_compact(A);
+}
+
+void refresh_compact(HostAssembly2D &A) {
+  const int ld = A.nspec_pad;
+  const size_t pld = size_t(NPT) * ld;
+  A.mat_e.assign(size_t(2) * ld, 0.0f);
+  std::vector<uint8_t> mat_const(ld, 0);
+  for (int sl = 0; sl < ld; ++sl) {
+    if (A.ecolor[sl] == 0xff)
+      continue;
+    bool same = true;
+    for (int c = 0; c < 2 && same; ++c)
+      for (int p = 1; p < NPT && same; ++p)
+        same = A.lam2mu_mu[c * pld + size_t(p) * ld + sl] ==
+               A.lam2mu_mu[c * pld + sl];
+    mat_const[sl] = same;
+    for (int c = 0; c < 2; ++c)
+      A.mat_e[c * size_t(ld) + sl] = A.lam2mu_mu[c * pld + sl];
+  }
+  for (int t = 0; t < A.ntiles; ++t) {
+    bool g = true, m = true;
+    for (int j = 0; j < A.tiles[t].ecount; ++j) {
+      const int sl = t * ELEM_TILE + j;
+      g = g && A.elem_affine[sl];
+      m = m && mat_const[sl];
+    }
+    A.tiles[t].flags = (g ? TILE_GEOM_ELEM : 0) | (m ? TILE_MAT_ELEM : 0);
+  }
 }
 
 void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
@@ -1105,6 +1163,8 @@ void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
   M.geom.assign(5 * pld, 0.0f);
   M.lam2mu_mu.assign(2 * pld, 0.0f);
   M.kappa_mu_rho.assign(3 * pld, 0.0f);
+  M.geom_e.assign(size_t(5) * ld, 0.0f);
+  M.elem_affine.assign(ld, 0);
   M.stacey_slot.assign(ld, -1);
   M.stacey.assign(6 * sp, 0.0f);
   M.src_ispec.assign(std::max(1, nsrc), 0);
@@ -1146,6 +1206,9 @@ void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
     for (int sl = 0; sl < sld0; ++sl) {
       const int d = slot_b + sl;
       M.ecolor[d] = S.ecolor[sl];
+      M.elem_affine[d] = S.elem_affine[sl];
+      for (int c = 0; c < 5; ++c)
+        M.geom_e[c * size_t(ld) + d] = S.geom_e[c * size_t(sld0) + sl];
       M.stacey_slot[d] = S.stacey_slot[sl] >= 0 ? st_b[k] + S.stacey_slot[sl] : -1;
       for (int p = 0; p < NPT; ++p) {
         const size_t so = size_t(p) * sld0 + sl, dO = size_t(p) * ld + d;
@@ -1208,6 +1271,7 @@ void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
   }
   build_shared_csr(M);
   finish_assembly(M);
+  refresh_compact(M);
 }
 
 } // namespace smx

@@ -54,7 +54,15 @@ struct TileInfo {
   int32_t nsrc;
   int32_t rec_off; // receiver entries on exclusive points
   int32_t nrec;
+  int32_t flags;   // TILE_GEOM_ELEM | TILE_MAT_ELEM
 };
+
+// Tile flags: every element of the tile has element-constant geometry
+// (affine element: the five Jacobian terms agree at all 25 points to within
+// FP32 rounding) / element-constant lambda+2mu and mu; the kernels then read
+// the per-element arrays geom_e / mat_e instead of the per-point ones.
+constexpr int32_t TILE_GEOM_ELEM = 1;
+constexpr int32_t TILE_MAT_ELEM = 2;
 
 struct HostAssembly2D {
   int nspec = 0, nspec_pad = 0, nglob = 0;
@@ -84,6 +92,11 @@ struct HostAssembly2D {
   std::vector<float> lam2mu_mu;
   // [3][NPT][nspec_pad]: kappa, mu, rho (model parameters)
   std::vector<float> kappa_mu_rho;
+  // Per-element copies for tiles flagged element-constant: geometry
+  // [5][nspec_pad] (evaluated at the element centre) and (lambda+2mu, mu)
+  // [2][nspec_pad]; elem_affine marks slots whose geometry qualifies.
+  std::vector<float> geom_e, mat_e;
+  std::vector<uint8_t> elem_affine;
 
   // Stacey: slot -> boundary slot (-1 if none); per slot [6][NPT][nst_pad]
   // (n0, n1, |n|, edge weight (0 if the point is not on an absorbing edge),
@@ -160,6 +173,10 @@ void assemble2d(const smx_mesh2d &mesh, const smx_sources2d &sources,
 // numbering order, tiling and summation order, so its results are those of
 // its own context bit for bit.
 void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &out);
+
+// Recomputes the per-element material copy and the tile flags from the
+// per-point arrays (after assembly or a model change).
+void refresh_compact(HostAssembly2D &A);
 
 // Replaces the material at every GLL point (arrays [nspec][5][5] in mesh
 // element order) and refreshes every quantity derived from it (lambda+2mu,

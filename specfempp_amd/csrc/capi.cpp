@@ -179,6 +179,19 @@ int smx_ctx_layout(const smx_ctx *ctx, int *ntiles, int *nexclusive,
   });
 }
 
+int smx_ctx_compact(const smx_ctx *ctx, int *geom_elements, int *mat_elements) {
+  return guarded([&] {
+    const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
+    int g = 0, m = 0;
+    for (const auto &t : A.tiles) {
+      if (t.flags & smx::TILE_GEOM_ELEM) g += t.ecount;
+      if (t.flags & smx::TILE_MAT_ELEM) m += t.ecount;
+    }
+    if (geom_elements) *geom_elements = g;
+    if (mat_elements) *mat_elements = m;
+  });
+}
+
 double smx_ctx_t0(const smx_ctx *ctx) {
   if (!ctx || !ctx->solver)
     return 0.0;

@@ -159,6 +159,7 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
            const int *__restrict__ sh_list, const uint16_t *__restrict__ lidx,
            const uint8_t *__restrict__ ecolor, int ncolors,
            const real *__restrict__ geom, const real *__restrict__ mat,
+           const real *__restrict__ geom_e, const real *__restrict__ mat_e,
            const int *__restrict__ sslot, const real *__restrict__ stacey,
            real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
            const real *__restrict__ Minv, real *__restrict__ part,
@@ -268,13 +269,20 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
 
   // ---- stress (stress.hpp:38-78) and F = J sigma dxi/dx (point/stress.hpp:166-183)
   real f1x[NGLL], f1z[NGLL];
+  // element-constant geometry / moduli of flagged tiles come from the
+  // per-element arrays (uniform choice of base and stride)
+  const bool g_el = (T.flags & TILE_GEOM_ELEM) != 0;
+  const bool m_el = (T.flags & TILE_MAT_ELEM) != 0;
+  const size_t gst = g_el ? size_t(ld) : pld, mst = m_el ? size_t(ld) : pld;
 #pragma unroll
   for (int iz = 0; iz < NGLL; ++iz) {
     const int p = iz * NGLL + ix;
     const size_t o = size_t(p) * ld + slot;
-    const real xix = geom[o], xiz = geom[pld + o], gax = geom[2 * pld + o],
-               gaz = geom[3 * pld + o], jac = geom[4 * pld + o];
-    const real l2m = mat[o], mu = mat[pld + o];
+    const real *gp = g_el ? geom_e + slot : geom + o;
+    const real *mp = m_el ? mat_e + slot : mat + o;
+    const real xix = gp[0], xiz = gp[gst], gax = gp[2 * gst], gaz = gp[3 * gst],
+               jac = gp[4 * gst];
+    const real l2m = mp[0], mu = mp[mst];
     const real lam = l2m - real(2.0) * mu;
     const real du_xx = xix * dxi_x[iz] + gax * dga_x[iz];
     const real du_xz = xiz * dxi_x[iz] + gaz * dga_x[iz];
@@ -757,6 +765,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
                const uint16_t *__restrict__ lidx,
                const uint8_t *__restrict__ ecolor, int ncolors,
                const real *__restrict__ geom, const real *__restrict__ mat,
+               const real *__restrict__ geom_e, const real *__restrict__ mat_e,
                const int *__restrict__ sslot, const real *__restrict__ stacey,
                real *__restrict__ RB0, real *__restrict__ RB1,
                real *__restrict__ UB, real *__restrict__ VB,
@@ -876,13 +885,20 @@ __global__ void __launch_bounds__(TILE_THREADS)
 
   // ---- gradient densities, stress of rb and the xi integrand
   real f1x[NGLL], f1z[NGLL];
+  // element-constant geometry / moduli of flagged tiles come from the
+  // per-element arrays (uniform choice of base and stride)
+  const bool g_el = (T.flags & TILE_GEOM_ELEM) != 0;
+  const bool m_el = (T.flags & TILE_MAT_ELEM) != 0;
+  const size_t gst = g_el ? size_t(ld) : pld, mst = m_el ? size_t(ld) : pld;
 #pragma unroll
   for (int iz = 0; iz < NGLL; ++iz) {
     const int p = iz * NGLL + ix;
     const size_t o = size_t(p) * ld + slot;
-    const real xix = geom[o], xiz = geom[pld + o], gax = geom[2 * pld + o],
-               gaz = geom[3 * pld + o], jac = geom[4 * pld + o];
-    const real l2m = mat[o], mu = mat[pld + o];
+    const real *gp = g_el ? geom_e + slot : geom + o;
+    const real *mp = m_el ? mat_e + slot : mat + o;
+    const real xix = gp[0], xiz = gp[gst], gax = gp[2 * gst], gaz = gp[3 * gst],
+               jac = gp[4 * gst];
+    const real l2m = mp[0], mu = mp[mst];
     const real lam = l2m - real(2.0) * mu;
     const real du_xx = xix * dxi_x[iz] + gax * dga_x[iz];
     const real du_xz = xiz * dxi_x[iz] + gaz * dga_x[iz];
@@ -1088,6 +1104,8 @@ public:
     ibool_ = upload_raw<int>(A.ibool);
     geom_ = upload_real(A.geom);
     mat_ = upload_real(A.lam2mu_mu);
+    geom_e_ = upload_real(A.geom_e);
+    mat_e_ = alloc<real>(A.mat_e.size());
     kmr_ = upload_real(A.kappa_mu_rho);
     sslot_ = upload_raw<int>(A.stacey_slot);
     stacey_ = upload_real(A.stacey);
@@ -1618,7 +1636,8 @@ private:
     k_tile<real, MODE><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_,
                          stream_>>>(
         tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
-        mat_, sslot_, stacey_, U, V, Aout, minv_, part ? part : part_, src_li_,
+        mat_, geom_e_, mat_e_, sslot_, stacey_, U, V, Aout, minv_,
+        part ? part : part_, src_li_,
         src_idx_, src_L_, stf_, std::max(1, A.nsrc), trec_li_, trec_entry_,
         rent_h_, terms, A.nrec * NPT, stypes_, ntypes, A.nsbs, step_ptr, istep,
         A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, rec,
@@ -1652,7 +1671,15 @@ private:
       std::copy(A.lam2mu64.begin(), A.lam2mu64.end(), m.begin());
       std::copy(A.imp64.begin(), A.imp64.end(), st.begin() + 4 * sp);
     }
+    std::vector<double> me(A.mat_e.begin(), A.mat_e.end());
+    if (sizeof(real) == 8)
+      std::copy(A.lam2mu64.begin(), A.lam2mu64.begin() + A.nspec_pad,
+                me.begin()); // point 0 of each slot
     copy_to(mat_, m);
+    copy_to(mat_e_, me);
+    SMX_HIP(hipMemcpyAsync(tiles_, A.tiles.data(),
+                           A.tiles.size() * sizeof(TileInfo),
+                           hipMemcpyHostToDevice, stream_));
     copy_to(stacey_, st);
     copy_to(kmr_, A.kappa_mu_rho);
     if (bsum_)
@@ -1671,7 +1698,8 @@ private:
     const HostAssembly2D &A = *asm_;
     k_adj_tile<real><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_, st>>>(
         tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
-        mat_, sslot_, stacey_, rb0_, rb1_, B.u, B.v, minv_, part_adj_, rec,
+        mat_, geom_e_, mat_e_, sslot_, stacey_, rb0_, rb1_, B.u, B.v, minv_,
+        part_adj_, rec,
         rec_stride(), rec_every_, bidx_, bsum_, gd_, ge_, racc_, sp_, ss_,
         step_ + 2, A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_);
     SMX_HIP(hipGetLastError());
@@ -1822,6 +1850,7 @@ private:
       *step_ = nullptr;
   uint16_t *lidx_ = nullptr;
   uint8_t *ecolor_ = nullptr;
+  real *geom_e_ = nullptr, *mat_e_ = nullptr;
   real *geom_ = nullptr, *mat_ = nullptr, *kmr_ = nullptr, *stacey_ = nullptr,
        *src_L_ = nullptr, *stf_ = nullptr, *rent_h_ = nullptr,
        *minv_ = nullptr, *part_ = nullptr, *seis_ = nullptr,

@@ -126,11 +126,15 @@ def box_mesh(xmin, xmax, zmin, zmax, nx, nz, *, ngnod=4, absorbing=("bottom", "r
 
     def per_elem(val):
         if callable(val):
-            out = np.empty((nz, nx), dtype=np.float64)
-            for jz in range(nz):
-                for jx in range(nx):
-                    out[jz, jx] = val(jx, jz)
-            return out.ravel()
+            JZ, JX = np.meshgrid(np.arange(nz), np.arange(nx), indexing="ij")
+            try:  # vectorised callables take index arrays
+                out = np.broadcast_to(np.asarray(val(JX, JZ), dtype=np.float64), (nz, nx))
+            except (TypeError, ValueError):
+                out = np.empty((nz, nx), dtype=np.float64)
+                for jz in range(nz):
+                    for jx in range(nx):
+                        out[jz, jx] = val(jx, jz)
+            return np.ascontiguousarray(out).ravel()
         return np.full(nx * nz, float(val))
 
     r, p, s = per_elem(rho), per_elem(vp), per_elem(vs)
@@ -143,19 +147,23 @@ def box_mesh(xmin, xmax, zmin, zmax, nx, nz, *, ngnod=4, absorbing=("bottom", "r
     # left (so at a corner GLL point the later edge's normal is kept, as in the
     # reference's last-edge-wins assembly).
     want = {EDGE_TYPES[a] for a in absorbing}
-    ai, at = [], []
-    for e in range(nx * nz):
-        jz, jx = divmod(e, nx)
-        for t in (BOTTOM, RIGHT, TOP, LEFT):
-            if t not in want:
-                continue
-            if (t == BOTTOM and jz == 0) or (t == TOP and jz == nz - 1) or \
-               (t == LEFT and jx == 0) or (t == RIGHT and jx == nx - 1):
-                ai.append(e)
-                at.append(t)
+    ids = np.arange(nx * nz)
+    jz_e, jx_e = ids // nx, ids % nx
+    on = {BOTTOM: jz_e == 0, RIGHT: jx_e == nx - 1, TOP: jz_e == nz - 1, LEFT: jx_e == 0}
+    order = {BOTTOM: 0, RIGHT: 1, TOP: 2, LEFT: 3}
+    parts = [(ids[on[t]], np.full(int(on[t].sum()), t)) for t in (BOTTOM, RIGHT, TOP, LEFT)
+             if t in want]
+    if parts:
+        ai = np.concatenate([p[0] for p in parts])
+        at = np.concatenate([p[1] for p in parts])
+        rank = np.array([order[t] for t in at.tolist()]) if len(at) else np.zeros(0, int)
+        key = np.lexsort((rank, ai))  # element order, then bottom/right/top/left
+        ai, at = ai[key], at[key]
+    else:
+        ai, at = np.zeros(0, np.int64), np.zeros(0, np.int64)
     return Mesh2D(coord=coord.astype(np.float64), knods=knods, kappa=kappa, mu=mu,
-                  rho=rho_pt, abs_ispec=np.array(ai, dtype=np.int32),
-                  abs_type=np.array(at, dtype=np.int32), nx=nx, nz=nz,
+                  rho=rho_pt, abs_ispec=ai.astype(np.int32),
+                  abs_type=at.astype(np.int32), nx=nx, nz=nz,
                   extent=(xmin, xmax, zmin, zmax)).validate()
 
 

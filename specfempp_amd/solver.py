@@ -102,8 +102,11 @@ class Context:
         nt, nx, nsh, npart = (ctypes.c_int() for _ in range(4))
         check(lib.smx_ctx_layout(h, ctypes.byref(nt), ctypes.byref(nx), ctypes.byref(nsh),
                                  ctypes.byref(npart)))
+        gel, mel = ctypes.c_int(), ctypes.c_int()
+        check(lib.smx_ctx_compact(h, ctypes.byref(gel), ctypes.byref(mel)))
         self.layout = {"ntiles": nt.value, "nexclusive": nx.value, "nshared": nsh.value,
-                       "npartials": npart.value}
+                       "npartials": npart.value, "geom_elem_elements": gel.value,
+                       "mat_elem_elements": mel.value}
         nsh_ = ctypes.c_int()
         check(lib.smx_ctx_shots(h, ctypes.byref(nsh_), None, None, None))
         k = nsh_.value
