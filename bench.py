@@ -224,6 +224,45 @@ def batch_leg(nshots=64, nstep=3000, baseline_steps=300):
             "host_s": {"generate": gen_s, "autotune": tune_s, "assemble": setup_s}}
 
 
+def hex_leg(n=64, nstep=4000):
+    """3D elastic forward: 64^3 hex8 elements of 100 m (6.4 km cube, 262,144
+    elements), vp linear 3000 -> 6000 m/s with depth, vs = vp/sqrt(3), rho by
+    Gardner, +-2 % seeded Gaussian random perturbation (correlation 500 m);
+    free top, Stacey on five faces; explosion moment tensor (1e15), Ricker
+    f0 = 2 Hz at 3 km depth; 256 surface receivers; dt = 1.25e-3 s (2e-3 s is
+    beyond the stability limit), 4000 steps, FP32."""
+    import numpy as np
+    from specfempp_amd.mesh3d import hex_box_case
+    from specfempp_amd.solver3d import Context3D
+
+    t = time.perf_counter()
+    cfg = hex_box_case(n=n, nstep=nstep)
+    gen_s = time.perf_counter() - t
+    t = time.perf_counter()
+    ctx = Context3D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, nstep)
+    ctx.initialize()
+    setup_s = time.perf_counter() - t
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.run_forward(0, nstep)
+    ctx.synchronize()
+    wall = time.perf_counter() - t
+    seis = ctx.seismograms()
+    stiff_ms = ctx.time_stiffness(10)
+    nglob, nspec = ctx.nglob, ctx.nspec
+    ctx.close()
+    canon = 76 * nglob + 6500 * nspec
+    return {"workload": f"3D elastic, {n}^3 hex8 of 100 m, vp 3000->6000 m/s +-2 % random, "
+                        f"free top + Stacey on 5 faces, explosion source, 256 receivers, "
+                        f"dt={cfg.dt}, {nstep} steps",
+            "nspec": nspec, "nglob": nglob, "wall_s": wall, "ms_per_step": wall * 1e3 / nstep,
+            "point_updates_per_s": nglob * nstep / wall,
+            "canonical_bytes_per_step": canon,
+            "canonical_frac": canon * nstep / wall / 1e9 / HBM_PEAK_GBS,
+            "element_passes_ms": stiff_ms, "seismograms_finite": bool(np.isfinite(seis).all()),
+            "host_s": {"generate": gen_s, "assemble": setup_s}}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,6 +275,8 @@ def main():
                     help="skip the adjoint-gradient leg (80x60 box, 2000 steps)")
     ap.add_argument("--no-batch", action="store_true",
                     help="skip the batched-shots leg (64 shots, 3000 steps)")
+    ap.add_argument("--no-3d", action="store_true",
+                    help="skip the 3D leg (64^3 hex, 4000 steps)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per k_tile launch measured with rocprofv3 --pmc "
@@ -329,6 +370,8 @@ def main():
         out["adjoint"] = adjoint_leg()
     if not args.no_batch:
         out["batch"] = batch_leg()
+    if not args.no_3d:
+        out["hex3d"] = hex_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     print(json.dumps(out), flush=True)

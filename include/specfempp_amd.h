@@ -82,6 +82,51 @@ typedef struct {
   int nstep_between_samples;
 } smx_receivers2d;
 
+/* 3D hex8 mesh (NGLL = 5).  Control nodes [nnodes][3] (x, y, z); element
+ * connectivity [nspec][8] in the reference's hex8 order (z- face
+ * counter-clockwise from (-,-,-), then the z+ face,
+ * core/specfem/shape_function/dim3/shape_functions3d.cpp); material per GLL
+ * point [nspec][5][5][5] (iz, iy, ix); absorbing faces (element, face with
+ * bottom=1 (z-), right=2 (x+), top=3 (z+), left=4 (x-), front=5 (y-),
+ * back=6 (y+), include/enumerations/dim3/mesh_entities.hpp). */
+typedef struct smx_mesh3d {
+  int nspec;
+  int nnodes;
+  const double *coord;
+  const int *knods;
+  const float *kappa;
+  const float *mu;
+  const float *rho;
+  int nabs;
+  const int *abs_ispec;
+  const int *abs_face;
+} smx_mesh3d;
+
+/* 3D point sources with a Ricker time function: kind 0 = force (comp[0..2]
+ * = fx, fy, fz), kind 1 = moment tensor (comp = Mxx, Myy, Mzz, Mxy, Mxz,
+ * Myz); comp is [nsrc][6]. */
+typedef struct smx_sources3d {
+  int nsrc;
+  const double *x;
+  const double *y;
+  const double *z;
+  const int *kind;
+  const float *comp;
+  const float *f0;
+  const float *tshift;
+  const float *factor;
+} smx_sources3d;
+
+typedef struct smx_receivers3d {
+  int nrec;
+  const double *x;
+  const double *y;
+  const double *z;
+  int ntypes;
+  const int *types; /* [ntypes] SMX_SEIS_* */
+  int nstep_between_samples;
+} smx_receivers3d;
+
 /* Last error message of the calling thread (never NULL). */
 const char *smx_last_error(void);
 
@@ -132,6 +177,18 @@ int smx_ctx_create2d_batch(int nshots, const smx_mesh2d *meshes,
 int smx_ctx_shots(const smx_ctx *ctx, int *nshots, int *elem_off, int *rec_off,
                   int *tile_off);
 
+/* A 3D elastic context (hex8, NGLL = 5).  Supported operations:
+ * smx_initialize, smx_run_forward, smx_synchronize, smx_get_seismograms
+ * ([nsig][ntypes][nrec][3]), smx_get_field / smx_set_field ([nglob][3]),
+ * smx_ctx_info, smx_last_run_stats, smx_time_stiffness, smx_ctx_destroy;
+ * others return SMX_ERR_STATE.  Replaces assembly<dim3> + the dim3
+ * instantiations of domain_kernels (src/kokkos_kernels/impl/
+ * compute_stiffness_interaction.cpp:53-63) and time_marching<forward, dim3>.
+ * Absorbing faces are this build's own 3D Stacey condition. */
+int smx_ctx_create3d(const smx_mesh3d *mesh, const smx_sources3d *sources,
+                     const smx_receivers3d *receivers, double dt, int nstep,
+                     int precision, smx_ctx **out);
+
 /* Frees every device buffer and the stream of the context. */
 int smx_ctx_destroy(smx_ctx *ctx);
 
@@ -148,6 +205,10 @@ int smx_ctx_layout(const smx_ctx *ctx, int *ntiles, int *nexclusive,
 
 /* Start time t0 of the simulation (src/io/sources/dim2/read_sources.cpp:142-172). */
 double smx_ctx_t0(const smx_ctx *ctx);
+
+/* Elements whose tiles read element-constant geometry / moduli (one value
+ * per element instead of one per GLL point; see the assembly's tile flags). */
+int smx_ctx_compact(const smx_ctx *ctx, int *geom_elements, int *mat_elements);
 
 /* Mass matrix + Stacey mass term, then its inverse.
  * Replaces domain_kernels::initialize(dt)

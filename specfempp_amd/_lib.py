@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(HERE, "libspecfempp_amd.so")
 
 SMX_FIELD_FORWARD, SMX_FIELD_ADJOINT, SMX_FIELD_BACKWARD = 0, 1, 2
 SMX_U, SMX_V, SMX_A, SMX_MASS_INVERSE = 0, 1, 2, 3
+SMX_OK, SMX_ERR_ARG, SMX_ERR_HIP, SMX_ERR_STATE, SMX_ERR_MESH = 0, -1, -2, -3, -4
 SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
 
 # Every symbol include/specfempp_amd.h declares.
@@ -26,6 +27,7 @@ EXPORTS = [
     "smx_time_stiffness", "smx_set_model", "smx_run_forward_checkpointed",
     "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
     "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots", "smx_ctx_compact",
+    "smx_ctx_create3d",
 ]
 
 
@@ -46,6 +48,25 @@ class SmxSources2D(ctypes.Structure):
 class SmxReceivers2D(ctypes.Structure):
     _fields_ = [("nrec", ctypes.c_int), ("x", ctypes.c_void_p), ("z", ctypes.c_void_p),
                 ("ntypes", ctypes.c_int), ("types", ctypes.c_void_p),
+                ("nstep_between_samples", ctypes.c_int)]
+
+
+class SmxMesh3D(ctypes.Structure):
+    _fields_ = [("nspec", ctypes.c_int), ("nnodes", ctypes.c_int), ("coord", ctypes.c_void_p),
+                ("knods", ctypes.c_void_p), ("kappa", ctypes.c_void_p), ("mu", ctypes.c_void_p),
+                ("rho", ctypes.c_void_p), ("nabs", ctypes.c_int), ("abs_ispec", ctypes.c_void_p),
+                ("abs_face", ctypes.c_void_p)]
+
+
+class SmxSources3D(ctypes.Structure):
+    _fields_ = [("nsrc", ctypes.c_int), ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("z", ctypes.c_void_p), ("kind", ctypes.c_void_p), ("comp", ctypes.c_void_p),
+                ("f0", ctypes.c_void_p), ("tshift", ctypes.c_void_p), ("factor", ctypes.c_void_p)]
+
+
+class SmxReceivers3D(ctypes.Structure):
+    _fields_ = [("nrec", ctypes.c_int), ("x", ctypes.c_void_p), ("y", ctypes.c_void_p),
+                ("z", ctypes.c_void_p), ("ntypes", ctypes.c_int), ("types", ctypes.c_void_p),
                 ("nstep_between_samples", ctypes.c_int)]
 
 
@@ -111,6 +132,8 @@ def load():
                                        ctypes.POINTER(SmxOptions), ctypes.POINTER(P)]),
         "smx_ctx_shots": (i, [P, IP, P, P, P]),
         "smx_ctx_compact": (i, [P, IP, IP]),
+        "smx_ctx_create3d": (i, [ctypes.POINTER(SmxMesh3D), ctypes.POINTER(SmxSources3D),
+                                 ctypes.POINTER(SmxReceivers3D), d, i, i, ctypes.POINTER(P)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

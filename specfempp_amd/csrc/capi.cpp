@@ -9,10 +9,12 @@
 #include <vector>
 
 #include "solver2d.hpp"
+#include "solver3d.hpp"
 #include "specfempp_amd.h"
 
 struct smx_ctx {
-  std::unique_ptr<smx::Solver2DBase> solver;
+  std::unique_ptr<smx::Solver2DBase> solver;   // 2D contexts
+  std::unique_ptr<smx::Solver3DBase> solver3d; // 3D contexts
 };
 
 namespace {
@@ -36,9 +38,18 @@ template <typename F> int guarded(F &&f) {
 }
 
 smx::Solver2DBase &S(smx_ctx *ctx) {
+  if (ctx && ctx->solver3d)
+    throw smx::SmxError(SMX_ERR_STATE, "operation not available for 3D contexts");
   if (!ctx || !ctx->solver)
     throw smx::SmxError(SMX_ERR_ARG, "null context");
   return *ctx->solver;
+}
+
+// The 3D solver of a context, or null for a 2D one.
+smx::Solver3DBase *S3(smx_ctx *ctx) {
+  if (!ctx)
+    throw smx::SmxError(SMX_ERR_ARG, "null context");
+  return ctx->solver3d.get();
 }
 
 } // namespace
@@ -153,6 +164,21 @@ int smx_ctx_shots(const smx_ctx *ctx, int *nshots, int *elem_off, int *rec_off,
   });
 }
 
+int smx_ctx_create3d(const smx_mesh3d *mesh, const smx_sources3d *sources,
+                     const smx_receivers3d *receivers, double dt, int nstep,
+                     int precision, smx_ctx **out) {
+  return guarded([&] {
+    if (!mesh || !sources || !receivers || !out)
+      throw smx::SmxError(SMX_ERR_ARG, "null argument");
+    *out = nullptr;
+    auto a = std::make_unique<smx::HostAssembly3D>();
+    smx::assemble3d(*mesh, *sources, *receivers, dt, nstep, *a);
+    auto ctx = std::make_unique<smx_ctx>();
+    ctx->solver3d = smx::make_solver3d(std::move(a), precision);
+    *out = ctx.release();
+  });
+}
+
 int smx_ctx_destroy(smx_ctx *ctx) {
   return guarded([&] { delete ctx; });
 }
@@ -160,6 +186,14 @@ int smx_ctx_destroy(smx_ctx *ctx) {
 int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
                  int *nrec) {
   return guarded([&] {
+    if (auto *s3 = S3(const_cast<smx_ctx *>(ctx))) {
+      const auto &A3 = s3->assembly();
+      if (nspec) *nspec = A3.nspec;
+      if (nglob) *nglob = A3.nglob;
+      if (nsig) *nsig = A3.nsig;
+      if (nrec) *nrec = A3.nrec;
+      return;
+    }
     const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
     if (nspec) *nspec = A.nspec;
     if (nglob) *nglob = A.nglob;
@@ -193,13 +227,19 @@ int smx_ctx_compact(const smx_ctx *ctx, int *geom_elements, int *mat_elements) {
 }
 
 double smx_ctx_t0(const smx_ctx *ctx) {
+  if (ctx && ctx->solver3d)
+    return ctx->solver3d->assembly().t0;
   if (!ctx || !ctx->solver)
     return 0.0;
   return ctx->solver->assembly().t0;
 }
 
 int smx_initialize(smx_ctx *ctx) {
-  return guarded([&] { S(ctx).initialize(); });
+  return guarded([&] {
+    if (auto *s3 = S3(ctx))
+      return s3->initialize();
+    S(ctx).initialize();
+  });
 }
 
 int smx_predictor(smx_ctx *ctx, int field, int *dofs) {
@@ -228,16 +268,26 @@ int smx_compute_seismograms(smx_ctx *ctx, int field, int isig) {
 }
 
 int smx_run_forward(smx_ctx *ctx, int istep0, int nstep) {
-  return guarded([&] { S(ctx).run_forward(istep0, nstep); });
+  return guarded([&] {
+    if (auto *s3 = S3(ctx))
+      return s3->run_forward(istep0, nstep);
+    S(ctx).run_forward(istep0, nstep);
+  });
 }
 
 int smx_synchronize(smx_ctx *ctx) {
-  return guarded([&] { S(ctx).synchronize(); });
+  return guarded([&] {
+    if (auto *s3 = S3(ctx))
+      return s3->synchronize();
+    S(ctx).synchronize();
+  });
 }
 
 int smx_get_seismograms(smx_ctx *ctx, void *host_out) {
   return guarded([&] {
     if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    if (auto *s3 = S3(ctx))
+      return s3->get_seismograms(host_out);
     S(ctx).get_seismograms(host_out);
   });
 }
@@ -245,6 +295,11 @@ int smx_get_seismograms(smx_ctx *ctx, void *host_out) {
 int smx_get_field(smx_ctx *ctx, int field, int which, void *host_out) {
   return guarded([&] {
     if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    if (auto *s3 = S3(ctx)) {
+      if (field != SMX_FIELD_FORWARD)
+        throw smx::SmxError(SMX_ERR_ARG, "3D contexts hold the forward field only");
+      return s3->get_field(which, host_out);
+    }
     S(ctx).get_field(field, which, host_out);
   });
 }
@@ -252,6 +307,11 @@ int smx_get_field(smx_ctx *ctx, int field, int which, void *host_out) {
 int smx_set_field(smx_ctx *ctx, int field, int which, const void *host_in) {
   return guarded([&] {
     if (!host_in) throw smx::SmxError(SMX_ERR_ARG, "null input");
+    if (auto *s3 = S3(ctx)) {
+      if (field != SMX_FIELD_FORWARD)
+        throw smx::SmxError(SMX_ERR_ARG, "3D contexts hold the forward field only");
+      return s3->set_field(which, host_in);
+    }
     S(ctx).set_field(field, which, host_in);
   });
 }
@@ -277,12 +337,17 @@ int smx_field_device_ptr(smx_ctx *ctx, int field, int which, void **dptr) {
 int smx_last_run_stats(smx_ctx *ctx, double *ms, double *stiffness_ms,
                        int *stiffness_launches) {
   return guarded(
-      [&] { S(ctx).last_run_stats(ms, stiffness_ms, stiffness_launches); });
+      [&] {
+        if (auto *s3 = S3(ctx))
+          return s3->last_run_stats(ms, stiffness_ms, stiffness_launches);
+        S(ctx).last_run_stats(ms, stiffness_ms, stiffness_launches);
+      });
 }
 
 int smx_time_stiffness(smx_ctx *ctx, int nlaunch, double *avg_ms) {
   return guarded([&] {
-    const double t = S(ctx).time_stiffness(nlaunch);
+    auto *s3 = S3(ctx);
+    const double t = s3 ? s3->time_stiffness(nlaunch) : S(ctx).time_stiffness(nlaunch);
     if (avg_ms) *avg_ms = t;
   });
 }

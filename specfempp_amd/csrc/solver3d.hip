@@ -1,0 +1,578 @@
+// gfx950 kernels and the device-resident solver for the 3D elastic isotropic
+// time step (hex8, NGLL = 5).
+//
+// Kernels (each cites the reference dispatch it replaces):
+//   k3_elem     compute_stiffness_interaction for dim3
+//               (include/kokkos_kernels/impl/compute_stiffness_interaction.tpp:
+//               140-247 with the dim3 gradient, stress and divergence,
+//               algorithms/gradient.hpp, medium/dim3/elastic/isotropic/
+//               stress.hpp:46-99, point/stress.hpp:217-240,
+//               algorithms/divergence.hpp:95-137), plus this build's 3D Stacey
+//               faces.  One 128-thread workgroup per element, one thread per
+//               GLL point; the element's displacement and its stress integrand
+//               are staged in LDS.  Assembly without atomics: one launch per
+//               element colour (no two elements of a colour share a point),
+//               each a plain read-modify-write of the accumulator.
+//   k3_src      compute_source_interaction (compute_source_interaction.tpp:
+//               17-104): sources in list order, one block.
+//   k3_seis     compute_seismograms (compute_seismogram.tpp:19-171).
+//   k3_node     divide_mass_matrix + Newmark corrector + next predictor
+//               (divide_mass_matrix.tpp:13-65, newmark.tpp:13-128).
+// A step is k3_src, the colour passes, k3_seis, k3_node; steps are captured
+// into HIP graphs and a device counter selects the source row and sample.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "solver3d.hpp"
+
+namespace smx {
+
+#define SMX3_HIP(call)                                                         \
+  do {                                                                         \
+    hipError_t err_ = (call);                                                  \
+    if (err_ != hipSuccess)                                                    \
+      throw SmxError(SMX_ERR_HIP, std::string(#call) + ": " +                  \
+                                      hipGetErrorString(err_));                \
+  } while (0)
+
+namespace {
+
+constexpr int ELEM_THREADS = 128; // >= NPT3 = 125
+constexpr int NODE3_THREADS = 256;
+constexpr int kGraphSteps3 = 8;
+
+template <typename real> struct Quad3 {
+  real hp[NGLL][NGLL]; // hp[i][j] = l'_j(xi_i)
+  real w[NGLL];
+};
+
+struct Types3 {
+  int t[4];
+};
+
+__device__ __forceinline__ int p3(int iz, int iy, int ix) {
+  return (iz * NGLL + iy) * NGLL + ix;
+}
+
+// One element of the colour pass.  GE / ME: geometry / moduli stored per
+// element instead of per GLL point.
+template <typename real, bool GE, bool ME>
+__global__ void __launch_bounds__(ELEM_THREADS)
+    k3_elem(int slot0, const int *__restrict__ ibool, int ld,
+            const real *__restrict__ geom, const real *__restrict__ mat,
+            const int *__restrict__ sslot, const real *__restrict__ stacey,
+            int nst, const real *__restrict__ U, const real *__restrict__ V,
+            real *__restrict__ A, long npad, Quad3<real> q) {
+  __shared__ real s_u[3][NPT3];
+  __shared__ real s_f[3][3][NPT3];
+  const int slot = slot0 + blockIdx.x;
+  const int t = threadIdx.x;
+  const bool act = t < NPT3;
+  const int p = act ? t : 0;
+  const int iz = p / (NGLL * NGLL), iy = (p / NGLL) % NGLL, ix = p % NGLL;
+  int g = 0;
+  if (act) {
+    g = ibool[size_t(p) * ld + slot];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      s_u[c][p] = U[c * npad + g];
+  }
+  __syncthreads();
+  if (act) {
+    real dxi[3], det[3], dga[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      real a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+      for (int l = 0; l < NGLL; ++l) {
+        a0 += q.hp[ix][l] * s_u[c][p3(iz, iy, l)];
+        a1 += q.hp[iy][l] * s_u[c][p3(iz, l, ix)];
+        a2 += q.hp[iz][l] * s_u[c][p3(l, iy, ix)];
+      }
+      dxi[c] = a0;
+      det[c] = a1;
+      dga[c] = a2;
+    }
+    real G[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+      G[k] = GE ? geom[size_t(k) * ld + slot]
+                : geom[(size_t(k) * NPT3 + p) * ld + slot];
+    real du[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      du[c][0] = G[0] * dxi[c] + G[3] * det[c] + G[6] * dga[c];
+      du[c][1] = G[1] * dxi[c] + G[4] * det[c] + G[7] * dga[c];
+      du[c][2] = G[2] * dxi[c] + G[5] * det[c] + G[8] * dga[c];
+    }
+    const real L = ME ? mat[slot] : mat[size_t(p) * ld + slot];
+    const real M = ME ? mat[size_t(ld) + slot]
+                      : mat[(size_t(NPT3) + p) * ld + slot];
+    const real lam = L - real(2.0) * M;
+    real T[3][3];
+    T[0][0] = L * du[0][0] + lam * (du[1][1] + du[2][2]);
+    T[1][1] = L * du[1][1] + lam * (du[0][0] + du[2][2]);
+    T[2][2] = L * du[2][2] + lam * (du[0][0] + du[1][1]);
+    T[0][1] = T[1][0] = M * (du[0][1] + du[1][0]);
+    T[0][2] = T[2][0] = M * (du[0][2] + du[2][0]);
+    T[1][2] = T[2][1] = M * (du[1][2] + du[2][1]);
+    const real J = G[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s_f[c][0][p] = J * (T[c][0] * G[0] + T[c][1] * G[1] + T[c][2] * G[2]);
+      s_f[c][1][p] = J * (T[c][0] * G[3] + T[c][1] * G[4] + T[c][2] * G[5]);
+      s_f[c][2][p] = J * (T[c][0] * G[6] + T[c][1] * G[7] + T[c][2] * G[8]);
+    }
+  }
+  __syncthreads();
+  if (!act)
+    return;
+  real r[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    real t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      t1 += s_f[c][0][p3(iz, iy, l)] * q.hp[l][ix] * q.w[l];
+      t2 += s_f[c][1][p3(iz, l, ix)] * q.hp[l][iy] * q.w[l];
+      t3 += s_f[c][2][p3(l, iy, ix)] * q.hp[l][iz] * q.w[l];
+    }
+    r[c] = q.w[iz] * q.w[iy] * t1 + q.w[iz] * q.w[ix] * t2 +
+           q.w[iy] * q.w[ix] * t3;
+    r[c] *= real(-1.0);
+  }
+  const int b = sslot[slot];
+  if (b >= 0) {
+    const size_t sp = size_t(NPT3) * nst, so = size_t(p) * nst + b;
+    const real wgt = stacey[4 * sp + so];
+    if (wgt != real(0)) {
+      const real n[3] = {stacey[so], stacey[sp + so], stacey[2 * sp + so]};
+      const real nrm = stacey[3 * sp + so], P = stacey[5 * sp + so],
+                 S = stacey[6 * sp + so];
+      const real v[3] = {V[g], V[npad + g], V[2 * npad + g]};
+      const real vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const real fc = ((vn * n[c] / (nrm * nrm)) * (P - S)) + v[c] * S;
+        r[c] += real(-1.0) * fc * nrm * wgt;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    A[c * npad + g] += r[c];
+}
+
+// Sources in list order (sources may share points), one block.
+template <typename real>
+__global__ void __launch_bounds__(ELEM_THREADS)
+    k3_src(int nsrc, const int *__restrict__ src_slot,
+           const real *__restrict__ src_array, const real *__restrict__ stf,
+           const int *__restrict__ ibool, int ld, real *__restrict__ A,
+           long npad, const int *__restrict__ step_ptr) {
+  const int step = *step_ptr;
+  const int p = threadIdx.x;
+  for (int s = 0; s < nsrc; ++s) {
+    if (p < NPT3) {
+      const int g = ibool[size_t(p) * ld + src_slot[s]];
+      const real f = stf[size_t(step) * nsrc + s];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        A[c * npad + g] += src_array[(size_t(s) * 3 + c) * NPT3 + p] * f;
+    }
+    __syncthreads();
+  }
+}
+
+// Seismograms of step `*step_ptr` (if sampled): displacement from U, and
+// velocity / acceleration from the corrected values the node pass is about
+// to form (same arithmetic).  One thread per (type, receiver), points summed
+// in order.
+template <typename real>
+__global__ void k3_seis(const real *__restrict__ U, const real *__restrict__ V,
+                        const real *__restrict__ A,
+                        const real *__restrict__ Minv,
+                        const int *__restrict__ rec_g,
+                        const real *__restrict__ rec_h, int nrec, Types3 types,
+                        int ntypes, int nsbs, long npad, real dto2,
+                        const int *__restrict__ step_ptr,
+                        real *__restrict__ out) {
+  const int step = *step_ptr;
+  if (step % nsbs != 0)
+    return;
+  const int isig = step / nsbs;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int r = k % nrec, it = k / nrec;
+  if (it >= ntypes)
+    return;
+  const int type = types.t[it];
+  real s[3] = {0, 0, 0};
+  for (int p = 0; p < NPT3; ++p) {
+    const int g = rec_g[size_t(r) * NPT3 + p];
+    const real h = rec_h[size_t(r) * NPT3 + p];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      real f;
+      if (type == 0) {
+        f = U[c * npad + g];
+      } else {
+        const real a = A[c * npad + g] * Minv[c * npad + g];
+        f = type == 2 ? a : V[c * npad + g] + dto2 * a;
+      }
+      s[c] += h * f;
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    out[((size_t(isig) * ntypes + it) * nrec + r) * 3 + c] = s[c];
+}
+
+// Node pass: a = A / M, v += dt/2 a; then (FINAL = false) the next
+// predictor u += dt v + dt^2/2 a, v += dt/2 a, a = 0.  Thread 0 advances the
+// step counter.
+template <typename real, bool FINAL>
+__global__ void __launch_bounds__(NODE3_THREADS)
+    k3_node(real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
+            const real *__restrict__ Minv, long n3, real dt, real dto2,
+            real dt2o2, int *__restrict__ step_ptr) {
+  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i == 0 && step_ptr)
+    *step_ptr += 1;
+  if (i >= n3)
+    return;
+  real a = A[i] * Minv[i];
+  real v = V[i];
+  v += dto2 * a;
+  if (FINAL) {
+    V[i] = v;
+    A[i] = a;
+    return;
+  }
+  real u = U[i];
+  u += dt * v + dt2o2 * a;
+  v += dto2 * a;
+  U[i] = u;
+  V[i] = v;
+  A[i] = 0;
+}
+
+template <typename real>
+__global__ void __launch_bounds__(NODE3_THREADS)
+    k3_pred(real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
+            long n3, real dt, real dto2, real dt2o2) {
+  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n3)
+    return;
+  real u = U[i], v = V[i], a = A[i];
+  u += dt * v + dt2o2 * a;
+  v += dto2 * a;
+  U[i] = u;
+  V[i] = v;
+  A[i] = 0;
+}
+
+__global__ void k3_set_step(int *__restrict__ step_ptr, int value) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *step_ptr = value;
+}
+
+} // namespace
+
+template <typename real> class Solver3D final : public Solver3DBase {
+public:
+  explicit Solver3D(std::unique_ptr<HostAssembly3D> a) : asm_(std::move(a)) {
+    const HostAssembly3D &A = *asm_;
+    SMX3_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+    SMX3_HIP(hipEventCreate(&ev0_));
+    SMX3_HIP(hipEventCreate(&ev1_));
+    npad_ = (long(A.nglob) + 255) / 256 * 256;
+    ibool_ = upload<int>(A.ibool);
+    geom_ = upload<real>(A.geom);
+    mat_ = sizeof(real) == 8 ? upload<real>(A.mat64) : upload<real>(A.mat);
+    sslot_ = upload<int>(A.stacey_slot);
+    nst_ = std::max(1, A.nstacey);
+    {
+      std::vector<double> st(A.stacey.begin(), A.stacey.end());
+      if (sizeof(real) == 8)
+        std::copy(A.stacey64.begin(), A.stacey64.end(),
+                  st.begin() + size_t(5) * NPT3 * nst_);
+      stacey_ = upload<real>(st);
+    }
+    {
+      std::vector<double> mi(size_t(3) * npad_, 1.0);
+      for (int c = 0; c < 3; ++c)
+        for (int g = 0; g < A.nglob; ++g)
+          mi[size_t(c) * npad_ + g] = sizeof(real) == 8
+                                          ? A.minv64[size_t(c) * A.nglob + g]
+                                          : A.minv[size_t(c) * A.nglob + g];
+      minv_ = upload<real>(mi);
+    }
+    src_slot_ = upload<int>(A.src_slot);
+    src_array_ = upload<real>(A.src_array);
+    stf_ = upload<real>(A.stf);
+    rec_g_ = upload<int>(A.rec_g);
+    rec_h_ = upload<real>(A.rec_h);
+    U_ = alloc<real>(3 * npad_);
+    V_ = alloc<real>(3 * npad_);
+    A_ = alloc<real>(3 * npad_);
+    step_ = alloc<int>(1);
+    const int ntypes = int(A.seis_types.size());
+    seis_ = alloc<real>(std::max<size_t>(
+        size_t(std::max(1, A.nsig)) * ntypes * std::max(1, A.nrec) * 3, 3));
+    for (int k = 0; k < 4; ++k)
+      types_.t[k] = k < ntypes ? A.seis_types[k] : 0;
+    for (int i = 0; i < NGLL; ++i) {
+      quad_.w[i] = A.quad.w[i];
+      for (int j = 0; j < NGLL; ++j)
+        quad_.hp[i][j] = A.quad.hprime[i][j];
+    }
+    dt_ = A.dt;
+    dto2_ = A.dt_over2;
+    dt2o2_ = A.dt2_over2;
+    SMX3_HIP(hipStreamSynchronize(stream_));
+  }
+
+  ~Solver3D() override {
+    if (graph_exec_)
+      (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_)
+      (void)hipGraphDestroy(graph_);
+    (void)hipStreamSynchronize(stream_);
+    for (void *p : allocs_)
+      (void)hipFree(p);
+    (void)hipEventDestroy(ev0_);
+    (void)hipEventDestroy(ev1_);
+    (void)hipStreamDestroy(stream_);
+  }
+
+  int precision() const override { return sizeof(real); }
+  const HostAssembly3D &assembly() const override { return *asm_; }
+
+  void initialize() override { initialized_ = true; }
+
+  void run_forward(int istep0, int nstep) override {
+    if (!initialized_)
+      throw SmxError(SMX_ERR_STATE, "smx_initialize has not been called");
+    const HostAssembly3D &A = *asm_;
+    if (istep0 < 0 || nstep < 1 || istep0 + nstep > A.nstep)
+      throw SmxError(SMX_ERR_ARG, "steps outside [0, nstep)");
+    SMX3_HIP(hipEventRecord(ev0_, stream_));
+    k3_set_step<<<1, 64, 0, stream_>>>(step_, istep0);
+    SMX3_HIP(hipGetLastError());
+    k3_pred<real><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
+        U_, V_, A_, 3 * npad_, dt_, dto2_, dt2o2_);
+    SMX3_HIP(hipGetLastError());
+    const int body = nstep - 1;
+    const int reps = body / kGraphSteps3;
+    if (reps > 0) {
+      build_graph();
+      for (int r = 0; r < reps; ++r)
+        SMX3_HIP(hipGraphLaunch(graph_exec_, stream_));
+    }
+    for (int s = reps * kGraphSteps3; s < body; ++s)
+      step<false>();
+    step<true>();
+    SMX3_HIP(hipEventRecord(ev1_, stream_));
+    have_stats_ = true;
+    launches_ = nstep * A.ncolors;
+  }
+
+  void synchronize() override { SMX3_HIP(hipStreamSynchronize(stream_)); }
+
+  void get_seismograms(void *host) override {
+    const HostAssembly3D &A = *asm_;
+    const size_t n = size_t(A.nsig) * A.seis_types.size() * A.nrec * 3;
+    SMX3_HIP(hipMemcpyAsync(host, seis_, n * sizeof(real),
+                            hipMemcpyDeviceToHost, stream_));
+    SMX3_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void get_field(int which, void *host) override {
+    const real *d = pick(which);
+    const int n = asm_->nglob;
+    std::vector<real> buf(size_t(3) * npad_);
+    SMX3_HIP(hipMemcpyAsync(buf.data(), d, buf.size() * sizeof(real),
+                            hipMemcpyDeviceToHost, stream_));
+    SMX3_HIP(hipStreamSynchronize(stream_));
+    real *h = static_cast<real *>(host);
+    for (int g = 0; g < n; ++g)
+      for (int c = 0; c < 3; ++c)
+        h[size_t(g) * 3 + c] = buf[size_t(c) * npad_ + g];
+  }
+
+  void set_field(int which, const void *host) override {
+    if (which == SMX_MASS_INVERSE)
+      throw SmxError(SMX_ERR_ARG, "the mass matrix is read-only");
+    real *d = pick(which);
+    const int n = asm_->nglob;
+    std::vector<real> buf(size_t(3) * npad_, real(0));
+    const real *h = static_cast<const real *>(host);
+    for (int g = 0; g < n; ++g)
+      for (int c = 0; c < 3; ++c)
+        buf[size_t(c) * npad_ + g] = h[size_t(g) * 3 + c];
+    SMX3_HIP(hipMemcpyAsync(d, buf.data(), buf.size() * sizeof(real),
+                            hipMemcpyHostToDevice, stream_));
+    SMX3_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void last_run_stats(double *ms, double *stiff_ms, int *launches) override {
+    float t = 0;
+    if (have_stats_) {
+      SMX3_HIP(hipEventSynchronize(ev1_));
+      SMX3_HIP(hipEventElapsedTime(&t, ev0_, ev1_));
+    }
+    if (ms)
+      *ms = t;
+    if (stiff_ms)
+      *stiff_ms = stiff_ms_;
+    if (launches)
+      *launches = launches_;
+  }
+
+  double time_stiffness(int n) override {
+    if (n < 1)
+      throw SmxError(SMX_ERR_ARG, "n must be positive");
+    if (!scratch_a_)
+      scratch_a_ = alloc<real>(3 * npad_);
+    elem_passes(scratch_a_); // warm-up
+    SMX3_HIP(hipEventRecord(ev0_, stream_));
+    for (int i = 0; i < n; ++i)
+      elem_passes(scratch_a_);
+    SMX3_HIP(hipEventRecord(ev1_, stream_));
+    SMX3_HIP(hipEventSynchronize(ev1_));
+    float t = 0;
+    SMX3_HIP(hipEventElapsedTime(&t, ev0_, ev1_));
+    stiff_ms_ = double(t) / n;
+    have_stats_ = false;
+    return stiff_ms_;
+  }
+
+private:
+  template <typename T, typename H> T *upload(const std::vector<H> &h) {
+    T *d = alloc<T>(h.size());
+    if (!h.empty()) {
+      std::vector<T> tmp(h.begin(), h.end());
+      SMX3_HIP(hipMemcpyAsync(d, tmp.data(), tmp.size() * sizeof(T),
+                              hipMemcpyHostToDevice, stream_));
+      SMX3_HIP(hipStreamSynchronize(stream_));
+    }
+    return d;
+  }
+  template <typename T> T *alloc(size_t n) {
+    void *p = nullptr;
+    SMX3_HIP(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)));
+    allocs_.push_back(p);
+    SMX3_HIP(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), stream_));
+    return static_cast<T *>(p);
+  }
+  real *pick(int which) {
+    switch (which) {
+    case SMX_U: return U_;
+    case SMX_V: return V_;
+    case SMX_A: return A_;
+    case SMX_MASS_INVERSE: return minv_;
+    }
+    throw SmxError(SMX_ERR_ARG, "unknown field component");
+  }
+  dim3 node_grid() const {
+    return dim3(unsigned((3 * npad_ + NODE3_THREADS - 1) / NODE3_THREADS));
+  }
+
+  // Every colour's element pass, accumulating into `acc`.
+  void elem_passes(real *acc) {
+    const HostAssembly3D &A = *asm_;
+    for (int c = 0; c < A.ncolors; ++c) {
+      const int s0 = A.color_off[c], cnt = A.color_off[c + 1] - s0;
+      if (cnt == 0)
+        continue;
+      auto launch = [&](auto kern) {
+        kern<<<dim3(cnt), dim3(ELEM_THREADS), 0, stream_>>>(
+            s0, ibool_, A.nspec, geom_, mat_, sslot_, stacey_, nst_, U_, V_,
+            acc, npad_, quad_);
+      };
+      if (A.geom_elem && A.mat_elem)
+        launch(k3_elem<real, true, true>);
+      else if (A.geom_elem)
+        launch(k3_elem<real, true, false>);
+      else if (A.mat_elem)
+        launch(k3_elem<real, false, true>);
+      else
+        launch(k3_elem<real, false, false>);
+      SMX3_HIP(hipGetLastError());
+    }
+  }
+
+  template <bool FINAL> void step() {
+    const HostAssembly3D &A = *asm_;
+    if (A.nsrc > 0) {
+      k3_src<real><<<1, ELEM_THREADS, 0, stream_>>>(
+          A.nsrc, src_slot_, src_array_, stf_, ibool_, A.nspec, A_, npad_,
+          step_);
+      SMX3_HIP(hipGetLastError());
+    }
+    elem_passes(A_);
+    const int ntypes = int(A.seis_types.size());
+    if (A.nrec > 0 && ntypes > 0) {
+      const int n = A.nrec * ntypes;
+      k3_seis<real><<<(n + 127) / 128, 128, 0, stream_>>>(
+          U_, V_, A_, minv_, rec_g_, rec_h_, A.nrec, types_, ntypes, A.nsbs,
+          npad_, dto2_, step_, seis_);
+      SMX3_HIP(hipGetLastError());
+    }
+    k3_node<real, FINAL><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
+        U_, V_, A_, minv_, 3 * npad_, dt_, dto2_, dt2o2_, step_);
+    SMX3_HIP(hipGetLastError());
+  }
+
+  void build_graph() {
+    if (graph_exec_)
+      return;
+    SMX3_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
+    try {
+      for (int s = 0; s < kGraphSteps3; ++s)
+        step<false>();
+    } catch (...) {
+      hipGraph_t g = nullptr;
+      (void)hipStreamEndCapture(stream_, &g);
+      if (g)
+        (void)hipGraphDestroy(g);
+      throw;
+    }
+    SMX3_HIP(hipStreamEndCapture(stream_, &graph_));
+    SMX3_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
+  }
+
+  std::unique_ptr<HostAssembly3D> asm_;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  hipGraph_t graph_ = nullptr;
+  hipGraphExec_t graph_exec_ = nullptr;
+  std::vector<void *> allocs_;
+  long npad_ = 0;
+  int nst_ = 1;
+  int *ibool_ = nullptr, *sslot_ = nullptr, *src_slot_ = nullptr,
+      *rec_g_ = nullptr, *step_ = nullptr;
+  real *geom_ = nullptr, *mat_ = nullptr, *stacey_ = nullptr, *minv_ = nullptr,
+       *src_array_ = nullptr, *stf_ = nullptr, *rec_h_ = nullptr, *U_ = nullptr,
+       *V_ = nullptr, *A_ = nullptr, *seis_ = nullptr, *scratch_a_ = nullptr;
+  Quad3<real> quad_;
+  Types3 types_;
+  real dt_ = 0, dto2_ = 0, dt2o2_ = 0;
+  bool initialized_ = false, have_stats_ = false;
+  double stiff_ms_ = 0;
+  int launches_ = 0;
+};
+
+std::unique_ptr<Solver3DBase> make_solver3d(std::unique_ptr<HostAssembly3D> a,
+                                            int precision) {
+  if (precision == 4)
+    return std::make_unique<Solver3D<float>>(std::move(a));
+  if (precision == 8)
+    return std::make_unique<Solver3D<double>>(std::move(a));
+  throw SmxError(SMX_ERR_ARG, "precision must be 4 or 8");
+}
+
+} // namespace smx

@@ -1,0 +1,32 @@
+// Device-resident 3D elastic solver: the object behind a 3D smx_ctx.
+#pragma once
+
+#include <memory>
+
+#include "assembly3d.hpp"
+
+namespace smx {
+
+class Solver3DBase {
+public:
+  virtual ~Solver3DBase() = default;
+  virtual int precision() const = 0;
+  virtual const HostAssembly3D &assembly() const = 0;
+  virtual void initialize() = 0;
+  // Fused, graph-replayed forward steps istep0 .. istep0+nstep-1 (the last
+  // one ends with the corrected velocity and acceleration stored).
+  virtual void run_forward(int istep0, int nstep) = 0;
+  virtual void synchronize() = 0;
+  virtual void get_seismograms(void *host) = 0;   // [nsig][ntypes][nrec][3]
+  virtual void get_field(int which, void *host) = 0;       // [nglob][3]
+  virtual void set_field(int which, const void *host) = 0; // [nglob][3]
+  virtual void last_run_stats(double *ms, double *stiffness_ms,
+                              int *stiffness_launches) = 0;
+  // Mean duration (ms) of one step's element passes (all colours).
+  virtual double time_stiffness(int n) = 0;
+};
+
+std::unique_ptr<Solver3DBase> make_solver3d(std::unique_ptr<HostAssembly3D> a,
+                                            int precision);
+
+} // namespace smx

@@ -1,0 +1,122 @@
+"""Host-side mirror of the 3D forward path (time_marching<forward, dim3>
+with the dim3 domain_kernels, core/specfem/solver/time_marching.tpp:8-131).
+
+Every operator runs on the GPU through libspecfempp_amd (C ABI,
+``smx_ctx_create3d``); the host only sequences calls.
+"""
+
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import SMX_A, SMX_FIELD_FORWARD, SMX_MASS_INVERSE, SMX_U, SMX_V, check
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None and a.size else None
+
+
+class Context3D:
+    """A 3D smx_ctx: hex8 mesh, sources, receivers and fields on the GPU."""
+
+    def __init__(self, mesh, sources, receivers, dt, nstep, *, nsbs=1,
+                 seismo_types=("displacement",), precision=4):
+        lib = _lib.load()
+        mesh.validate()
+        keep = []
+
+        def arr(x, dt_):
+            a = np.ascontiguousarray(np.asarray(x, dtype=dt_))
+            keep.append(a)
+            return a
+
+        m = _lib.SmxMesh3D(mesh.nspec, mesh.nnodes, _p(arr(mesh.coord, np.float64)),
+                           _p(arr(mesh.knods, np.int32)), _p(arr(mesh.kappa, np.float32)),
+                           _p(arr(mesh.mu, np.float32)), _p(arr(mesh.rho, np.float32)),
+                           len(mesh.abs_ispec), _p(arr(mesh.abs_ispec, np.int32)),
+                           _p(arr(mesh.abs_face, np.int32)))
+        s = _lib.SmxSources3D(len(sources), _p(arr(sources.x, np.float64)),
+                              _p(arr(sources.y, np.float64)), _p(arr(sources.z, np.float64)),
+                              _p(arr(sources.kind, np.int32)), _p(arr(sources.comp, np.float32)),
+                              _p(arr(sources.f0, np.float32)), _p(arr(sources.tshift, np.float32)),
+                              _p(arr(sources.factor, np.float32)))
+        types = arr([_lib.SEIS_TYPES[t] for t in seismo_types], np.int32)
+        r = _lib.SmxReceivers3D(len(receivers), _p(arr(receivers.x, np.float64)),
+                                _p(arr(receivers.y, np.float64)), _p(arr(receivers.z, np.float64)),
+                                len(types), _p(types), int(nsbs))
+        h = ctypes.c_void_p()
+        check(lib.smx_ctx_create3d(ctypes.byref(m), ctypes.byref(s), ctypes.byref(r),
+                                   float(dt), int(nstep), int(precision), ctypes.byref(h)))
+        del keep
+        self.lib, self.h = lib, h
+        self.precision = precision
+        self.dtype = np.float32 if precision == 4 else np.float64
+        self.dt, self.nstep, self.nsbs = float(dt), int(nstep), int(nsbs)
+        self.seismo_types = tuple(seismo_types)
+        ns, ng, nsig, nr = (ctypes.c_int() for _ in range(4))
+        check(lib.smx_ctx_info(h, ctypes.byref(ns), ctypes.byref(ng), ctypes.byref(nsig),
+                               ctypes.byref(nr)))
+        self.nspec, self.nglob, self.nsig, self.nrec = ns.value, ng.value, nsig.value, nr.value
+        self.t0 = lib.smx_ctx_t0(h)
+
+    def close(self):
+        if getattr(self, "h", None):
+            check(self.lib.smx_ctx_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self):
+        check(self.lib.smx_initialize(self.h))
+
+    def run_forward(self, istep0, nstep):
+        check(self.lib.smx_run_forward(self.h, int(istep0), int(nstep)))
+
+    def synchronize(self):
+        check(self.lib.smx_synchronize(self.h))
+
+    def seismograms(self):
+        """(nsig, ntypes, nrec, 3) array of the recorded seismograms."""
+        out = np.empty((self.nsig, len(self.seismo_types), self.nrec, 3), dtype=self.dtype)
+        check(self.lib.smx_get_seismograms(self.h, _p(out)))
+        return out
+
+    def get_field(self, which):
+        idx = {"u": SMX_U, "v": SMX_V, "a": SMX_A, "minv": SMX_MASS_INVERSE}[which]
+        out = np.empty((self.nglob, 3), dtype=self.dtype)
+        check(self.lib.smx_get_field(self.h, SMX_FIELD_FORWARD, idx, _p(out)))
+        return out
+
+    def set_field(self, which, values):
+        idx = {"u": SMX_U, "v": SMX_V, "a": SMX_A}[which]
+        v = np.ascontiguousarray(np.asarray(values, dtype=self.dtype).reshape(self.nglob, 3))
+        check(self.lib.smx_set_field(self.h, SMX_FIELD_FORWARD, idx, _p(v)))
+
+    def last_run_ms(self):
+        ms, sms, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        check(self.lib.smx_last_run_stats(self.h, ctypes.byref(ms), ctypes.byref(sms),
+                                          ctypes.byref(n)))
+        return ms.value
+
+    def time_stiffness(self, nlaunch=10):
+        t = ctypes.c_double()
+        check(self.lib.smx_time_stiffness(self.h, int(nlaunch), ctypes.byref(t)))
+        return t.value
+
+
+def simulate3d(config, *, precision=4, nstep=None):
+    """Runs a forward simulation of a Config3D; returns (seismograms, ctx)."""
+    n = config.nstep if nstep is None else int(nstep)
+    ctx = Context3D(config.mesh, config.sources, config.receivers, config.dt, n,
+                    nsbs=config.nsbs, seismo_types=config.seismo_types, precision=precision)
+    ctx.initialize()
+    ctx.run_forward(0, n)
+    ctx.synchronize()
+    return ctx.seismograms(), ctx
