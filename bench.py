@@ -249,7 +249,7 @@ def hex_leg(n=64, nstep=4000):
     wall = time.perf_counter() - t
     seis = ctx.seismograms()
     stiff_ms = ctx.time_stiffness(10)
-    nglob, nspec = ctx.nglob, ctx.nspec
+    nglob, nspec, geometry = ctx.nglob, ctx.nspec, ctx.geometry
     ctx.close()
     canon = 76 * nglob + 6500 * nspec
     return {"workload": f"3D elastic, {n}^3 hex8 of 100 m, vp 3000->6000 m/s +-2 % random, "
@@ -260,6 +260,7 @@ def hex_leg(n=64, nstep=4000):
             "canonical_bytes_per_step": canon,
             "canonical_frac": canon * nstep / wall / 1e9 / HBM_PEAK_GBS,
             "element_passes_ms": stiff_ms, "seismograms_finite": bool(np.isfinite(seis).all()),
+            "geometry": geometry,
             "host_s": {"generate": gen_s, "assemble": setup_s}}
 
 
@@ -277,6 +278,8 @@ def main():
                     help="skip the batched-shots leg (64 shots, 3000 steps)")
     ap.add_argument("--no-3d", action="store_true",
                     help="skip the 3D leg (64^3 hex, 4000 steps)")
+    ap.add_argument("--only-leg", choices=("adjoint", "batch", "hex3d"), default=None,
+                    help="profiling aid: run only this leg and print its JSON")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per k_tile launch measured with rocprofv3 --pmc "
@@ -296,6 +299,10 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": 4 * n, "copy_probe_bytes_written": 4 * n,
                           "launches": 5}))
+        return
+    if args.only_leg:
+        leg = {"adjoint": adjoint_leg, "batch": batch_leg, "hex3d": hex_leg}[args.only_leg]
+        print(json.dumps({args.only_leg: leg()}), flush=True)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
@@ -327,6 +334,10 @@ def main():
     seis = ctx.seismograms()
     import numpy as np
     finite = bool(np.isfinite(seis).all())
+    import ctypes
+    gm, mis = ctypes.c_int(), ctypes.c_longlong()
+    ctx.lib.smx_ctx_geometry(ctx.h, ctypes.byref(gm), ctypes.byref(mis))
+    geometry = {"mode": ("point", "element", "nodes")[gm.value], "mismatches": mis.value}
 
     stiff_ms = ctx.time_stiffness(50)
     sb = tile_kernel_bytes(ctx.nspec, ctx.layout, nstacey)
@@ -364,6 +375,7 @@ def main():
                           / HBM_PEAK_GBS,
                           "event_ms": ev_ms},
         "seismograms_finite": finite,
+        "geometry": geometry,
     }
     del ctx
     if not args.no_adjoint:

@@ -151,6 +151,10 @@ int smx_ctx_create2d(const smx_mesh2d *mesh, const smx_sources2d *sources,
 typedef struct smx_options {
   int tile_elems;  /* elements per tile, 1..64 (default 64) */
   int graph_steps; /* time steps per replayed HIP graph, 1..256 (default 8) */
+  /* 1: tiles re-form per-point geometry from the element control nodes when
+   * the device reproduces every stored value bit for bit (fewer HBM bytes,
+   * more registers); 0 (default): read the stored per-point geometry. */
+  int geometry_from_nodes;
 } smx_options;
 
 /* smx_ctx_create2d with layout options (opts may be NULL). */
@@ -189,6 +193,12 @@ int smx_ctx_create3d(const smx_mesh3d *mesh, const smx_sources3d *sources,
                      const smx_receivers3d *receivers, double dt, int nstep,
                      int precision, smx_ctx **out);
 
+/* smx_ctx_create3d with options (opts may be NULL; tile_elems is ignored). */
+int smx_ctx_create3d_opts(const smx_mesh3d *mesh, const smx_sources3d *sources,
+                          const smx_receivers3d *receivers, double dt,
+                          int nstep, int precision, const smx_options *opts,
+                          smx_ctx **out);
+
 /* Frees every device buffer and the stream of the context. */
 int smx_ctx_destroy(smx_ctx *ctx);
 
@@ -209,6 +219,13 @@ double smx_ctx_t0(const smx_ctx *ctx);
 /* Elements whose tiles read element-constant geometry / moduli (one value
  * per element instead of one per GLL point; see the assembly's tile flags). */
 int smx_ctx_compact(const smx_ctx *ctx, int *geom_elements, int *mat_elements);
+
+/* Where the stiffness kernels take the geometry from: 0 stored per GLL point,
+ * 1 stored per element (affine elements), 2 re-formed on the device from the
+ * control nodes with the host's FP32 operations (used only when a set-up
+ * check reproduced every stored value bit for bit; `mismatches` counts the
+ * values it did not).  A measurement aid with no reference counterpart. */
+int smx_ctx_geometry(const smx_ctx *ctx, int *mode, long long *mismatches);
 
 /* Mass matrix + Stacey mass term, then its inverse.
  * Replaces domain_kernels::initialize(dt)

@@ -27,7 +27,7 @@ EXPORTS = [
     "smx_time_stiffness", "smx_set_model", "smx_run_forward_checkpointed",
     "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
     "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots", "smx_ctx_compact",
-    "smx_ctx_create3d",
+    "smx_ctx_create3d", "smx_ctx_create3d_opts", "smx_ctx_geometry",
 ]
 
 
@@ -71,7 +71,8 @@ class SmxReceivers3D(ctypes.Structure):
 
 
 class SmxOptions(ctypes.Structure):
-    _fields_ = [("tile_elems", ctypes.c_int), ("graph_steps", ctypes.c_int)]
+    _fields_ = [("tile_elems", ctypes.c_int), ("graph_steps", ctypes.c_int),
+                ("geometry_from_nodes", ctypes.c_int)]
 
 
 class SmxError(RuntimeError):
@@ -132,8 +133,12 @@ def load():
                                        ctypes.POINTER(SmxOptions), ctypes.POINTER(P)]),
         "smx_ctx_shots": (i, [P, IP, P, P, P]),
         "smx_ctx_compact": (i, [P, IP, IP]),
+        "smx_ctx_geometry": (i, [P, IP, ctypes.POINTER(ctypes.c_longlong)]),
         "smx_ctx_create3d": (i, [ctypes.POINTER(SmxMesh3D), ctypes.POINTER(SmxSources3D),
                                  ctypes.POINTER(SmxReceivers3D), d, i, i, ctypes.POINTER(P)]),
+        "smx_ctx_create3d_opts": (i, [ctypes.POINTER(SmxMesh3D), ctypes.POINTER(SmxSources3D),
+                                      ctypes.POINTER(SmxReceivers3D), d, i, i,
+                                      ctypes.POINTER(SmxOptions), ctypes.POINTER(P)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -576,6 +576,14 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
   // largest term)
   A.geom_e.assign(size_t(5) * ld, 0.0f);
   A.elem_affine.assign(ld, 0);
+  if (ngnod == 4) {
+    A.nodes.assign(size_t(8) * ld, 0.0f);
+    for (int e = 0; e < nspec; ++e)
+      for (int n = 0; n < 4; ++n) {
+        A.nodes[size_t(n) * ld + slot_of[e]] = cx[size_t(e) * 4 + n];
+        A.nodes[size_t(4 + n) * ld + slot_of[e]] = cz[size_t(e) * 4 + n];
+      }
+  }
   {
     float c0[9], c1[9];
     shape_deriv(0.0f, 0.0f, ngnod, c0, c1);
@@ -1105,7 +1113,9 @@ void refresh_compact(HostAssembly2D &A) {
       g = g && A.elem_affine[sl];
       m = m && mat_const[sl];
     }
-    A.tiles[t].flags = (g ? TILE_GEOM_ELEM : 0) | (m ? TILE_MAT_ELEM : 0);
+    // the geometry source chosen by the solver does not depend on the model
+    A.tiles[t].flags = (g ? TILE_GEOM_ELEM : 0) | (m ? TILE_MAT_ELEM : 0) |
+                       (A.tiles[t].flags & TILE_GEOM_NODES);
   }
 }
 
@@ -1165,6 +1175,11 @@ void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
   M.kappa_mu_rho.assign(3 * pld, 0.0f);
   M.geom_e.assign(size_t(5) * ld, 0.0f);
   M.elem_affine.assign(ld, 0);
+  bool all_q4 = true;
+  for (const auto &S : shots)
+    all_q4 = all_q4 && !S.nodes.empty();
+  if (all_q4)
+    M.nodes.assign(size_t(8) * ld, 0.0f);
   M.stacey_slot.assign(ld, -1);
   M.stacey.assign(6 * sp, 0.0f);
   M.src_ispec.assign(std::max(1, nsrc), 0);
@@ -1207,6 +1222,9 @@ void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
       const int d = slot_b + sl;
       M.ecolor[d] = S.ecolor[sl];
       M.elem_affine[d] = S.elem_affine[sl];
+      if (all_q4)
+        for (int c = 0; c < 8; ++c)
+          M.nodes[c * size_t(ld) + d] = S.nodes[c * size_t(sld0) + sl];
       for (int c = 0; c < 5; ++c)
         M.geom_e[c * size_t(ld) + d] = S.geom_e[c * size_t(sld0) + sl];
       M.stacey_slot[d] = S.stacey_slot[sl] >= 0 ? st_b[k] + S.stacey_slot[sl] : -1;

@@ -63,6 +63,9 @@ struct TileInfo {
 // the per-element arrays geom_e / mat_e instead of the per-point ones.
 constexpr int32_t TILE_GEOM_ELEM = 1;
 constexpr int32_t TILE_MAT_ELEM = 2;
+// The tile re-forms its per-point geometry from the elements' four control
+// nodes (set by the solver after a bitwise check on the device).
+constexpr int32_t TILE_GEOM_NODES = 4;
 
 struct HostAssembly2D {
   int nspec = 0, nspec_pad = 0, nglob = 0;
@@ -97,6 +100,9 @@ struct HostAssembly2D {
   // [2][nspec_pad]; elem_affine marks slots whose geometry qualifies.
   std::vector<float> geom_e, mat_e;
   std::vector<uint8_t> elem_affine;
+  // Control nodes of 4-node elements [8][nspec_pad] (x0..x3, z0..z3), FP32;
+  // empty for 9-node meshes.
+  std::vector<float> nodes;
 
   // Stacey: slot -> boundary slot (-1 if none); per slot [6][NPT][nst_pad]
   // (n0, n1, |n|, edge weight (0 if the point is not on an absorbing edge),

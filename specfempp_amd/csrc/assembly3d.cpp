@@ -184,6 +184,14 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
       A.elem_slot[e] = fill[color[e]]++;
   }
   const int ld = nspec;
+  A.nodes.assign(size_t(24) * ld, 0.0f);
+  for (int e = 0; e < nspec; ++e)
+    for (int n = 0; n < 8; ++n) {
+      const size_t o = size_t(A.elem_slot[e]) * 24;
+      A.nodes[o + n] = cx[e * 8 + n];
+      A.nodes[o + 8 + n] = cy[e * 8 + n];
+      A.nodes[o + 16 + n] = cz[e * 8 + n];
+    }
 
   // ---- geometry at the GLL points
   float dN[NPT3][3][8];
@@ -203,14 +211,14 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
     for (int k = 0; k < 9; ++k)
       scale_d = std::max(scale_d, std::fabs(gc.t[k]));
     for (int k = 0; k < 10; ++k)
-      gel[size_t(k) * ld + s] = gc.t[k];
+      gel[size_t(s) * 10 + k] = gc.t[k];
     for (int p = 0; p < NPT3; ++p) {
       const G3 g = jac3(&cx[e * 8], &cy[e * 8], &cz[e * 8], dN[p]);
       if (!(g.t[9] > 0))
         throw SmxError(SMX_ERR_MESH,
                        "non-positive Jacobian in element " + std::to_string(e));
       for (int k = 0; k < 10; ++k) {
-        gpt[(size_t(k) * NPT3 + p) * ld + s] = g.t[k];
+        gpt[(size_t(s) * 10 + k) * NPT3 + p] = g.t[k];
         const float sc = k < 9 ? scale_d : std::fabs(gc.t[9]);
         if (std::fabs(g.t[k] - gc.t[k]) > 4e-7f * sc)
           all_affine = false;
@@ -220,15 +228,12 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
   A.geom_elem = all_affine;
   A.geom = all_affine ? std::move(gel) : std::move(gpt);
   std::vector<float> jac_pt; // J per point for the mass matrix
-  if (!all_affine) {
-    jac_pt.assign(A.geom.begin() + size_t(9) * NPT3 * ld,
-                  A.geom.begin() + size_t(10) * NPT3 * ld);
-  } else {
-    jac_pt.resize(size_t(NPT3) * ld);
+  jac_pt.resize(size_t(NPT3) * ld);
+  for (int s = 0; s < ld; ++s)
     for (int p = 0; p < NPT3; ++p)
-      for (int s = 0; s < ld; ++s)
-        jac_pt[size_t(p) * ld + s] = A.geom[size_t(9) * ld + s];
-  }
+      jac_pt[size_t(s) * NPT3 + p] =
+          all_affine ? A.geom[size_t(s) * 10 + 9]
+                     : A.geom[(size_t(s) * 10 + 9) * NPT3 + p];
 
   // ---- numbering: equal GLL coordinates are one point; ids by first touch
   // in Morton element order
@@ -269,7 +274,7 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
         auto it = ids.emplace((kx << 42) | (ky << 21) | kz, next);
         if (it.second)
           ++next;
-        A.ibool[size_t(p) * ld + s] = it.first->second;
+        A.ibool[size_t(s) * NPT3 + p] = it.first->second;
       }
     }
     A.nglob = next;
@@ -284,19 +289,20 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
   for (int e = 0; e < nspec; ++e) {
     const int s = A.elem_slot[e];
     for (int p = 0; p < NPT3; ++p) {
-      const size_t i = size_t(e) * NPT3 + p, o = size_t(p) * ld + s;
+      const size_t i = size_t(e) * NPT3 + p;
+      const size_t o3 = size_t(s) * 3 * NPT3 + p, o2 = size_t(s) * 2 * NPT3 + p;
       const float k = m.kappa[i], mu = m.mu[i], r = m.rho[i];
       if (!(r > 0) || !(mu >= 0) || !(k > 0))
         throw SmxError(SMX_ERR_ARG,
                        "non-physical material at element " + std::to_string(e));
-      A.kappa_mu_rho[o] = k;
-      A.kappa_mu_rho[size_t(NPT3) * ld + o] = mu;
-      A.kappa_mu_rho[size_t(2) * NPT3 * ld + o] = r;
-      mpt[o] = k + static_cast<float>(4.0 / 3.0) * mu;
-      mpt[size_t(NPT3) * ld + o] = mu;
-      mpt64[o] = double(k) + 4.0 / 3.0 * double(mu);
-      mpt64[size_t(NPT3) * ld + o] = mu;
-      if (mpt[o] != mpt[s] || mu != mpt[size_t(NPT3) * ld + s])
+      A.kappa_mu_rho[o3] = k;
+      A.kappa_mu_rho[o3 + NPT3] = mu;
+      A.kappa_mu_rho[o3 + 2 * NPT3] = r;
+      mpt[o2] = k + static_cast<float>(4.0 / 3.0) * mu;
+      mpt[o2 + NPT3] = mu;
+      mpt64[o2] = double(k) + 4.0 / 3.0 * double(mu);
+      mpt64[o2 + NPT3] = mu;
+      if (mpt[o2] != mpt[o2 - p] || mu != mpt[o2 - p + NPT3])
         mat_const = false;
     }
   }
@@ -306,8 +312,8 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
     A.mat64.assign(size_t(2) * ld, 0.0);
     for (int c = 0; c < 2; ++c)
       for (int s = 0; s < ld; ++s) {
-        A.mat[size_t(c) * ld + s] = mpt[size_t(c) * NPT3 * ld + s];
-        A.mat64[size_t(c) * ld + s] = mpt64[size_t(c) * NPT3 * ld + s];
+        A.mat[size_t(s) * 2 + c] = mpt[(size_t(s) * 2 + c) * NPT3];
+        A.mat64[size_t(s) * 2 + c] = mpt64[(size_t(s) * 2 + c) * NPT3];
       }
   } else {
     A.mat = std::move(mpt);
@@ -335,8 +341,8 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
       const int e = kv.first, s = A.elem_slot[e];
       A.stacey_slot[s] = b;
       for (int p = 0; p < NPT3; ++p) {
-        A.stacey[(size_t(0) * NPT3 + p) * nst + b] = 1.0f;
-        A.stacey[(size_t(3) * NPT3 + p) * nst + b] = 1.0f;
+        A.stacey[(size_t(b) * 7 + 0) * NPT3 + p] = 1.0f;
+        A.stacey[(size_t(b) * 7 + 3) * NPT3 + p] = 1.0f;
       }
       for (int f : kv.second)
         for (int iz = 0; iz < NGLL; ++iz)
@@ -361,19 +367,19 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
               const float n0 = sg * g.t[3 * d] * g.t[9];
               const float n1 = sg * g.t[3 * d + 1] * g.t[9];
               const float n2 = sg * g.t[3 * d + 2] * g.t[9];
-              const size_t o = size_t(p) * ld + s;
-              const float kk = A.kappa_mu_rho[o],
-                          mm = A.kappa_mu_rho[size_t(NPT3) * ld + o],
-                          rr = A.kappa_mu_rho[size_t(2) * NPT3 * ld + o];
+              const size_t o3 = size_t(s) * 3 * NPT3 + p;
+              const float kk = A.kappa_mu_rho[o3],
+                          mm = A.kappa_mu_rho[o3 + NPT3],
+                          rr = A.kappa_mu_rho[o3 + 2 * NPT3];
               const float l2m = kk + static_cast<float>(4.0 / 3.0) * mm;
               const float vals[7] = {n0, n1, n2,
                                      std::sqrt(n0 * n0 + n1 * n1 + n2 * n2), w,
                                      std::sqrt(l2m / rr) * rr, std::sqrt(rr * mm)};
               for (int c = 0; c < 7; ++c)
-                A.stacey[(size_t(c) * NPT3 + p) * nst + b] = vals[c];
+                A.stacey[(size_t(b) * 7 + c) * NPT3 + p] = vals[c];
               const double l2m64 = double(kk) + 4.0 / 3.0 * double(mm);
-              A.stacey64[size_t(p) * nst + b] = std::sqrt(l2m64 / rr) * rr;
-              A.stacey64[(size_t(NPT3) + p) * nst + b] = std::sqrt(double(rr) * mm);
+              A.stacey64[(size_t(b) * 2) * NPT3 + p] = std::sqrt(l2m64 / rr) * rr;
+              A.stacey64[(size_t(b) * 2 + 1) * NPT3 + p] = std::sqrt(double(rr) * mm);
             }
       ++b;
     }
@@ -395,19 +401,18 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
         for (int iy = 0; iy < NGLL; ++iy)
           for (int ix = 0; ix < NGLL; ++ix) {
             const int p = pidx(iz, iy, ix);
-            const size_t o = size_t(p) * ld + s;
-            const float f = q.w[iz] * q.w[iy] * q.w[ix] * jac_pt[o];
-            const float rr = A.kappa_mu_rho[size_t(2) * NPT3 * ld + o];
+            const float f = q.w[iz] * q.w[iy] * q.w[ix] * jac_pt[size_t(s) * NPT3 + p];
+            const float rr = A.kappa_mu_rho[(size_t(s) * 3 + 2) * NPT3 + p];
             double add[3] = {double(rr * f), double(rr * f), double(rr * f)};
             double add64[3] = {double(rr) * f, double(rr) * f, double(rr) * f};
-            if (b >= 0 && A.stacey[(size_t(4) * NPT3 + p) * nst + b] != 0.0f) {
+            if (b >= 0 && A.stacey[(size_t(b) * 7 + 4) * NPT3 + p] != 0.0f) {
               const float *S = A.stacey.data();
-              auto at = [&](int c) { return S[(size_t(c) * NPT3 + p) * nst + b]; };
+              auto at = [&](int c) { return S[(size_t(b) * 7 + c) * NPT3 + p]; };
               const float nn[3] = {at(0), at(1), at(2)}, nrm = at(3), w = at(4);
               const float vf = -1.0f * dt * 0.5f;
               const float vn = vf * nn[0] + vf * nn[1] + vf * nn[2];
-              const double P64 = A.stacey64[size_t(p) * nst + b],
-                           S64 = A.stacey64[(size_t(NPT3) + p) * nst + b];
+              const double P64 = A.stacey64[(size_t(b) * 2) * NPT3 + p],
+                           S64 = A.stacey64[(size_t(b) * 2 + 1) * NPT3 + p];
               for (int c = 0; c < 3; ++c) {
                 const float fc = ((vn * nn[c] / (nrm * nrm)) * (at(5) - at(6))) + vf * at(6);
                 add[c] += double(-1.0f * fc * nrm * w);
@@ -416,7 +421,7 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
                 add64[c] += -fc64 * nrm * w;
               }
             }
-            const int g = A.ibool[o];
+            const int g = A.ibool[size_t(s) * NPT3 + p];
             for (int c = 0; c < 3; ++c) {
               M[size_t(c) * A.nglob + g] += add[c];
               M64[size_t(c) * A.nglob + g] += add64[c];
@@ -634,7 +639,7 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
       for (int iy = 0; iy < NGLL; ++iy)
         for (int ix = 0; ix < NGLL; ++ix) {
           const int p = pidx(iz, iy, ix);
-          A.rec_g[size_t(r) * NPT3 + p] = A.ibool[size_t(p) * ld + s];
+          A.rec_g[size_t(r) * NPT3 + p] = A.ibool[size_t(s) * NPT3 + p];
           A.rec_h[size_t(r) * NPT3 + p] = h[0][ix] * h[1][iy] * h[2][iz];
         }
   }

@@ -11,9 +11,11 @@
 // Device layout.  Elements are grouped by colour (no two elements of a
 // colour share a GLL point; 8 colours on a structured hex box), each colour
 // contiguous and in Morton order of the element centroids; slot = position
-// in that order.  Per-element arrays are [point][slot] (point p = (iz*5 +
-// iy)*5 + ix).  Global points are numbered by first touch in Morton element
-// order.  Fields are three component planes [3][nglob_pad].
+// in that order.  Per-element arrays are element-major, [slot][term][point]
+// (point p = (iz*5 + iy)*5 + ix), so that the 125 threads of an element's
+// workgroup read contiguous bytes.  Global points are numbered by first touch
+// in Morton element order.  Fields are three component planes
+// [3][nglob_pad].
 #pragma once
 
 #include <cstdint>
@@ -33,25 +35,28 @@ struct HostAssembly3D {
   int ncolors = 0;
   std::vector<int32_t> color_off;
   std::vector<int32_t> elem_slot; // mesh element -> slot
-  std::vector<int32_t> ibool;     // [NPT3][nspec] global ids
-  // geometry: per element [10][nspec] (xix, xiy, xiz, etax, etay, etaz,
+  std::vector<int32_t> ibool;     // [nspec][NPT3] global ids
+  // geometry: per element [nspec][10] (xix, xiy, xiz, etax, etay, etaz,
   // gammax, gammay, gammaz, J) when every element is affine (geom_elem),
-  // else per point [10][NPT3][nspec]
+  // else per point [nspec][10][NPT3]
   bool geom_elem = false;
   std::vector<float> geom;
-  // moduli (lambda+2mu, mu): per element [2][nspec] when constant inside
-  // every element (mat_elem), else per point [2][NPT3][nspec]
+  // control-node coordinates [nspec][24] (x0..x7, y0..y7, z0..z7) in FP32:
+  // the kernels can re-form the per-point geometry from them
+  std::vector<float> nodes;
+  // moduli (lambda+2mu, mu): per element [nspec][2] when constant inside
+  // every element (mat_elem), else per point [nspec][2][NPT3]
   bool mat_elem = false;
   std::vector<float> mat;
-  std::vector<float> kappa_mu_rho; // [3][NPT3][nspec] model
+  std::vector<float> kappa_mu_rho; // [nspec][3][NPT3] model
   // FP64 copies formed in double (for FP64 contexts)
   std::vector<double> mat64;
-  // absorbing faces: slot -> boundary slot (-1); [7][NPT3][nst] (n0, n1,
+  // absorbing faces: slot -> boundary slot (-1); [nst][7][NPT3] (n0, n1,
   // n2, |n|, face weight (0 off the face), rho vp, rho vs)
   std::vector<int32_t> stacey_slot;
   int nstacey = 0;
   std::vector<float> stacey;
-  std::vector<double> stacey64; // impedances formed in double: [2][NPT3][nst]
+  std::vector<double> stacey64; // impedances formed in double: [nst][2][NPT3]
   // mass matrix inverse [3][nglob] (with the Stacey term), and in double
   std::vector<float> minv;
   std::vector<double> minv64;

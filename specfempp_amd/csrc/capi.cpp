@@ -85,6 +85,9 @@ smx::AssemblyOptions assembly_options(const smx_options *o) {
 int graph_steps(const smx_options *o) {
   return (o && o->graph_steps) ? o->graph_steps : 8;
 }
+bool geometry_from_nodes(const smx_options *o) {
+  return o && o->geometry_from_nodes;
+}
 } // namespace
 
 int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
@@ -99,7 +102,8 @@ int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
     smx::assemble2d(*mesh, *sources, *receivers, dt, nstep, *a,
                     assembly_options(opts));
     auto ctx = std::make_unique<smx_ctx>();
-    ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts));
+    ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts),
+                                      geometry_from_nodes(opts));
     *out = ctx.release();
   });
 }
@@ -145,7 +149,8 @@ int smx_ctx_create2d_batch(int nshots, const smx_mesh2d *meshes,
     smx::merge_assemblies(shots, *a);
     shots.clear();
     auto ctx = std::make_unique<smx_ctx>();
-    ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts));
+    ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts),
+                                      geometry_from_nodes(opts));
     *out = ctx.release();
   });
 }
@@ -167,6 +172,14 @@ int smx_ctx_shots(const smx_ctx *ctx, int *nshots, int *elem_off, int *rec_off,
 int smx_ctx_create3d(const smx_mesh3d *mesh, const smx_sources3d *sources,
                      const smx_receivers3d *receivers, double dt, int nstep,
                      int precision, smx_ctx **out) {
+  return smx_ctx_create3d_opts(mesh, sources, receivers, dt, nstep, precision,
+                               nullptr, out);
+}
+
+int smx_ctx_create3d_opts(const smx_mesh3d *mesh, const smx_sources3d *sources,
+                          const smx_receivers3d *receivers, double dt,
+                          int nstep, int precision, const smx_options *opts,
+                          smx_ctx **out) {
   return guarded([&] {
     if (!mesh || !sources || !receivers || !out)
       throw smx::SmxError(SMX_ERR_ARG, "null argument");
@@ -174,7 +187,8 @@ int smx_ctx_create3d(const smx_mesh3d *mesh, const smx_sources3d *sources,
     auto a = std::make_unique<smx::HostAssembly3D>();
     smx::assemble3d(*mesh, *sources, *receivers, dt, nstep, *a);
     auto ctx = std::make_unique<smx_ctx>();
-    ctx->solver3d = smx::make_solver3d(std::move(a), precision);
+    ctx->solver3d = smx::make_solver3d(std::move(a), precision, graph_steps(opts),
+                                       geometry_from_nodes(opts));
     *out = ctx.release();
   });
 }
@@ -223,6 +237,15 @@ int smx_ctx_compact(const smx_ctx *ctx, int *geom_elements, int *mat_elements) {
     }
     if (geom_elements) *geom_elements = g;
     if (mat_elements) *mat_elements = m;
+  });
+}
+
+int smx_ctx_geometry(const smx_ctx *ctx, int *mode, long long *mismatches) {
+  return guarded([&] {
+    auto *c = const_cast<smx_ctx *>(ctx);
+    if (auto *s3 = S3(c))
+      return s3->geometry_mode(mode, mismatches);
+    S(c).geometry_mode(mode, mismatches);
   });
 }
 

@@ -52,7 +52,63 @@ namespace smx {
 template <typename real> struct QuadArgs {
   real hp[NGLL][NGLL]; // hp[i][j] = l'_j(xi_i)
   real w[NGLL];
+  float xi[NGLL]; // GLL nodes (FP32, as the host assembly uses them)
 };
+
+// The host assembly's per-point geometry of a 4-node element at (xi, gamma),
+// re-formed from its control nodes with the same FP32 operations in the same
+// order and no contraction (shape_function_derivatives2d.cpp:7-20,
+// core/specfem/jacobian/dim2/jacobian.cpp:25-51).  nd points at node 0 of
+// the element's slot in [8][ld] (x0..x3, z0..z3).
+__device__ __forceinline__ void geom_from_nodes2(const float *__restrict__ nd,
+                                                 int ld, float xi, float ga,
+                                                 float &xix, float &xiz,
+                                                 float &gax, float &gaz,
+                                                 float &jac) {
+#pragma clang fp contract(off)
+  const float sp = xi + 1, sm = xi - 1, tp = ga + 1, tm = ga - 1;
+  const float dxi[4] = {0.25f * tm, -0.25f * tm, 0.25f * tp, -0.25f * tp};
+  const float dga[4] = {0.25f * sm, -0.25f * sp, 0.25f * sp, -0.25f * sm};
+  float xxi = 0.0f, zxi = 0.0f, xga = 0.0f, zga = 0.0f;
+#pragma unroll
+  for (int n = 0; n < 4; ++n) {
+    const float cx = nd[size_t(n) * ld], cz = nd[size_t(4 + n) * ld];
+    xxi += dxi[n] * cx;
+    zxi += dxi[n] * cz;
+    xga += dga[n] * cx;
+    zga += dga[n] * cz;
+  }
+  jac = xxi * zga - xga * zxi;
+  xix = zga / jac;
+  gax = -zxi / jac;
+  xiz = -xga / jac;
+  gaz = xxi / jac;
+}
+
+// Set-up check: 1 where the re-formed geometry of (point, slot) differs from
+// the stored FP32 value in any bit (slots of padding elements count as equal).
+__global__ void k_geom_check2(const float *__restrict__ nodes,
+                              const float *__restrict__ geom,
+                              const uint8_t *__restrict__ ecolor, int ld,
+                              QuadArgs<float> q,
+                              unsigned char *__restrict__ bad) {
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(NPT) * ld)
+    return;
+  const int p = int(t / ld), slot = int(t % ld);
+  if (ecolor[slot] == 0xff) {
+    bad[t] = 0;
+    return;
+  }
+  float g[5];
+  geom_from_nodes2(nodes + slot, ld, q.xi[p % NGLL], q.xi[p / NGLL], g[0],
+                   g[1], g[2], g[3], g[4]);
+  const size_t pld = size_t(NPT) * ld;
+  bool diff = false;
+  for (int c = 0; c < 5; ++c)
+    diff = diff || (__float_as_uint(g[c]) != __float_as_uint(geom[c * pld + t]));
+  bad[t] = diff ? 1 : 0;
+}
 
 template <typename real> struct Vec2;
 template <> struct Vec2<float> {
@@ -152,14 +208,17 @@ __host__ __device__ constexpr size_t tile_lds_bytes(int nloc_max) {
 // The FP32 time-step variant is held to 7 waves per SIMD (at most 73
 // VGPRs) so that register use does not cut below the 5 tiles per CU the LDS
 // footprint allows.
-template <typename real, int MODE>
+// GN: the tiles not flagged element-constant re-form their geometry from
+// the control nodes (instead of reading it per point).
+template <typename real, int MODE, bool GN = false>
 __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
-    amdgpu_waves_per_eu(sizeof(real) == 4 && MODE == MODE_STEP ? 7 : 1, 8)))
-    k_tile(const TileInfo *__restrict__ tiles, int nloc_max,
+    amdgpu_waves_per_eu(sizeof(real) == 4 && MODE == MODE_STEP ? (GN ? 4 : 7) : 1,
+                        8))) k_tile(const TileInfo *__restrict__ tiles, int nloc_max,
            const int *__restrict__ sh_list, const uint16_t *__restrict__ lidx,
            const uint8_t *__restrict__ ecolor, int ncolors,
            const real *__restrict__ geom, const real *__restrict__ mat,
            const real *__restrict__ geom_e, const real *__restrict__ mat_e,
+           const float *__restrict__ nodes,
            const int *__restrict__ sslot, const real *__restrict__ stacey,
            real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
            const real *__restrict__ Minv, real *__restrict__ part,
@@ -278,10 +337,18 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
   for (int iz = 0; iz < NGLL; ++iz) {
     const int p = iz * NGLL + ix;
     const size_t o = size_t(p) * ld + slot;
-    const real *gp = g_el ? geom_e + slot : geom + o;
     const real *mp = m_el ? mat_e + slot : mat + o;
-    const real xix = gp[0], xiz = gp[gst], gax = gp[2 * gst], gaz = gp[3 * gst],
-               jac = gp[4 * gst];
+    real xix, xiz, gax, gaz, jac;
+    if (GN && !g_el) {
+      float a0, a1, a2, a3, a4;
+      geom_from_nodes2(nodes + slot, ld, q.xi[ix], q.xi[iz], a0, a1, a2, a3,
+                       a4);
+      xix = a0, xiz = a1, gax = a2, gaz = a3, jac = a4;
+    } else {
+      const real *gp = g_el ? geom_e + slot : geom + o;
+      xix = gp[0], xiz = gp[gst], gax = gp[2 * gst], gaz = gp[3 * gst];
+      jac = gp[4 * gst];
+    }
     const real l2m = mp[0], mu = mp[mst];
     const real lam = l2m - real(2.0) * mu;
     const real du_xx = xix * dxi_x[iz] + gax * dga_x[iz];
@@ -758,7 +825,7 @@ __device__ __forceinline__ void adj_inject(V2 &ub, int j,
 // assembled in LDS as in k_tile, gradient-density accumulation from rb and
 // the recorded u~, the Stacey impedance terms, and the node update of the
 // tile's exclusive points; shared points get their partial of -K rb.
-template <typename real>
+template <typename real, bool GN = false>
 __global__ void __launch_bounds__(TILE_THREADS)
     k_adj_tile(const TileInfo *__restrict__ tiles, int nloc_max,
                const int *__restrict__ sh_list,
@@ -766,6 +833,7 @@ __global__ void __launch_bounds__(TILE_THREADS)
                const uint8_t *__restrict__ ecolor, int ncolors,
                const real *__restrict__ geom, const real *__restrict__ mat,
                const real *__restrict__ geom_e, const real *__restrict__ mat_e,
+               const float *__restrict__ nodes,
                const int *__restrict__ sslot, const real *__restrict__ stacey,
                real *__restrict__ RB0, real *__restrict__ RB1,
                real *__restrict__ UB, real *__restrict__ VB,
@@ -894,10 +962,18 @@ __global__ void __launch_bounds__(TILE_THREADS)
   for (int iz = 0; iz < NGLL; ++iz) {
     const int p = iz * NGLL + ix;
     const size_t o = size_t(p) * ld + slot;
-    const real *gp = g_el ? geom_e + slot : geom + o;
     const real *mp = m_el ? mat_e + slot : mat + o;
-    const real xix = gp[0], xiz = gp[gst], gax = gp[2 * gst], gaz = gp[3 * gst],
-               jac = gp[4 * gst];
+    real xix, xiz, gax, gaz, jac;
+    if (GN && !g_el) {
+      float a0, a1, a2, a3, a4;
+      geom_from_nodes2(nodes + slot, ld, q.xi[ix], q.xi[iz], a0, a1, a2, a3,
+                       a4);
+      xix = a0, xiz = a1, gax = a2, gaz = a3, jac = a4;
+    } else {
+      const real *gp = g_el ? geom_e + slot : geom + o;
+      xix = gp[0], xiz = gp[gst], gax = gp[2 * gst], gaz = gp[3 * gst];
+      jac = gp[4 * gst];
+    }
     const real l2m = mp[0], mu = mp[mst];
     const real lam = l2m - real(2.0) * mu;
     const real du_xx = xix * dxi_x[iz] + gax * dga_x[iz];
@@ -1085,7 +1161,8 @@ __global__ void k_adj_inject(real *__restrict__ UB,
 // ========================================================================
 template <typename real> class Solver2D final : public Solver2DBase {
 public:
-  Solver2D(std::unique_ptr<HostAssembly2D> a, int graph_steps)
+  Solver2D(std::unique_ptr<HostAssembly2D> a, int graph_steps,
+           bool nodes_geometry)
       : asm_(std::move(a)) {
     gsteps_ = graph_steps;
     const HostAssembly2D &A = *asm_;
@@ -1136,6 +1213,13 @@ public:
     SMX_HIP(hipFuncSetAttribute(
         reinterpret_cast<const void *>(&k_adj_tile<real>),
         hipFuncAttributeMaxDynamicSharedMemorySize, int(lds_bytes_)));
+    for (const void *f : {reinterpret_cast<const void *>(&k_tile<real, MODE_STEP, true>),
+                          reinterpret_cast<const void *>(&k_tile<real, MODE_FINAL, true>),
+                          reinterpret_cast<const void *>(&k_tile<real, MODE_ASSEMBLE, true>),
+                          reinterpret_cast<const void *>(&k_tile<real, MODE_RECORD, true>),
+                          reinterpret_cast<const void *>(&k_adj_tile<real, true>)})
+      SMX_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  int(lds_bytes_)));
     // [0] step of the forward / recompute, [1] its copy for k_shared,
     // [2] step of the adjoint, [3] its copy for k_adj_shared
     step_ = alloc<int>(4);
@@ -1147,6 +1231,7 @@ public:
         std::max<size_t>(size_t(ntypes) * A.nrec * NPT * 2, 2));
     for (int i = 0; i < NGLL; ++i) {
       quad_.w[i] = A.quad.w[i];
+      quad_.xi[i] = A.quad.xi[i];
       for (int j = 0; j < NGLL; ++j)
         quad_.hp[i][j] = A.quad.hprime[i][j];
     }
@@ -1159,6 +1244,8 @@ public:
     dt2o2_ = A.dt2_over2;
     field(SMX_FIELD_FORWARD);
     upload_material();
+    if (nodes_geometry && !A.nodes.empty())
+      try_geometry_from_nodes();
     SMX_HIP(hipStreamSynchronize(stream_));
   }
 
@@ -1542,6 +1629,13 @@ public:
     }
   }
 
+  void geometry_mode(int *mode, long long *mismatches) const override {
+    if (mode)
+      *mode = gmode_;
+    if (mismatches)
+      *mismatches = geom_mismatch_;
+  }
+
   void last_adjoint_stats(double *fwd_ms, double *adj_ms) override {
     float tf = 0, ta = 0;
     if (have_adj_stats_) {
@@ -1633,10 +1727,10 @@ private:
                     real *rec = nullptr) {
     const HostAssembly2D &A = *asm_;
     const int ntypes = int(A.seis_types.size());
-    k_tile<real, MODE><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_,
-                         stream_>>>(
+    auto kern = gmode_ == 2 ? &k_tile<real, MODE, true> : &k_tile<real, MODE, false>;
+    kern<<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_, stream_>>>(
         tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
-        mat_, geom_e_, mat_e_, sslot_, stacey_, U, V, Aout, minv_,
+        mat_, geom_e_, mat_e_, nodes_, sslot_, stacey_, U, V, Aout, minv_,
         part ? part : part_, src_li_,
         src_idx_, src_L_, stf_, std::max(1, A.nsrc), trec_li_, trec_entry_,
         rent_h_, terms, A.nrec * NPT, stypes_, ntypes, A.nsbs, step_ptr, istep,
@@ -1657,6 +1751,49 @@ private:
     SMX_HIP(hipGetLastError());
   }
   long rec_stride() const { return long(n2_ * sizeof(real)); }
+
+  // Tiles whose geometry is not element-constant re-form it from the control
+  // nodes when the device reproduces every stored per-point value bit for bit
+  // (checked once here); otherwise they keep reading the stored values.
+  void try_geometry_from_nodes() {
+    HostAssembly2D &A = *asm_;
+    const int ld = A.nspec_pad;
+    const size_t n = size_t(NPT) * ld;
+    nodes_ = alloc<float>(A.nodes.size());
+    SMX_HIP(hipMemcpyAsync(nodes_, A.nodes.data(), A.nodes.size() * sizeof(float),
+                           hipMemcpyHostToDevice, stream_));
+    float *gf = nullptr;
+    unsigned char *bad = nullptr;
+    SMX_HIP(hipMalloc(&gf, size_t(5) * n * sizeof(float)));
+    SMX_HIP(hipMalloc(&bad, n));
+    SMX_HIP(hipMemcpyAsync(gf, A.geom.data(), size_t(5) * n * sizeof(float),
+                           hipMemcpyHostToDevice, stream_));
+    QuadArgs<float> qf{};
+    for (int i = 0; i < NGLL; ++i)
+      qf.xi[i] = A.quad.xi[i];
+    k_geom_check2<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_>>>(
+        nodes_, gf, ecolor_, ld, qf, bad);
+    SMX_HIP(hipGetLastError());
+    std::vector<unsigned char> h(n);
+    SMX_HIP(hipMemcpyAsync(h.data(), bad, n, hipMemcpyDeviceToHost, stream_));
+    SMX_HIP(hipStreamSynchronize(stream_));
+    (void)hipFree(gf);
+    (void)hipFree(bad);
+    long long nbad = 0;
+    for (unsigned char b : h)
+      nbad += b;
+    geom_mismatch_ = nbad;
+    if (nbad != 0)
+      return;
+    gmode_ = 2;
+    for (auto &t : A.tiles)
+      if (!(t.flags & TILE_GEOM_ELEM))
+        t.flags |= TILE_GEOM_NODES;
+    SMX_HIP(hipMemcpyAsync(tiles_, A.tiles.data(),
+                           A.tiles.size() * sizeof(TileInfo),
+                           hipMemcpyHostToDevice, stream_));
+    SMX_HIP(hipStreamSynchronize(stream_));
+  }
 
   // Material arrays on the device.  An FP32 context uses the FP32 values
   // the host formed (the reference's rounding); an FP64 context takes
@@ -1696,9 +1833,11 @@ private:
   // step_[2]; own partials buffer, so that it may overlap a recompute).
   void adjoint_step(Field &B, real *rec, hipStream_t st) {
     const HostAssembly2D &A = *asm_;
-    k_adj_tile<real><<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_, st>>>(
+    auto kern = gmode_ == 2 ? &k_adj_tile<real, true> : &k_adj_tile<real, false>;
+    kern<<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_, st>>>(
         tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
-        mat_, geom_e_, mat_e_, sslot_, stacey_, rb0_, rb1_, B.u, B.v, minv_,
+        mat_, geom_e_, mat_e_, nodes_, sslot_, stacey_, rb0_, rb1_, B.u, B.v,
+        minv_,
         part_adj_, rec,
         rec_stride(), rec_every_, bidx_, bsum_, gd_, ge_, racc_, sp_, ss_,
         step_ + 2, A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_);
@@ -1851,6 +1990,9 @@ private:
   uint16_t *lidx_ = nullptr;
   uint8_t *ecolor_ = nullptr;
   real *geom_e_ = nullptr, *mat_e_ = nullptr;
+  float *nodes_ = nullptr;
+  int gmode_ = 0;
+  long long geom_mismatch_ = 0;
   real *geom_ = nullptr, *mat_ = nullptr, *kmr_ = nullptr, *stacey_ = nullptr,
        *src_L_ = nullptr, *stf_ = nullptr, *rent_h_ = nullptr,
        *minv_ = nullptr, *part_ = nullptr, *seis_ = nullptr,
@@ -1882,13 +2024,16 @@ private:
 };
 
 std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
-                                            int precision, int graph_steps) {
+                                            int precision, int graph_steps,
+                                            bool nodes_geometry) {
   if (graph_steps < 1 || graph_steps > 256)
     throw SmxError(SMX_ERR_ARG, "graph_steps must lie in [1, 256]");
   if (precision == 4)
-    return std::make_unique<Solver2D<float>>(std::move(a), graph_steps);
+    return std::make_unique<Solver2D<float>>(std::move(a), graph_steps,
+                                             nodes_geometry);
   if (precision == 8)
-    return std::make_unique<Solver2D<double>>(std::move(a), graph_steps);
+    return std::make_unique<Solver2D<double>>(std::move(a), graph_steps,
+                                              nodes_geometry);
   throw SmxError(SMX_ERR_ARG, "precision must be 4 or 8");
 }
 

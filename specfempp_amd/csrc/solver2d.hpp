@@ -47,10 +47,15 @@ public:
   // Gradient [3][nspec][5][5] (kappa, mu, rho) as double.
   virtual void get_gradient(double *host) = 0;
   virtual void last_adjoint_stats(double *forward_ms, double *adjoint_ms) = 0;
+  // Geometry source of the tile kernels: 0 per point, 1 per element (some
+  // tiles), 2 re-formed from the control nodes; mismatches as in 3D.
+  virtual void geometry_mode(int *mode, long long *mismatches) const = 0;
 };
 
-// graph_steps: time steps per replayed HIP graph.
+// graph_steps: time steps per replayed HIP graph; nodes_geometry: re-form
+// per-point geometry from the control nodes when bitwise reproducible.
 std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
-                                            int precision, int graph_steps = 8);
+                                            int precision, int graph_steps = 8,
+                                            bool nodes_geometry = false);
 
 } // namespace smx

@@ -57,12 +57,87 @@ __device__ __forceinline__ int p3(int iz, int iy, int ix) {
   return (iz * NGLL + iy) * NGLL + ix;
 }
 
-// One element of the colour pass.  GE / ME: geometry / moduli stored per
-// element instead of per GLL point.
-template <typename real, bool GE, bool ME>
+// Where an element pass takes its geometry from.
+enum GeomMode : int {
+  GEOM_POINT = 0, // stored per GLL point
+  GEOM_ELEM = 1,  // stored per element (affine elements)
+  GEOM_NODES = 2  // re-formed from the 8 control nodes at every point
+};
+
+// The host assembly's per-point geometry, re-formed from the element's
+// control nodes with the same FP32 operations in the same order and no
+// contraction (shape-function derivatives in double rounded to float,
+// core/specfem/shape_function/dim3/shape_function_derivatives3d.cpp:9-40;
+// Jacobian and inverse in float, core/specfem/jacobian/dim3/jacobian.cpp:
+// 27-74).  The solver checks at set-up that it reproduces the stored values
+// bit for bit before using it.
+__device__ __noinline__ void geom_from_nodes3(const float *__restrict__ nd,
+                                              float xi, float et, float ga,
+                                              float G[10]) {
+#pragma clang fp contract(off)
+  const double x = xi, e = et, g = ga;
+  float xxi = 0, yxi = 0, zxi = 0, xet = 0, yet = 0, zet = 0, xga = 0,
+        yga = 0, zga = 0;
+#pragma unroll
+  for (int n = 0; n < 8; ++n) {
+    const double sx = (n == 1 || n == 2 || n == 5 || n == 6) ? 1.0 : -1.0;
+    const double se = (n == 2 || n == 3 || n == 6 || n == 7) ? 1.0 : -1.0;
+    const double sg = n >= 4 ? 1.0 : -1.0;
+    const float d0 = static_cast<float>(0.125 * sx * (1.0 + se * e) * (1.0 + sg * g));
+    const float d1 = static_cast<float>(0.125 * se * (1.0 + sx * x) * (1.0 + sg * g));
+    const float d2 = static_cast<float>(0.125 * sg * (1.0 + sx * x) * (1.0 + se * e));
+    const float cx = nd[n], cy = nd[8 + n], cz = nd[16 + n];
+    xxi += d0 * cx;
+    yxi += d0 * cy;
+    zxi += d0 * cz;
+    xet += d1 * cx;
+    yet += d1 * cy;
+    zet += d1 * cz;
+    xga += d2 * cx;
+    yga += d2 * cy;
+    zga += d2 * cz;
+  }
+  const float J = xxi * (yet * zga - yga * zet) - xet * (yxi * zga - yga * zxi) +
+                  xga * (yxi * zet - yet * zxi);
+  G[0] = (yet * zga - yga * zet) / J;
+  G[1] = (xga * zet - xet * zga) / J;
+  G[2] = (xet * yga - xga * yet) / J;
+  G[3] = (yga * zxi - yxi * zga) / J;
+  G[4] = (xxi * zga - xga * zxi) / J;
+  G[5] = (xga * yxi - xxi * yga) / J;
+  G[6] = (yxi * zet - yet * zxi) / J;
+  G[7] = (xet * zxi - xxi * zet) / J;
+  G[8] = (xxi * yet - xet * yxi) / J;
+  G[9] = J;
+}
+
+// Set-up check: 1 where the re-formed geometry of (element, point) differs
+// from the stored FP32 value in any bit.
+__global__ void k3_geom_check(const float *__restrict__ nodes,
+                              const float *__restrict__ geom, int nspec,
+                              const float *__restrict__ xi,
+                              unsigned char *__restrict__ bad) {
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(nspec) * NPT3)
+    return;
+  const int slot = int(t / NPT3), p = int(t % NPT3);
+  const int iz = p / (NGLL * NGLL), iy = (p / NGLL) % NGLL, ix = p % NGLL;
+  float G[10];
+  geom_from_nodes3(nodes + size_t(slot) * 24, xi[ix], xi[iy], xi[iz], G);
+  bool diff = false;
+  for (int k = 0; k < 10; ++k)
+    diff = diff || (__float_as_uint(G[k]) !=
+                    __float_as_uint(geom[(size_t(slot) * 10 + k) * NPT3 + p]));
+  bad[t] = diff ? 1 : 0;
+}
+
+// One element of the colour pass.  GM: where the geometry comes from
+// (GeomMode); ME: moduli stored per element instead of per GLL point.
+template <typename real, int GM, bool ME>
 __global__ void __launch_bounds__(ELEM_THREADS)
     k3_elem(int slot0, const int *__restrict__ ibool, int ld,
-            const real *__restrict__ geom, const real *__restrict__ mat,
+            const real *__restrict__ geom, const float *__restrict__ nodes,
+            Quad3<float> xi3, const real *__restrict__ mat,
             const int *__restrict__ sslot, const real *__restrict__ stacey,
             int nst, const real *__restrict__ U, const real *__restrict__ V,
             real *__restrict__ A, long npad, Quad3<real> q) {
@@ -75,7 +150,7 @@ __global__ void __launch_bounds__(ELEM_THREADS)
   const int iz = p / (NGLL * NGLL), iy = (p / NGLL) % NGLL, ix = p % NGLL;
   int g = 0;
   if (act) {
-    g = ibool[size_t(p) * ld + slot];
+    g = ibool[size_t(slot) * NPT3 + p];
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       s_u[c][p] = U[c * npad + g];
@@ -97,10 +172,19 @@ __global__ void __launch_bounds__(ELEM_THREADS)
       dga[c] = a2;
     }
     real G[10];
+    if (GM == GEOM_NODES) {
+      float Gf[10];
+      geom_from_nodes3(nodes + size_t(slot) * 24, xi3.w[ix], xi3.w[iy],
+                       xi3.w[iz], Gf);
 #pragma unroll
-    for (int k = 0; k < 10; ++k)
-      G[k] = GE ? geom[size_t(k) * ld + slot]
-                : geom[(size_t(k) * NPT3 + p) * ld + slot];
+      for (int k = 0; k < 10; ++k)
+        G[k] = Gf[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 10; ++k)
+        G[k] = GM == GEOM_ELEM ? geom[size_t(slot) * 10 + k]
+                               : geom[(size_t(slot) * 10 + k) * NPT3 + p];
+    }
     real du[3][3];
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -108,9 +192,9 @@ __global__ void __launch_bounds__(ELEM_THREADS)
       du[c][1] = G[1] * dxi[c] + G[4] * det[c] + G[7] * dga[c];
       du[c][2] = G[2] * dxi[c] + G[5] * det[c] + G[8] * dga[c];
     }
-    const real L = ME ? mat[slot] : mat[size_t(p) * ld + slot];
-    const real M = ME ? mat[size_t(ld) + slot]
-                      : mat[(size_t(NPT3) + p) * ld + slot];
+    const real L = ME ? mat[size_t(slot) * 2] : mat[size_t(slot) * 2 * NPT3 + p];
+    const real M = ME ? mat[size_t(slot) * 2 + 1]
+                      : mat[(size_t(slot) * 2 + 1) * NPT3 + p];
     const real lam = L - real(2.0) * M;
     real T[3][3];
     T[0][0] = L * du[0][0] + lam * (du[1][1] + du[2][2]);
@@ -146,12 +230,11 @@ __global__ void __launch_bounds__(ELEM_THREADS)
   }
   const int b = sslot[slot];
   if (b >= 0) {
-    const size_t sp = size_t(NPT3) * nst, so = size_t(p) * nst + b;
-    const real wgt = stacey[4 * sp + so];
+    const real *st = stacey + size_t(b) * 7 * NPT3 + p;
+    const real wgt = st[4 * NPT3];
     if (wgt != real(0)) {
-      const real n[3] = {stacey[so], stacey[sp + so], stacey[2 * sp + so]};
-      const real nrm = stacey[3 * sp + so], P = stacey[5 * sp + so],
-                 S = stacey[6 * sp + so];
+      const real n[3] = {st[0], st[NPT3], st[2 * NPT3]};
+      const real nrm = st[3 * NPT3], P = st[5 * NPT3], S = st[6 * NPT3];
       const real v[3] = {V[g], V[npad + g], V[2 * npad + g]};
       const real vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
 #pragma unroll
@@ -177,7 +260,7 @@ __global__ void __launch_bounds__(ELEM_THREADS)
   const int p = threadIdx.x;
   for (int s = 0; s < nsrc; ++s) {
     if (p < NPT3) {
-      const int g = ibool[size_t(p) * ld + src_slot[s]];
+      const int g = ibool[size_t(src_slot[s]) * NPT3 + p];
       const real f = stf[size_t(step) * nsrc + s];
 #pragma unroll
       for (int c = 0; c < 3; ++c)
@@ -283,7 +366,9 @@ __global__ void k3_set_step(int *__restrict__ step_ptr, int value) {
 
 template <typename real> class Solver3D final : public Solver3DBase {
 public:
-  explicit Solver3D(std::unique_ptr<HostAssembly3D> a) : asm_(std::move(a)) {
+  Solver3D(std::unique_ptr<HostAssembly3D> a, int graph_steps,
+           bool nodes_geometry)
+      : asm_(std::move(a)), gsteps_(graph_steps) {
     const HostAssembly3D &A = *asm_;
     SMX3_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
     SMX3_HIP(hipEventCreate(&ev0_));
@@ -291,14 +376,22 @@ public:
     npad_ = (long(A.nglob) + 255) / 256 * 256;
     ibool_ = upload<int>(A.ibool);
     geom_ = upload<real>(A.geom);
+    gmode_ = A.geom_elem ? GEOM_ELEM : GEOM_POINT;
+    for (int i = 0; i < NGLL; ++i)
+      xi3_.w[i] = A.quad.xi[i];
+    if (nodes_geometry && !A.geom_elem)
+      try_geometry_from_nodes();
     mat_ = sizeof(real) == 8 ? upload<real>(A.mat64) : upload<real>(A.mat);
     sslot_ = upload<int>(A.stacey_slot);
     nst_ = std::max(1, A.nstacey);
     {
       std::vector<double> st(A.stacey.begin(), A.stacey.end());
       if (sizeof(real) == 8)
-        std::copy(A.stacey64.begin(), A.stacey64.end(),
-                  st.begin() + size_t(5) * NPT3 * nst_);
+        for (int b = 0; b < A.nstacey; ++b)
+          for (int c = 0; c < 2; ++c)
+            for (int p = 0; p < NPT3; ++p)
+              st[(size_t(b) * 7 + 5 + c) * NPT3 + p] =
+                  A.stacey64[(size_t(b) * 2 + c) * NPT3 + p];
       stacey_ = upload<real>(st);
     }
     {
@@ -353,6 +446,13 @@ public:
 
   void initialize() override { initialized_ = true; }
 
+  void geometry_mode(int *mode, long long *mismatches) const override {
+    if (mode)
+      *mode = gmode_;
+    if (mismatches)
+      *mismatches = static_cast<long long>(geom_mismatch_);
+  }
+
   void run_forward(int istep0, int nstep) override {
     if (!initialized_)
       throw SmxError(SMX_ERR_STATE, "smx_initialize has not been called");
@@ -366,13 +466,13 @@ public:
         U_, V_, A_, 3 * npad_, dt_, dto2_, dt2o2_);
     SMX3_HIP(hipGetLastError());
     const int body = nstep - 1;
-    const int reps = body / kGraphSteps3;
+    const int reps = body / gsteps_;
     if (reps > 0) {
       build_graph();
       for (int r = 0; r < reps; ++r)
         SMX3_HIP(hipGraphLaunch(graph_exec_, stream_));
     }
-    for (int s = reps * kGraphSteps3; s < body; ++s)
+    for (int s = reps * gsteps_; s < body; ++s)
       step<false>();
     step<true>();
     SMX3_HIP(hipEventRecord(ev1_, stream_));
@@ -490,19 +590,58 @@ private:
         continue;
       auto launch = [&](auto kern) {
         kern<<<dim3(cnt), dim3(ELEM_THREADS), 0, stream_>>>(
-            s0, ibool_, A.nspec, geom_, mat_, sslot_, stacey_, nst_, U_, V_,
-            acc, npad_, quad_);
+            s0, ibool_, A.nspec, geom_, nodes_, xi3_, mat_, sslot_, stacey_,
+            nst_, U_, V_, acc, npad_, quad_);
       };
-      if (A.geom_elem && A.mat_elem)
-        launch(k3_elem<real, true, true>);
-      else if (A.geom_elem)
-        launch(k3_elem<real, true, false>);
-      else if (A.mat_elem)
-        launch(k3_elem<real, false, true>);
-      else
-        launch(k3_elem<real, false, false>);
+      const bool me = A.mat_elem;
+      switch (gmode_) {
+      case GEOM_ELEM:
+        me ? launch(k3_elem<real, GEOM_ELEM, true>)
+           : launch(k3_elem<real, GEOM_ELEM, false>);
+        break;
+      case GEOM_NODES:
+        me ? launch(k3_elem<real, GEOM_NODES, true>)
+           : launch(k3_elem<real, GEOM_NODES, false>);
+        break;
+      default:
+        me ? launch(k3_elem<real, GEOM_POINT, true>)
+           : launch(k3_elem<real, GEOM_POINT, false>);
+      }
       SMX3_HIP(hipGetLastError());
     }
+  }
+
+  // Uses the geometry re-formed from the control nodes when the device
+  // reproduces every stored per-point value bit for bit (checked once here);
+  // otherwise the element passes keep reading the stored values.
+  void try_geometry_from_nodes() {
+    const HostAssembly3D &A = *asm_;
+    nodes_ = upload<float>(A.nodes);
+    float *gf = nullptr, *xi = nullptr;
+    unsigned char *bad = nullptr;
+    const size_t n = size_t(A.nspec) * NPT3;
+    SMX3_HIP(hipMalloc(&gf, A.geom.size() * sizeof(float)));
+    SMX3_HIP(hipMalloc(&xi, NGLL * sizeof(float)));
+    SMX3_HIP(hipMalloc(&bad, n));
+    SMX3_HIP(hipMemcpyAsync(gf, A.geom.data(), A.geom.size() * sizeof(float),
+                            hipMemcpyHostToDevice, stream_));
+    SMX3_HIP(hipMemcpyAsync(xi, A.quad.xi, NGLL * sizeof(float),
+                            hipMemcpyHostToDevice, stream_));
+    k3_geom_check<<<dim3(unsigned((n + 255) / 256)), dim3(256), 0, stream_>>>(
+        nodes_, gf, A.nspec, xi, bad);
+    SMX3_HIP(hipGetLastError());
+    std::vector<unsigned char> h(n);
+    SMX3_HIP(hipMemcpyAsync(h.data(), bad, n, hipMemcpyDeviceToHost, stream_));
+    SMX3_HIP(hipStreamSynchronize(stream_));
+    (void)hipFree(gf);
+    (void)hipFree(xi);
+    (void)hipFree(bad);
+    size_t nbad = 0;
+    for (unsigned char b : h)
+      nbad += b;
+    geom_mismatch_ = nbad;
+    if (nbad == 0)
+      gmode_ = GEOM_NODES;
   }
 
   template <bool FINAL> void step() {
@@ -532,7 +671,7 @@ private:
       return;
     SMX3_HIP(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal));
     try {
-      for (int s = 0; s < kGraphSteps3; ++s)
+      for (int s = 0; s < gsteps_; ++s)
         step<false>();
     } catch (...) {
       hipGraph_t g = nullptr;
@@ -546,6 +685,7 @@ private:
   }
 
   std::unique_ptr<HostAssembly3D> asm_;
+  int gsteps_ = kGraphSteps3;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   hipGraph_t graph_ = nullptr;
@@ -555,6 +695,10 @@ private:
   int nst_ = 1;
   int *ibool_ = nullptr, *sslot_ = nullptr, *src_slot_ = nullptr,
       *rec_g_ = nullptr, *step_ = nullptr;
+  int gmode_ = GEOM_POINT;
+  size_t geom_mismatch_ = 0;
+  float *nodes_ = nullptr;
+  Quad3<float> xi3_{}; // GLL nodes in w[] (re-formed geometry)
   real *geom_ = nullptr, *mat_ = nullptr, *stacey_ = nullptr, *minv_ = nullptr,
        *src_array_ = nullptr, *stf_ = nullptr, *rec_h_ = nullptr, *U_ = nullptr,
        *V_ = nullptr, *A_ = nullptr, *seis_ = nullptr, *scratch_a_ = nullptr;
@@ -567,11 +711,16 @@ private:
 };
 
 std::unique_ptr<Solver3DBase> make_solver3d(std::unique_ptr<HostAssembly3D> a,
-                                            int precision) {
+                                            int precision, int graph_steps,
+                                            bool nodes_geometry) {
+  if (graph_steps < 1 || graph_steps > 256)
+    throw SmxError(SMX_ERR_ARG, "graph_steps must lie in [1, 256]");
   if (precision == 4)
-    return std::make_unique<Solver3D<float>>(std::move(a));
+    return std::make_unique<Solver3D<float>>(std::move(a), graph_steps,
+                                             nodes_geometry);
   if (precision == 8)
-    return std::make_unique<Solver3D<double>>(std::move(a));
+    return std::make_unique<Solver3D<double>>(std::move(a), graph_steps,
+                                              nodes_geometry);
   throw SmxError(SMX_ERR_ARG, "precision must be 4 or 8");
 }
 

@@ -24,9 +24,16 @@ public:
                               int *stiffness_launches) = 0;
   // Mean duration (ms) of one step's element passes (all colours).
   virtual double time_stiffness(int n) = 0;
+  // Geometry source of the element passes (0 per point, 1 per element,
+  // 2 re-formed from the control nodes) and, for 2, the number of stored
+  // values the device did not reproduce at set-up (then 0 is used).
+  virtual void geometry_mode(int *mode, long long *mismatches) const = 0;
 };
 
+// graph_steps: time steps per replayed HIP graph; nodes_geometry: re-form
+// per-point geometry from the control nodes when bitwise reproducible.
 std::unique_ptr<Solver3DBase> make_solver3d(std::unique_ptr<HostAssembly3D> a,
-                                            int precision);
+                                            int precision, int graph_steps = 8,
+                                            bool nodes_geometry = false);
 
 } // namespace smx

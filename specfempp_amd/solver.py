@@ -56,16 +56,19 @@ def _structs(mesh, sources, receivers, seismo_types, nsbs, keep):
 def _options(options):
     if options is None:
         return None
-    o = _lib.SmxOptions(int(options.get("tile_elems", 0)), int(options.get("graph_steps", 0)))
+    o = _lib.SmxOptions(int(options.get("tile_elems", 0)), int(options.get("graph_steps", 0)),
+                        int(bool(options.get("geometry_from_nodes", False))))
     return ctypes.byref(o)
 
 
 class Context:
     """An smx_ctx: the device-resident assembly, fields and seismograms.
 
-    ``options`` (dict, optional): ``tile_elems`` (elements per tile, 1..64)
-    and ``graph_steps`` (time steps per replayed graph), as chosen by
-    ``specfempp_amd.autotune``.  ``Context.batch`` builds one context over
+    ``options`` (dict, optional): ``tile_elems`` (elements per tile, 1..64),
+    ``graph_steps`` (time steps per replayed graph), as chosen by
+    ``specfempp_amd.autotune``, and ``geometry_from_nodes`` (re-form the
+    per-point geometry from the control nodes when the device reproduces
+    it bit for bit; off by default).  ``Context.batch`` builds one context over
     many independent shots."""
 
     def __init__(self, mesh, sources, receivers, dt, nstep, *, nsbs=1,

@@ -13,6 +13,7 @@ import numpy as np
 
 from . import _lib
 from ._lib import SMX_A, SMX_FIELD_FORWARD, SMX_MASS_INVERSE, SMX_U, SMX_V, check
+from .solver import _options
 
 
 def _p(a):
@@ -20,10 +21,13 @@ def _p(a):
 
 
 class Context3D:
-    """A 3D smx_ctx: hex8 mesh, sources, receivers and fields on the GPU."""
+    """A 3D smx_ctx: hex8 mesh, sources, receivers and fields on the GPU.
+
+    ``options`` (dict, optional): ``graph_steps`` and ``geometry_from_nodes``
+    as for the 2D ``Context``."""
 
     def __init__(self, mesh, sources, receivers, dt, nstep, *, nsbs=1,
-                 seismo_types=("displacement",), precision=4):
+                 seismo_types=("displacement",), precision=4, options=None):
         lib = _lib.load()
         mesh.validate()
         keep = []
@@ -48,8 +52,9 @@ class Context3D:
                                 _p(arr(receivers.y, np.float64)), _p(arr(receivers.z, np.float64)),
                                 len(types), _p(types), int(nsbs))
         h = ctypes.c_void_p()
-        check(lib.smx_ctx_create3d(ctypes.byref(m), ctypes.byref(s), ctypes.byref(r),
-                                   float(dt), int(nstep), int(precision), ctypes.byref(h)))
+        check(lib.smx_ctx_create3d_opts(ctypes.byref(m), ctypes.byref(s), ctypes.byref(r),
+                                        float(dt), int(nstep), int(precision),
+                                        _options(options), ctypes.byref(h)))
         del keep
         self.lib, self.h = lib, h
         self.precision = precision
@@ -61,6 +66,10 @@ class Context3D:
                                ctypes.byref(nr)))
         self.nspec, self.nglob, self.nsig, self.nrec = ns.value, ng.value, nsig.value, nr.value
         self.t0 = lib.smx_ctx_t0(h)
+        gm, mis = ctypes.c_int(), ctypes.c_longlong()
+        check(lib.smx_ctx_geometry(h, ctypes.byref(gm), ctypes.byref(mis)))
+        self.geometry = {"mode": ("point", "element", "nodes")[gm.value],
+                         "mismatches": mis.value}
 
     def close(self):
         if getattr(self, "h", None):
@@ -111,11 +120,12 @@ class Context3D:
         return t.value
 
 
-def simulate3d(config, *, precision=4, nstep=None):
+def simulate3d(config, *, precision=4, nstep=None, options=None):
     """Runs a forward simulation of a Config3D; returns (seismograms, ctx)."""
     n = config.nstep if nstep is None else int(nstep)
     ctx = Context3D(config.mesh, config.sources, config.receivers, config.dt, n,
-                    nsbs=config.nsbs, seismo_types=config.seismo_types, precision=precision)
+                    nsbs=config.nsbs, seismo_types=config.seismo_types, precision=precision,
+                    options=options)
     ctx.initialize()
     ctx.run_forward(0, n)
     ctx.synchronize()

@@ -43,6 +43,7 @@ def test_3d_halfspace_fixture(oracle_lib):
     _say("3D halfspace fixture (8x6x6 hex), 75 steps, d/v/a")
     s, ctx = simulate3d(cfg)
     assert ctx.nglob == 33 * 25 * 25
+    _say(f"  geometry {ctx.geometry}")
     err_ref = _metric(s, g["traces"])
     ref = oracle_lib.Oracle3D(mesh, src, rec, 0.035, 75).run(types=types)
     errs = [rel_l2(s[:, k], ref[:, k]) for k in range(3)]
@@ -80,25 +81,43 @@ def test_3d_stacey_heterogeneous_matches_oracle(oracle_lib, moment):
                                 precision=8).run()
     e64 = rel_l2(s64, ref64)
     e32 = rel_l2(s, s64)
-    _say(f"  fp32 vs oracle {err:.3e}; fp64 vs fp64 oracle {e64:.3e}; fp32 vs fp64 {e32:.3e}")
+    _say(f"  fp32 vs oracle {err:.3e}; fp64 vs fp64 oracle {e64:.3e}; fp32 vs fp64 {e32:.3e}; "
+         f"geometry {_.geometry}")
     assert err <= 1e-5
     assert e64 <= 1e-5
     assert e32 <= 1e-5
 
 
-def test_3d_absorbing_boundary_removes_energy():
+def _kinetic(ctx):
+    v = ctx.get_field("v").astype(np.float64)
+    m = 1.0 / ctx.get_field("minv").astype(np.float64)
+    return float(np.sum(m * v * v))
+
+
+@pytest.mark.parametrize("absorbing", [True, False])
+def test_3d_absorbing_boundary_removes_energy(absorbing):
+    """Kinetic energy sum(M v^2) at step 1500 against step 300: with Stacey on
+    five faces (free top) the oracle gives 6.6 %; with every face free the
+    energy stays in the box (97 %)."""
     cfg = _stacey_case(nstep=1500)
-    _say("3D Stacey: kinetic energy after the wave leaves the box")
+    if not absorbing:
+        cfg.mesh.abs_ispec = cfg.mesh.abs_ispec[:0]
+        cfg.mesh.abs_face = cfg.mesh.abs_face[:0]
+    _say(f"3D kinetic energy decay, absorbing={absorbing}")
     ctx = Context3D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
     ctx.initialize()
     ctx.run_forward(0, 300)
     ctx.synchronize()
-    v_early = np.linalg.norm(ctx.get_field("v"))
+    e_early = _kinetic(ctx)
     ctx.run_forward(300, 1200)
     ctx.synchronize()
-    v_late = np.linalg.norm(ctx.get_field("v"))
-    _say(f"  |v| at step 300: {v_early:.3e}, at step 1500: {v_late:.3e}")
-    assert np.isfinite(v_late) and v_late < 0.05 * v_early
+    e_late = _kinetic(ctx)
+    _say(f"  E_kin step 300: {e_early:.3e}, step 1500: {e_late:.3e}")
+    assert np.isfinite(e_late)
+    if absorbing:
+        assert e_late < 0.1 * e_early
+    else:
+        assert e_late > 0.5 * e_early
 
 
 def test_3d_errors():
@@ -113,3 +132,22 @@ def test_3d_errors():
     import ctypes
     n = ctypes.c_int()
     assert _lib.load().smx_update_wavefields(ctx.h, 0, 0, ctypes.byref(n)) == _lib.SMX_ERR_STATE
+
+
+def test_3d_geometry_from_nodes_is_exact():
+    """Coordinates far from the origin leave FP32 rounding in the per-point
+    Jacobian (the elements are not flagged affine); the device re-forms the
+    stored values from the control nodes bit for bit, and the run matches a
+    context that reads the stored values."""
+    mesh = box_mesh3d(37000.0, 37600.0, 52000.0, 52500.0, -3400.0, -3000.0, 6, 5, 4,
+                      rho=2600.0, vp=3200.0, vs=1800.0)
+    src = Sources3D.make([37310.0], [52240.0], [-3210.0], force=(0.0, 1e10, 1e10), f0=15.0)
+    rec = Receivers3D(x=np.array([37100.0, 37500.0]), y=np.array([52100.0, 52400.0]),
+                      z=np.array([-3000.0, -3300.0]))
+    cfg = Config3D("far", mesh, src, rec, 5e-4, 200)
+    s, ctx = simulate3d(cfg, options={"geometry_from_nodes": True})
+    _say(f"3D far-from-origin box: geometry {ctx.geometry}")
+    assert ctx.geometry["mode"] == "nodes" and ctx.geometry["mismatches"] == 0
+    s_stored, ctx2 = simulate3d(cfg)
+    assert ctx2.geometry["mode"] == "point"
+    assert np.linalg.norm(s - s_stored) <= 1e-6 * np.linalg.norm(s_stored)

@@ -148,3 +148,24 @@ def test_operator_errors():
     tm = TimeMarching(ctx)
     tm.run(fused=False)
     assert np.isfinite(ctx.seismograms()).all()
+
+
+def test_geometry_from_nodes_is_bitwise(oracle_lib):
+    """Q4 meshes re-form the per-point Jacobian on the device from the control
+    nodes; the set-up check must find every stored FP32 value reproduced bit
+    for bit, and the run keeps the oracle parity."""
+    import ctypes
+    cfg = small_layered(nstep=200)
+    _say("geometry from control nodes: layered 120x90, 200 steps")
+    ours, ctx = simulate(cfg, options={"geometry_from_nodes": True})
+    gm, mis = ctypes.c_int(), ctypes.c_longlong()
+    _lib.check(ctx.lib.smx_ctx_geometry(ctx.h, ctypes.byref(gm), ctypes.byref(mis)))
+    _say(f"  geometry mode {gm.value}, mismatches {mis.value}")
+    assert mis.value == 0 and gm.value == 2
+    ref, _ = _oracle_run(oracle_lib, cfg)
+    assert rel_l2(ours, ref) <= 1e-5
+    stored, ctx2 = simulate(cfg)
+    _lib.check(ctx2.lib.smx_ctx_geometry(ctx2.h, ctypes.byref(gm), ctypes.byref(mis)))
+    assert gm.value != 2
+    # same geometry bits, same arithmetic after it
+    assert rel_l2(ours, stored) <= 1e-6
