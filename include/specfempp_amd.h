@@ -184,8 +184,10 @@ int smx_ctx_shots(const smx_ctx *ctx, int *nshots, int *elem_off, int *rec_off,
 /* A 3D elastic context (hex8, NGLL = 5).  Supported operations:
  * smx_initialize, smx_run_forward, smx_synchronize, smx_get_seismograms
  * ([nsig][ntypes][nrec][3]), smx_get_field / smx_set_field ([nglob][3]),
- * smx_ctx_info, smx_last_run_stats, smx_time_stiffness, smx_ctx_destroy;
- * others return SMX_ERR_STATE.  Replaces assembly<dim3> + the dim3
+ * smx_ctx_info, smx_last_run_stats, smx_time_stiffness, smx_ctx_geometry,
+ * the adjoint path (smx_set_model, smx_run_forward_checkpointed,
+ * smx_run_adjoint, smx_get_gradient, smx_last_adjoint_stats) and
+ * smx_ctx_destroy; others return SMX_ERR_STATE.  Replaces assembly<dim3> + the dim3
  * instantiations of domain_kernels (src/kokkos_kernels/impl/
  * compute_stiffness_interaction.cpp:53-63) and time_marching<forward, dim3>.
  * Absorbing faces are this build's own 3D Stacey condition. */
@@ -298,7 +300,8 @@ int smx_time_stiffness(smx_ctx *ctx, int nlaunch, double *avg_ms);
 /* ---- adjoint / Frechet path ------------------------------------------ */
 
 /* Replaces the material at every GLL point (kappa, mu, rho, each
- * [nspec][5][5] in mesh element order), refreshes every derived array
+ * [nspec][5][5] in mesh element order; [nspec][5][5][5] for a 3D context),
+ * refreshes every derived array
  * (lambda+2mu, the Stacey impedances) and rebuilds the mass matrix.
  * Checkpoints of an earlier run are dropped.  Replaces reassembling
  * specfem::assembly::properties (core/specfem/assembly/properties) for a new
@@ -314,7 +317,8 @@ int smx_set_model(smx_ctx *ctx, const float *kappa, const float *mu,
 int smx_run_forward_checkpointed(smx_ctx *ctx, int every);
 
 /* Discrete-adjoint gradient of a functional chi of the displacement
- * seismograms, given dchi/ds = gsrc[nstep][nrec][2] (element size = the
+ * seismograms, given dchi/ds = gsrc[nstep][nrec][2] ([nstep][nrec][3] for a
+ * 3D context, whose forward fields the recompute overwrites) (element size = the
  * context's precision; row k is the seismogram after step k).  Recomputes
  * each checkpoint segment (newest first), recording its states, then runs
  * the adjoint steps of the segment backwards, accumulating the kappa, mu and
@@ -326,7 +330,8 @@ int smx_run_forward_checkpointed(smx_ctx *ctx, int every);
 int smx_run_adjoint(smx_ctx *ctx, const void *gsrc);
 
 /* Gradient [3][nspec][5][5] (d chi / d kappa, d mu, d rho at every GLL
- * point, mesh element order) of the last smx_run_adjoint, as double.
+ * point, mesh element order; [3][nspec][5][5][5] in 3D) of the last
+ * smx_run_adjoint, as double.
  * Replaces reading assembly::kernels (core/specfem/assembly/kernels). */
 int smx_get_gradient(smx_ctx *ctx, double *host_out);
 

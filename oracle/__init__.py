@@ -201,6 +201,8 @@ def _load3():
     lib.oracle3d_get.argtypes = [P, i, P]
     lib.oracle3d_get.restype = ctypes.c_long
     lib.oracle3d_apply_stiffness.argtypes = [P, P, P, i]
+    lib.oracle3d_forward_store.argtypes = [P, i, P]
+    lib.oracle3d_adjoint.argtypes = [P, i, P, P]
     lib.oracle3d_last_error.restype = ctypes.c_char_p
     _lib3 = lib
     return lib
@@ -265,6 +267,27 @@ class Oracle3D:
             np.asarray(v, dtype=np.float64).ravel())
         self.lib.oracle3d_apply_stiffness(self.h, _ptr(u), _ptr(v), int(with_stacey))
         return self.get("a").reshape(-1, 3)
+
+    def forward_store(self, nstep=None):
+        """Displacement seismograms (nstep, nrec, 3) from rest, keeping every
+        step's state for ``adjoint``."""
+        n = self.nstep if nstep is None else int(nstep)
+        out = np.zeros((n, self.nrec, 3), dtype=np.float64)
+        if self.lib.oracle3d_forward_store(self.h, n, _ptr(out)) != 0:
+            raise RuntimeError(self.lib.oracle3d_last_error().decode())
+        self._stored = n
+        return out
+
+    def adjoint(self, gsrc):
+        """Discrete-adjoint gradient (3, nspec, 5, 5, 5) of a functional with
+        d chi / d s = gsrc (nstep, nrec, 3), after ``forward_store``."""
+        g = np.ascontiguousarray(np.asarray(gsrc, dtype=np.float64))
+        n = g.shape[0]
+        nspec = len(self.get("jacobian")) // 125
+        out = np.zeros((3, nspec, 5, 5, 5), dtype=np.float64)
+        if self.lib.oracle3d_adjoint(self.h, n, _ptr(g), _ptr(out)) != 0:
+            raise RuntimeError(self.lib.oracle3d_last_error().decode())
+        return out
 
     def run(self, nstep=None, types=("displacement",)):
         """Seismograms (nstep, ntypes, nrec, 3) from rest."""

@@ -20,6 +20,9 @@
 //   * rigid motion gives zero force, K symmetric, sum(M) = rho * volume,
 //     affine Jacobian, moment-tensor source array moments
 //   * Stacey: normal-incidence impedance of a face, energy decay
+//   * discrete adjoint (tests/test_oracle3d_adjoint.py): FP64 central
+//     differences of a waveform misfit along random model directions (error
+//     scaling as the step squared), zero source -> zero gradient, linearity
 #include <algorithm>
 #include <array>
 #include <cmath>
@@ -641,6 +644,196 @@ template <typename real> struct Oracle3D {
       }
     }
   }
+
+  // ------------------------------------------- discrete adjoint (checker) --
+  // Exact adjoint of step() above (not a reference routine: the reference's
+  // 3D path has no adjoint).  Forward step k maps (u, v, a)_k to
+  // (u, v, a)_{k+1}:
+  //   u~ = u + dt v + dt^2/2 a,  v~ = v + dt/2 a,
+  //   a' = W (f_k - K u~ - B v~)  with W = 1/M' (M' = M + the Stacey term),
+  //   u' = u~,  v' = v~ + dt/2 a'.
+  // Stored per step: u_{k+1} (= u~_k), v~_k (before the corrector), a_{k+1}.
+  std::vector<std::vector<real>> st_u, st_v, st_a;
+
+  void forward_store(int nsteps, real *seis /* [nsteps][nrec][3] */) {
+    u.assign(size_t(nglob) * 3, 0);
+    v.assign(u.size(), 0);
+    a.assign(u.size(), 0);
+    st_u.assign(nsteps, {});
+    st_v.assign(nsteps, {});
+    st_a.assign(nsteps, {});
+    const size_t n = u.size();
+    for (int it = 0; it < nsteps; ++it) {
+      for (size_t k = 0; k < n; ++k) {
+        u[k] += dt * v[k] + dt2o2 * a[k];
+        v[k] += dto2 * a[k];
+        a[k] = 0;
+      }
+      add_sources(it, a);
+      stiffness(u, v, a);
+      for (size_t k = 0; k < n; ++k)
+        a[k] = a[k] * minv[k];
+      st_u[it] = u;
+      st_v[it] = v;
+      st_a[it] = a;
+      for (size_t k = 0; k < n; ++k)
+        v[k] += dto2 * a[k];
+      for (int r = 0; r < nrec; ++r) {
+        real sr[3] = {0, 0, 0};
+        for (int p = 0; p < NP3; ++p) {
+          const int g = ibool[size_t(rec_e[r]) * NP3 + p];
+          const real hw = rec_h[size_t(r) * NP3 + p];
+          for (int c = 0; c < 3; ++c)
+            sr[c] += hw * u[3 * size_t(g) + c];
+        }
+        for (int c = 0; c < 3; ++c)
+          seis[(size_t(it) * nrec + r) * 3 + c] = sr[c];
+      }
+    }
+  }
+
+  // Displacement gradient du[p][c][d] = d u_c / d x_d of element e.
+  void element_gradient(int e, const std::vector<real> &U, real du[NP3][3][3]) const {
+    real f[NP3][3];
+    for (int p = 0; p < NP3; ++p) {
+      const int g = ibool[size_t(e) * NP3 + p];
+      for (int c = 0; c < 3; ++c)
+        f[p][c] = U[3 * size_t(g) + c];
+    }
+    for (int iz = 0; iz < NGLL; ++iz)
+      for (int iy = 0; iy < NGLL; ++iy)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const int p = pidx(iz, iy, ix);
+          const Jac3<real> &J = jac[size_t(e) * NP3 + p];
+          real dxi[3] = {0, 0, 0}, deta[3] = {0, 0, 0}, dga[3] = {0, 0, 0};
+          for (int l = 0; l < NGLL; ++l)
+            for (int c = 0; c < 3; ++c) {
+              dxi[c] += gll.hprime[ix][l] * f[pidx(iz, iy, l)][c];
+              deta[c] += gll.hprime[iy][l] * f[pidx(iz, l, ix)][c];
+              dga[c] += gll.hprime[iz][l] * f[pidx(l, iy, ix)][c];
+            }
+          for (int c = 0; c < 3; ++c) {
+            du[p][c][0] = J.xix * dxi[c] + J.etax * deta[c] + J.gax * dga[c];
+            du[p][c][1] = J.xiy * dxi[c] + J.etay * deta[c] + J.gay * dga[c];
+            du[p][c][2] = J.xiz * dxi[c] + J.etaz * deta[c] + J.gaz * dga[c];
+          }
+        }
+  }
+
+  // y += B x at the absorbing points (B v is minus the Stacey traction).
+  void apply_B(const std::vector<real> &x, std::vector<real> &y) const {
+    for (int e = 0; e < nspec; ++e) {
+      if (!elem_st[e])
+        continue;
+      for (int p = 0; p < NP3; ++p) {
+        const size_t i = size_t(e) * NP3 + p;
+        if (st[i][4] == real(0))
+          continue;
+        const int g = ibool[i];
+        const real xv[3] = {x[3 * size_t(g)], x[3 * size_t(g) + 1], x[3 * size_t(g) + 2]};
+        real t[3];
+        traction(i, xv, t);
+        for (int c = 0; c < 3; ++c)
+          y[3 * size_t(g) + c] -= t[c];
+      }
+    }
+  }
+
+  // Gradient of sum_k chi_k(u_{k+1}) with respect to (kappa, mu, rho) at
+  // every local point, given dchi/ds_k = gsrc[k][r][c] for the displacement
+  // seismograms (after forward_store).  grad [3][nspec][125].
+  void adjoint(int nsteps, const double *gsrc, double *grad) {
+    const size_t n3 = size_t(nglob) * 3, nloc = size_t(nspec) * NP3;
+    std::vector<real> ub(n3, 0), vb(n3, 0), ab(n3, 0), rb(n3, 0), Kr(n3), Br(n3),
+        zero(n3, 0);
+    std::vector<double> gk(nloc, 0), gm(nloc, 0), gr(nloc, 0);
+    const real c = dto2, h = dt2o2;
+    real dr[NP3][3][3], du[NP3][3][3];
+    for (int k = nsteps - 1; k >= 0; --k) {
+      for (int r = 0; r < nrec; ++r)
+        for (int p = 0; p < NP3; ++p) {
+          const int g = ibool[size_t(rec_e[r]) * NP3 + p];
+          const real hw = rec_h[size_t(r) * NP3 + p];
+          for (int cc = 0; cc < 3; ++cc)
+            ub[3 * size_t(g) + cc] += hw * real(gsrc[(size_t(k) * nrec + r) * 3 + cc]);
+        }
+      for (size_t i = 0; i < n3; ++i)
+        rb[i] = minv[i] * (ab[i] + c * vb[i]);
+      std::fill(Kr.begin(), Kr.end(), real(0));
+      stiffness(rb, zero, Kr, false);
+      for (auto &x : Kr)
+        x = -x;
+      std::fill(Br.begin(), Br.end(), real(0));
+      apply_B(rb, Br);
+      const std::vector<real> &U1 = st_u[k], &V0 = st_v[k], &A1 = st_a[k];
+      for (int e = 0; e < nspec; ++e) {
+        element_gradient(e, rb, dr);
+        element_gradient(e, U1, du);
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int iy = 0; iy < NGLL; ++iy)
+            for (int ix = 0; ix < NGLL; ++ix) {
+              const int p = pidx(iz, iy, ix);
+              const size_t i = size_t(e) * NP3 + p;
+              const size_t g = size_t(ibool[i]);
+              const double wj = double(gll.w[iz]) * gll.w[iy] * gll.w[ix] * jac[i].jac;
+              const double divr = double(dr[p][0][0]) + dr[p][1][1] + dr[p][2][2];
+              const double divu = double(du[p][0][0]) + du[p][1][1] + du[p][2][2];
+              double ee = 0;
+              for (int a1 = 0; a1 < 3; ++a1)
+                for (int b1 = 0; b1 < 3; ++b1)
+                  ee += 0.25 * (double(dr[p][a1][b1]) + dr[p][b1][a1]) *
+                        (double(du[p][a1][b1]) + du[p][b1][a1]);
+              double ra = 0;
+              for (int cc = 0; cc < 3; ++cc)
+                ra += double(rb[3 * g + cc]) * A1[3 * g + cc];
+              gk[i] -= wj * divr * divu;
+              gm[i] -= wj * (2.0 * ee - 2.0 / 3.0 * divr * divu);
+              gr[i] -= wj * ra;
+              if (elem_st[e] && st[i][4] != real(0)) {
+                const auto &sp = st[i];
+                const double nv[3] = {sp[0], sp[1], sp[2]};
+                const double nrm = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+                const double nh[3] = {nv[0] / nrm, nv[1] / nrm, nv[2] / nrm};
+                const double wt = sp[4];
+                const double s1 = nh[0] + nh[1] + nh[2];
+                double rn = 0, vn = 0, rv = 0, tp = 0, ts = 0;
+                for (int cc = 0; cc < 3; ++cc) {
+                  const double r0 = rb[3 * g + cc], v0 = V0[3 * g + cc],
+                               a0 = A1[3 * g + cc];
+                  rn += r0 * nh[cc];
+                  vn += v0 * nh[cc];
+                  rv += r0 * v0;
+                  tp += r0 * a0 * nh[cc] * s1;
+                  ts += r0 * a0 * (1.0 - nh[cc] * s1);
+                }
+                const double dP = -(nrm * wt * rn * vn + 0.5 * dt * nrm * wt * tp);
+                const double dS = -(nrm * wt * (rv - rn * vn) + 0.5 * dt * nrm * wt * ts);
+                const double L = l2m[i], m = mu[i], rh = rho[i];
+                const double P = std::sqrt(rh * L), S = std::sqrt(rh * m);
+                gk[i] += dP * rh / (2.0 * P);
+                gm[i] += dP * (4.0 / 3.0) * rh / (2.0 * P);
+                gr[i] += dP * L / (2.0 * P);
+                if (S > 0) {
+                  gm[i] += dS * rh / (2.0 * S);
+                  gr[i] += dS * m / (2.0 * S);
+                }
+              }
+            }
+      }
+      for (size_t i = 0; i < n3; ++i) {
+        const real uu = ub[i] - Kr[i];
+        const real vv = vb[i] - Br[i];
+        ub[i] = uu;
+        vb[i] = vv + dt * uu;
+        ab[i] = c * vv + h * uu;
+      }
+    }
+    for (size_t i = 0; i < nloc; ++i) {
+      grad[i] = gk[i];
+      grad[nloc + i] = gm[i];
+      grad[2 * nloc + i] = gr[i];
+    }
+  }
 };
 
 struct Handle3 {
@@ -773,6 +966,51 @@ int oracle3d_apply_stiffness(void *hv, const double *u, const double *v,
     return 0;
   };
   return h->f ? app(h->f) : app(h->d);
+}
+
+// Forward run from rest storing every step's state for the adjoint;
+// displacement seismograms [nsteps][nrec][3].
+int oracle3d_forward_store(void *hv, int nsteps, double *seis) {
+  auto *h = static_cast<Handle3 *>(hv);
+  try {
+    auto go = [&](auto *o) {
+      const size_t n = size_t(nsteps) * o->nrec * 3;
+      std::vector<decltype(o->dt)> buf(std::max<size_t>(n, 1));
+      o->forward_store(nsteps, buf.data());
+      for (size_t k = 0; k < n; ++k)
+        seis[k] = buf[k];
+    };
+    if (h->f)
+      go(h->f);
+    else
+      go(h->d);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err3 = e.what();
+    return -1;
+  }
+}
+
+// Discrete-adjoint gradient for dchi/ds = gsrc[nsteps][nrec][3] (after
+// oracle3d_forward_store with the same nsteps); grad [3][nspec][125] =
+// dchi / d(kappa, mu, rho) per local point.
+int oracle3d_adjoint(void *hv, int nsteps, const double *gsrc, double *grad) {
+  auto *h = static_cast<Handle3 *>(hv);
+  try {
+    auto go = [&](auto *o) {
+      if (int(o->st_u.size()) != nsteps)
+        throw std::runtime_error("run oracle3d_forward_store with the same nsteps first");
+      o->adjoint(nsteps, gsrc, grad);
+    };
+    if (h->f)
+      go(h->f);
+    else
+      go(h->d);
+    return 0;
+  } catch (const std::exception &e) {
+    g_err3 = e.what();
+    return -1;
+  }
 }
 
 } // extern "C"

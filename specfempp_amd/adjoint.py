@@ -16,7 +16,9 @@ quadrature weights and Jacobian.
 
 ``Seismograms`` wraps this as a ``torch.autograd.Function``: its forward
 returns the displacement seismograms, its backward turns the incoming
-d loss / d seismogram into the adjoint source.
+d loss / d seismogram into the adjoint source.  Every function here takes a
+2D ``Context`` (2 components, material (nspec, 5, 5)) or a 3D ``Context3D``
+(3 components, material (nspec, 5, 5, 5)).
 """
 
 from __future__ import annotations
@@ -24,6 +26,9 @@ from __future__ import annotations
 import numpy as np
 
 from .solver import Context
+
+# Context or Context3D: both expose ncomp, nspec, nrec, nsig, nstep, nsbs, dt,
+# seismo_types and the adjoint methods.
 
 
 def l2_misfit(s, d, dt):
@@ -40,7 +45,7 @@ def _displacement_index(sim: Context) -> int:
 
 def forward_seismograms(sim: Context, kappa, mu, rho, every=100):
     """Sets the model, runs the checkpointed forward and returns the
-    displacement seismograms (nsig, nrec, 2)."""
+    displacement seismograms (nsig, nrec, ncomp)."""
     sim.set_model(kappa, mu, rho)
     sim.run_forward_checkpointed(every)
     sim.synchronize()
@@ -49,20 +54,21 @@ def forward_seismograms(sim: Context, kappa, mu, rho, every=100):
 
 
 def adjoint_gradient(sim: Context, dchi_ds):
-    """Gradient (3, nspec, 5, 5) for dchi/ds = dchi_ds (nsig, nrec, 2), after
-    forward_seismograms on the same context.  Seismogram i was recorded after
-    step i * nsbs; the other steps get no adjoint source."""
+    """Gradient (3, nspec, 5, 5[, 5]) for dchi/ds = dchi_ds (nsig, nrec,
+    ncomp), after forward_seismograms on the same context.  Seismogram i was
+    recorded after step i * nsbs; the other steps get no adjoint source."""
     g = np.asarray(dchi_ds, dtype=np.float64)
-    if g.shape != (sim.nsig, sim.nrec, 2):
-        raise ValueError(f"dchi/ds must have shape {(sim.nsig, sim.nrec, 2)}, got {g.shape}")
-    gsrc = np.zeros((sim.nstep, sim.nrec, 2), dtype=np.float64)
+    nc = sim.ncomp
+    if g.shape != (sim.nsig, sim.nrec, nc):
+        raise ValueError(f"dchi/ds must have shape {(sim.nsig, sim.nrec, nc)}, got {g.shape}")
+    gsrc = np.zeros((sim.nstep, sim.nrec, nc), dtype=np.float64)
     gsrc[::sim.nsbs][:sim.nsig] = g
     sim.run_adjoint(gsrc)
     return sim.gradient()
 
 
 def misfit_and_gradient(sim: Context, model, observed, every=100):
-    """L2 waveform misfit against ``observed`` (nsig, nrec, 2) and its
+    """L2 waveform misfit against ``observed`` (nsig, nrec, ncomp) and its
     gradient with respect to ``model`` = (kappa, mu, rho)."""
     s = forward_seismograms(sim, *model, every=every)
     dt_rec = sim.dt * sim.nsbs
@@ -76,8 +82,8 @@ try:  # torch is plumbing for the autograd interface, not a compute path
 
     class Seismograms(torch.autograd.Function):
         """s = Seismograms.apply(kappa, mu, rho, sim, every): displacement
-        seismograms (nsig, nrec, 2) of context ``sim`` for the material
-        tensors (nspec, 5, 5); differentiable in kappa, mu and rho."""
+        seismograms (nsig, nrec, ncomp) of context ``sim`` for the material
+        tensors (nspec, 5, 5[, 5]); differentiable in kappa, mu and rho."""
 
         @staticmethod
         def forward(ctx, kappa, mu, rho, sim, every=100):

@@ -85,7 +85,135 @@ uint64_t spread3(uint32_t v) { // 21 bits -> every third bit
   return x;
 }
 
+// Material (medium/dim2/elastic/isotropic/properties.hpp:71-91, shared by
+// dim3): lambda+2mu = kappa + 4/3 mu, in float and in double.  kappa, mu, rho
+// are [nspec][125] in mesh element order; everything is validated before A
+// changes.
+void material3d(HostAssembly3D &A, const float *kappa, const float *mu_in,
+                const float *rho) {
+  const int nspec = A.nspec, ld = nspec;
+  for (size_t i = 0; i < size_t(nspec) * NPT3; ++i)
+    if (!(rho[i] > 0) || !(mu_in[i] >= 0) || !(kappa[i] > 0))
+      throw SmxError(SMX_ERR_ARG, "non-physical material at element " +
+                                      std::to_string(i / NPT3));
+  A.kappa_mu_rho.assign(size_t(3) * NPT3 * ld, 0.0f);
+  std::vector<float> mpt(size_t(2) * NPT3 * ld);
+  std::vector<double> mpt64(size_t(2) * NPT3 * ld);
+  bool mat_const = true;
+  for (int e = 0; e < nspec; ++e) {
+    const int s = A.elem_slot[e];
+    for (int p = 0; p < NPT3; ++p) {
+      const size_t i = size_t(e) * NPT3 + p;
+      const size_t o3 = size_t(s) * 3 * NPT3 + p, o2 = size_t(s) * 2 * NPT3 + p;
+      const float k = kappa[i], mu = mu_in[i], r = rho[i];
+      A.kappa_mu_rho[o3] = k;
+      A.kappa_mu_rho[o3 + NPT3] = mu;
+      A.kappa_mu_rho[o3 + 2 * NPT3] = r;
+      mpt[o2] = k + static_cast<float>(4.0 / 3.0) * mu;
+      mpt[o2 + NPT3] = mu;
+      mpt64[o2] = double(k) + 4.0 / 3.0 * double(mu);
+      mpt64[o2 + NPT3] = mu;
+      if (mpt[o2] != mpt[o2 - p] || mu != mpt[o2 - p + NPT3])
+        mat_const = false;
+    }
+  }
+  A.mat_elem = mat_const;
+  if (mat_const) {
+    A.mat.assign(size_t(2) * ld, 0.0f);
+    A.mat64.assign(size_t(2) * ld, 0.0);
+    for (int c = 0; c < 2; ++c)
+      for (int s = 0; s < ld; ++s) {
+        A.mat[size_t(s) * 2 + c] = mpt[(size_t(s) * 2 + c) * NPT3];
+        A.mat64[size_t(s) * 2 + c] = mpt64[(size_t(s) * 2 + c) * NPT3];
+      }
+  } else {
+    A.mat = std::move(mpt);
+    A.mat64 = std::move(mpt64);
+  }
+}
+
+// Stacey impedances rho vp = sqrt((lambda+2mu)/rho) rho and rho vs =
+// sqrt(rho mu) at the absorbing face points (medium/dim2/elastic/isotropic/
+// properties.hpp:77-87), in float and, for FP64 contexts, in double.
+void impedances3d(HostAssembly3D &A) {
+  for (int s = 0; s < A.nspec; ++s) {
+    const int b = A.stacey_slot[s];
+    if (b < 0)
+      continue;
+    for (int p = 0; p < NPT3; ++p) {
+      if (A.stacey[(size_t(b) * 7 + 4) * NPT3 + p] == 0.0f)
+        continue;
+      const size_t o3 = size_t(s) * 3 * NPT3 + p;
+      const float kk = A.kappa_mu_rho[o3], mm = A.kappa_mu_rho[o3 + NPT3],
+                  rr = A.kappa_mu_rho[o3 + 2 * NPT3];
+      const float l2m = kk + static_cast<float>(4.0 / 3.0) * mm;
+      A.stacey[(size_t(b) * 7 + 5) * NPT3 + p] = std::sqrt(l2m / rr) * rr;
+      A.stacey[(size_t(b) * 7 + 6) * NPT3 + p] = std::sqrt(rr * mm);
+      const double l2m64 = double(kk) + 4.0 / 3.0 * double(mm);
+      A.stacey64[(size_t(b) * 2) * NPT3 + p] = std::sqrt(l2m64 / rr) * rr;
+      A.stacey64[(size_t(b) * 2 + 1) * NPT3 + p] = std::sqrt(double(rr) * mm);
+    }
+  }
+}
+
+// Mass with the Stacey term at v = (-dt/2, -dt/2, -dt/2) (compute_mass_matrix.
+// tpp:23-122, stacey.tpp:817-853 carried to faces), summed in double,
+// inverted per component.
+void mass3d(HostAssembly3D &A) {
+  const Quadrature &q = A.quad;
+  const float dt = A.dt;
+  std::vector<double> M(size_t(3) * A.nglob, 0.0), M64(M.size(), 0.0);
+  for (int s = 0; s < A.nspec; ++s) {
+    const int b = A.stacey_slot[s];
+    for (int iz = 0; iz < NGLL; ++iz)
+      for (int iy = 0; iy < NGLL; ++iy)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const int p = pidx(iz, iy, ix);
+          const float f = q.w[iz] * q.w[iy] * q.w[ix] * A.jac_pt[size_t(s) * NPT3 + p];
+          const float rr = A.kappa_mu_rho[(size_t(s) * 3 + 2) * NPT3 + p];
+          double add[3] = {double(rr * f), double(rr * f), double(rr * f)};
+          double add64[3] = {double(rr) * f, double(rr) * f, double(rr) * f};
+          if (b >= 0 && A.stacey[(size_t(b) * 7 + 4) * NPT3 + p] != 0.0f) {
+            const float *S = A.stacey.data();
+            auto at = [&](int c) { return S[(size_t(b) * 7 + c) * NPT3 + p]; };
+            const float nn[3] = {at(0), at(1), at(2)}, nrm = at(3), w = at(4);
+            const float vf = -1.0f * dt * 0.5f;
+            const float vn = vf * nn[0] + vf * nn[1] + vf * nn[2];
+            const double P64 = A.stacey64[(size_t(b) * 2) * NPT3 + p],
+                         S64 = A.stacey64[(size_t(b) * 2 + 1) * NPT3 + p];
+            for (int c = 0; c < 3; ++c) {
+              const float fc = ((vn * nn[c] / (nrm * nrm)) * (at(5) - at(6))) + vf * at(6);
+              add[c] += double(-1.0f * fc * nrm * w);
+              const double fc64 =
+                  ((double(vn) * nn[c] / (double(nrm) * nrm)) * (P64 - S64)) + double(vf) * S64;
+              add64[c] += -fc64 * nrm * w;
+            }
+          }
+          const int g = A.ibool[size_t(s) * NPT3 + p];
+          for (int c = 0; c < 3; ++c) {
+            M[size_t(c) * A.nglob + g] += add[c];
+            M64[size_t(c) * A.nglob + g] += add64[c];
+          }
+        }
+  }
+  A.minv.resize(M.size());
+  A.minv64.resize(M.size());
+  for (size_t k = 0; k < M.size(); ++k) {
+    A.minv[k] = static_cast<float>(1.0 / M[k]);
+    A.minv64[k] = 1.0 / M64[k];
+  }
+}
+
 } // namespace
+
+void set_model3d(HostAssembly3D &A, const float *kappa, const float *mu,
+                 const float *rho) {
+  if (!kappa || !mu || !rho)
+    throw SmxError(SMX_ERR_ARG, "null model");
+  material3d(A, kappa, mu, rho);
+  impedances3d(A);
+  mass3d(A);
+}
 
 void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
                 const smx_receivers3d &rec, double dt_in, int nstep,
@@ -227,11 +355,10 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
   }
   A.geom_elem = all_affine;
   A.geom = all_affine ? std::move(gel) : std::move(gpt);
-  std::vector<float> jac_pt; // J per point for the mass matrix
-  jac_pt.resize(size_t(NPT3) * ld);
+  A.jac_pt.resize(size_t(NPT3) * ld); // J per point for the mass matrix
   for (int s = 0; s < ld; ++s)
     for (int p = 0; p < NPT3; ++p)
-      jac_pt[size_t(s) * NPT3 + p] =
+      A.jac_pt[size_t(s) * NPT3 + p] =
           all_affine ? A.geom[size_t(s) * 10 + 9]
                      : A.geom[(size_t(s) * 10 + 9) * NPT3 + p];
 
@@ -280,45 +407,7 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
     A.nglob = next;
   }
 
-  // ---- material (medium/dim2/elastic/isotropic/properties.hpp:71-91, shared
-  // by dim3): lambda+2mu = kappa + 4/3 mu
-  A.kappa_mu_rho.assign(size_t(3) * NPT3 * ld, 0.0f);
-  std::vector<float> mpt(size_t(2) * NPT3 * ld);
-  std::vector<double> mpt64(size_t(2) * NPT3 * ld);
-  bool mat_const = true;
-  for (int e = 0; e < nspec; ++e) {
-    const int s = A.elem_slot[e];
-    for (int p = 0; p < NPT3; ++p) {
-      const size_t i = size_t(e) * NPT3 + p;
-      const size_t o3 = size_t(s) * 3 * NPT3 + p, o2 = size_t(s) * 2 * NPT3 + p;
-      const float k = m.kappa[i], mu = m.mu[i], r = m.rho[i];
-      if (!(r > 0) || !(mu >= 0) || !(k > 0))
-        throw SmxError(SMX_ERR_ARG,
-                       "non-physical material at element " + std::to_string(e));
-      A.kappa_mu_rho[o3] = k;
-      A.kappa_mu_rho[o3 + NPT3] = mu;
-      A.kappa_mu_rho[o3 + 2 * NPT3] = r;
-      mpt[o2] = k + static_cast<float>(4.0 / 3.0) * mu;
-      mpt[o2 + NPT3] = mu;
-      mpt64[o2] = double(k) + 4.0 / 3.0 * double(mu);
-      mpt64[o2 + NPT3] = mu;
-      if (mpt[o2] != mpt[o2 - p] || mu != mpt[o2 - p + NPT3])
-        mat_const = false;
-    }
-  }
-  A.mat_elem = mat_const;
-  if (mat_const) {
-    A.mat.assign(size_t(2) * ld, 0.0f);
-    A.mat64.assign(size_t(2) * ld, 0.0);
-    for (int c = 0; c < 2; ++c)
-      for (int s = 0; s < ld; ++s) {
-        A.mat[size_t(s) * 2 + c] = mpt[(size_t(s) * 2 + c) * NPT3];
-        A.mat64[size_t(s) * 2 + c] = mpt64[(size_t(s) * 2 + c) * NPT3];
-      }
-  } else {
-    A.mat = std::move(mpt);
-    A.mat64 = std::move(mpt64);
-  }
+  material3d(A, m.kappa, m.mu, m.rho);
 
   // ---- absorbing faces (this build's 3D Stacey: the 2D point formula on
   // face points, normal n = +-J grad(xi_d), weight = the two in-face GLL
@@ -367,19 +456,10 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
               const float n0 = sg * g.t[3 * d] * g.t[9];
               const float n1 = sg * g.t[3 * d + 1] * g.t[9];
               const float n2 = sg * g.t[3 * d + 2] * g.t[9];
-              const size_t o3 = size_t(s) * 3 * NPT3 + p;
-              const float kk = A.kappa_mu_rho[o3],
-                          mm = A.kappa_mu_rho[o3 + NPT3],
-                          rr = A.kappa_mu_rho[o3 + 2 * NPT3];
-              const float l2m = kk + static_cast<float>(4.0 / 3.0) * mm;
-              const float vals[7] = {n0, n1, n2,
-                                     std::sqrt(n0 * n0 + n1 * n1 + n2 * n2), w,
-                                     std::sqrt(l2m / rr) * rr, std::sqrt(rr * mm)};
-              for (int c = 0; c < 7; ++c)
+              const float vals[5] = {n0, n1, n2,
+                                     std::sqrt(n0 * n0 + n1 * n1 + n2 * n2), w};
+              for (int c = 0; c < 5; ++c)
                 A.stacey[(size_t(b) * 7 + c) * NPT3 + p] = vals[c];
-              const double l2m64 = double(kk) + 4.0 / 3.0 * double(mm);
-              A.stacey64[(size_t(b) * 2) * NPT3 + p] = std::sqrt(l2m64 / rr) * rr;
-              A.stacey64[(size_t(b) * 2 + 1) * NPT3 + p] = std::sqrt(double(rr) * mm);
             }
       ++b;
     }
@@ -391,50 +471,8 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
   A.dt_over2 = dt / 2.0;
   A.dt2_over2 = dt * dt / 2.0;
 
-  // ---- mass with the Stacey term at v = (-dt/2, -dt/2, -dt/2), summed in
-  // double, inverted per component
-  {
-    std::vector<double> M(size_t(3) * A.nglob, 0.0), M64(M.size(), 0.0);
-    for (int s = 0; s < ld; ++s) {
-      const int b = A.stacey_slot[s];
-      for (int iz = 0; iz < NGLL; ++iz)
-        for (int iy = 0; iy < NGLL; ++iy)
-          for (int ix = 0; ix < NGLL; ++ix) {
-            const int p = pidx(iz, iy, ix);
-            const float f = q.w[iz] * q.w[iy] * q.w[ix] * jac_pt[size_t(s) * NPT3 + p];
-            const float rr = A.kappa_mu_rho[(size_t(s) * 3 + 2) * NPT3 + p];
-            double add[3] = {double(rr * f), double(rr * f), double(rr * f)};
-            double add64[3] = {double(rr) * f, double(rr) * f, double(rr) * f};
-            if (b >= 0 && A.stacey[(size_t(b) * 7 + 4) * NPT3 + p] != 0.0f) {
-              const float *S = A.stacey.data();
-              auto at = [&](int c) { return S[(size_t(b) * 7 + c) * NPT3 + p]; };
-              const float nn[3] = {at(0), at(1), at(2)}, nrm = at(3), w = at(4);
-              const float vf = -1.0f * dt * 0.5f;
-              const float vn = vf * nn[0] + vf * nn[1] + vf * nn[2];
-              const double P64 = A.stacey64[(size_t(b) * 2) * NPT3 + p],
-                           S64 = A.stacey64[(size_t(b) * 2 + 1) * NPT3 + p];
-              for (int c = 0; c < 3; ++c) {
-                const float fc = ((vn * nn[c] / (nrm * nrm)) * (at(5) - at(6))) + vf * at(6);
-                add[c] += double(-1.0f * fc * nrm * w);
-                const double fc64 =
-                    ((double(vn) * nn[c] / (double(nrm) * nrm)) * (P64 - S64)) + double(vf) * S64;
-                add64[c] += -fc64 * nrm * w;
-              }
-            }
-            const int g = A.ibool[size_t(s) * NPT3 + p];
-            for (int c = 0; c < 3; ++c) {
-              M[size_t(c) * A.nglob + g] += add[c];
-              M64[size_t(c) * A.nglob + g] += add64[c];
-            }
-          }
-    }
-    A.minv.resize(M.size());
-    A.minv64.resize(M.size());
-    for (size_t k = 0; k < M.size(); ++k) {
-      A.minv[k] = static_cast<float>(1.0 / M[k]);
-      A.minv64[k] = 1.0 / M64[k];
-    }
-  }
+  impedances3d(A);
+  mass3d(A);
 
   // ---- point location: candidate elements from a grid of element bounding
   // boxes, Newton in each (src/algorithms/dim3/locate_point_impl.cpp), the

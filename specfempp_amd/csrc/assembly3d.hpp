@@ -41,6 +41,7 @@ struct HostAssembly3D {
   // else per point [nspec][10][NPT3]
   bool geom_elem = false;
   std::vector<float> geom;
+  std::vector<float> jac_pt; // J per GLL point [nspec][NPT3] (mass, gradient)
   // control-node coordinates [nspec][24] (x0..x7, y0..y7, z0..z7) in FP32:
   // the kernels can re-form the per-point geometry from them
   std::vector<float> nodes;
@@ -76,5 +77,11 @@ struct HostAssembly3D {
 void assemble3d(const smx_mesh3d &mesh, const smx_sources3d &sources,
                 const smx_receivers3d &receivers, double dt, int nstep,
                 HostAssembly3D &out);
+
+// Replaces the material ([nspec][125] each, mesh element order) of an
+// assembled problem: moduli, Stacey impedances and mass.  A is unchanged
+// when the model is rejected.
+void set_model3d(HostAssembly3D &A, const float *kappa, const float *mu,
+                 const float *rho);
 
 } // namespace smx

@@ -379,16 +379,30 @@ int smx_set_model(smx_ctx *ctx, const float *kappa, const float *mu,
                   const float *rho) {
   return guarded([&] {
     if (!kappa || !mu || !rho) throw smx::SmxError(SMX_ERR_ARG, "null model");
-    S(ctx).set_model(kappa, mu, rho);
+    if (auto *s3 = S3(ctx))
+      s3->set_model(kappa, mu, rho);
+    else
+      S(ctx).set_model(kappa, mu, rho);
   });
 }
 
 int smx_run_forward_checkpointed(smx_ctx *ctx, int every) {
-  return guarded([&] { S(ctx).run_forward_checkpointed(every); });
+  return guarded([&] {
+    if (auto *s3 = S3(ctx))
+      s3->run_forward_checkpointed(every);
+    else
+      S(ctx).run_forward_checkpointed(every);
+  });
 }
 
 int smx_run_adjoint(smx_ctx *ctx, const void *gsrc) {
   return guarded([&] {
+    if (auto *s3 = S3(ctx)) {
+      if (!gsrc && s3->assembly().nrec > 0)
+        throw smx::SmxError(SMX_ERR_ARG, "null adjoint source");
+      s3->run_adjoint(gsrc);
+      return;
+    }
     if (!gsrc && S(ctx).assembly().nrec > 0)
       throw smx::SmxError(SMX_ERR_ARG, "null adjoint source");
     S(ctx).run_adjoint(gsrc);
@@ -398,13 +412,21 @@ int smx_run_adjoint(smx_ctx *ctx, const void *gsrc) {
 int smx_get_gradient(smx_ctx *ctx, double *host_out) {
   return guarded([&] {
     if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
-    S(ctx).get_gradient(host_out);
+    if (auto *s3 = S3(ctx))
+      s3->get_gradient(host_out);
+    else
+      S(ctx).get_gradient(host_out);
   });
 }
 
 int smx_last_adjoint_stats(smx_ctx *ctx, double *forward_ms,
                            double *adjoint_ms) {
-  return guarded([&] { S(ctx).last_adjoint_stats(forward_ms, adjoint_ms); });
+  return guarded([&] {
+    if (auto *s3 = S3(ctx))
+      s3->last_adjoint_stats(forward_ms, adjoint_ms);
+    else
+      S(ctx).last_adjoint_stats(forward_ms, adjoint_ms);
+  });
 }
 
 } // extern "C"

@@ -313,22 +313,39 @@ __global__ void k3_seis(const real *__restrict__ U, const real *__restrict__ V,
     out[((size_t(isig) * ntypes + it) * nrec + r) * 3 + c] = s[c];
 }
 
+// Record buffers of the adjoint's recompute: per step j of a segment,
+// a' [every][3][npad], the predicted u~ [every][3][npad] and v~ at the
+// absorbing nodes [every][3][nb].
+template <typename real> struct Rec3 {
+  real *a, *u, *vb;
+  const int *bidx; // node -> absorbing-node index or -1
+  int every;
+  long npad, nb;
+};
+
 // Node pass: a = A / M, v += dt/2 a; then (FINAL = false) the next
 // predictor u += dt v + dt^2/2 a, v += dt/2 a, a = 0.  Thread 0 advances the
-// step counter.
-template <typename real, bool FINAL>
+// step counter, except when recording (REC): then every thread reads the
+// counter (j = step % every) and a separate launch advances it; the
+// arithmetic is the same, so a recompute reproduces the forward run.
+template <typename real, bool FINAL, bool REC>
 __global__ void __launch_bounds__(NODE3_THREADS)
     k3_node(real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
             const real *__restrict__ Minv, long n3, real dt, real dto2,
-            real dt2o2, int *__restrict__ step_ptr) {
+            real dt2o2, int *__restrict__ step_ptr, Rec3<real> rec) {
   const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (i == 0 && step_ptr)
+  int j = 0;
+  if (REC)
+    j = *step_ptr % rec.every;
+  else if (i == 0 && step_ptr)
     *step_ptr += 1;
   if (i >= n3)
     return;
   real a = A[i] * Minv[i];
   real v = V[i];
   v += dto2 * a;
+  if (REC)
+    rec.a[size_t(j) * n3 + i] = a;
   if (FINAL) {
     V[i] = v;
     A[i] = a;
@@ -340,6 +357,240 @@ __global__ void __launch_bounds__(NODE3_THREADS)
   U[i] = u;
   V[i] = v;
   A[i] = 0;
+  if (REC && j + 1 < rec.every) {
+    rec.u[size_t(j + 1) * n3 + i] = u;
+    const long g = i % rec.npad;
+    const int c = int(i / rec.npad);
+    const int b = rec.bidx[g];
+    if (b >= 0)
+      rec.vb[(size_t(j + 1) * 3 + c) * rec.nb + b] = v;
+  }
+}
+
+// v~ at the absorbing nodes into record slot 0 (segment start).
+template <typename real>
+__global__ void k3_gather_vb(const real *__restrict__ V,
+                             const int *__restrict__ bnode, long nb, long npad,
+                             real *__restrict__ vb0) {
+  const long b = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (b >= nb)
+    return;
+  const int g = bnode[b];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    vb0[c * nb + b] = V[c * npad + g];
+}
+
+__global__ void k3_step_add(int *__restrict__ step_ptr, int d) {
+  if (threadIdx.x == 0 && blockIdx.x == 0)
+    *step_ptr += d;
+}
+
+// Adjoint misfit source of step `*adj_step`: ub += H^T g_k at every node a
+// receiver touches, each node summing its (receiver, weight) entries in
+// order (no atomics; receivers may share nodes).
+template <typename real>
+__global__ void k3_adj_inject(const int *__restrict__ rin_node,
+                              const int *__restrict__ rin_off,
+                              const int *__restrict__ rin_r,
+                              const real *__restrict__ rin_h, int nnode,
+                              const real *__restrict__ gsrc, int nrec,
+                              real *__restrict__ Ub, long npad,
+                              const int *__restrict__ adj_step) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nnode)
+    return;
+  const real *gk = gsrc + size_t(*adj_step) * nrec * 3;
+  real s[3] = {0, 0, 0};
+  for (int e = rin_off[t]; e < rin_off[t + 1]; ++e) {
+    const int r = rin_r[e];
+    const real h = rin_h[e];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      s[c] += h * gk[size_t(r) * 3 + c];
+  }
+  const int g = rin_node[t];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    Ub[c * npad + g] += s[c];
+}
+
+// Adjoint colour pass of step k = *adj_step: with rb the adjoint
+// acceleration field and u~_k the recorded forward displacement, assembles
+// -K rb (as k3_elem assembles -K u, without the Stacey term) and
+// accumulates the material densities div(rb) div(u~) (Gd) and
+// eps(rb) : eps(u~) (Ge) per local point.
+template <typename real, int GM, bool ME>
+__global__ void __launch_bounds__(ELEM_THREADS)
+    k3_adj_elem(int slot0, const int *__restrict__ ibool,
+                const real *__restrict__ geom, const float *__restrict__ nodes,
+                Quad3<float> xi3, const real *__restrict__ mat,
+                const real *__restrict__ Rb, const real *__restrict__ rec_u,
+                int every, const int *__restrict__ adj_step,
+                real *__restrict__ Kacc, real *__restrict__ Gd,
+                real *__restrict__ Ge, long npad, Quad3<real> q) {
+  __shared__ real s_r[3][NPT3];
+  __shared__ real s_u[3][NPT3];
+  __shared__ real s_f[3][3][NPT3];
+  const int slot = slot0 + blockIdx.x;
+  const int t = threadIdx.x;
+  const bool act = t < NPT3;
+  const int p = act ? t : 0;
+  const int iz = p / (NGLL * NGLL), iy = (p / NGLL) % NGLL, ix = p % NGLL;
+  const real *Ut = rec_u + size_t(*adj_step % every) * 3 * npad;
+  int g = 0;
+  if (act) {
+    g = ibool[size_t(slot) * NPT3 + p];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s_r[c][p] = Rb[c * npad + g];
+      s_u[c][p] = Ut[c * npad + g];
+    }
+  }
+  __syncthreads();
+  if (act) {
+    real G[10];
+    if (GM == GEOM_NODES) {
+      float Gf[10];
+      geom_from_nodes3(nodes + size_t(slot) * 24, xi3.w[ix], xi3.w[iy],
+                       xi3.w[iz], Gf);
+#pragma unroll
+      for (int k = 0; k < 10; ++k)
+        G[k] = Gf[k];
+    } else {
+#pragma unroll
+      for (int k = 0; k < 10; ++k)
+        G[k] = GM == GEOM_ELEM ? geom[size_t(slot) * 10 + k]
+                               : geom[(size_t(slot) * 10 + k) * NPT3 + p];
+    }
+    real dr[3][3], du[3][3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      real a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+      for (int l = 0; l < NGLL; ++l) {
+        a0 += q.hp[ix][l] * s_r[c][p3(iz, iy, l)];
+        a1 += q.hp[iy][l] * s_r[c][p3(iz, l, ix)];
+        a2 += q.hp[iz][l] * s_r[c][p3(l, iy, ix)];
+        b0 += q.hp[ix][l] * s_u[c][p3(iz, iy, l)];
+        b1 += q.hp[iy][l] * s_u[c][p3(iz, l, ix)];
+        b2 += q.hp[iz][l] * s_u[c][p3(l, iy, ix)];
+      }
+      dr[c][0] = G[0] * a0 + G[3] * a1 + G[6] * a2;
+      dr[c][1] = G[1] * a0 + G[4] * a1 + G[7] * a2;
+      dr[c][2] = G[2] * a0 + G[5] * a1 + G[8] * a2;
+      du[c][0] = G[0] * b0 + G[3] * b1 + G[6] * b2;
+      du[c][1] = G[1] * b0 + G[4] * b1 + G[7] * b2;
+      du[c][2] = G[2] * b0 + G[5] * b1 + G[8] * b2;
+    }
+    const real divr = dr[0][0] + dr[1][1] + dr[2][2];
+    const real divu = du[0][0] + du[1][1] + du[2][2];
+    const real ee = dr[0][0] * du[0][0] + dr[1][1] * du[1][1] +
+                    dr[2][2] * du[2][2] +
+                    real(0.5) * ((dr[0][1] + dr[1][0]) * (du[0][1] + du[1][0]) +
+                                 (dr[0][2] + dr[2][0]) * (du[0][2] + du[2][0]) +
+                                 (dr[1][2] + dr[2][1]) * (du[1][2] + du[2][1]));
+    const size_t o = size_t(slot) * NPT3 + p;
+    Gd[o] += divr * divu;
+    Ge[o] += ee;
+    const real L = ME ? mat[size_t(slot) * 2] : mat[size_t(slot) * 2 * NPT3 + p];
+    const real M = ME ? mat[size_t(slot) * 2 + 1]
+                      : mat[(size_t(slot) * 2 + 1) * NPT3 + p];
+    const real lam = L - real(2.0) * M;
+    real T[3][3];
+    T[0][0] = L * dr[0][0] + lam * (dr[1][1] + dr[2][2]);
+    T[1][1] = L * dr[1][1] + lam * (dr[0][0] + dr[2][2]);
+    T[2][2] = L * dr[2][2] + lam * (dr[0][0] + dr[1][1]);
+    T[0][1] = T[1][0] = M * (dr[0][1] + dr[1][0]);
+    T[0][2] = T[2][0] = M * (dr[0][2] + dr[2][0]);
+    T[1][2] = T[2][1] = M * (dr[1][2] + dr[2][1]);
+    const real J = G[9];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      s_f[c][0][p] = J * (T[c][0] * G[0] + T[c][1] * G[1] + T[c][2] * G[2]);
+      s_f[c][1][p] = J * (T[c][0] * G[3] + T[c][1] * G[4] + T[c][2] * G[5]);
+      s_f[c][2][p] = J * (T[c][0] * G[6] + T[c][1] * G[7] + T[c][2] * G[8]);
+    }
+  }
+  __syncthreads();
+  if (!act)
+    return;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    real t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      t1 += s_f[c][0][p3(iz, iy, l)] * q.hp[l][ix] * q.w[l];
+      t2 += s_f[c][1][p3(iz, l, ix)] * q.hp[l][iy] * q.w[l];
+      t3 += s_f[c][2][p3(l, iy, ix)] * q.hp[l][iz] * q.w[l];
+    }
+    const real r = q.w[iz] * q.w[iy] * t1 + q.w[iz] * q.w[ix] * t2 +
+                   q.w[iy] * q.w[ix] * t3;
+    Kacc[c * npad + g] += real(-1.0) * r;
+  }
+}
+
+// Adjoint node pass of step k = *adj_step (the transpose of the forward
+// step): with rb = W (ab + dt/2 vb),
+//   uu = ub - K rb,  vv = vb - B rb,
+//   ub = uu,  vb = vv + dt uu,  ab = dt/2 vv + dt^2/2 uu,  rb = W (ab + dt/2 vb)
+// (rb of step k-1 is formed here; ab is not kept).  Accumulates rb . a'_k
+// per node (density term) and, at the absorbing nodes, rb v~_k^T (9) and
+// rb * a'_k per component (3) for the impedance terms; B at a node is the
+// sum of its face points' symmetric 3x3 Stacey matrices [6][nb].
+template <typename real>
+__global__ void __launch_bounds__(NODE3_THREADS)
+    k3_adj_node(real *__restrict__ Ub, real *__restrict__ Vb,
+                real *__restrict__ Rb, real *__restrict__ Kacc,
+                const real *__restrict__ Minv, const real *__restrict__ rec_a,
+                const real *__restrict__ rec_vb, int every,
+                real *__restrict__ R, const int *__restrict__ bidx,
+                const real *__restrict__ bmat, real *__restrict__ T9,
+                real *__restrict__ D3, long nb, long npad, int nglob, real dt,
+                real dto2, real dt2o2, const int *__restrict__ adj_step) {
+  const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nglob)
+    return;
+  const int j = *adj_step % every;
+  const real *ap = rec_a + size_t(j) * 3 * npad;
+  real rb[3], a[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    rb[c] = Rb[c * npad + g];
+    a[c] = ap[c * npad + g];
+  }
+  R[g] += rb[0] * a[0] + rb[1] * a[1] + rb[2] * a[2];
+  real br[3] = {0, 0, 0};
+  const int b = bidx[g];
+  if (b >= 0) {
+    const real mxx = bmat[b], myy = bmat[nb + b], mzz = bmat[2 * nb + b],
+               mxy = bmat[3 * nb + b], mxz = bmat[4 * nb + b],
+               myz = bmat[5 * nb + b];
+    br[0] = mxx * rb[0] + mxy * rb[1] + mxz * rb[2];
+    br[1] = mxy * rb[0] + myy * rb[1] + myz * rb[2];
+    br[2] = mxz * rb[0] + myz * rb[1] + mzz * rb[2];
+    const real *vt = rec_vb + size_t(j) * 3 * nb;
+    const real v[3] = {vt[b], vt[nb + b], vt[2 * nb + b]};
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+#pragma unroll
+      for (int d = 0; d < 3; ++d)
+        T9[(c * 3 + d) * nb + b] += rb[c] * v[d];
+      D3[c * nb + b] += rb[c] * a[c];
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    const long i = c * npad + g;
+    const real uu = Ub[i] + Kacc[i];
+    const real vv = Vb[i] - br[c];
+    const real vbn = vv + dt * uu;
+    const real abn = dto2 * vv + dt2o2 * uu;
+    Ub[i] = uu;
+    Vb[i] = vbn;
+    Rb[i] = Minv[i] * (abn + dto2 * vbn);
+    Kacc[i] = 0;
+  }
 }
 
 template <typename real>
@@ -381,28 +632,9 @@ public:
       xi3_.w[i] = A.quad.xi[i];
     if (nodes_geometry && !A.geom_elem)
       try_geometry_from_nodes();
-    mat_ = sizeof(real) == 8 ? upload<real>(A.mat64) : upload<real>(A.mat);
     sslot_ = upload<int>(A.stacey_slot);
     nst_ = std::max(1, A.nstacey);
-    {
-      std::vector<double> st(A.stacey.begin(), A.stacey.end());
-      if (sizeof(real) == 8)
-        for (int b = 0; b < A.nstacey; ++b)
-          for (int c = 0; c < 2; ++c)
-            for (int p = 0; p < NPT3; ++p)
-              st[(size_t(b) * 7 + 5 + c) * NPT3 + p] =
-                  A.stacey64[(size_t(b) * 2 + c) * NPT3 + p];
-      stacey_ = upload<real>(st);
-    }
-    {
-      std::vector<double> mi(size_t(3) * npad_, 1.0);
-      for (int c = 0; c < 3; ++c)
-        for (int g = 0; g < A.nglob; ++g)
-          mi[size_t(c) * npad_ + g] = sizeof(real) == 8
-                                          ? A.minv64[size_t(c) * A.nglob + g]
-                                          : A.minv[size_t(c) * A.nglob + g];
-      minv_ = upload<real>(mi);
-    }
+    upload_material();
     src_slot_ = upload<int>(A.src_slot);
     src_array_ = upload<real>(A.src_array);
     stf_ = upload<real>(A.stf);
@@ -425,6 +657,7 @@ public:
     dt_ = A.dt;
     dto2_ = A.dt_over2;
     dt2o2_ = A.dt2_over2;
+    build_boundary_nodes();
     SMX3_HIP(hipStreamSynchronize(stream_));
   }
 
@@ -438,6 +671,9 @@ public:
       (void)hipFree(p);
     (void)hipEventDestroy(ev0_);
     (void)hipEventDestroy(ev1_);
+    for (hipEvent_t e : adj_ev_)
+      if (e)
+        (void)hipEventDestroy(e);
     (void)hipStreamDestroy(stream_);
   }
 
@@ -550,6 +786,143 @@ public:
     return stiff_ms_;
   }
 
+  // ---------------------------------------------------------------- adjoint
+  void set_model(const float *kappa, const float *mu,
+                 const float *rho) override {
+    SMX3_HIP(hipStreamSynchronize(stream_));
+    set_model3d(*asm_, kappa, mu, rho);
+    upload_material();
+    build_boundary_nodes();
+    drop_graph(); // the captured launches hold the old material buffers
+    ckpt_every_ = 0;
+    have_gradient_ = false;
+    SMX3_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void run_forward_checkpointed(int every) override {
+    if (!initialized_)
+      throw SmxError(SMX_ERR_STATE, "smx_initialize has not been called");
+    const HostAssembly3D &A = *asm_;
+    if (every < 1)
+      throw SmxError(SMX_ERR_ARG, "checkpoint interval must be positive");
+    every = std::min(every, A.nstep);
+    const int nseg = (A.nstep + every - 1) / every;
+    const size_t f3 = size_t(3) * npad_;
+    if (nseg > ckpt_cap_) {
+      release(ckpt_);
+      ckpt_ = alloc<real>(size_t(nseg) * 2 * f3);
+      ckpt_cap_ = nseg;
+    }
+    ensure_events();
+    SMX3_HIP(hipEventRecord(adj_ev_[0], stream_));
+    for (real *p : {U_, V_, A_}) // from rest
+      SMX3_HIP(hipMemsetAsync(p, 0, f3 * sizeof(real), stream_));
+    k3_set_step<<<1, 64, 0, stream_>>>(step_, 0);
+    SMX3_HIP(hipGetLastError());
+    k3_pred<real><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
+        U_, V_, A_, 3 * npad_, dt_, dto2_, dt2o2_);
+    SMX3_HIP(hipGetLastError());
+    for (int sg = 0; sg < nseg; ++sg) {
+      const int k0 = sg * every, len = std::min(every, A.nstep - k0);
+      real *ck = ckpt_ + size_t(sg) * 2 * f3;
+      SMX3_HIP(hipMemcpyAsync(ck, U_, f3 * sizeof(real), hipMemcpyDeviceToDevice,
+                              stream_));
+      SMX3_HIP(hipMemcpyAsync(ck + f3, V_, f3 * sizeof(real),
+                              hipMemcpyDeviceToDevice, stream_));
+      const bool last = sg == nseg - 1;
+      const int body = last ? len - 1 : len;
+      const int reps = body / gsteps_;
+      if (reps > 0) {
+        build_graph();
+        for (int r = 0; r < reps; ++r)
+          SMX3_HIP(hipGraphLaunch(graph_exec_, stream_));
+      }
+      for (int st = reps * gsteps_; st < body; ++st)
+        step<false>();
+      if (last)
+        step<true>();
+    }
+    SMX3_HIP(hipEventRecord(adj_ev_[1], stream_));
+    ckpt_every_ = every;
+    have_gradient_ = false;
+    have_adj_stats_ = false;
+  }
+
+  void run_adjoint(const void *gsrc_host) override {
+    const HostAssembly3D &A = *asm_;
+    if (!ckpt_every_)
+      throw SmxError(SMX_ERR_STATE,
+                     "no checkpointed forward run for the current model");
+    const int every = ckpt_every_;
+    const int nseg = (A.nstep + every - 1) / every;
+    const size_t f3 = size_t(3) * npad_;
+    ensure_adjoint(every);
+    const size_t ng = size_t(A.nstep) * A.nrec * 3;
+    if (ng)
+      SMX3_HIP(hipMemcpyAsync(gsrc_, gsrc_host, ng * sizeof(real),
+                              hipMemcpyHostToDevice, stream_));
+    SMX3_HIP(hipEventRecord(adj_ev_[2], stream_));
+    for (real *p : {ub_, vb_, rb_, kacc_})
+      SMX3_HIP(hipMemsetAsync(p, 0, f3 * sizeof(real), stream_));
+    for (real *p : {gd_, ge_})
+      SMX3_HIP(hipMemsetAsync(p, 0, size_t(A.nspec) * NPT3 * sizeof(real), stream_));
+    SMX3_HIP(hipMemsetAsync(racc_, 0, size_t(npad_) * sizeof(real), stream_));
+    SMX3_HIP(hipMemsetAsync(t9_, 0, size_t(9) * nbn_ * sizeof(real), stream_));
+    SMX3_HIP(hipMemsetAsync(d3_, 0, size_t(3) * nbn_ * sizeof(real), stream_));
+    const Rec3<real> rec{rec_a_, rec_u_, rec_vb_, bidx_, every, npad_, nbn_alloc_};
+    for (int sg = nseg - 1; sg >= 0; --sg) {
+      const int k0 = sg * every, len = std::min(every, A.nstep - k0);
+      const real *ck = ckpt_ + size_t(sg) * 2 * f3;
+      // recompute the segment from its checkpoint, recording every step
+      SMX3_HIP(hipMemcpyAsync(U_, ck, f3 * sizeof(real), hipMemcpyDeviceToDevice,
+                              stream_));
+      SMX3_HIP(hipMemcpyAsync(V_, ck + f3, f3 * sizeof(real),
+                              hipMemcpyDeviceToDevice, stream_));
+      SMX3_HIP(hipMemcpyAsync(rec_u_, ck, f3 * sizeof(real),
+                              hipMemcpyDeviceToDevice, stream_));
+      SMX3_HIP(hipMemsetAsync(A_, 0, f3 * sizeof(real), stream_));
+      if (nbn_ > 0) {
+        k3_gather_vb<real><<<dim3(unsigned((nbn_ + 255) / 256)), 256, 0, stream_>>>(
+            V_, bnode_, nbn_alloc_, npad_, rec_vb_);
+        SMX3_HIP(hipGetLastError());
+      }
+      k3_set_step<<<1, 64, 0, stream_>>>(step_, k0);
+      SMX3_HIP(hipGetLastError());
+      for (int st = 0; st < len; ++st)
+        record_step(rec);
+      // adjoint steps k0+len-1 .. k0
+      k3_set_step<<<1, 64, 0, stream_>>>(adj_step_, k0 + len - 1);
+      SMX3_HIP(hipGetLastError());
+      for (int st = 0; st < len; ++st)
+        adjoint_step(every);
+    }
+    SMX3_HIP(hipEventRecord(adj_ev_[3], stream_));
+    have_gradient_ = true;
+    have_adj_stats_ = true;
+    grad_valid_ = false;
+  }
+
+  void get_gradient(double *out) override {
+    if (!have_gradient_)
+      throw SmxError(SMX_ERR_STATE, "no adjoint run");
+    if (!grad_valid_)
+      finalize_gradient();
+    std::copy(grad_.begin(), grad_.end(), out);
+  }
+
+  void last_adjoint_stats(double *forward_ms, double *adjoint_ms) override {
+    float f = 0, a = 0;
+    if (have_adj_stats_) {
+      SMX3_HIP(hipEventSynchronize(adj_ev_[3]));
+      SMX3_HIP(hipEventElapsedTime(&f, adj_ev_[0], adj_ev_[1]));
+      SMX3_HIP(hipEventElapsedTime(&a, adj_ev_[2], adj_ev_[3]));
+    }
+    if (forward_ms)
+      *forward_ms = f;
+    if (adjoint_ms)
+      *adjoint_ms = a;
+  }
+
 private:
   template <typename T, typename H> T *upload(const std::vector<H> &h) {
     T *d = alloc<T>(h.size());
@@ -568,6 +941,308 @@ private:
     SMX3_HIP(hipMemsetAsync(p, 0, std::max<size_t>(n, 1) * sizeof(T), stream_));
     return static_cast<T *>(p);
   }
+  template <typename T> void release(T *&p) {
+    if (!p)
+      return;
+    auto it = std::find(allocs_.begin(), allocs_.end(), static_cast<void *>(p));
+    if (it != allocs_.end())
+      allocs_.erase(it);
+    SMX3_HIP(hipStreamSynchronize(stream_));
+    (void)hipFree(p);
+    p = nullptr;
+  }
+  void drop_graph() {
+    if (graph_exec_)
+      (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_)
+      (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+  }
+  void ensure_events() {
+    for (hipEvent_t &e : adj_ev_)
+      if (!e)
+        SMX3_HIP(hipEventCreate(&e));
+  }
+
+  // Moduli, Stacey terms (impedances in double for FP64) and inverse mass.
+  void upload_material() {
+    const HostAssembly3D &A = *asm_;
+    release(mat_);
+    mat_ = sizeof(real) == 8 ? upload<real>(A.mat64) : upload<real>(A.mat);
+    std::vector<double> st(A.stacey.begin(), A.stacey.end());
+    if (sizeof(real) == 8)
+      for (int b = 0; b < A.nstacey; ++b)
+        for (int c = 0; c < 2; ++c)
+          for (int p = 0; p < NPT3; ++p)
+            st[(size_t(b) * 7 + 5 + c) * NPT3 + p] =
+                A.stacey64[(size_t(b) * 2 + c) * NPT3 + p];
+    release(stacey_);
+    stacey_ = upload<real>(st);
+    std::vector<double> mi(size_t(3) * npad_, 1.0);
+    for (int c = 0; c < 3; ++c)
+      for (int g = 0; g < A.nglob; ++g)
+        mi[size_t(c) * npad_ + g] = sizeof(real) == 8
+                                        ? A.minv64[size_t(c) * A.nglob + g]
+                                        : A.minv[size_t(c) * A.nglob + g];
+    release(minv_);
+    minv_ = upload<real>(mi);
+  }
+
+  // Absorbing nodes (first touch in slot order), node -> index, and each
+  // node's Stacey matrix B = sum over its face points of
+  // |n| w [(rho vp - rho vs) n n^T / |n|^2 + rho vs I], formed in double.
+  void build_boundary_nodes() {
+    const HostAssembly3D &A = *asm_;
+    std::vector<int> bidx(size_t(npad_), -1), bnode;
+    std::vector<double> bm;
+    for (int s = 0; s < A.nspec; ++s) {
+      const int b = A.stacey_slot[s];
+      if (b < 0)
+        continue;
+      auto at = [&](int c, int p) { return A.stacey[(size_t(b) * 7 + c) * NPT3 + p]; };
+      for (int p = 0; p < NPT3; ++p) {
+        if (at(4, p) == 0.0f)
+          continue;
+        const int g = A.ibool[size_t(s) * NPT3 + p];
+        if (bidx[g] < 0) {
+          bidx[g] = int(bnode.size());
+          bnode.push_back(g);
+          bm.resize(bm.size() + 6, 0.0);
+        }
+        const double n[3] = {at(0, p), at(1, p), at(2, p)};
+        const double nrm = at(3, p), w = at(4, p);
+        const double P = sizeof(real) == 8 ? A.stacey64[(size_t(b) * 2) * NPT3 + p]
+                                           : double(at(5, p));
+        const double S = sizeof(real) == 8 ? A.stacey64[(size_t(b) * 2 + 1) * NPT3 + p]
+                                           : double(at(6, p));
+        const double f = nrm * w, q = (P - S) / (nrm * nrm);
+        double *m = &bm[size_t(bidx[g]) * 6];
+        m[0] += f * (q * n[0] * n[0] + S);
+        m[1] += f * (q * n[1] * n[1] + S);
+        m[2] += f * (q * n[2] * n[2] + S);
+        m[3] += f * q * n[0] * n[1];
+        m[4] += f * q * n[0] * n[2];
+        m[5] += f * q * n[1] * n[2];
+      }
+    }
+    nbn_ = long(bnode.size());
+    const long nb = std::max(1L, nbn_);
+    std::vector<double> bmt(size_t(6) * nb, 0.0);
+    for (long b = 0; b < nbn_; ++b)
+      for (int k = 0; k < 6; ++k)
+        bmt[size_t(k) * nb + b] = bm[size_t(b) * 6 + k];
+    bidx_h_ = bidx;
+    release(bidx_);
+    release(bnode_);
+    release(bmat_);
+    bidx_ = upload<int>(bidx);
+    bnode_ = upload<int>(bnode);
+    bmat_ = upload<real>(bmt);
+    nbn_alloc_ = nb;
+    // adjoint buffers sized by nb are re-made on the next adjoint run
+    release(t9_);
+    release(d3_);
+    rec_every_ = 0;
+  }
+
+  // Device buffers of the adjoint run (kept between runs).
+  void ensure_adjoint(int every) {
+    const HostAssembly3D &A = *asm_;
+    const size_t f3 = size_t(3) * npad_;
+    if (!ub_) {
+      ub_ = alloc<real>(f3);
+      vb_ = alloc<real>(f3);
+      rb_ = alloc<real>(f3);
+      kacc_ = alloc<real>(f3);
+      racc_ = alloc<real>(size_t(npad_));
+      gd_ = alloc<real>(size_t(A.nspec) * NPT3);
+      ge_ = alloc<real>(size_t(A.nspec) * NPT3);
+      adj_step_ = alloc<int>(1);
+      gsrc_ = alloc<real>(std::max<size_t>(size_t(A.nstep) * A.nrec * 3, 1));
+      // receiver injection: nodes touched by receivers, entries in
+      // (receiver, point) order
+      std::vector<std::vector<std::pair<int, float>>> lists;
+      std::vector<int> idx(size_t(npad_), -1), nodes;
+      for (int r = 0; r < A.nrec; ++r)
+        for (int p = 0; p < NPT3; ++p) {
+          const float h = A.rec_h[size_t(r) * NPT3 + p];
+          if (h == 0.0f)
+            continue;
+          const int g = A.rec_g[size_t(r) * NPT3 + p];
+          if (idx[g] < 0) {
+            idx[g] = int(nodes.size());
+            nodes.push_back(g);
+            lists.emplace_back();
+          }
+          lists[idx[g]].push_back({r, h});
+        }
+      std::vector<int> off(1, 0), er;
+      std::vector<float> eh;
+      for (auto &l : lists) {
+        for (auto &e : l) {
+          er.push_back(e.first);
+          eh.push_back(e.second);
+        }
+        off.push_back(int(er.size()));
+      }
+      nrin_ = int(nodes.size());
+      rin_node_ = upload<int>(nodes);
+      rin_off_ = upload<int>(off);
+      rin_r_ = upload<int>(er);
+      rin_h_ = upload<real>(eh);
+    }
+    if (!t9_) {
+      t9_ = alloc<real>(size_t(9) * nbn_alloc_);
+      d3_ = alloc<real>(size_t(3) * nbn_alloc_);
+    }
+    if (every != rec_every_) {
+      release(rec_u_);
+      release(rec_a_);
+      release(rec_vb_);
+      rec_u_ = alloc<real>(size_t(every) * f3);
+      rec_a_ = alloc<real>(size_t(every) * f3);
+      rec_vb_ = alloc<real>(size_t(every) * 3 * nbn_alloc_);
+      rec_every_ = every;
+    }
+  }
+
+  // One forward step of the recompute, recording a', u~ and v~.
+  void record_step(const Rec3<real> &rec) {
+    const HostAssembly3D &A = *asm_;
+    if (A.nsrc > 0) {
+      k3_src<real><<<1, ELEM_THREADS, 0, stream_>>>(
+          A.nsrc, src_slot_, src_array_, stf_, ibool_, A.nspec, A_, npad_,
+          step_);
+      SMX3_HIP(hipGetLastError());
+    }
+    elem_passes(A_);
+    k3_node<real, false, true><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
+        U_, V_, A_, minv_, 3 * npad_, dt_, dto2_, dt2o2_, step_, rec);
+    SMX3_HIP(hipGetLastError());
+    k3_step_add<<<1, 64, 0, stream_>>>(step_, 1);
+    SMX3_HIP(hipGetLastError());
+  }
+
+  // One adjoint step (k = *adj_step, then k - 1).
+  void adjoint_step(int every) {
+    const HostAssembly3D &A = *asm_;
+    if (nrin_ > 0) {
+      k3_adj_inject<real><<<dim3(unsigned((nrin_ + 127) / 128)), 128, 0, stream_>>>(
+          rin_node_, rin_off_, rin_r_, rin_h_, nrin_, gsrc_, A.nrec, ub_, npad_,
+          adj_step_);
+      SMX3_HIP(hipGetLastError());
+    }
+    for (int c = 0; c < A.ncolors; ++c) {
+      const int s0 = A.color_off[c], cnt = A.color_off[c + 1] - s0;
+      if (cnt == 0)
+        continue;
+      auto launch = [&](auto kern) {
+        kern<<<dim3(cnt), dim3(ELEM_THREADS), 0, stream_>>>(
+            s0, ibool_, geom_, nodes_, xi3_, mat_, rb_, rec_u_, every,
+            adj_step_, kacc_, gd_, ge_, npad_, quad_);
+      };
+      const bool me = A.mat_elem;
+      switch (gmode_) {
+      case GEOM_ELEM:
+        me ? launch(k3_adj_elem<real, GEOM_ELEM, true>)
+           : launch(k3_adj_elem<real, GEOM_ELEM, false>);
+        break;
+      case GEOM_NODES:
+        me ? launch(k3_adj_elem<real, GEOM_NODES, true>)
+           : launch(k3_adj_elem<real, GEOM_NODES, false>);
+        break;
+      default:
+        me ? launch(k3_adj_elem<real, GEOM_POINT, true>)
+           : launch(k3_adj_elem<real, GEOM_POINT, false>);
+      }
+      SMX3_HIP(hipGetLastError());
+    }
+    k3_adj_node<real><<<dim3(unsigned((npad_ + NODE3_THREADS - 1) / NODE3_THREADS)),
+                        NODE3_THREADS, 0, stream_>>>(
+        ub_, vb_, rb_, kacc_, minv_, rec_a_, rec_vb_, every, racc_, bidx_,
+        bmat_, t9_, d3_, nbn_alloc_, npad_, A.nglob, dt_, dto2_, dt2o2_,
+        adj_step_);
+    SMX3_HIP(hipGetLastError());
+    k3_step_add<<<1, 64, 0, stream_>>>(adj_step_, -1);
+    SMX3_HIP(hipGetLastError());
+  }
+
+  // Chain rule and quadrature on the host, in double:
+  //   kappa: -wJ Gd,  mu: -wJ (2 Ge - 2/3 Gd),  rho: -wJ (rb . a')_node,
+  // plus, at absorbing points, the rho vp / rho vs terms of B and of the
+  // Stacey mass, differentiated through rho vp = sqrt(rho (kappa + 4/3 mu))
+  // and rho vs = sqrt(rho mu).
+  void finalize_gradient() {
+    const HostAssembly3D &A = *asm_;
+    SMX3_HIP(hipStreamSynchronize(stream_));
+    const size_t nloc = size_t(A.nspec) * NPT3;
+    std::vector<real> gd(nloc), ge(nloc), R(static_cast<size_t>(npad_)),
+        t9(size_t(9) * nbn_alloc_), d3(size_t(3) * nbn_alloc_);
+    SMX3_HIP(hipMemcpy(gd.data(), gd_, nloc * sizeof(real), hipMemcpyDeviceToHost));
+    SMX3_HIP(hipMemcpy(ge.data(), ge_, nloc * sizeof(real), hipMemcpyDeviceToHost));
+    SMX3_HIP(hipMemcpy(R.data(), racc_, R.size() * sizeof(real), hipMemcpyDeviceToHost));
+    SMX3_HIP(hipMemcpy(t9.data(), t9_, t9.size() * sizeof(real), hipMemcpyDeviceToHost));
+    SMX3_HIP(hipMemcpy(d3.data(), d3_, d3.size() * sizeof(real), hipMemcpyDeviceToHost));
+    grad_.assign(3 * nloc, 0.0);
+    std::vector<int> slot_elem(A.nspec);
+    for (int e = 0; e < A.nspec; ++e)
+      slot_elem[A.elem_slot[e]] = e;
+    const long nb = nbn_alloc_;
+    const double hdt = 0.5 * double(A.dt);
+    for (int s = 0; s < A.nspec; ++s) {
+      const size_t oe = size_t(slot_elem[s]) * NPT3;
+      const int b = A.stacey_slot[s];
+      for (int iz = 0; iz < NGLL; ++iz)
+        for (int iy = 0; iy < NGLL; ++iy)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const int p = (iz * NGLL + iy) * NGLL + ix;
+            const size_t i = size_t(s) * NPT3 + p;
+            const int g = A.ibool[i];
+            const double wj = double(A.quad.w[iz]) * A.quad.w[iy] * A.quad.w[ix] *
+                              A.jac_pt[i];
+            double gk = -wj * double(gd[i]);
+            double gm = -wj * (2.0 * double(ge[i]) - 2.0 / 3.0 * double(gd[i]));
+            double gr = -wj * double(R[g]);
+            if (b >= 0 && A.stacey[(size_t(b) * 7 + 4) * NPT3 + p] != 0.0f) {
+              auto at = [&](int c) { return double(A.stacey[(size_t(b) * 7 + c) * NPT3 + p]); };
+              const double nv[3] = {at(0), at(1), at(2)};
+              const double nrm = std::sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+              const double nh[3] = {nv[0] / nrm, nv[1] / nrm, nv[2] / nrm};
+              const double wt = at(4), s1 = nh[0] + nh[1] + nh[2];
+              const long bi = bidx_h_[g];
+              double rnvn = 0, tr = 0, tp = 0, ts = 0;
+              for (int c = 0; c < 3; ++c) {
+                tr += double(t9[size_t(c * 3 + c) * nb + bi]);
+                for (int d = 0; d < 3; ++d)
+                  rnvn += nh[c] * nh[d] * double(t9[size_t(c * 3 + d) * nb + bi]);
+                const double D = d3[size_t(c) * nb + bi];
+                tp += D * nh[c] * s1;
+                ts += D * (1.0 - nh[c] * s1);
+              }
+              const double dP = -(nrm * wt * rnvn + hdt * nrm * wt * tp);
+              const double dS = -(nrm * wt * (tr - rnvn) + hdt * nrm * wt * ts);
+              const size_t o3 = size_t(s) * 3 * NPT3 + p;
+              const double kk = A.kappa_mu_rho[o3], m = A.kappa_mu_rho[o3 + NPT3],
+                           rh = A.kappa_mu_rho[o3 + 2 * NPT3];
+              const double L = kk + 4.0 / 3.0 * m;
+              const double P = std::sqrt(rh * L), S = std::sqrt(rh * m);
+              gk += dP * rh / (2.0 * P);
+              gm += dP * (4.0 / 3.0) * rh / (2.0 * P);
+              gr += dP * L / (2.0 * P);
+              if (S > 0) {
+                gm += dS * rh / (2.0 * S);
+                gr += dS * m / (2.0 * S);
+              }
+            }
+            grad_[oe + p] = gk;
+            grad_[nloc + oe + p] = gm;
+            grad_[2 * nloc + oe + p] = gr;
+          }
+    }
+    grad_valid_ = true;
+  }
+
   real *pick(int which) {
     switch (which) {
     case SMX_U: return U_;
@@ -661,8 +1336,8 @@ private:
           npad_, dto2_, step_, seis_);
       SMX3_HIP(hipGetLastError());
     }
-    k3_node<real, FINAL><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
-        U_, V_, A_, minv_, 3 * npad_, dt_, dto2_, dt2o2_, step_);
+    k3_node<real, FINAL, false><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
+        U_, V_, A_, minv_, 3 * npad_, dt_, dto2_, dt2o2_, step_, Rec3<real>{});
     SMX3_HIP(hipGetLastError());
   }
 
@@ -708,6 +1383,20 @@ private:
   bool initialized_ = false, have_stats_ = false;
   double stiff_ms_ = 0;
   int launches_ = 0;
+  // adjoint
+  int ckpt_every_ = 0, ckpt_cap_ = 0, rec_every_ = 0, nrin_ = 0;
+  long nbn_ = 0, nbn_alloc_ = 1;
+  std::vector<int> bidx_h_;
+  int *bidx_ = nullptr, *bnode_ = nullptr, *adj_step_ = nullptr,
+      *rin_node_ = nullptr, *rin_off_ = nullptr, *rin_r_ = nullptr;
+  real *ckpt_ = nullptr, *rec_u_ = nullptr, *rec_a_ = nullptr,
+       *rec_vb_ = nullptr, *ub_ = nullptr, *vb_ = nullptr, *rb_ = nullptr,
+       *kacc_ = nullptr, *racc_ = nullptr, *gd_ = nullptr, *ge_ = nullptr,
+       *t9_ = nullptr, *d3_ = nullptr, *bmat_ = nullptr, *gsrc_ = nullptr,
+       *rin_h_ = nullptr;
+  hipEvent_t adj_ev_[4] = {};
+  bool have_gradient_ = false, have_adj_stats_ = false, grad_valid_ = false;
+  std::vector<double> grad_;
 };
 
 std::unique_ptr<Solver3DBase> make_solver3d(std::unique_ptr<HostAssembly3D> a,

@@ -28,6 +28,20 @@ public:
   // 2 re-formed from the control nodes) and, for 2, the number of stored
   // values the device did not reproduce at set-up (then 0 is used).
   virtual void geometry_mode(int *mode, long long *mismatches) const = 0;
+
+  // ---- discrete adjoint (same contract as the 2D solver's)
+  // Replaces the material ([nspec][125] each, mesh element order).
+  virtual void set_model(const float *kappa, const float *mu,
+                         const float *rho) = 0;
+  // All nstep steps from rest, keeping (u~, v~) every `every` steps.
+  virtual void run_forward_checkpointed(int every) = 0;
+  // Adjoint run for dchi/ds = gsrc [nstep][nrec][3] (context precision), s
+  // the displacement seismogram of each step; recomputes each segment from
+  // its checkpoint (the forward fields are left at some segment's end).
+  virtual void run_adjoint(const void *gsrc) = 0;
+  // dchi / d(kappa, mu, rho) [3][nspec][125], mesh element order.
+  virtual void get_gradient(double *out) = 0;
+  virtual void last_adjoint_stats(double *forward_ms, double *adjoint_ms) = 0;
 };
 
 // graph_steps: time steps per replayed HIP graph; nodes_geometry: re-form

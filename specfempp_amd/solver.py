@@ -194,6 +194,8 @@ class Context:
         return t.value
 
     # ---- adjoint / Frechet path ------------------------------------------
+    ncomp = 2
+
     def set_model(self, kappa, mu, rho):
         """Replaces the material (each (nspec, 5, 5)) and rebuilds the mass."""
         k, m, r = (np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(self.nspec, 5, 5))

@@ -119,6 +119,38 @@ class Context3D:
         check(self.lib.smx_time_stiffness(self.h, int(nlaunch), ctypes.byref(t)))
         return t.value
 
+    # ---- adjoint (same contract as the 2D Context)
+    ncomp = 3
+
+    def set_model(self, kappa, mu, rho):
+        """Replaces the material (each (nspec, 5, 5, 5)) and rebuilds the mass."""
+        k, m, r = (np.ascontiguousarray(np.asarray(x, dtype=np.float32).reshape(self.nspec, 125))
+                   for x in (kappa, mu, rho))
+        check(self.lib.smx_set_model(self.h, _p(k), _p(m), _p(r)))
+
+    def run_forward_checkpointed(self, every=100):
+        """All nstep steps from rest, keeping the state every ``every`` steps."""
+        check(self.lib.smx_run_forward_checkpointed(self.h, int(every)))
+
+    def run_adjoint(self, gsrc):
+        """Adjoint run for dchi/ds = gsrc (nstep, nrec, 3), s the displacement
+        seismogram after each step.  The recompute overwrites the forward
+        fields (the seismograms stay)."""
+        g = np.ascontiguousarray(np.asarray(gsrc, dtype=self.dtype).reshape(self.nstep, self.nrec, 3))
+        check(self.lib.smx_run_adjoint(self.h, _p(g)))
+
+    def gradient(self):
+        """(3, nspec, 5, 5, 5) float64: d chi / d (kappa, mu, rho) per GLL point."""
+        out = np.empty((3, self.nspec, 5, 5, 5), dtype=np.float64)
+        check(self.lib.smx_get_gradient(self.h, _p(out)))
+        return out
+
+    def last_adjoint_ms(self):
+        """(checkpointed forward ms, adjoint ms incl. recompute) of the last runs."""
+        f, a = ctypes.c_double(), ctypes.c_double()
+        check(self.lib.smx_last_adjoint_stats(self.h, ctypes.byref(f), ctypes.byref(a)))
+        return f.value, a.value
+
 
 def simulate3d(config, *, precision=4, nstep=None, options=None):
     """Runs a forward simulation of a Config3D; returns (seismograms, ctx)."""

@@ -90,3 +90,35 @@ def perturbed_model(cfg, scale=(1.02, 0.97, 1.01)):
     m = cfg.mesh
     return tuple((x.astype(np.float64) * s).astype(np.float32)
                  for x, s in zip((m.kappa, m.mu, m.rho), scale))
+
+
+def adjoint_small3d(n=(4, 3, 3), nstep=160, dt=1e-3, seed=7, moment=True):
+    """Small heterogeneous 3D box for adjoint checks: 100 m hex8 elements,
+    free top, Stacey on the five other faces, one Ricker source (f0 = 15 Hz,
+    moment tensor or force), four receivers; kappa, mu, rho carry smooth
+    seeded +-3% perturbations per GLL point."""
+    from specfempp_amd.mesh3d import (Config3D, Receivers3D, Sources3D, box_mesh3d,
+                                      gll_coordinates3d)
+    h = 100.0
+    nx, ny, nz = n
+    Lx, Ly, Lz = nx * h, ny * h, nz * h
+    mesh = box_mesh3d(0.0, Lx, 0.0, Ly, -Lz, 0.0, nx, ny, nz,
+                      rho=2600.0, vp=3200.0, vs=1850.0)
+    rng = np.random.default_rng(seed)
+    xyz = gll_coordinates3d(mesh)
+    for name in ("kappa", "mu", "rho"):
+        k = rng.uniform(1.0, 3.0, 3)
+        ph = rng.uniform(0, 2 * np.pi)
+        f = (np.sin(k[0] * np.pi * xyz[..., 0] / Lx + ph) * np.cos(k[1] * np.pi * xyz[..., 1] / Ly)
+             * np.cos(k[2] * np.pi * xyz[..., 2] / Lz))
+        setattr(mesh, name, (getattr(mesh, name) * (1.0 + 0.03 * f)).astype(np.float32))
+    if moment:
+        src = Sources3D.make([Lx * 0.45], [Ly * 0.55], [-Lz * 0.6],
+                             moment=(1.0, 0.8, 1.2, 0.3, -0.2, 0.1), f0=15.0, factor=1e13)
+    else:
+        src = Sources3D.make([Lx * 0.45], [Ly * 0.55], [-Lz * 0.6], force=(0.3, -0.2, 1.0),
+                             f0=15.0, factor=1e10)
+    rec = Receivers3D(x=np.array([60.0, Lx - 70.0, Lx * 0.5, Lx * 0.3]),
+                      y=np.array([50.0, Ly * 0.5, Ly - 40.0, Ly * 0.3]),
+                      z=np.array([0.0, -30.0, -Lz * 0.5, -Lz + 40.0]))
+    return Config3D(f"adjoint_small3d_{nx}x{ny}x{nz}", mesh, src, rec, dt, nstep)
