@@ -224,13 +224,15 @@ def batch_leg(nshots=64, nstep=3000, baseline_steps=300):
             "host_s": {"generate": gen_s, "autotune": tune_s, "assemble": setup_s}}
 
 
-def hex_leg(n=64, nstep=4000):
+def hex_leg(n=64, nstep=4000, with_adjoint=True):
     """3D elastic forward: 64^3 hex8 elements of 100 m (6.4 km cube, 262,144
     elements), vp linear 3000 -> 6000 m/s with depth, vs = vp/sqrt(3), rho by
     Gardner, +-2 % seeded Gaussian random perturbation (correlation 500 m);
     free top, Stacey on five faces; explosion moment tensor (1e15), Ricker
     f0 = 2 Hz at 3 km depth; 256 surface receivers; dt = 1.25e-3 s (2e-3 s is
-    beyond the stability limit), 4000 steps, FP32."""
+    beyond the stability limit), 4000 steps, FP32.  Then the adjoint
+    gradient (kappa, mu, rho) of the L2 misfit against zero data, with
+    checkpoints every 100 steps."""
     import numpy as np
     from specfempp_amd.mesh3d import hex_box_case
     from specfempp_amd.solver3d import Context3D
@@ -250,6 +252,28 @@ def hex_leg(n=64, nstep=4000):
     seis = ctx.seismograms()
     stiff_ms = ctx.time_stiffness(10)
     nglob, nspec, geometry = ctx.nglob, ctx.nspec, ctx.geometry
+    adjoint = None
+    if with_adjoint:
+        # gradient of chi = 1/2 dt sum |s|^2 (the L2 misfit against zero data):
+        # checkpointed forward, then recompute + adjoint, one GPU
+        every = 100
+        t = time.perf_counter()
+        ctx.run_forward_checkpointed(every)
+        ctx.synchronize()
+        s = ctx.seismograms()[:, 0].astype(np.float64)
+        ctx.run_adjoint((cfg.dt * s).astype(np.float32))
+        ctx.synchronize()
+        grad_wall = time.perf_counter() - t
+        fwd_ms, adj_ms = ctx.last_adjoint_ms()
+        g = ctx.gradient()
+        adjoint = {
+            "every": every, "checkpointed_forward_ms": fwd_ms, "adjoint_ms": adj_ms,
+            "gradient_ms": fwd_ms + adj_ms, "gradient_wall_s": grad_wall,
+            "gradient_over_forward": (fwd_ms + adj_ms) / (wall * 1e3),
+            "gradient_finite": bool(np.isfinite(g).all()),
+            "gradient_norms": [float(np.linalg.norm(g[i])) for i in range(3)],
+            "device_buffers_gb": (2 * ((nstep + every - 1) // every) + 2 * every)
+                                 * 3 * nglob * 4 / 1e9}
     ctx.close()
     canon = 76 * nglob + 6500 * nspec
     return {"workload": f"3D elastic, {n}^3 hex8 of 100 m, vp 3000->6000 m/s +-2 % random, "
@@ -260,8 +284,112 @@ def hex_leg(n=64, nstep=4000):
             "canonical_bytes_per_step": canon,
             "canonical_frac": canon * nstep / wall / 1e9 / HBM_PEAK_GBS,
             "element_passes_ms": stiff_ms, "seismograms_finite": bool(np.isfinite(seis).all()),
-            "geometry": geometry,
+            "geometry": geometry, "adjoint": adjoint,
             "host_s": {"generate": gen_s, "assemble": setup_s}}
+
+
+def _rel(a, b):
+    import numpy as np
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def errors_leg(c1_oracle_steps=200):
+    """Accuracy at the full workload sizes (run with --only-leg errors):
+    every workload's FP32 seismograms against an FP64 run of the same GPU
+    code, plus the FP32 oracle where it finishes in seconds to minutes:
+    config 80x60 (all 2000 steps, seismograms), 800x600 (displacement field
+    at every GLL point after `c1_oracle_steps` steps, compared point by point
+    through both numberings), the two smallest batched shots (all 3000 steps,
+    seismograms).  The 3D oracle's brute-force point location makes the
+    64^3 set-up take hours; its parity is tested on the same generator at
+    12^3."""
+    import numpy as np
+    import oracle
+    from specfempp_amd.adjoint import misfit_and_gradient
+    from specfempp_amd.config import adjoint_case, batched_shots, homogeneous_box, layered_box
+    from specfempp_amd.mesh3d import hex_box_case
+    from specfempp_amd.solver import Context, simulate, simulate_batch
+    from specfempp_amd.solver3d import simulate3d
+
+    out = {}
+    # 80x60 homogeneous box, 2000 steps
+    cfg = homogeneous_box()
+    s32, _ = simulate(cfg)
+    s64, _ = simulate(cfg, precision=8)
+    o = oracle.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    so = o.run()
+    out["80x60"] = {"fp32_vs_fp64": _rel(s32, s64), "fp32_vs_oracle_fp32": _rel(s32[:, 0], so[:, 0]),
+                    "quantity": "displacement seismograms, 50 receivers, 2000 steps"}
+    # 800x600 layered, 5000 steps
+    cfg = layered_box()
+    s32, c32 = simulate(cfg)
+    s64, _ = simulate(cfg, precision=8)
+    from specfempp_amd.config import Receivers2D
+    n = c1_oracle_steps
+    ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, n)
+    ctx.initialize()
+    ctx.run_forward(0, n)
+    ctx.synchronize()
+    ug = ctx.get_field("u")
+    ibg = ctx.ibool().reshape(-1)
+    o = oracle.Oracle2D(cfg.mesh, cfg.sources, Receivers2D(x=cfg.receivers.x[:1],
+                        z=cfg.receivers.z[:1]), cfg.dt, n)
+    o.run(nstep=n)
+    uo = o.get("u").reshape(-1, 2)
+    ibo = o.get("ibool").astype(np.int64).reshape(-1)
+    out["800x600"] = {"fp32_vs_fp64": _rel(s32, s64),
+                      "quantity": "displacement seismograms, 500 receivers, 5000 steps",
+                      "field_fp32_vs_oracle_fp32": _rel(ug[ibg], uo[ibo]),
+                      "field_quantity": f"displacement at all 12,000,000 local GLL points "
+                                        f"after {n} steps"}
+    del c32, ctx, o
+    # adjoint case: gradient and seismograms, FP32 vs FP64
+    cfg, true = adjoint_case()
+    m0 = (cfg.mesh.kappa, cfg.mesh.mu, cfg.mesh.rho)
+    res = {}
+    for prec in (4, 8):
+        c = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep, precision=prec)
+        c.initialize()
+        c.set_model(*true)
+        c.run_forward(0, cfg.nstep)
+        c.synchronize()
+        obs = c.seismograms()[:, 0].astype(np.float64)
+        res[prec] = misfit_and_gradient(c, m0, obs.astype(c.dtype), every=100)
+        res[prec] = (res[prec][1], c.seismograms()[:, 0].copy())
+    out["adjoint_80x60"] = {"gradient_fp32_vs_fp64": [_rel(res[4][0][i], res[8][0][i])
+                                                      for i in range(3)],
+                            "seismograms_fp32_vs_fp64": _rel(res[4][1], res[8][1])}
+    # 64 batched shots
+    shots = batched_shots()
+    b32, _ = simulate_batch(shots)
+    b64, _ = simulate_batch(shots, precision=8)
+    small = sorted(range(len(shots)), key=lambda k: shots[k].mesh.nspec)[:2]
+    orc = []
+    for k in small:
+        c = shots[k]
+        so = oracle.Oracle2D(c.mesh, c.sources, c.receivers, c.dt, c.nstep).run()
+        orc.append(_rel(b32[k][:, 0], so[:, 0]))
+    # shots whose receivers see an arrival within the 3000 steps (the others
+    # record only the round-off tail of the discrete operator)
+    energy = np.array([float(np.sum(np.asarray(b, np.float64) ** 2)) for b in b64])
+    live = [k for k in range(len(shots)) if energy[k] >= 1e-6 * energy.max()]
+    out["64_shots"] = {"fp32_vs_fp64": _rel(np.concatenate([x.ravel() for x in b32]),
+                                            np.concatenate([x.ravel() for x in b64])),
+                       "shots_with_arrivals": len(live),
+                       "worst_shot_with_arrivals_fp32_vs_fp64":
+                           max(_rel(b32[k], b64[k]) for k in live),
+                       "smallest_two_shots_fp32_vs_oracle_fp32": orc}
+    # 64^3 hex, 4000 steps
+    cfg = hex_box_case()
+    h32, c = simulate3d(cfg)
+    del c
+    h64, c = simulate3d(cfg, precision=8)
+    del c
+    out["hex64"] = {"fp32_vs_fp64": _rel(h32, h64),
+                    "quantity": "displacement seismograms, 256 receivers, 4000 steps"}
+    return out
 
 
 def main():
@@ -278,8 +406,11 @@ def main():
                     help="skip the batched-shots leg (64 shots, 3000 steps)")
     ap.add_argument("--no-3d", action="store_true",
                     help="skip the 3D leg (64^3 hex, 4000 steps)")
-    ap.add_argument("--only-leg", choices=("adjoint", "batch", "hex3d"), default=None,
+    ap.add_argument("--only-leg", choices=("adjoint", "batch", "hex3d", "errors"), default=None,
                     help="profiling aid: run only this leg and print its JSON")
+    ap.add_argument("--hex-steps", type=int, default=4000,
+                    help="time steps of the 3D leg (4000 is the workload; fewer only "
+                         "to profile it)")
     ap.add_argument("--cpu-steps", type=int, default=40)
     ap.add_argument("--traffic", type=float, default=None,
                     help="HBM bytes per k_tile launch measured with rocprofv3 --pmc "
@@ -301,7 +432,9 @@ def main():
                           "launches": 5}))
         return
     if args.only_leg:
-        leg = {"adjoint": adjoint_leg, "batch": batch_leg, "hex3d": hex_leg}[args.only_leg]
+        leg = {"adjoint": adjoint_leg, "batch": batch_leg,
+               "hex3d": lambda: hex_leg(nstep=args.hex_steps),
+               "errors": errors_leg}[args.only_leg]
         print(json.dumps({args.only_leg: leg()}), flush=True)
         return
     if args.gpus != 1:
@@ -383,7 +516,7 @@ def main():
     if not args.no_batch:
         out["batch"] = batch_leg()
     if not args.no_3d:
-        out["hex3d"] = hex_leg()
+        out["hex3d"] = hex_leg(nstep=args.hex_steps)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
     print(json.dumps(out), flush=True)

@@ -272,28 +272,26 @@ __global__ void __launch_bounds__(ELEM_THREADS)
 
 // Seismograms of step `*step_ptr` (if sampled): displacement from U, and
 // velocity / acceleration from the corrected values the node pass is about
-// to form (same arithmetic).  One thread per (type, receiver), points summed
-// in order.
+// to form (same arithmetic).  One wavefront per (type, receiver): lane l sums
+// points l and l + 64 in order, then a fixed butterfly reduction (lane 0's
+// order of additions is fixed, so the result is deterministic).  Grid:
+// exactly nrec * ntypes blocks of 64 threads.
 template <typename real>
-__global__ void k3_seis(const real *__restrict__ U, const real *__restrict__ V,
-                        const real *__restrict__ A,
-                        const real *__restrict__ Minv,
-                        const int *__restrict__ rec_g,
-                        const real *__restrict__ rec_h, int nrec, Types3 types,
-                        int ntypes, int nsbs, long npad, real dto2,
-                        const int *__restrict__ step_ptr,
-                        real *__restrict__ out) {
+__global__ void __launch_bounds__(64)
+    k3_seis(const real *__restrict__ U, const real *__restrict__ V,
+            const real *__restrict__ A, const real *__restrict__ Minv,
+            const int *__restrict__ rec_g, const real *__restrict__ rec_h,
+            int nrec, Types3 types, int ntypes, int nsbs, long npad, real dto2,
+            const int *__restrict__ step_ptr, real *__restrict__ out) {
   const int step = *step_ptr;
   if (step % nsbs != 0)
     return;
   const int isig = step / nsbs;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = k % nrec, it = k / nrec;
-  if (it >= ntypes)
-    return;
+  const int r = blockIdx.x % nrec, it = blockIdx.x / nrec;
   const int type = types.t[it];
+  const int lane = threadIdx.x;
   real s[3] = {0, 0, 0};
-  for (int p = 0; p < NPT3; ++p) {
+  for (int p = lane; p < NPT3; p += 64) {
     const int g = rec_g[size_t(r) * NPT3 + p];
     const real h = rec_h[size_t(r) * NPT3 + p];
 #pragma unroll
@@ -309,8 +307,14 @@ __global__ void k3_seis(const real *__restrict__ U, const real *__restrict__ V,
     }
   }
 #pragma unroll
-  for (int c = 0; c < 3; ++c)
-    out[((size_t(isig) * ntypes + it) * nrec + r) * 3 + c] = s[c];
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      s[c] += __shfl_xor(s[c], off);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      out[((size_t(isig) * ntypes + it) * nrec + r) * 3 + c] = s[c];
 }
 
 // Record buffers of the adjoint's recompute: per step j of a segment,
@@ -1331,7 +1335,7 @@ private:
     const int ntypes = int(A.seis_types.size());
     if (A.nrec > 0 && ntypes > 0) {
       const int n = A.nrec * ntypes;
-      k3_seis<real><<<(n + 127) / 128, 128, 0, stream_>>>(
+      k3_seis<real><<<n, 64, 0, stream_>>>(
           U_, V_, A_, minv_, rec_g_, rec_h_, A.nrec, types_, ntypes, A.nsbs,
           npad_, dto2_, step_, seis_);
       SMX3_HIP(hipGetLastError());

@@ -88,6 +88,25 @@ def test_3d_stacey_heterogeneous_matches_oracle(oracle_lib, moment):
     assert e32 <= 1e-5
 
 
+def test_3d_hex_workload_generator_at_12cubed_matches_oracle(oracle_lib):
+    """The 64^3 workload's generator at 12^3 (same 6.4 km cube, random
+    medium, explosion source at 3 km depth, 256 surface receivers, free top,
+    Stacey on five faces), dt scaled with the element size, 300 steps (2 s):
+    FP32 against the FP32 oracle and against the FP64 run of the same code."""
+    from specfempp_amd.mesh3d import hex_box_case
+    h = 6400.0 / 12
+    cfg = hex_box_case(n=12, h=h, dt=1.25e-3 * h / 100.0, nstep=300)
+    _say("3D workload generator at 12^3, 300 steps")
+    s, ctx = simulate3d(cfg)
+    ref = oracle_lib.Oracle3D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep).run()
+    s64, _ = simulate3d(cfg, precision=8)
+    err, e32 = rel_l2(s, ref), rel_l2(s, s64)
+    _say(f"  fp32 vs oracle {err:.3e}; fp32 vs fp64 {e32:.3e}; geometry {ctx.geometry}")
+    assert np.abs(ref).max() > 0
+    assert err <= 1e-5
+    assert e32 <= 1e-5
+
+
 def _kinetic(ctx):
     v = ctx.get_field("v").astype(np.float64)
     m = 1.0 / ctx.get_field("minv").astype(np.float64)
