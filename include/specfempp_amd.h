@@ -1,6 +1,7 @@
 /*
- * specfempp_amd — MI355X (gfx950) implementation of SPECFEM++'s 2D elastic
- * isotropic P-SV time-stepping path, exposed through a plain C ABI.
+ * specfempp_amd — MI355X (gfx950) implementation of SPECFEM++'s elastic
+ * isotropic time-stepping path (2D P-SV; 3D hex8), its discrete adjoint and
+ * batched multi-shot runs, exposed through a plain C ABI.
  *
  * Every entry point returns 0 on success and a negative status on error;
  * smx_last_error() returns the message of the last failure on the calling
