@@ -204,6 +204,79 @@ void mass3d(HostAssembly3D &A) {
   }
 }
 
+// Consistency of the tile layout (every index the tile kernels use stays
+// in bounds and maps back to the global numbering); throws on violation.
+void validate_tiles(const HostAssembly3D &A) {
+  auto fail = [](const std::string &w) {
+    throw SmxError(SMX_ERR_MESH, "3D tile layout: " + w);
+  };
+  const int RW = A.round_width;
+  if (int(A.round_slots.size()) % RW)
+    fail("round table size");
+  const int nrounds = int(A.round_slots.size()) / RW;
+  std::vector<int> seen_slot(A.nspec, 0), stamp(A.nglob, -1);
+  long long ex_sum = 0;
+  for (int t = 0; t < A.ntiles; ++t) {
+    const TileInfo3 &T = A.tiles[t];
+    const int nloc = T.nex + T.nsh;
+    if (T.slot0 < 0 || T.nelem < 1 || T.slot0 + T.nelem > A.nspec ||
+        nloc > A.nloc_max || T.ex_off < 0 || T.ex_off + T.nex > A.nex_total ||
+        T.sh_off < 0 || T.sh_off + T.nsh > A.npart || T.round_off < 0 ||
+        T.round_off + T.nround > nrounds || T.src_off < 0 ||
+        T.src_off + T.nsrc > int(A.tsrc_id.size()))
+      fail("tile " + std::to_string(t) + " header out of range");
+    ex_sum += T.nex;
+    for (int sl = T.slot0; sl < T.slot0 + T.nelem; ++sl)
+      for (int p = 0; p < NPT3; ++p) {
+        const int l = A.lidx[size_t(sl) * NPT3 + p];
+        const int g = A.ibool[size_t(sl) * NPT3 + p];
+        if (l >= nloc)
+          fail("local id out of range");
+        const int gl = l < T.nex ? T.ex_off + l : A.sh_list[T.sh_off + l - T.nex];
+        if (gl != g || g < 0 || g >= A.nglob)
+          fail("local id does not map back to the global id");
+      }
+    for (int r = 0; r < T.nround; ++r) {
+      for (int k = 0; k < RW; ++k) {
+        const int sl = A.round_slots[size_t(T.round_off + r) * RW + k];
+        if (sl < 0)
+          continue;
+        if (sl < T.slot0 || sl >= T.slot0 + T.nelem)
+          fail("round slot outside its tile");
+        seen_slot[sl]++;
+        for (int p = 0; p < NPT3; ++p) {
+          const int g = A.ibool[size_t(sl) * NPT3 + p];
+          if (stamp[g] == t * 4096 + r)
+            fail("two elements of a round share a point");
+          stamp[g] = t * 4096 + r;
+        }
+      }
+    }
+    for (int k = T.src_off; k < T.src_off + T.nsrc; ++k)
+      if (A.tsrc_li[k] < 0 || A.tsrc_li[k] >= nloc || A.tsrc_id[k] < 0 ||
+          A.tsrc_id[k] >= std::max(1, A.nsrc))
+        fail("source entry out of range");
+  }
+  for (int sl = 0; sl < A.nspec; ++sl)
+    if (seen_slot[sl] != 1)
+      fail("slot not in exactly one round");
+  if (ex_sum != A.nex_total || A.nex_total + A.nsh_total != A.nglob)
+    fail("exclusive / shared counts");
+  if (int(A.sh_poff.size()) != A.nsh_total + 1 || A.sh_poff[A.nsh_total] != A.npart)
+    fail("shared CSR size");
+  std::vector<int> used(A.npart, 0);
+  for (int s = 0; s < A.nsh_total; ++s)
+    for (int k = A.sh_poff[s]; k < A.sh_poff[s + 1]; ++k) {
+      const int q = A.sh_pidx[k];
+      if (q < 0 || q >= A.npart || A.sh_list[q] != A.nex_total + s)
+        fail("partial slot of the wrong point");
+      used[q]++;
+    }
+  for (int q = 0; q < A.npart; ++q)
+    if (used[q] != 1)
+      fail("partial slot not summed exactly once");
+}
+
 } // namespace
 
 void set_model3d(HostAssembly3D &A, const float *kappa, const float *mu,
@@ -217,7 +290,10 @@ void set_model3d(HostAssembly3D &A, const float *kappa, const float *mu,
 
 void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
                 const smx_receivers3d &rec, double dt_in, int nstep,
-                HostAssembly3D &A) {
+                HostAssembly3D &A, const AssemblyOptions3D &opt) {
+  if (opt.tile_elems < 1 || opt.tile_elems > 64 || opt.round_width < 1 ||
+      opt.round_width > 4)
+    throw SmxError(SMX_ERR_ARG, "3D tile options out of range");
   if (m.nspec <= 0 || m.nnodes <= 0 || !m.coord || !m.knods || !m.kappa ||
       !m.mu || !m.rho)
     throw SmxError(SMX_ERR_ARG, "invalid 3D mesh description");
@@ -266,7 +342,8 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
     A.ncolors = nc;
   }
 
-  // ---- Morton order of the element centroids; slots by colour, then Morton
+  // ---- Morton order of the element centroids; tiles of consecutive
+  // elements, ordered by colour inside a tile
   std::vector<int> morton(nspec);
   {
     float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
@@ -284,9 +361,17 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
         hi[d] = std::max(hi[d], c[e][d]);
       }
     }
+    // grid spacing: the edge of an element of average volume, from the
+    // control nodes' bounding box (on a structured box the centroids then
+    // sit exactly on the grid, and tiles of 32 are 4x4x2 blocks)
     double vol = 1;
-    for (int d = 0; d < 3; ++d)
-      vol *= std::max(double(hi[d] - lo[d]), 1e-6);
+    {
+      const std::vector<float> *cs[3] = {&cx, &cy, &cz};
+      for (int d = 0; d < 3; ++d) {
+        const auto mm = std::minmax_element(cs[d]->begin(), cs[d]->end());
+        vol *= std::max(double(*mm.second) - double(*mm.first), 1e-6);
+      }
+    }
     const double h = std::max(std::cbrt(vol / nspec), 1e-6);
     std::vector<std::pair<uint64_t, int>> key(nspec);
     for (int e = 0; e < nspec; ++e) {
@@ -300,16 +385,47 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
       morton[k] = key[k].second;
   }
   A.elem_slot.assign(nspec, 0);
+  const int TE = opt.tile_elems, RW = opt.round_width;
+  A.tile_elems = TE;
+  A.round_width = RW;
+  A.ntiles = (nspec + TE - 1) / TE;
+  A.tiles.assign(A.ntiles, TileInfo3{});
+  std::vector<int> slot_color(nspec);
+  for (int t = 0; t < A.ntiles; ++t) {
+    const int k0 = t * TE, k1 = std::min(nspec, k0 + TE);
+    std::vector<int> el(morton.begin() + k0, morton.begin() + k1);
+    std::stable_sort(el.begin(), el.end(),
+                     [&](int a, int b) { return color[a] < color[b]; });
+    TileInfo3 &T = A.tiles[t];
+    T.slot0 = k0;
+    T.nelem = k1 - k0;
+    T.round_off = int(A.round_slots.size()) / RW;
+    for (size_t i = 0; i < el.size(); ++i) {
+      A.elem_slot[el[i]] = k0 + int(i);
+      slot_color[k0 + i] = color[el[i]];
+    }
+    // rounds: up to RW elements of one colour
+    for (size_t i = 0; i < el.size();) {
+      size_t j = i;
+      while (j < el.size() && j - i < size_t(RW) && color[el[j]] == color[el[i]])
+        ++j;
+      for (int k = 0; k < RW; ++k)
+        A.round_slots.push_back(i + k < j ? k0 + int(i) + k : -1);
+      ++T.nround;
+      i = j;
+    }
+  }
   A.color_off.assign(A.ncolors + 1, 0);
+  A.color_slots.assign(nspec, 0);
   {
     std::vector<int> count(A.ncolors, 0);
-    for (int e = 0; e < nspec; ++e)
-      count[color[e]]++;
+    for (int sl = 0; sl < nspec; ++sl)
+      count[slot_color[sl]]++;
     for (int c = 0; c < A.ncolors; ++c)
       A.color_off[c + 1] = A.color_off[c] + count[c];
     std::vector<int> fill(A.color_off.begin(), A.color_off.end() - 1);
-    for (int e : morton)
-      A.elem_slot[e] = fill[color[e]]++;
+    for (int sl = 0; sl < nspec; ++sl)
+      A.color_slots[fill[slot_color[sl]]++] = sl;
   }
   const int ld = nspec;
   A.nodes.assign(size_t(24) * ld, 0.0f);
@@ -405,6 +521,86 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
       }
     }
     A.nglob = next;
+  }
+
+  // ---- exclusive / shared points: renumber tile by tile (each tile's
+  // exclusive points contiguous, in first-touch order), then the shared
+  // points; tile-local ids (exclusive first, then the tile's shared points
+  // in first-touch order); one partial slot per (tile, shared point)
+  {
+    const int ng = A.nglob;
+    std::vector<int> owner(ng, -1); // tile, or -2 when shared
+    for (int t = 0; t < A.ntiles; ++t) {
+      const TileInfo3 &T = A.tiles[t];
+      for (size_t i = size_t(T.slot0) * NPT3; i < size_t(T.slot0 + T.nelem) * NPT3; ++i) {
+        const int g = A.ibool[i];
+        owner[g] = owner[g] == -1 ? t : (owner[g] == t ? t : -2);
+      }
+    }
+    std::vector<int> nid(ng, -1);
+    int next = 0;
+    for (int t = 0; t < A.ntiles; ++t) {
+      TileInfo3 &T = A.tiles[t];
+      T.ex_off = next;
+      for (size_t i = size_t(T.slot0) * NPT3; i < size_t(T.slot0 + T.nelem) * NPT3; ++i) {
+        const int g = A.ibool[i];
+        if (owner[g] == t && nid[g] < 0)
+          nid[g] = next++;
+      }
+      T.nex = next - T.ex_off;
+    }
+    A.nex_total = next;
+    for (size_t i = 0; i < A.ibool.size(); ++i) {
+      const int g = A.ibool[i];
+      if (owner[g] == -2 && nid[g] < 0)
+        nid[g] = next++;
+    }
+    if (next != ng)
+      throw SmxError(SMX_ERR_MESH, "3D point renumbering is inconsistent");
+    A.nsh_total = ng - A.nex_total;
+    A.lidx.assign(size_t(nspec) * NPT3, 0);
+    A.sh_list.clear();
+    std::vector<int> stamp(ng, -1), lid(ng, 0);
+    A.nloc_max = 0;
+    for (int t = 0; t < A.ntiles; ++t) {
+      TileInfo3 &T = A.tiles[t];
+      T.sh_off = int(A.sh_list.size());
+      int k = 0;
+      for (size_t i = size_t(T.slot0) * NPT3; i < size_t(T.slot0 + T.nelem) * NPT3; ++i) {
+        const int g = A.ibool[i];
+        int l;
+        if (owner[g] == t) {
+          l = nid[g] - T.ex_off;
+        } else {
+          if (stamp[g] != t) {
+            stamp[g] = t;
+            lid[g] = T.nex + k++;
+            A.sh_list.push_back(nid[g]);
+          }
+          l = lid[g];
+        }
+        if (l < 0 || l > 65535)
+          throw SmxError(SMX_ERR_MESH, "tile has more than 65536 points");
+        A.lidx[i] = static_cast<uint16_t>(l);
+      }
+      T.nsh = k;
+      A.nloc_max = std::max(A.nloc_max, T.nex + T.nsh);
+    }
+    A.npart = int(A.sh_list.size());
+    // shared point -> its partial slots, in tile order
+    A.sh_poff.assign(A.nsh_total + 1, 0);
+    for (int g : A.sh_list)
+      A.sh_poff[g - A.nex_total + 1]++;
+    for (int k = 0; k < A.nsh_total; ++k)
+      A.sh_poff[k + 1] += A.sh_poff[k];
+    A.sh_pidx.assign(A.npart, 0);
+    {
+      std::vector<int> fill(A.sh_poff.begin(), A.sh_poff.end() - 1);
+      for (int k = 0; k < A.npart; ++k)
+        A.sh_pidx[fill[A.sh_list[k] - A.nex_total]++] = k;
+    }
+    for (auto &g : A.ibool)
+      g = nid[g];
   }
 
   material3d(A, m.kappa, m.mu, m.rho);
@@ -652,6 +848,31 @@ void assemble3d(const smx_mesh3d &m, const smx_sources3d &src,
       }
     }
   }
+
+  // ---- per-tile source entries (every point of each source element, in
+  // source order)
+  {
+    std::vector<std::vector<int>> of_tile(A.ntiles);
+    for (int s2 = 0; s2 < src.nsrc; ++s2)
+      of_tile[A.src_slot[s2] / TE].push_back(s2);
+    A.tsrc_li.clear();
+    A.tsrc_id.clear();
+    A.tsrc_val.clear();
+    for (int t = 0; t < A.ntiles; ++t) {
+      TileInfo3 &T = A.tiles[t];
+      T.src_off = int(A.tsrc_id.size());
+      for (int s2 : of_tile[t])
+        for (int p = 0; p < NPT3; ++p) {
+          A.tsrc_li.push_back(A.lidx[size_t(A.src_slot[s2]) * NPT3 + p]);
+          A.tsrc_id.push_back(s2);
+          for (int c = 0; c < 3; ++c)
+            A.tsrc_val.push_back(A.src_array[(size_t(s2) * 3 + c) * NPT3 + p]);
+        }
+      T.nsrc = int(A.tsrc_id.size()) - T.src_off;
+    }
+  }
+
+  validate_tiles(A);
 
   // ---- receivers (core/specfem/assembly/receivers/dim3/receivers.cpp)
   A.nrec = rec.nrec;

@@ -8,14 +8,19 @@
 // (compute_source_array/dim3/impl/*), receiver interpolants, plus absorbing
 // faces (this build's 3D Stacey condition; the reference's 3D path has none).
 //
-// Device layout.  Elements are grouped by colour (no two elements of a
-// colour share a GLL point; 8 colours on a structured hex box), each colour
-// contiguous and in Morton order of the element centroids; slot = position
-// in that order.  Per-element arrays are element-major, [slot][term][point]
-// (point p = (iz*5 + iy)*5 + ix), so that the 125 threads of an element's
-// workgroup read contiguous bytes.  Global points are numbered by first touch
-// in Morton element order.  Fields are three component planes
-// [3][nglob_pad].
+// Device layout.  Elements are sorted along a Morton curve of their
+// centroids and cut into tiles of `tile_elems` consecutive elements (4x4x2
+// blocks on a structured box); inside a tile the elements are ordered by
+// colour (no two elements of a colour share a GLL point; 8 colours on a
+// structured box); slot = position in that order.  A tile's elements of one
+// colour are processed in rounds of up to `round_width` elements.
+// Per-element arrays are element-major, [slot][term][point] (point p =
+// (iz*5 + iy)*5 + ix), so that the 125 threads of an element read contiguous
+// bytes.  A GLL point touched by one tile only is exclusive to it; global
+// points are numbered tile by tile (each tile's exclusive points
+// contiguous), then the shared points.  Each tile numbers its points
+// locally (exclusive first, then its shared points) and owns one partial
+// slot per shared point.  Fields are three component planes [3][nglob_pad].
 #pragma once
 
 #include <cstdint>
@@ -28,14 +33,41 @@ namespace smx {
 
 constexpr int NPT3 = NGLL * NGLL * NGLL;
 
+// One tile of the device layout (slots [slot0, slot0 + nelem)).
+struct TileInfo3 {
+  int32_t slot0, nelem;
+  int32_t ex_off, nex;   // exclusive points: global ids ex_off .. ex_off+nex-1
+  int32_t sh_off, nsh;   // shared points: sh_list / partial slots sh_off ..
+  int32_t round_off, nround; // rounds: round_slots[(round_off + r) * rw + k]
+  int32_t src_off, nsrc;     // source entries
+};
+
+struct AssemblyOptions3D {
+  int tile_elems = 32;  // elements per tile (1 .. 64)
+  int round_width = 4;  // elements processed together in a tile (1 .. 4)
+};
+
 struct HostAssembly3D {
   int nspec = 0, nglob = 0;
   Quadrature quad;
-  // colours: slots [color_off[c], color_off[c+1])
+  // global colours (adjoint colour passes): slots color_slots[color_off[c]
+  // .. color_off[c+1]), ascending
   int ncolors = 0;
-  std::vector<int32_t> color_off;
+  std::vector<int32_t> color_off, color_slots;
   std::vector<int32_t> elem_slot; // mesh element -> slot
   std::vector<int32_t> ibool;     // [nspec][NPT3] global ids
+  // tiles
+  int tile_elems = 32, round_width = 4, ntiles = 0;
+  int nex_total = 0, nsh_total = 0, npart = 0, nloc_max = 0;
+  std::vector<TileInfo3> tiles;
+  std::vector<uint16_t> lidx;       // [nspec][NPT3] tile-local ids
+  std::vector<int32_t> sh_list;     // [npart] global id of each partial slot
+  std::vector<int32_t> sh_poff;     // [nsh_total + 1] shared point -> partials
+  std::vector<int32_t> sh_pidx;     // partial slots, in tile order
+  std::vector<int32_t> round_slots; // [nrounds][round_width], -1 = none
+  // per-tile point sources: tile-local point, source, array value [3]
+  std::vector<int32_t> tsrc_li, tsrc_id;
+  std::vector<float> tsrc_val;
   // geometry: per element [nspec][10] (xix, xiy, xiz, etax, etay, etaz,
   // gammax, gammay, gammaz, J) when every element is affine (geom_elem),
   // else per point [nspec][10][NPT3]
@@ -76,7 +108,8 @@ struct HostAssembly3D {
 
 void assemble3d(const smx_mesh3d &mesh, const smx_sources3d &sources,
                 const smx_receivers3d &receivers, double dt, int nstep,
-                HostAssembly3D &out);
+                HostAssembly3D &out,
+                const AssemblyOptions3D &opt = AssemblyOptions3D());
 
 // Replaces the material ([nspec][125] each, mesh element order) of an
 // assembled problem: moduli, Stacey impedances and mass.  A is unchanged

@@ -184,8 +184,14 @@ int smx_ctx_create3d_opts(const smx_mesh3d *mesh, const smx_sources3d *sources,
     if (!mesh || !sources || !receivers || !out)
       throw smx::SmxError(SMX_ERR_ARG, "null argument");
     *out = nullptr;
+    // tiles of 32 elements in rounds of 4 (FP32); 8 elements, one per round,
+    // in FP64 (twice the LDS per value)
+    smx::AssemblyOptions3D ao;
+    ao.tile_elems = (opts && opts->tile_elems) ? opts->tile_elems
+                                               : (precision == 8 ? 8 : 32);
+    ao.round_width = precision == 8 ? 1 : 4;
     auto a = std::make_unique<smx::HostAssembly3D>();
-    smx::assemble3d(*mesh, *sources, *receivers, dt, nstep, *a);
+    smx::assemble3d(*mesh, *sources, *receivers, dt, nstep, *a, ao);
     auto ctx = std::make_unique<smx_ctx>();
     ctx->solver3d = smx::make_solver3d(std::move(a), precision, graph_steps(opts),
                                        geometry_from_nodes(opts));

@@ -2,24 +2,26 @@
 // time step (hex8, NGLL = 5).
 //
 // Kernels (each cites the reference dispatch it replaces):
-//   k3_elem     compute_stiffness_interaction for dim3
+//   k3_tile     compute_stiffness_interaction for dim3
 //               (include/kokkos_kernels/impl/compute_stiffness_interaction.tpp:
 //               140-247 with the dim3 gradient, stress and divergence,
 //               algorithms/gradient.hpp, medium/dim3/elastic/isotropic/
 //               stress.hpp:46-99, point/stress.hpp:217-240,
 //               algorithms/divergence.hpp:95-137), plus this build's 3D Stacey
-//               faces.  One 128-thread workgroup per element, one thread per
-//               GLL point; the element's displacement and its stress integrand
-//               are staged in LDS.  Assembly without atomics: one launch per
-//               element colour (no two elements of a colour share a point),
-//               each a plain read-modify-write of the accumulator.
-//   k3_src      compute_source_interaction (compute_source_interaction.tpp:
-//               17-104): sources in list order, one block.
+//               faces, the source interaction (compute_source_interaction.tpp:
+//               17-104), and for the tile's exclusive points divide_mass_matrix
+//               + Newmark corrector + next predictor (divide_mass_matrix.tpp:
+//               13-65, newmark.tpp:13-128).  One workgroup per tile of
+//               elements: the tile's accumulator lives in LDS; its elements
+//               are processed colour by colour in rounds of up to four
+//               elements (128 threads each, one per GLL point), so that no two
+//               elements of a round share a point and the assembly needs no
+//               atomics.  Shared points leave one partial per tile.
+//   k3_shared   the same finish for shared points: their partials summed in
+//               tile order.
 //   k3_seis     compute_seismograms (compute_seismogram.tpp:19-171).
-//   k3_node     divide_mass_matrix + Newmark corrector + next predictor
-//               (divide_mass_matrix.tpp:13-65, newmark.tpp:13-128).
-// A step is k3_src, the colour passes, k3_seis, k3_node; steps are captured
-// into HIP graphs and a device counter selects the source row and sample.
+// A step is k3_seis (displacement), k3_tile, k3_shared, k3_seis (velocity /
+// acceleration) and the step counter; steps are captured into HIP graphs.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -131,192 +133,6 @@ __global__ void k3_geom_check(const float *__restrict__ nodes,
   bad[t] = diff ? 1 : 0;
 }
 
-// One element of the colour pass.  GM: where the geometry comes from
-// (GeomMode); ME: moduli stored per element instead of per GLL point.
-template <typename real, int GM, bool ME>
-__global__ void __launch_bounds__(ELEM_THREADS)
-    k3_elem(int slot0, const int *__restrict__ ibool, int ld,
-            const real *__restrict__ geom, const float *__restrict__ nodes,
-            Quad3<float> xi3, const real *__restrict__ mat,
-            const int *__restrict__ sslot, const real *__restrict__ stacey,
-            int nst, const real *__restrict__ U, const real *__restrict__ V,
-            real *__restrict__ A, long npad, Quad3<real> q) {
-  __shared__ real s_u[3][NPT3];
-  __shared__ real s_f[3][3][NPT3];
-  const int slot = slot0 + blockIdx.x;
-  const int t = threadIdx.x;
-  const bool act = t < NPT3;
-  const int p = act ? t : 0;
-  const int iz = p / (NGLL * NGLL), iy = (p / NGLL) % NGLL, ix = p % NGLL;
-  int g = 0;
-  if (act) {
-    g = ibool[size_t(slot) * NPT3 + p];
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      s_u[c][p] = U[c * npad + g];
-  }
-  __syncthreads();
-  if (act) {
-    real dxi[3], det[3], dga[3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      real a0 = 0, a1 = 0, a2 = 0;
-#pragma unroll
-      for (int l = 0; l < NGLL; ++l) {
-        a0 += q.hp[ix][l] * s_u[c][p3(iz, iy, l)];
-        a1 += q.hp[iy][l] * s_u[c][p3(iz, l, ix)];
-        a2 += q.hp[iz][l] * s_u[c][p3(l, iy, ix)];
-      }
-      dxi[c] = a0;
-      det[c] = a1;
-      dga[c] = a2;
-    }
-    real G[10];
-    if (GM == GEOM_NODES) {
-      float Gf[10];
-      geom_from_nodes3(nodes + size_t(slot) * 24, xi3.w[ix], xi3.w[iy],
-                       xi3.w[iz], Gf);
-#pragma unroll
-      for (int k = 0; k < 10; ++k)
-        G[k] = Gf[k];
-    } else {
-#pragma unroll
-      for (int k = 0; k < 10; ++k)
-        G[k] = GM == GEOM_ELEM ? geom[size_t(slot) * 10 + k]
-                               : geom[(size_t(slot) * 10 + k) * NPT3 + p];
-    }
-    real du[3][3];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      du[c][0] = G[0] * dxi[c] + G[3] * det[c] + G[6] * dga[c];
-      du[c][1] = G[1] * dxi[c] + G[4] * det[c] + G[7] * dga[c];
-      du[c][2] = G[2] * dxi[c] + G[5] * det[c] + G[8] * dga[c];
-    }
-    const real L = ME ? mat[size_t(slot) * 2] : mat[size_t(slot) * 2 * NPT3 + p];
-    const real M = ME ? mat[size_t(slot) * 2 + 1]
-                      : mat[(size_t(slot) * 2 + 1) * NPT3 + p];
-    const real lam = L - real(2.0) * M;
-    real T[3][3];
-    T[0][0] = L * du[0][0] + lam * (du[1][1] + du[2][2]);
-    T[1][1] = L * du[1][1] + lam * (du[0][0] + du[2][2]);
-    T[2][2] = L * du[2][2] + lam * (du[0][0] + du[1][1]);
-    T[0][1] = T[1][0] = M * (du[0][1] + du[1][0]);
-    T[0][2] = T[2][0] = M * (du[0][2] + du[2][0]);
-    T[1][2] = T[2][1] = M * (du[1][2] + du[2][1]);
-    const real J = G[9];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      s_f[c][0][p] = J * (T[c][0] * G[0] + T[c][1] * G[1] + T[c][2] * G[2]);
-      s_f[c][1][p] = J * (T[c][0] * G[3] + T[c][1] * G[4] + T[c][2] * G[5]);
-      s_f[c][2][p] = J * (T[c][0] * G[6] + T[c][1] * G[7] + T[c][2] * G[8]);
-    }
-  }
-  __syncthreads();
-  if (!act)
-    return;
-  real r[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    real t1 = 0, t2 = 0, t3 = 0;
-#pragma unroll
-    for (int l = 0; l < NGLL; ++l) {
-      t1 += s_f[c][0][p3(iz, iy, l)] * q.hp[l][ix] * q.w[l];
-      t2 += s_f[c][1][p3(iz, l, ix)] * q.hp[l][iy] * q.w[l];
-      t3 += s_f[c][2][p3(l, iy, ix)] * q.hp[l][iz] * q.w[l];
-    }
-    r[c] = q.w[iz] * q.w[iy] * t1 + q.w[iz] * q.w[ix] * t2 +
-           q.w[iy] * q.w[ix] * t3;
-    r[c] *= real(-1.0);
-  }
-  const int b = sslot[slot];
-  if (b >= 0) {
-    const real *st = stacey + size_t(b) * 7 * NPT3 + p;
-    const real wgt = st[4 * NPT3];
-    if (wgt != real(0)) {
-      const real n[3] = {st[0], st[NPT3], st[2 * NPT3]};
-      const real nrm = st[3 * NPT3], P = st[5 * NPT3], S = st[6 * NPT3];
-      const real v[3] = {V[g], V[npad + g], V[2 * npad + g]};
-      const real vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        const real fc = ((vn * n[c] / (nrm * nrm)) * (P - S)) + v[c] * S;
-        r[c] += real(-1.0) * fc * nrm * wgt;
-      }
-    }
-  }
-#pragma unroll
-  for (int c = 0; c < 3; ++c)
-    A[c * npad + g] += r[c];
-}
-
-// Sources in list order (sources may share points), one block.
-template <typename real>
-__global__ void __launch_bounds__(ELEM_THREADS)
-    k3_src(int nsrc, const int *__restrict__ src_slot,
-           const real *__restrict__ src_array, const real *__restrict__ stf,
-           const int *__restrict__ ibool, int ld, real *__restrict__ A,
-           long npad, const int *__restrict__ step_ptr) {
-  const int step = *step_ptr;
-  const int p = threadIdx.x;
-  for (int s = 0; s < nsrc; ++s) {
-    if (p < NPT3) {
-      const int g = ibool[size_t(src_slot[s]) * NPT3 + p];
-      const real f = stf[size_t(step) * nsrc + s];
-#pragma unroll
-      for (int c = 0; c < 3; ++c)
-        A[c * npad + g] += src_array[(size_t(s) * 3 + c) * NPT3 + p] * f;
-    }
-    __syncthreads();
-  }
-}
-
-// Seismograms of step `*step_ptr` (if sampled): displacement from U, and
-// velocity / acceleration from the corrected values the node pass is about
-// to form (same arithmetic).  One wavefront per (type, receiver): lane l sums
-// points l and l + 64 in order, then a fixed butterfly reduction (lane 0's
-// order of additions is fixed, so the result is deterministic).  Grid:
-// exactly nrec * ntypes blocks of 64 threads.
-template <typename real>
-__global__ void __launch_bounds__(64)
-    k3_seis(const real *__restrict__ U, const real *__restrict__ V,
-            const real *__restrict__ A, const real *__restrict__ Minv,
-            const int *__restrict__ rec_g, const real *__restrict__ rec_h,
-            int nrec, Types3 types, int ntypes, int nsbs, long npad, real dto2,
-            const int *__restrict__ step_ptr, real *__restrict__ out) {
-  const int step = *step_ptr;
-  if (step % nsbs != 0)
-    return;
-  const int isig = step / nsbs;
-  const int r = blockIdx.x % nrec, it = blockIdx.x / nrec;
-  const int type = types.t[it];
-  const int lane = threadIdx.x;
-  real s[3] = {0, 0, 0};
-  for (int p = lane; p < NPT3; p += 64) {
-    const int g = rec_g[size_t(r) * NPT3 + p];
-    const real h = rec_h[size_t(r) * NPT3 + p];
-#pragma unroll
-    for (int c = 0; c < 3; ++c) {
-      real f;
-      if (type == 0) {
-        f = U[c * npad + g];
-      } else {
-        const real a = A[c * npad + g] * Minv[c * npad + g];
-        f = type == 2 ? a : V[c * npad + g] + dto2 * a;
-      }
-      s[c] += h * f;
-    }
-  }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      s[c] += __shfl_xor(s[c], off);
-  if (lane == 0)
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      out[((size_t(isig) * ntypes + it) * nrec + r) * 3 + c] = s[c];
-}
-
 // Record buffers of the adjoint's recompute: per step j of a segment,
 // a' [every][3][npad], the predicted u~ [every][3][npad] and v~ at the
 // absorbing nodes [every][3][nb].
@@ -327,30 +143,49 @@ template <typename real> struct Rec3 {
   long npad, nb;
 };
 
-// Node pass: a = A / M, v += dt/2 a; then (FINAL = false) the next
-// predictor u += dt v + dt^2/2 a, v += dt/2 a, a = 0.  Thread 0 advances the
-// step counter, except when recording (REC): then every thread reads the
-// counter (j = step % every) and a separate launch advances it; the
-// arithmetic is the same, so a recompute reproduces the forward run.
-template <typename real, bool FINAL, bool REC>
-__global__ void __launch_bounds__(NODE3_THREADS)
-    k3_node(real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
-            const real *__restrict__ Minv, long n3, real dt, real dto2,
-            real dt2o2, int *__restrict__ step_ptr, Rec3<real> rec) {
-  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  int j = 0;
-  if (REC)
-    j = *step_ptr % rec.every;
-  else if (i == 0 && step_ptr)
-    *step_ptr += 1;
-  if (i >= n3)
+// Modes of the tile and shared-point kernels.
+enum Mode3 : int {
+  M3_STEP = 0,     // finish the step and apply the next predictor
+  M3_FINAL = 1,    // finish the step (corrected v and a stored)
+  M3_RECORD = 2,   // as M3_STEP, and record a', u~, v~ for the adjoint
+  M3_ASSEMBLE = 3  // a = (f - K u - B v) / M only (stiffness timing)
+};
+
+// Sampled-step outputs for velocity / acceleration seismograms (v, a null
+// when no such seismogram is recorded).
+template <typename real> struct Aux3 {
+  real *v, *a;
+  int nsbs;
+};
+
+// Finish one (point, component), i = c * npad + g, from its assembled force
+// acc: a = acc / M, v += dt/2 a, then the next predictor
+// u += dt v + dt^2/2 a, v += dt/2 a (divide_mass_matrix.tpp:13-65,
+// newmark.tpp:13-128).
+template <typename real, int MODE>
+__device__ __forceinline__ void finish_point(long i, long n3, real acc,
+                                             real *__restrict__ U,
+                                             real *__restrict__ V,
+                                             real *__restrict__ A,
+                                             const real *__restrict__ Minv,
+                                             real dt, real dto2, real dt2o2,
+                                             const Rec3<real> &rec, int j,
+                                             const Aux3<real> &aux,
+                                             bool sample) {
+  const real a = acc * Minv[i];
+  if (MODE == M3_ASSEMBLE) {
+    A[i] = a;
     return;
-  real a = A[i] * Minv[i];
+  }
   real v = V[i];
   v += dto2 * a;
-  if (REC)
+  if (sample) {
+    aux.v[i] = v;
+    aux.a[i] = a;
+  }
+  if (MODE == M3_RECORD)
     rec.a[size_t(j) * n3 + i] = a;
-  if (FINAL) {
+  if (MODE == M3_FINAL) {
     V[i] = v;
     A[i] = a;
     return;
@@ -360,8 +195,7 @@ __global__ void __launch_bounds__(NODE3_THREADS)
   v += dto2 * a;
   U[i] = u;
   V[i] = v;
-  A[i] = 0;
-  if (REC && j + 1 < rec.every) {
+  if (MODE == M3_RECORD && j + 1 < rec.every) {
     rec.u[size_t(j + 1) * n3 + i] = u;
     const long g = i % rec.npad;
     const int c = int(i / rec.npad);
@@ -369,6 +203,275 @@ __global__ void __launch_bounds__(NODE3_THREADS)
     if (b >= 0)
       rec.vb[(size_t(j + 1) * 3 + c) * rec.nb + b] = v;
   }
+}
+
+// One tile of elements (see the file comment).  GM: where the geometry
+// comes from (GeomMode); ME: moduli per element instead of per GLL point;
+// RW: elements per round (block = RW x 128 threads).  LDS: the tile's
+// accumulator [3][nloc_max], then per round slot the element displacement
+// [3][125] and stress integrand [3][3][125].
+template <typename real, int MODE, int GM, bool ME, int RW>
+__global__ void __launch_bounds__(RW * ELEM_THREADS)
+    k3_tile(const TileInfo3 *__restrict__ tiles,
+            const int *__restrict__ round_slots,
+            const uint16_t *__restrict__ lidx, const int *__restrict__ sh_list,
+            const real *__restrict__ geom, const float *__restrict__ nodes,
+            Quad3<float> xi3, const real *__restrict__ mat,
+            const int *__restrict__ sslot, const real *__restrict__ stacey,
+            real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
+            const real *__restrict__ Minv, real *__restrict__ part, long npart,
+            const int *__restrict__ tsrc_li, const int *__restrict__ tsrc_id,
+            const real *__restrict__ tsrc_val, const real *__restrict__ stf,
+            int nsrc, int nloc_max, long npad, real dt, real dto2, real dt2o2,
+            Quad3<real> q, const int *__restrict__ step_ptr, Rec3<real> rec,
+            Aux3<real> aux) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  real *s_a = reinterpret_cast<real *>(lds_raw);
+  const TileInfo3 T = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int sub = tid / ELEM_THREADS, p = tid % ELEM_THREADS;
+  const bool pact = p < NPT3;
+  const int pp = pact ? p : 0;
+  const int iz = pp / (NGLL * NGLL), iy = (pp / NGLL) % NGLL, ix = pp % NGLL;
+  real *se = s_a + 3 * size_t(nloc_max) + size_t(sub) * 12 * NPT3;
+  real *sf = se + 3 * NPT3;
+  const int nloc = T.nex + T.nsh;
+  const int step = MODE == M3_ASSEMBLE ? 0 : *step_ptr;
+  for (int i = tid; i < nloc; i += RW * ELEM_THREADS) {
+    s_a[i] = 0;
+    s_a[nloc_max + i] = 0;
+    s_a[2 * nloc_max + i] = 0;
+  }
+  __syncthreads();
+  // point sources first, in list order (one thread: fixed order)
+  if (MODE != M3_ASSEMBLE && tid == 0)
+    for (int k = T.src_off; k < T.src_off + T.nsrc; ++k) {
+      const int l = tsrc_li[k];
+      const real f = stf[size_t(step) * nsrc + tsrc_id[k]];
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        s_a[c * nloc_max + l] += tsrc_val[size_t(k) * 3 + c] * f;
+    }
+  __syncthreads();
+
+  // Rounds are software-pipelined: the indices, displacement, geometry and
+  // moduli of round r + 1 are requested while round r computes (no point
+  // of the tile is written before all rounds are done, and shared points
+  // only by the next kernel, so the early reads see this step's values).
+  int slot_n = -1, l_n = 0;
+  long g_n = 0;
+  real u_n[3] = {0, 0, 0}, G_n[10], L_n = 0, M_n = 0;
+  auto fetch = [&](int r) {
+    slot_n = r < T.nround ? round_slots[(size_t(T.round_off) + r) * RW + sub] : -1;
+    if (!(pact && slot_n >= 0))
+      return;
+    l_n = lidx[size_t(slot_n) * NPT3 + p];
+    g_n = l_n < T.nex ? long(T.ex_off) + l_n
+                      : long(sh_list[T.sh_off + l_n - T.nex]);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      u_n[c] = U[c * npad + g_n];
+    if (GM != GEOM_NODES) {
+#pragma unroll
+      for (int k = 0; k < 10; ++k)
+        G_n[k] = GM == GEOM_ELEM ? geom[size_t(slot_n) * 10 + k]
+                                 : geom[(size_t(slot_n) * 10 + k) * NPT3 + p];
+    }
+    L_n = ME ? mat[size_t(slot_n) * 2] : mat[size_t(slot_n) * 2 * NPT3 + p];
+    M_n = ME ? mat[size_t(slot_n) * 2 + 1] : mat[(size_t(slot_n) * 2 + 1) * NPT3 + p];
+  };
+  fetch(0);
+  for (int r = 0; r < T.nround; ++r) {
+    const int slot = slot_n;
+    const bool act = pact && slot >= 0;
+    const int l = l_n;
+    const long g = g_n;
+    real G[10];
+#pragma unroll
+    for (int k = 0; k < 10; ++k)
+      G[k] = G_n[k];
+    const real L = L_n, M = M_n;
+    if (act)
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        se[c * NPT3 + p] = u_n[c];
+    __syncthreads();
+    fetch(r + 1);
+    if (act) {
+      real dxi[3], det[3], dga[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        real a0 = 0, a1 = 0, a2 = 0;
+#pragma unroll
+        for (int k = 0; k < NGLL; ++k) {
+          a0 += q.hp[ix][k] * se[c * NPT3 + p3(iz, iy, k)];
+          a1 += q.hp[iy][k] * se[c * NPT3 + p3(iz, k, ix)];
+          a2 += q.hp[iz][k] * se[c * NPT3 + p3(k, iy, ix)];
+        }
+        dxi[c] = a0;
+        det[c] = a1;
+        dga[c] = a2;
+      }
+      if (GM == GEOM_NODES) {
+        float Gf[10];
+        geom_from_nodes3(nodes + size_t(slot) * 24, xi3.w[ix], xi3.w[iy],
+                         xi3.w[iz], Gf);
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+          G[k] = Gf[k];
+      }
+      real du[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        du[c][0] = G[0] * dxi[c] + G[3] * det[c] + G[6] * dga[c];
+        du[c][1] = G[1] * dxi[c] + G[4] * det[c] + G[7] * dga[c];
+        du[c][2] = G[2] * dxi[c] + G[5] * det[c] + G[8] * dga[c];
+      }
+      const real lam = L - real(2.0) * M;
+      real Ts[3][3];
+      Ts[0][0] = L * du[0][0] + lam * (du[1][1] + du[2][2]);
+      Ts[1][1] = L * du[1][1] + lam * (du[0][0] + du[2][2]);
+      Ts[2][2] = L * du[2][2] + lam * (du[0][0] + du[1][1]);
+      Ts[0][1] = Ts[1][0] = M * (du[0][1] + du[1][0]);
+      Ts[0][2] = Ts[2][0] = M * (du[0][2] + du[2][0]);
+      Ts[1][2] = Ts[2][1] = M * (du[1][2] + du[2][1]);
+      const real J = G[9];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sf[(c * 3 + 0) * NPT3 + p] =
+            J * (Ts[c][0] * G[0] + Ts[c][1] * G[1] + Ts[c][2] * G[2]);
+        sf[(c * 3 + 1) * NPT3 + p] =
+            J * (Ts[c][0] * G[3] + Ts[c][1] * G[4] + Ts[c][2] * G[5]);
+        sf[(c * 3 + 2) * NPT3 + p] =
+            J * (Ts[c][0] * G[6] + Ts[c][1] * G[7] + Ts[c][2] * G[8]);
+      }
+    }
+    __syncthreads();
+    if (act) {
+      real rr[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        real t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+        for (int k = 0; k < NGLL; ++k) {
+          t1 += sf[(c * 3 + 0) * NPT3 + p3(iz, iy, k)] * q.hp[k][ix] * q.w[k];
+          t2 += sf[(c * 3 + 1) * NPT3 + p3(iz, k, ix)] * q.hp[k][iy] * q.w[k];
+          t3 += sf[(c * 3 + 2) * NPT3 + p3(k, iy, ix)] * q.hp[k][iz] * q.w[k];
+        }
+        rr[c] = q.w[iz] * q.w[iy] * t1 + q.w[iz] * q.w[ix] * t2 +
+                q.w[iy] * q.w[ix] * t3;
+        rr[c] *= real(-1.0);
+      }
+      const int b = sslot[slot];
+      if (b >= 0) {
+        const real *st = stacey + size_t(b) * 7 * NPT3 + p;
+        const real wgt = st[4 * NPT3];
+        if (wgt != real(0)) {
+          const real n[3] = {st[0], st[NPT3], st[2 * NPT3]};
+          const real nrm = st[3 * NPT3], P = st[5 * NPT3], S = st[6 * NPT3];
+          const real v[3] = {V[g], V[npad + g], V[2 * npad + g]};
+          const real vn = v[0] * n[0] + v[1] * n[1] + v[2] * n[2];
+#pragma unroll
+          for (int c = 0; c < 3; ++c) {
+            const real fc = ((vn * n[c] / (nrm * nrm)) * (P - S)) + v[c] * S;
+            rr[c] += real(-1.0) * fc * nrm * wgt;
+          }
+        }
+      }
+#pragma unroll
+      for (int c = 0; c < 3; ++c)
+        s_a[c * nloc_max + l] += rr[c];
+    }
+    __syncthreads();
+  }
+
+  // exclusive points: finish the step; shared points: store the partials
+  const long n3 = 3 * npad;
+  const bool sample = MODE != M3_ASSEMBLE && aux.v != nullptr && step % aux.nsbs == 0;
+  const int j = MODE == M3_RECORD ? step % rec.every : 0;
+  for (int i = tid; i < T.nex; i += RW * ELEM_THREADS) {
+    const long g = long(T.ex_off) + i;
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      finish_point<real, MODE>(c * npad + g, n3, s_a[c * nloc_max + i], U, V, A,
+                               Minv, dt, dto2, dt2o2, rec, j, aux, sample);
+  }
+  for (int i = tid; i < T.nsh; i += RW * ELEM_THREADS)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      part[c * npart + T.sh_off + i] = s_a[c * nloc_max + T.nex + i];
+}
+
+// Shared points (global ids nex_total + s): the tiles' partials summed in
+// tile order, then the same finish as the tiles' exclusive points.
+template <typename real, int MODE>
+__global__ void __launch_bounds__(NODE3_THREADS)
+    k3_shared(int nsh, int nex_total, const int *__restrict__ sh_poff,
+              const int *__restrict__ sh_pidx, const real *__restrict__ part,
+              long npart, real *__restrict__ U, real *__restrict__ V,
+              real *__restrict__ A, const real *__restrict__ Minv, long npad,
+              real dt, real dto2, real dt2o2, const int *__restrict__ step_ptr,
+              Rec3<real> rec, Aux3<real> aux) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsh)
+    return;
+  const int step = MODE == M3_ASSEMBLE ? 0 : *step_ptr;
+  const bool sample = MODE != M3_ASSEMBLE && aux.v != nullptr && step % aux.nsbs == 0;
+  const int j = MODE == M3_RECORD ? step % rec.every : 0;
+  const long g = long(nex_total) + s;
+  const long n3 = 3 * npad;
+  const int k0 = sh_poff[s], k1 = sh_poff[s + 1];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    real acc = 0;
+    for (int k = k0; k < k1; ++k)
+      acc += part[c * npart + sh_pidx[k]];
+    finish_point<real, MODE>(c * npad + g, n3, acc, U, V, A, Minv, dt, dto2,
+                             dt2o2, rec, j, aux, sample);
+  }
+}
+
+// Seismograms of step `*step_ptr` (if sampled).  POST = false, before the
+// step's tile pass: displacement types from U (= u~ of the step).  POST =
+// true, after the shared-point pass: velocity / acceleration types from the
+// step's corrected values (aux).  One wavefront per (type, receiver): lane l
+// sums points l and l + 64 in order, then a fixed butterfly reduction (lane
+// 0's order of additions is fixed, so the result is deterministic).  Grid:
+// exactly nrec * ntypes blocks of 64 threads.
+template <typename real, bool POST>
+__global__ void __launch_bounds__(64)
+    k3_seis(const real *__restrict__ U, const real *__restrict__ auxv,
+            const real *__restrict__ auxa, const int *__restrict__ rec_g,
+            const real *__restrict__ rec_h, int nrec, Types3 types, int ntypes,
+            int nsbs, long npad, const int *__restrict__ step_ptr,
+            real *__restrict__ out) {
+  const int step = *step_ptr;
+  if (step % nsbs != 0)
+    return;
+  const int isig = step / nsbs;
+  const int r = blockIdx.x % nrec, it = blockIdx.x / nrec;
+  const int type = types.t[it];
+  if ((type != 0) != POST)
+    return;
+  const real *F = type == 0 ? U : (type == 1 ? auxv : auxa);
+  const int lane = threadIdx.x;
+  real s[3] = {0, 0, 0};
+  for (int p = lane; p < NPT3; p += 64) {
+    const int g = rec_g[size_t(r) * NPT3 + p];
+    const real h = rec_h[size_t(r) * NPT3 + p];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      s[c] += h * F[c * npad + g];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      s[c] += __shfl_xor(s[c], off);
+  if (lane == 0)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      out[((size_t(isig) * ntypes + it) * nrec + r) * 3 + c] = s[c];
 }
 
 // v~ at the absorbing nodes into record slot 0 (segment start).
@@ -419,14 +522,15 @@ __global__ void k3_adj_inject(const int *__restrict__ rin_node,
     Ub[c * npad + g] += s[c];
 }
 
-// Adjoint colour pass of step k = *adj_step: with rb the adjoint
-// acceleration field and u~_k the recorded forward displacement, assembles
-// -K rb (as k3_elem assembles -K u, without the Stacey term) and
+// Adjoint colour pass of step k = *adj_step over the elements `slots` of
+// one colour (one workgroup per element): with rb the adjoint acceleration
+// field and u~_k the recorded forward displacement, assembles -K rb (as
+// k3_tile assembles -K u, without the Stacey term) and
 // accumulates the material densities div(rb) div(u~) (Gd) and
 // eps(rb) : eps(u~) (Ge) per local point.
 template <typename real, int GM, bool ME>
 __global__ void __launch_bounds__(ELEM_THREADS)
-    k3_adj_elem(int slot0, const int *__restrict__ ibool,
+    k3_adj_elem(const int *__restrict__ slots, const int *__restrict__ ibool,
                 const real *__restrict__ geom, const float *__restrict__ nodes,
                 Quad3<float> xi3, const real *__restrict__ mat,
                 const real *__restrict__ Rb, const real *__restrict__ rec_u,
@@ -436,7 +540,7 @@ __global__ void __launch_bounds__(ELEM_THREADS)
   __shared__ real s_r[3][NPT3];
   __shared__ real s_u[3][NPT3];
   __shared__ real s_f[3][3][NPT3];
-  const int slot = slot0 + blockIdx.x;
+  const int slot = slots[blockIdx.x];
   const int t = threadIdx.x;
   const bool act = t < NPT3;
   const int p = act ? t : 0;
@@ -639,9 +743,38 @@ public:
     sslot_ = upload<int>(A.stacey_slot);
     nst_ = std::max(1, A.nstacey);
     upload_material();
-    src_slot_ = upload<int>(A.src_slot);
-    src_array_ = upload<real>(A.src_array);
     stf_ = upload<real>(A.stf);
+    // tile layout
+    if (A.round_width != kRoundWidth)
+      throw SmxError(SMX_ERR_ARG, "3D tile rounds must be " +
+                                      std::to_string(kRoundWidth) +
+                                      " elements wide for this precision");
+    tiles_ = upload<TileInfo3>(A.tiles);
+    round_slots_ = upload<int>(A.round_slots);
+    lidx_ = upload<uint16_t>(A.lidx);
+    sh_list_ = upload<int>(A.sh_list);
+    sh_poff_ = upload<int>(A.sh_poff);
+    sh_pidx_ = upload<int>(A.sh_pidx);
+    color_slots_ = upload<int>(A.color_slots);
+    tsrc_li_ = upload<int>(A.tsrc_li);
+    tsrc_id_ = upload<int>(A.tsrc_id);
+    tsrc_val_ = upload<real>(A.tsrc_val);
+    npart_ = std::max(1L, long(A.npart));
+    part_ = alloc<real>(3 * size_t(npart_));
+    tile_lds_ = (size_t(3) * std::max(1, A.nloc_max) + size_t(kRoundWidth) * 12 * NPT3) *
+                sizeof(real);
+    {
+      int dev = 0;
+      SMX3_HIP(hipGetDevice(&dev));
+      hipDeviceProp_t prop;
+      SMX3_HIP(hipGetDeviceProperties(&prop, dev));
+      if (tile_lds_ > prop.sharedMemPerBlock)
+        throw SmxError(SMX_ERR_MESH,
+                       "a 3D tile needs " + std::to_string(tile_lds_) +
+                           " bytes of LDS (device limit " +
+                           std::to_string(prop.sharedMemPerBlock) +
+                           "); use fewer elements per tile");
+    }
     rec_g_ = upload<int>(A.rec_g);
     rec_h_ = upload<real>(A.rec_h);
     U_ = alloc<real>(3 * npad_);
@@ -653,6 +786,13 @@ public:
         size_t(std::max(1, A.nsig)) * ntypes * std::max(1, A.nrec) * 3, 3));
     for (int k = 0; k < 4; ++k)
       types_.t[k] = k < ntypes ? A.seis_types[k] : 0;
+    for (int t : A.seis_types) {
+      need_disp_ = need_disp_ || t == 0;
+      if (t != 0 && !aux_v_) {
+        aux_v_ = alloc<real>(3 * npad_);
+        aux_a_ = alloc<real>(3 * npad_);
+      }
+    }
     for (int i = 0; i < NGLL; ++i) {
       quad_.w[i] = A.quad.w[i];
       for (int j = 0; j < NGLL; ++j)
@@ -717,7 +857,7 @@ public:
     step<true>();
     SMX3_HIP(hipEventRecord(ev1_, stream_));
     have_stats_ = true;
-    launches_ = nstep * A.ncolors;
+    launches_ = nstep;
   }
 
   void synchronize() override { SMX3_HIP(hipStreamSynchronize(stream_)); }
@@ -777,10 +917,10 @@ public:
       throw SmxError(SMX_ERR_ARG, "n must be positive");
     if (!scratch_a_)
       scratch_a_ = alloc<real>(3 * npad_);
-    elem_passes(scratch_a_); // warm-up
+    launch_tile<M3_ASSEMBLE>(scratch_a_, Rec3<real>{}); // warm-up
     SMX3_HIP(hipEventRecord(ev0_, stream_));
     for (int i = 0; i < n; ++i)
-      elem_passes(scratch_a_);
+      launch_tile<M3_ASSEMBLE>(scratch_a_, Rec3<real>{});
     SMX3_HIP(hipEventRecord(ev1_, stream_));
     SMX3_HIP(hipEventSynchronize(ev1_));
     float t = 0;
@@ -1113,17 +1253,8 @@ private:
 
   // One forward step of the recompute, recording a', u~ and v~.
   void record_step(const Rec3<real> &rec) {
-    const HostAssembly3D &A = *asm_;
-    if (A.nsrc > 0) {
-      k3_src<real><<<1, ELEM_THREADS, 0, stream_>>>(
-          A.nsrc, src_slot_, src_array_, stf_, ibool_, A.nspec, A_, npad_,
-          step_);
-      SMX3_HIP(hipGetLastError());
-    }
-    elem_passes(A_);
-    k3_node<real, false, true><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
-        U_, V_, A_, minv_, 3 * npad_, dt_, dto2_, dt2o2_, step_, rec);
-    SMX3_HIP(hipGetLastError());
+    launch_tile<M3_RECORD>(A_, rec);
+    launch_shared<M3_RECORD>(rec);
     k3_step_add<<<1, 64, 0, stream_>>>(step_, 1);
     SMX3_HIP(hipGetLastError());
   }
@@ -1143,8 +1274,8 @@ private:
         continue;
       auto launch = [&](auto kern) {
         kern<<<dim3(cnt), dim3(ELEM_THREADS), 0, stream_>>>(
-            s0, ibool_, geom_, nodes_, xi3_, mat_, rb_, rec_u_, every,
-            adj_step_, kacc_, gd_, ge_, npad_, quad_);
+            color_slots_ + s0, ibool_, geom_, nodes_, xi3_, mat_, rb_, rec_u_,
+            every, adj_step_, kacc_, gd_, ge_, npad_, quad_);
       };
       const bool me = A.mat_elem;
       switch (gmode_) {
@@ -1260,34 +1391,57 @@ private:
     return dim3(unsigned((3 * npad_ + NODE3_THREADS - 1) / NODE3_THREADS));
   }
 
-  // Every colour's element pass, accumulating into `acc`.
-  void elem_passes(real *acc) {
+  // The tile pass of one step (A: where M3_FINAL / M3_ASSEMBLE store a).
+  template <int MODE> void launch_tile(real *acc_out, const Rec3<real> &rec) {
     const HostAssembly3D &A = *asm_;
-    for (int c = 0; c < A.ncolors; ++c) {
-      const int s0 = A.color_off[c], cnt = A.color_off[c + 1] - s0;
-      if (cnt == 0)
-        continue;
-      auto launch = [&](auto kern) {
-        kern<<<dim3(cnt), dim3(ELEM_THREADS), 0, stream_>>>(
-            s0, ibool_, A.nspec, geom_, nodes_, xi3_, mat_, sslot_, stacey_,
-            nst_, U_, V_, acc, npad_, quad_);
-      };
-      const bool me = A.mat_elem;
-      switch (gmode_) {
-      case GEOM_ELEM:
-        me ? launch(k3_elem<real, GEOM_ELEM, true>)
-           : launch(k3_elem<real, GEOM_ELEM, false>);
-        break;
-      case GEOM_NODES:
-        me ? launch(k3_elem<real, GEOM_NODES, true>)
-           : launch(k3_elem<real, GEOM_NODES, false>);
-        break;
-      default:
-        me ? launch(k3_elem<real, GEOM_POINT, true>)
-           : launch(k3_elem<real, GEOM_POINT, false>);
-      }
-      SMX3_HIP(hipGetLastError());
+    const Aux3<real> aux{aux_v_, aux_a_, A.nsbs};
+    auto launch = [&](auto kern) {
+      kern<<<dim3(unsigned(A.ntiles)), dim3(kRoundWidth * ELEM_THREADS),
+             tile_lds_, stream_>>>(
+          tiles_, round_slots_, lidx_, sh_list_, geom_, nodes_, xi3_, mat_,
+          sslot_, stacey_, U_, V_, acc_out, minv_, part_, npart_, tsrc_li_,
+          tsrc_id_, tsrc_val_, stf_, std::max(1, A.nsrc), std::max(1, A.nloc_max),
+          npad_, dt_, dto2_, dt2o2_, quad_, step_, rec, aux);
+    };
+    const bool me = A.mat_elem;
+    switch (gmode_) {
+    case GEOM_ELEM:
+      me ? launch(k3_tile<real, MODE, GEOM_ELEM, true, kRoundWidth>)
+         : launch(k3_tile<real, MODE, GEOM_ELEM, false, kRoundWidth>);
+      break;
+    case GEOM_NODES:
+      me ? launch(k3_tile<real, MODE, GEOM_NODES, true, kRoundWidth>)
+         : launch(k3_tile<real, MODE, GEOM_NODES, false, kRoundWidth>);
+      break;
+    default:
+      me ? launch(k3_tile<real, MODE, GEOM_POINT, true, kRoundWidth>)
+         : launch(k3_tile<real, MODE, GEOM_POINT, false, kRoundWidth>);
     }
+    SMX3_HIP(hipGetLastError());
+  }
+
+  template <int MODE> void launch_shared(const Rec3<real> &rec) {
+    const HostAssembly3D &A = *asm_;
+    if (A.nsh_total == 0)
+      return;
+    const Aux3<real> aux{aux_v_, aux_a_, A.nsbs};
+    k3_shared<real, MODE><<<dim3(unsigned((A.nsh_total + NODE3_THREADS - 1) /
+                                          NODE3_THREADS)),
+                            NODE3_THREADS, 0, stream_>>>(
+        A.nsh_total, A.nex_total, sh_poff_, sh_pidx_, part_, npart_, U_, V_, A_,
+        minv_, npad_, dt_, dto2_, dt2o2_, step_, rec, aux);
+    SMX3_HIP(hipGetLastError());
+  }
+
+  template <bool POST> void launch_seis() {
+    const HostAssembly3D &A = *asm_;
+    const int ntypes = int(A.seis_types.size());
+    if (A.nrec == 0 || ntypes == 0 || (POST ? !aux_v_ : !need_disp_))
+      return;
+    k3_seis<real, POST><<<dim3(unsigned(A.nrec * ntypes)), 64, 0, stream_>>>(
+        U_, aux_v_, aux_a_, rec_g_, rec_h_, A.nrec, types_, ntypes, A.nsbs,
+        npad_, step_, seis_);
+    SMX3_HIP(hipGetLastError());
   }
 
   // Uses the geometry re-formed from the control nodes when the device
@@ -1323,25 +1477,15 @@ private:
       gmode_ = GEOM_NODES;
   }
 
+  // One forward step: displacement seismograms (u~ of the step), the tile
+  // pass, the shared points, velocity / acceleration seismograms, and the
+  // step counter.
   template <bool FINAL> void step() {
-    const HostAssembly3D &A = *asm_;
-    if (A.nsrc > 0) {
-      k3_src<real><<<1, ELEM_THREADS, 0, stream_>>>(
-          A.nsrc, src_slot_, src_array_, stf_, ibool_, A.nspec, A_, npad_,
-          step_);
-      SMX3_HIP(hipGetLastError());
-    }
-    elem_passes(A_);
-    const int ntypes = int(A.seis_types.size());
-    if (A.nrec > 0 && ntypes > 0) {
-      const int n = A.nrec * ntypes;
-      k3_seis<real><<<n, 64, 0, stream_>>>(
-          U_, V_, A_, minv_, rec_g_, rec_h_, A.nrec, types_, ntypes, A.nsbs,
-          npad_, dto2_, step_, seis_);
-      SMX3_HIP(hipGetLastError());
-    }
-    k3_node<real, FINAL, false><<<node_grid(), NODE3_THREADS, 0, stream_>>>(
-        U_, V_, A_, minv_, 3 * npad_, dt_, dto2_, dt2o2_, step_, Rec3<real>{});
+    launch_seis<false>();
+    launch_tile<FINAL ? M3_FINAL : M3_STEP>(A_, Rec3<real>{});
+    launch_shared<FINAL ? M3_FINAL : M3_STEP>(Rec3<real>{});
+    launch_seis<true>();
+    k3_step_add<<<1, 64, 0, stream_>>>(step_, 1);
     SMX3_HIP(hipGetLastError());
   }
 
@@ -1363,8 +1507,21 @@ private:
     SMX3_HIP(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0));
   }
 
+  // rounds of four elements (512 threads) in FP32; one element per round in
+  // FP64, whose LDS staging is twice as large
+  static constexpr int kRoundWidth = sizeof(real) == 4 ? 4 : 1;
   std::unique_ptr<HostAssembly3D> asm_;
   int gsteps_ = kGraphSteps3;
+  TileInfo3 *tiles_ = nullptr;
+  uint16_t *lidx_ = nullptr;
+  int *round_slots_ = nullptr, *sh_list_ = nullptr, *sh_poff_ = nullptr,
+      *sh_pidx_ = nullptr, *color_slots_ = nullptr, *tsrc_li_ = nullptr,
+      *tsrc_id_ = nullptr;
+  real *tsrc_val_ = nullptr, *part_ = nullptr, *aux_v_ = nullptr,
+       *aux_a_ = nullptr;
+  long npart_ = 1;
+  size_t tile_lds_ = 0;
+  bool need_disp_ = false;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   hipGraph_t graph_ = nullptr;
@@ -1372,14 +1529,14 @@ private:
   std::vector<void *> allocs_;
   long npad_ = 0;
   int nst_ = 1;
-  int *ibool_ = nullptr, *sslot_ = nullptr, *src_slot_ = nullptr,
+  int *ibool_ = nullptr, *sslot_ = nullptr,
       *rec_g_ = nullptr, *step_ = nullptr;
   int gmode_ = GEOM_POINT;
   size_t geom_mismatch_ = 0;
   float *nodes_ = nullptr;
   Quad3<float> xi3_{}; // GLL nodes in w[] (re-formed geometry)
   real *geom_ = nullptr, *mat_ = nullptr, *stacey_ = nullptr, *minv_ = nullptr,
-       *src_array_ = nullptr, *stf_ = nullptr, *rec_h_ = nullptr, *U_ = nullptr,
+       *stf_ = nullptr, *rec_h_ = nullptr, *U_ = nullptr,
        *V_ = nullptr, *A_ = nullptr, *seis_ = nullptr, *scratch_a_ = nullptr;
   Quad3<real> quad_;
   Types3 types_;
