@@ -638,67 +638,264 @@ __global__ void __launch_bounds__(ELEM_THREADS)
   }
 }
 
-// Adjoint node pass of step k = *adj_step (the transpose of the forward
-// step): with rb = W (ab + dt/2 vb),
+// Adjoint state of the nodal update (device pointers and sizes).
+template <typename real> struct AdjState3 {
+  real *ub, *vb, *rb;          // [3][npad]
+  const real *minv;            // [3][npad]
+  const real *rec_a, *rec_vb;  // record buffers (a' per step, v~ at absorbing nodes)
+  int every;
+  real *R;                     // [npad] rb . a' per node
+  const int *bidx;             // node -> absorbing-node index or -1
+  const real *bmat;            // [6][nb] Stacey matrix per absorbing node
+  real *T9, *D3;               // [9][nb], [3][nb]
+  long nb, npad;
+  real dt, dto2, dt2o2;
+};
+
+// Adjoint update of node g at step k (the transpose of the forward step),
+// given its assembled -K rb (kr): with rb = W (ab + dt/2 vb),
 //   uu = ub - K rb,  vv = vb - B rb,
 //   ub = uu,  vb = vv + dt uu,  ab = dt/2 vv + dt^2/2 uu,  rb = W (ab + dt/2 vb)
 // (rb of step k-1 is formed here; ab is not kept).  Accumulates rb . a'_k
 // per node (density term) and, at the absorbing nodes, rb v~_k^T (9) and
 // rb * a'_k per component (3) for the impedance terms; B at a node is the
-// sum of its face points' symmetric 3x3 Stacey matrices [6][nb].
+// sum of its face points' symmetric 3x3 Stacey matrices.
 template <typename real>
-__global__ void __launch_bounds__(NODE3_THREADS)
-    k3_adj_node(real *__restrict__ Ub, real *__restrict__ Vb,
-                real *__restrict__ Rb, real *__restrict__ Kacc,
-                const real *__restrict__ Minv, const real *__restrict__ rec_a,
-                const real *__restrict__ rec_vb, int every,
-                real *__restrict__ R, const int *__restrict__ bidx,
-                const real *__restrict__ bmat, real *__restrict__ T9,
-                real *__restrict__ D3, long nb, long npad, int nglob, real dt,
-                real dto2, real dt2o2, const int *__restrict__ adj_step) {
-  const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= nglob)
-    return;
-  const int j = *adj_step % every;
-  const real *ap = rec_a + size_t(j) * 3 * npad;
+__device__ __forceinline__ void adj_finish(long g, const real kr[3], int j,
+                                           const AdjState3<real> &S) {
+  const long npad = S.npad, nb = S.nb;
+  const real *ap = S.rec_a + size_t(j) * 3 * npad;
   real rb[3], a[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    rb[c] = Rb[c * npad + g];
+    rb[c] = S.rb[c * npad + g];
     a[c] = ap[c * npad + g];
   }
-  R[g] += rb[0] * a[0] + rb[1] * a[1] + rb[2] * a[2];
+  S.R[g] += rb[0] * a[0] + rb[1] * a[1] + rb[2] * a[2];
   real br[3] = {0, 0, 0};
-  const int b = bidx[g];
+  const int b = S.bidx[g];
   if (b >= 0) {
-    const real mxx = bmat[b], myy = bmat[nb + b], mzz = bmat[2 * nb + b],
-               mxy = bmat[3 * nb + b], mxz = bmat[4 * nb + b],
-               myz = bmat[5 * nb + b];
+    const real *bm = S.bmat;
+    const real mxx = bm[b], myy = bm[nb + b], mzz = bm[2 * nb + b],
+               mxy = bm[3 * nb + b], mxz = bm[4 * nb + b], myz = bm[5 * nb + b];
     br[0] = mxx * rb[0] + mxy * rb[1] + mxz * rb[2];
     br[1] = mxy * rb[0] + myy * rb[1] + myz * rb[2];
     br[2] = mxz * rb[0] + myz * rb[1] + mzz * rb[2];
-    const real *vt = rec_vb + size_t(j) * 3 * nb;
+    const real *vt = S.rec_vb + size_t(j) * 3 * nb;
     const real v[3] = {vt[b], vt[nb + b], vt[2 * nb + b]};
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
 #pragma unroll
       for (int d = 0; d < 3; ++d)
-        T9[(c * 3 + d) * nb + b] += rb[c] * v[d];
-      D3[c * nb + b] += rb[c] * a[c];
+        S.T9[(c * 3 + d) * nb + b] += rb[c] * v[d];
+      S.D3[c * nb + b] += rb[c] * a[c];
     }
   }
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const long i = c * npad + g;
-    const real uu = Ub[i] + Kacc[i];
-    const real vv = Vb[i] - br[c];
-    const real vbn = vv + dt * uu;
-    const real abn = dto2 * vv + dt2o2 * uu;
-    Ub[i] = uu;
-    Vb[i] = vbn;
-    Rb[i] = Minv[i] * (abn + dto2 * vbn);
-    Kacc[i] = 0;
+    const real uu = S.ub[i] + kr[c];
+    const real vv = S.vb[i] - br[c];
+    const real vbn = vv + S.dt * uu;
+    const real abn = S.dto2 * vv + S.dt2o2 * uu;
+    S.ub[i] = uu;
+    S.vb[i] = vbn;
+    S.rb[i] = S.minv[i] * (abn + S.dto2 * vbn);
   }
+}
+
+// Adjoint node pass after the colour passes: every node with its assembled
+// -K rb in Kacc (then cleared for the next step).
+template <typename real>
+__global__ void __launch_bounds__(NODE3_THREADS)
+    k3_adj_node(real *__restrict__ Kacc, AdjState3<real> S, int nglob,
+                const int *__restrict__ adj_step) {
+  const long g = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nglob)
+    return;
+  const int j = *adj_step % S.every;
+  real kr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    kr[c] = Kacc[c * S.npad + g];
+  adj_finish(g, kr, j, S);
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    Kacc[c * S.npad + g] = 0;
+}
+
+// Adjoint step k = *adj_step as element tiles (the layout of k3_tile): per
+// round, each element gathers rb and the recorded u~_k, forms both
+// gradients, accumulates the densities div(rb) div(u~) (Gd) and
+// eps(rb) : eps(u~) (Ge) per local point and assembles -K rb (without the
+// Stacey term, which adj_finish applies as B rb) into the tile's LDS
+// accumulator; the tile's exclusive nodes are then updated in place
+// (adj_finish) and its shared nodes get partials.  LDS: accumulator
+// [3][nloc_max], then per round slot rb [3][125], u~ [3][125] and the
+// stress integrand [3][3][125].
+template <typename real, int GM, bool ME, int RW>
+__global__ void __launch_bounds__(RW * ELEM_THREADS)
+    k3_adj_tile(const TileInfo3 *__restrict__ tiles,
+                const int *__restrict__ round_slots,
+                const uint16_t *__restrict__ lidx,
+                const int *__restrict__ sh_list, const real *__restrict__ geom,
+                const float *__restrict__ nodes, Quad3<float> xi3,
+                const real *__restrict__ mat, const real *__restrict__ rec_u,
+                real *__restrict__ part, long npart, real *__restrict__ Gd,
+                real *__restrict__ Ge, int nloc_max, Quad3<real> q,
+                const int *__restrict__ adj_step, AdjState3<real> S) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  real *s_a = reinterpret_cast<real *>(lds_raw);
+  const TileInfo3 T = tiles[blockIdx.x];
+  const int tid = threadIdx.x;
+  const int sub = tid / ELEM_THREADS, p = tid % ELEM_THREADS;
+  const bool pact = p < NPT3;
+  const int pp = pact ? p : 0;
+  const int iz = pp / (NGLL * NGLL), iy = (pp / NGLL) % NGLL, ix = pp % NGLL;
+  real *sr = s_a + 3 * size_t(nloc_max) + size_t(sub) * 15 * NPT3;
+  real *su = sr + 3 * NPT3;
+  real *sf = su + 3 * NPT3;
+  const long npad = S.npad;
+  const int nloc = T.nex + T.nsh;
+  const int j = *adj_step % S.every;
+  const real *Ut = rec_u + size_t(j) * 3 * npad;
+  for (int i = tid; i < nloc; i += RW * ELEM_THREADS) {
+    s_a[i] = 0;
+    s_a[nloc_max + i] = 0;
+    s_a[2 * nloc_max + i] = 0;
+  }
+  __syncthreads();
+  for (int r = 0; r < T.nround; ++r) {
+    const int slot = round_slots[(size_t(T.round_off) + r) * RW + sub];
+    const bool act = pact && slot >= 0;
+    int l = 0;
+    if (act) {
+      l = lidx[size_t(slot) * NPT3 + p];
+      const long g = l < T.nex ? long(T.ex_off) + l
+                               : long(sh_list[T.sh_off + l - T.nex]);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sr[c * NPT3 + p] = S.rb[c * npad + g];
+        su[c * NPT3 + p] = Ut[c * npad + g];
+      }
+    }
+    __syncthreads();
+    if (act) {
+      real G[10];
+      if (GM == GEOM_NODES) {
+        float Gf[10];
+        geom_from_nodes3(nodes + size_t(slot) * 24, xi3.w[ix], xi3.w[iy],
+                         xi3.w[iz], Gf);
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+          G[k] = Gf[k];
+      } else {
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+          G[k] = GM == GEOM_ELEM ? geom[size_t(slot) * 10 + k]
+                                 : geom[(size_t(slot) * 10 + k) * NPT3 + p];
+      }
+      real dr[3][3], du[3][3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        real a0 = 0, a1 = 0, a2 = 0, b0 = 0, b1 = 0, b2 = 0;
+#pragma unroll
+        for (int k = 0; k < NGLL; ++k) {
+          a0 += q.hp[ix][k] * sr[c * NPT3 + p3(iz, iy, k)];
+          a1 += q.hp[iy][k] * sr[c * NPT3 + p3(iz, k, ix)];
+          a2 += q.hp[iz][k] * sr[c * NPT3 + p3(k, iy, ix)];
+          b0 += q.hp[ix][k] * su[c * NPT3 + p3(iz, iy, k)];
+          b1 += q.hp[iy][k] * su[c * NPT3 + p3(iz, k, ix)];
+          b2 += q.hp[iz][k] * su[c * NPT3 + p3(k, iy, ix)];
+        }
+        dr[c][0] = G[0] * a0 + G[3] * a1 + G[6] * a2;
+        dr[c][1] = G[1] * a0 + G[4] * a1 + G[7] * a2;
+        dr[c][2] = G[2] * a0 + G[5] * a1 + G[8] * a2;
+        du[c][0] = G[0] * b0 + G[3] * b1 + G[6] * b2;
+        du[c][1] = G[1] * b0 + G[4] * b1 + G[7] * b2;
+        du[c][2] = G[2] * b0 + G[5] * b1 + G[8] * b2;
+      }
+      const real divr = dr[0][0] + dr[1][1] + dr[2][2];
+      const real divu = du[0][0] + du[1][1] + du[2][2];
+      const real ee = dr[0][0] * du[0][0] + dr[1][1] * du[1][1] +
+                      dr[2][2] * du[2][2] +
+                      real(0.5) * ((dr[0][1] + dr[1][0]) * (du[0][1] + du[1][0]) +
+                                   (dr[0][2] + dr[2][0]) * (du[0][2] + du[2][0]) +
+                                   (dr[1][2] + dr[2][1]) * (du[1][2] + du[2][1]));
+      const size_t o = size_t(slot) * NPT3 + p;
+      Gd[o] += divr * divu;
+      Ge[o] += ee;
+      const real L = ME ? mat[size_t(slot) * 2] : mat[size_t(slot) * 2 * NPT3 + p];
+      const real M = ME ? mat[size_t(slot) * 2 + 1]
+                        : mat[(size_t(slot) * 2 + 1) * NPT3 + p];
+      const real lam = L - real(2.0) * M;
+      real Ts[3][3];
+      Ts[0][0] = L * dr[0][0] + lam * (dr[1][1] + dr[2][2]);
+      Ts[1][1] = L * dr[1][1] + lam * (dr[0][0] + dr[2][2]);
+      Ts[2][2] = L * dr[2][2] + lam * (dr[0][0] + dr[1][1]);
+      Ts[0][1] = Ts[1][0] = M * (dr[0][1] + dr[1][0]);
+      Ts[0][2] = Ts[2][0] = M * (dr[0][2] + dr[2][0]);
+      Ts[1][2] = Ts[2][1] = M * (dr[1][2] + dr[2][1]);
+      const real J = G[9];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        sf[(c * 3 + 0) * NPT3 + p] =
+            J * (Ts[c][0] * G[0] + Ts[c][1] * G[1] + Ts[c][2] * G[2]);
+        sf[(c * 3 + 1) * NPT3 + p] =
+            J * (Ts[c][0] * G[3] + Ts[c][1] * G[4] + Ts[c][2] * G[5]);
+        sf[(c * 3 + 2) * NPT3 + p] =
+            J * (Ts[c][0] * G[6] + Ts[c][1] * G[7] + Ts[c][2] * G[8]);
+      }
+    }
+    __syncthreads();
+    if (act) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        real t1 = 0, t2 = 0, t3 = 0;
+#pragma unroll
+        for (int k = 0; k < NGLL; ++k) {
+          t1 += sf[(c * 3 + 0) * NPT3 + p3(iz, iy, k)] * q.hp[k][ix] * q.w[k];
+          t2 += sf[(c * 3 + 1) * NPT3 + p3(iz, k, ix)] * q.hp[k][iy] * q.w[k];
+          t3 += sf[(c * 3 + 2) * NPT3 + p3(k, iy, ix)] * q.hp[k][iz] * q.w[k];
+        }
+        const real rr = q.w[iz] * q.w[iy] * t1 + q.w[iz] * q.w[ix] * t2 +
+                        q.w[iy] * q.w[ix] * t3;
+        s_a[c * nloc_max + l] += real(-1.0) * rr;
+      }
+    }
+    __syncthreads();
+  }
+  for (int i = tid; i < T.nex; i += RW * ELEM_THREADS) {
+    const real kr[3] = {s_a[i], s_a[nloc_max + i], s_a[2 * nloc_max + i]};
+    adj_finish(long(T.ex_off) + i, kr, j, S);
+  }
+  for (int i = tid; i < T.nsh; i += RW * ELEM_THREADS)
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      part[c * npart + T.sh_off + i] = s_a[c * nloc_max + T.nex + i];
+}
+
+// Shared nodes of the adjoint step: partials summed in tile order, then
+// adj_finish.
+template <typename real>
+__global__ void __launch_bounds__(NODE3_THREADS)
+    k3_adj_shared(int nsh, int nex_total, const int *__restrict__ sh_poff,
+                  const int *__restrict__ sh_pidx, const real *__restrict__ part,
+                  long npart, AdjState3<real> S,
+                  const int *__restrict__ adj_step) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= nsh)
+    return;
+  const int j = *adj_step % S.every;
+  real kr[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    real acc = 0;
+    for (int k = sh_poff[s]; k < sh_poff[s + 1]; ++k)
+      acc += part[c * npart + sh_pidx[k]];
+    kr[c] = acc;
+  }
+  adj_finish(long(nex_total) + s, kr, j, S);
 }
 
 template <typename real>
@@ -774,6 +971,10 @@ public:
                            " bytes of LDS (device limit " +
                            std::to_string(prop.sharedMemPerBlock) +
                            "); use fewer elements per tile");
+      adj_lds_ = (size_t(3) * std::max(1, A.nloc_max) +
+                  size_t(kRoundWidth) * 15 * NPT3) *
+                 sizeof(real);
+      adj_tiles_ = adj_lds_ <= prop.sharedMemPerBlock;
     }
     rec_g_ = upload<int>(A.rec_g);
     rec_h_ = upload<real>(A.rec_h);
@@ -1259,7 +1460,15 @@ private:
     SMX3_HIP(hipGetLastError());
   }
 
-  // One adjoint step (k = *adj_step, then k - 1).
+  AdjState3<real> adj_state(int every) const {
+    return AdjState3<real>{ub_,    vb_,    rb_,   minv_, rec_a_, rec_vb_,
+                           every,  racc_,  bidx_, bmat_, t9_,    d3_,
+                           nbn_alloc_, npad_, dt_, dto2_, dt2o2_};
+  }
+
+  // One adjoint step (k = *adj_step, then k - 1): misfit source, then the
+  // tile pass and the shared nodes, or (when a tile's adjoint staging does
+  // not fit in LDS) the colour passes and the node pass.
   void adjoint_step(int every) {
     const HostAssembly3D &A = *asm_;
     if (nrin_ > 0) {
@@ -1267,6 +1476,42 @@ private:
           rin_node_, rin_off_, rin_r_, rin_h_, nrin_, gsrc_, A.nrec, ub_, npad_,
           adj_step_);
       SMX3_HIP(hipGetLastError());
+    }
+    if (adj_tiles_) {
+      const AdjState3<real> S = adj_state(every);
+      auto launch = [&](auto kern) {
+        kern<<<dim3(unsigned(A.ntiles)), dim3(kRoundWidth * ELEM_THREADS),
+               adj_lds_, stream_>>>(tiles_, round_slots_, lidx_, sh_list_, geom_,
+                                    nodes_, xi3_, mat_, rec_u_, part_, npart_,
+                                    gd_, ge_, std::max(1, A.nloc_max), quad_,
+                                    adj_step_, S);
+      };
+      const bool me = A.mat_elem;
+      switch (gmode_) {
+      case GEOM_ELEM:
+        me ? launch(k3_adj_tile<real, GEOM_ELEM, true, kRoundWidth>)
+           : launch(k3_adj_tile<real, GEOM_ELEM, false, kRoundWidth>);
+        break;
+      case GEOM_NODES:
+        me ? launch(k3_adj_tile<real, GEOM_NODES, true, kRoundWidth>)
+           : launch(k3_adj_tile<real, GEOM_NODES, false, kRoundWidth>);
+        break;
+      default:
+        me ? launch(k3_adj_tile<real, GEOM_POINT, true, kRoundWidth>)
+           : launch(k3_adj_tile<real, GEOM_POINT, false, kRoundWidth>);
+      }
+      SMX3_HIP(hipGetLastError());
+      if (A.nsh_total > 0) {
+        k3_adj_shared<real><<<dim3(unsigned((A.nsh_total + NODE3_THREADS - 1) /
+                                            NODE3_THREADS)),
+                              NODE3_THREADS, 0, stream_>>>(
+            A.nsh_total, A.nex_total, sh_poff_, sh_pidx_, part_, npart_, S,
+            adj_step_);
+        SMX3_HIP(hipGetLastError());
+      }
+      k3_step_add<<<1, 64, 0, stream_>>>(adj_step_, -1);
+      SMX3_HIP(hipGetLastError());
+      return;
     }
     for (int c = 0; c < A.ncolors; ++c) {
       const int s0 = A.color_off[c], cnt = A.color_off[c + 1] - s0;
@@ -1294,10 +1539,8 @@ private:
       SMX3_HIP(hipGetLastError());
     }
     k3_adj_node<real><<<dim3(unsigned((npad_ + NODE3_THREADS - 1) / NODE3_THREADS)),
-                        NODE3_THREADS, 0, stream_>>>(
-        ub_, vb_, rb_, kacc_, minv_, rec_a_, rec_vb_, every, racc_, bidx_,
-        bmat_, t9_, d3_, nbn_alloc_, npad_, A.nglob, dt_, dto2_, dt2o2_,
-        adj_step_);
+                        NODE3_THREADS, 0, stream_>>>(kacc_, adj_state(every),
+                                                     A.nglob, adj_step_);
     SMX3_HIP(hipGetLastError());
     k3_step_add<<<1, 64, 0, stream_>>>(adj_step_, -1);
     SMX3_HIP(hipGetLastError());
@@ -1520,7 +1763,8 @@ private:
   real *tsrc_val_ = nullptr, *part_ = nullptr, *aux_v_ = nullptr,
        *aux_a_ = nullptr;
   long npart_ = 1;
-  size_t tile_lds_ = 0;
+  size_t tile_lds_ = 0, adj_lds_ = 0;
+  bool adj_tiles_ = false; // adjoint as tiles (else colour passes)
   bool need_disp_ = false;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
