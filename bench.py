@@ -47,6 +47,27 @@ def tile_kernel_bytes(nspec, layout, nstacey):
             + 8 * layout["npartials"] + 800 * nstacey)
 
 
+def tile3d_kernel_bytes(nspec, layout, nstacey, assemble=False):
+    """Algorithmic bytes of one 3D tile pass (k3_tile, one time step, FP32;
+    `assemble`: the timing variant that stores a = (f - Ku)/M instead of
+    finishing the step, 24 B per exclusive point instead of 60 B):
+    * per element: 2-byte tile-local indices (250 B), geometry (10 terms per
+      GLL point, 5000 B, or 40 B for an affine element), moduli (2 per GLL
+      point, 1000 B, or 8 B), the absorbing-slot index (4 B);
+    * the tile's points gathered once: 12 B per exclusive point and per
+      partial slot (a tile's share of a shared point);
+    * per exclusive point: v, 1/M, u read and u, v written, 3 components
+      (60 B); per partial slot: 12 B written;
+    * absorbing elements: 7 face terms per point (3500 B) and the velocity."""
+    ge = layout.get("geom_elem_elements", 0)
+    me = layout.get("mat_elem_elements", 0)
+    per_elem = (254 * nspec + 5000 * (nspec - ge) + 40 * ge + 1000 * (nspec - me)
+                + 8 * me)
+    nx, npart = layout["nexclusive"], layout["npartials"]
+    per_ex = 24 if assemble else 60
+    return per_elem + 12 * (nx + npart) + per_ex * nx + 12 * npart + 3500 * nstacey
+
+
 def measured_traffic(path=os.path.join(ROOT, "profiles", "k_tile_layered_800x600.md")):
     """HBM bytes per k_tile launch on the 800x600 workload as recorded from
     the rocprofv3 FETCH_SIZE / WRITE_SIZE passes in the profile summary."""
@@ -251,7 +272,9 @@ def hex_leg(n=64, nstep=4000, with_adjoint=True):
     wall = time.perf_counter() - t
     seis = ctx.seismograms()
     stiff_ms = ctx.time_stiffness(10)
-    nglob, nspec, geometry = ctx.nglob, ctx.nspec, ctx.geometry
+    nglob, nspec, geometry, layout = ctx.nglob, ctx.nspec, ctx.geometry, dict(ctx.layout)
+    nstacey = len(set(cfg.mesh.abs_ispec.tolist()))
+    tb = tile3d_kernel_bytes(nspec, layout, nstacey, assemble=True)
     adjoint = None
     if with_adjoint:
         # gradient of chi = 1/2 dt sum |s|^2 (the L2 misfit against zero data):
@@ -283,7 +306,14 @@ def hex_leg(n=64, nstep=4000, with_adjoint=True):
             "point_updates_per_s": nglob * nstep / wall,
             "canonical_bytes_per_step": canon,
             "canonical_frac": canon * nstep / wall / 1e9 / HBM_PEAK_GBS,
-            "element_passes_ms": stiff_ms, "seismograms_finite": bool(np.isfinite(seis).all()),
+            "tile_pass": {"kernel": "k3_tile<float, assemble> (timing variant: a = "
+                                    "(f - K u - B v)/M stored, no Newmark update)",
+                          "mean_launch_ms": stiff_ms,
+                          "algorithmic_bytes_per_launch": tb,
+                          "achieved_GBs": tb / (stiff_ms * 1e-3) / 1e9,
+                          "frac": tb / (stiff_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                          "layout": layout},
+            "seismograms_finite": bool(np.isfinite(seis).all()),
             "geometry": geometry, "adjoint": adjoint,
             "host_s": {"generate": gen_s, "assemble": setup_s}}
 

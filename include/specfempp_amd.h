@@ -209,9 +209,9 @@ int smx_ctx_destroy(smx_ctx *ctx);
 int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
                  int *nrec);
 
-/* Device layout of the assembly: element tiles, GLL points exclusive to one
- * tile, points shared between tiles, and per-tile partial slots of shared
- * points.  No reference counterpart (the reference's layout is Kokkos
+/* Device layout of the assembly (2D or 3D): element tiles, GLL points
+ * exclusive to one tile, points shared between tiles, and per-tile partial
+ * slots of shared points.  No reference counterpart (the reference's layout is Kokkos
  * views); used to state the kernels' algorithmic byte counts. */
 int smx_ctx_layout(const smx_ctx *ctx, int *ntiles, int *nexclusive,
                    int *nshared, int *npartials);

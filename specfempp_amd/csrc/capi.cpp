@@ -225,6 +225,14 @@ int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
 int smx_ctx_layout(const smx_ctx *ctx, int *ntiles, int *nexclusive,
                    int *nshared, int *npartials) {
   return guarded([&] {
+    if (auto *s3 = S3(const_cast<smx_ctx *>(ctx))) {
+      const auto &B = s3->assembly();
+      if (ntiles) *ntiles = B.ntiles;
+      if (nexclusive) *nexclusive = B.nex_total;
+      if (nshared) *nshared = B.nsh_total;
+      if (npartials) *npartials = B.npart;
+      return;
+    }
     const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
     if (ntiles) *ntiles = A.ntiles;
     if (nexclusive) *nexclusive = A.nex_total;
@@ -235,6 +243,12 @@ int smx_ctx_layout(const smx_ctx *ctx, int *ntiles, int *nexclusive,
 
 int smx_ctx_compact(const smx_ctx *ctx, int *geom_elements, int *mat_elements) {
   return guarded([&] {
+    if (auto *s3 = S3(const_cast<smx_ctx *>(ctx))) {
+      const auto &B = s3->assembly();
+      if (geom_elements) *geom_elements = B.geom_elem ? B.nspec : 0;
+      if (mat_elements) *mat_elements = B.mat_elem ? B.nspec : 0;
+      return;
+    }
     const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
     int g = 0, m = 0;
     for (const auto &t : A.tiles) {

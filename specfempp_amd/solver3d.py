@@ -66,6 +66,14 @@ class Context3D:
                                ctypes.byref(nr)))
         self.nspec, self.nglob, self.nsig, self.nrec = ns.value, ng.value, nsig.value, nr.value
         self.t0 = lib.smx_ctx_t0(h)
+        nt, nx, nsh, npart = (ctypes.c_int() for _ in range(4))
+        check(lib.smx_ctx_layout(h, ctypes.byref(nt), ctypes.byref(nx), ctypes.byref(nsh),
+                                 ctypes.byref(npart)))
+        gel, mel = ctypes.c_int(), ctypes.c_int()
+        check(lib.smx_ctx_compact(h, ctypes.byref(gel), ctypes.byref(mel)))
+        self.layout = {"ntiles": nt.value, "nexclusive": nx.value, "nshared": nsh.value,
+                       "npartials": npart.value, "geom_elem_elements": gel.value,
+                       "mat_elem_elements": mel.value}
         gm, mis = ctypes.c_int(), ctypes.c_longlong()
         check(lib.smx_ctx_geometry(h, ctypes.byref(gm), ctypes.byref(mis)))
         self.geometry = {"mode": ("point", "element", "nodes")[gm.value],
