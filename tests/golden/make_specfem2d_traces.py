@@ -5,6 +5,9 @@ Source directories (read as text, in this container only):
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomain
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomainStaceyDirichletBC
 
+Each case's mesh database (database.bin, written by the reference's mesher)
+is also copied, gzip-compressed, as specfem2d_<case>.database.bin.gz.
+
 Each fixture holds the station table (STATIONS: name, network, x, z), the
 sample times, and traces[type][station][component][t] for the seismogram types
 present (d, v, a; pressure 'p' traces are stored separately).  The problem set
@@ -14,6 +17,7 @@ tests/test_oracle_golden.py from the fixtures' Par_file / sources.yaml.
 Usage: python tests/golden/make_specfem2d_traces.py [/root/reference]
 """
 
+import gzip
 import os
 import sys
 
@@ -27,6 +31,10 @@ def main(ref="/root/reference"):
     out_dir = os.path.dirname(os.path.abspath(__file__))
     for case in CASES:
         d = os.path.join(ref, BASE, case)
+        with open(os.path.join(d, "database.bin"), "rb") as fh:
+            db = fh.read()
+        with open(os.path.join(out_dir, f"specfem2d_{case}.database.bin.gz"), "wb") as fh:
+            fh.write(gzip.compress(db, 9, mtime=0))
         names, nets, xs, zs = [], [], [], []
         with open(os.path.join(d, "STATIONS")) as fh:
             for line in fh:

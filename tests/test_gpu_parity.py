@@ -169,3 +169,27 @@ def test_geometry_from_nodes_is_bitwise(oracle_lib):
     assert gm.value != 2
     # same geometry bits, same arithmetic after it
     assert rel_l2(ours, stored) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["HomogeneousElasticDomain",
+                                  "HomogeneousElasticDomainStaceyDirichletBC"])
+def test_database_mesh_fixtures(oracle_lib, name):
+    """The reference's own database.bin of the fixture (read by
+    specfempp_amd.meshdb) through the fused GPU path: against the oracle on
+    the same mesh (1e-5) and against the SPECFEM2D traces with the
+    reference's criterion."""
+    import dataclasses
+    import os
+    from specfempp_amd.meshdb import read_mesh_database_2d
+    from tests._cases import GOLDEN, fixture_case, reference_metric
+    cfg, g = fixture_case(name)
+    mesh, _ = read_mesh_database_2d(os.path.join(GOLDEN, f"specfem2d_{name}.database.bin.gz"))
+    cfg = dataclasses.replace(cfg, mesh=mesh)
+    _say(f"database mesh {name}: GPU vs oracle and SPECFEM2D")
+    ours, _ = simulate(cfg)
+    ref = oracle_lib.Oracle2D(mesh, cfg.sources, cfg.receivers, cfg.dt,
+                              cfg.nstep).run(types=cfg.seismo_types)
+    e_or, e_sp = rel_l2(ours[:, 0], ref[:, 0]), reference_metric(ours, g["traces"])
+    _say(f"  vs oracle {e_or:.3e}; vs SPECFEM2D (reference metric) {e_sp:.3e}")
+    assert e_or <= 1e-5
+    assert e_sp < 1e-3

@@ -1,0 +1,48 @@
+"""Seismograms in the reference's ASCII format: names, line format, time
+axis, a write/read round trip, and the reference's criterion applied to the
+SPECFEM2D golden traces through files."""
+
+import re
+
+import numpy as np
+
+from specfempp_amd.seismo_io import (band_code, read_seismograms, reference_error,
+                                     station_filenames, write_seismograms)
+from tests._cases import fixture_case
+
+
+def test_names_and_band_codes():
+    assert station_filenames("AA", "S0001", "displacement") == [
+        "AA.S0001.S2.BXX.semd", "AA.S0001.S2.BXZ.semd"]
+    assert station_filenames("TEST", "X20", "velocity", dim=3, dt=0.035) == [
+        "TEST.X20.S3.BXX.semv", "TEST.X20.S3.BXY.semv", "TEST.X20.S3.BXZ.semv"]
+    assert [band_code(d) for d in (2.0, 0.5, 0.02, 0.005, 0.002, 5e-4)] == list("LMBHCF")
+
+
+def test_round_trip_and_line_format(tmp_path):
+    rng = np.random.default_rng(0)
+    for ncomp, types in ((2, ("displacement", "velocity")), (3, ("acceleration",))):
+        seis = rng.standard_normal((37, len(types), 3, ncomp)).astype(np.float32) * 1e-3
+        st = [("AA", f"S{k:04d}") for k in range(3)]
+        paths = write_seismograms(tmp_path / f"d{ncomp}", seis, st, types, dt=1.1e-3,
+                                  t0=-0.12, nsbs=2)
+        assert len(paths) == len(types) * 3 * ncomp
+        line = open(paths[0]).readline().rstrip("\n")
+        assert re.fullmatch(r"-?\d\.\d{6}e[+-]\d{2} -?\d\.\d{6}e[+-]\d{2}", line), line
+        t, back = read_seismograms(tmp_path / f"d{ncomp}", st, types, dim=ncomp, dt=2.2e-3)
+        np.testing.assert_allclose(t, -0.12 + 2.2e-3 * np.arange(37), atol=1e-9)
+        np.testing.assert_allclose(back, seis, rtol=1e-6, atol=1e-12)
+
+
+def test_golden_traces_through_files(oracle_lib, tmp_path):
+    """Oracle seismograms of the Stacey fixture, written in the reference's
+    format, read back and judged with the reference's criterion."""
+    cfg, g = fixture_case("HomogeneousElasticDomainStaceyDirichletBC")
+    o = oracle_lib.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    s = o.run(types=cfg.seismo_types)
+    st = [("AA", n) for n in cfg.receivers.names]
+    write_seismograms(tmp_path, s, st, cfg.seismo_types, cfg.dt, o.t0)
+    t, back = read_seismograms(tmp_path, st, cfg.seismo_types)
+    np.testing.assert_allclose(t, g["times"], atol=1e-6)
+    theirs = np.transpose(g["traces"], (3, 0, 1, 2))  # (nt, ntypes, nsta, 2)
+    assert reference_error(back, theirs) < 1e-3
