@@ -1,0 +1,231 @@
+"""YAML front-end with the reference's Python entry points.
+
+``_initialize(argv)``, ``_execute(parameter_yaml, default_yaml)`` and
+``_finalize()`` mirror the reference's binding (src/python/core.cpp:10-80):
+``_execute`` reads a ``specfem_config.yaml`` (as text), the sources file, the
+STATIONS file and the mesh database it names, runs the forward simulation
+on the GPU and writes the seismograms in the reference's ASCII format.
+Keys follow the reference's parameter files (include/parameter_parser/*):
+
+  parameters:
+    header: {title, description}
+    simulation-setup:
+      quadrature: {quadrature-type: GLL4}
+      solver: {time-marching: {time-scheme: {type: Newmark, dt, nstep}}}
+      simulation-mode: {forward: {writer: {seismogram: {output-format: ascii,
+                                                        output-folder}}}}
+    receivers: {stations, angle, seismogram-type: [...], nstep_between_samples}
+    databases: {mesh-database}
+    sources: <path to a sources YAML>
+
+Sources YAML: ``sources: [{force: {x, z, angle, Ricker: {factor, tshift,
+f0}}}, ...]`` (include/parameter_parser/sources.hpp; force sources with a
+Ricker time function, the elastic path built here).  Relative paths are
+resolved against ``base_dir`` (the reference resolves them against the
+working directory).  Only what this build implements is accepted (elastic
+isotropic P-SV, GLL4, Newmark, forward mode, force sources with Ricker time
+functions, ASCII seismograms); anything else raises ``ValueError`` naming
+the key.
+"""
+
+from __future__ import annotations
+
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+import yaml
+
+from .config import Config2D, Receivers2D, Sources2D
+from .meshdb import read_mesh_database_2d
+from .seismo_io import write_seismograms
+
+_SEISMO = {"displacement", "velocity", "acceleration"}
+
+
+@dataclass
+class Stations:
+    names: list
+    networks: list
+    x: np.ndarray
+    z: np.ndarray
+
+
+@dataclass
+class Program:
+    """A parsed run: the configuration and where its seismograms go."""
+    config: Config2D
+    stations: Stations
+    output_folder: str
+    title: str = ""
+    extra: dict = field(default_factory=dict)
+
+
+def read_stations(path):
+    """STATIONS file: name, network, x, z, (elevation, burial) per line."""
+    names, nets, xs, zs = [], [], [], []
+    with open(path) as fh:
+        for line in fh:
+            p = line.split()
+            if not p:
+                continue
+            if len(p) < 4:
+                raise ValueError(f"{path}: station line needs name, network, x, z: {line!r}")
+            names.append(p[0])
+            nets.append(p[1])
+            xs.append(float(p[2]))
+            zs.append(float(p[3]))
+    return Stations(names, nets, np.array(xs), np.array(zs))
+
+
+def _get(d, *keys, default=None, required=True):
+    cur = d
+    for k in keys:
+        if not isinstance(cur, dict) or k not in cur:
+            if required:
+                raise ValueError("missing key " + "/".join(keys))
+            return default
+        cur = cur[k]
+    return cur
+
+
+def read_sources(node):
+    """Force sources with Ricker time functions from a sources YAML node."""
+    items = node.get("sources", [])
+    n_decl = node.get("number-of-sources")
+    if n_decl is not None and int(n_decl) != len(items):
+        raise ValueError(f"number-of-sources is {n_decl} but {len(items)} are listed")
+    x, z, ang, f0, ts, fac = [], [], [], [], [], []
+    for k, item in enumerate(items):
+        if "force" not in item:
+            raise ValueError(f"source {k}: only force sources are supported ({list(item)})")
+        f = item["force"]
+        if "Ricker" not in f:
+            raise ValueError(f"source {k}: only Ricker time functions are supported")
+        r = f["Ricker"]
+        x.append(float(f["x"]))
+        z.append(float(f["z"]))
+        ang.append(float(f.get("angle", 0.0)))
+        f0.append(float(r["f0"]))
+        ts.append(float(r.get("tshift", 0.0)))
+        fac.append(float(r.get("factor", 1.0)))
+    return Sources2D(x=np.array(x), z=np.array(z), angle=np.array(ang, np.float32),
+                     f0=np.array(f0, np.float32), tshift=np.array(ts, np.float32),
+                     factor=np.array(fac, np.float32))
+
+
+def parse_config(params, base_dir="."):
+    """A Program from a parsed specfem_config (dict with a 'parameters'
+    root, as the reference's files have)."""
+    p = params.get("parameters", params)
+
+    def path(rel):
+        return rel if os.path.isabs(rel) else os.path.join(base_dir, rel)
+
+    setup = _get(p, "simulation-setup")
+    qt = _get(setup, "quadrature", "quadrature-type", required=False, default="GLL4")
+    if qt != "GLL4":
+        raise ValueError(f"quadrature-type {qt!r}: only GLL4 (NGLL = 5) is built")
+    ts = _get(setup, "solver", "time-marching", "time-scheme")
+    if str(ts.get("type", "Newmark")).lower() != "newmark":
+        raise ValueError(f"time-scheme type {ts.get('type')!r}: only Newmark is built")
+    dt, nstep = float(ts["dt"]), int(ts["nstep"])
+    mode = _get(setup, "simulation-mode")
+    if "forward" not in mode:
+        raise ValueError(f"simulation-mode {list(mode)}: only forward runs are driven from YAML "
+                         "(gradients: specfempp_amd.adjoint)")
+    seis_w = _get(mode, "forward", "writer", "seismogram", required=False, default={}) or {}
+    fmt = seis_w.get("output-format", "ascii")
+    if fmt != "ascii":
+        raise ValueError(f"seismogram output-format {fmt!r}: only ascii is written")
+    out = path(str(seis_w.get("output-folder", ".")))
+
+    rec = _get(p, "receivers")
+    st = read_stations(path(rec["stations"]))
+    types = tuple(rec.get("seismogram-type", ["displacement"]))
+    bad = [t for t in types if t not in _SEISMO]
+    if bad:
+        raise ValueError(f"seismogram-type {bad}: only displacement, velocity, acceleration")
+    nsbs = int(rec.get("nstep_between_samples", 1))
+    if float(rec.get("angle", 0.0)) != 0.0:
+        raise ValueError("receiver angle: only 0 is supported")
+
+    mesh, info = read_mesh_database_2d(path(_get(p, "databases", "mesh-database")))
+    src_node = p.get("sources")
+    if isinstance(src_node, str):
+        with open(path(src_node)) as fh:
+            src_node = yaml.safe_load(fh)
+    if not isinstance(src_node, dict):
+        raise ValueError("sources: expected a path or a mapping")
+    sources = read_sources(src_node)
+    receivers = Receivers2D(x=st.x, z=st.z, names=tuple(st.names))
+    title = str(_get(p, "header", "title", required=False, default=""))
+    cfg = Config2D(title or "yaml", mesh, sources, receivers, dt, nstep, nsbs=nsbs,
+                   seismo_types=types, meta={"database": info})
+    return Program(cfg, st, out, title)
+
+
+class _State:
+    program = None
+    initialized = False
+    last = None
+
+
+def _initialize(argv=None):
+    """Loads the GPU library (``argv`` as the reference's; unused).  False if
+    already initialized."""
+    if _State.initialized:
+        return False
+    from . import _lib
+    _lib.load()
+    _State.initialized = True
+    return True
+
+
+def _execute(parameter_string, default_string="", base_dir="."):
+    """Runs the configuration given as YAML text (keys missing from it are
+    taken from ``default_string``) and writes its seismograms; returns True
+    on success, False when not initialized (as the reference)."""
+    if not _State.initialized:
+        return False
+    params = yaml.safe_load(parameter_string) or {}
+    defaults = yaml.safe_load(default_string) if default_string else {}
+    merged = _merge(defaults or {}, params)
+    prog = parse_config(merged, base_dir=base_dir)
+    seis, ctx = run_program(prog)
+    _State.last = (prog, seis, ctx)
+    return True
+
+
+def _finalize():
+    if not _State.initialized:
+        return False
+    _State.initialized = False
+    _State.last = None
+    return True
+
+
+def _merge(base, over):
+    out = dict(base)
+    for k, v in over.items():
+        out[k] = _merge(out[k], v) if isinstance(v, dict) and isinstance(out.get(k), dict) else v
+    return out
+
+
+def run_program(prog: Program, precision=4):
+    """Forward run of a parsed Program; writes the seismograms and returns
+    (seismograms (nsig, ntypes, nrec, 2), ctx)."""
+    from .solver import simulate
+    seis, ctx = simulate(prog.config, precision=precision)
+    write_seismograms(prog.output_folder, seis,
+                      list(zip(prog.stations.networks, prog.stations.names)),
+                      prog.config.seismo_types, prog.config.dt, ctx.t0, prog.config.nsbs)
+    return seis, ctx
+
+
+def run(config_path, precision=4):
+    """Runs a specfem_config.yaml file (paths relative to its folder)."""
+    with open(config_path) as fh:
+        params = yaml.safe_load(fh)
+    prog = parse_config(params, base_dir=os.path.dirname(os.path.abspath(config_path)))
+    return run_program(prog, precision)
