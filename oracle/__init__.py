@@ -53,6 +53,7 @@ def _load():
     lib.oracle2d_forward_store.argtypes = [P, i, P]
     lib.oracle2d_adjoint.argtypes = [P, i, P, P]
     lib.oracle2d_ref_kernels.argtypes = [P, i, P, P, P]
+    lib.oracle2d_time_parallel.argtypes = [P, i, i, i, P]
     lib.oracle2d_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -129,6 +130,17 @@ class Oracle2D:
     def apply_stiffness(self, with_stacey=True):
         self.lib.oracle2d_apply_stiffness(self.h, int(with_stacey))
         return self.get("a")
+
+    def time_parallel(self, nstep, threads, istep0=0):
+        """CPU-baseline timing only: wall seconds of `nstep` colour-parallel
+        forward steps on `threads` OpenMP threads; returns (seconds, threads
+        used)."""
+        sec = np.zeros(1, dtype=np.float64)
+        used = self.lib.oracle2d_time_parallel(self.h, int(istep0), int(nstep), int(threads),
+                                               _ptr(sec))
+        if used < 0:
+            raise RuntimeError(self.lib.oracle2d_last_error().decode())
+        return float(sec[0]), int(used)
 
     def forward_store(self, nstep=None):
         """Forward run from rest keeping every state for the adjoint;

@@ -22,6 +22,8 @@
 //                                reference fixture holds kernel values); pinned
 //                                by finite differences of the misfit instead.
 #include <algorithm>
+#include <chrono>
+#include <omp.h>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -479,7 +481,65 @@ template <typename real> struct Oracle2D {
   // include/kokkos_kernels/impl/compute_stiffness_interaction.tpp:140-247
   void stiffness(const std::vector<real> &U, const std::vector<real> &V,
                  std::vector<real> &A, bool apply_stacey = true) {
+    for (int e : order)
+      element_force(e, U, V, A, apply_stacey);
+  }
+
+  // CPU-baseline timing only (never a checker): the same element work over
+  // colour groups (no two elements of a group share a GLL point), each
+  // group split across OpenMP threads.  The summation order differs from
+  // stiffness(), so results agree to rounding, not bit for bit.
+  std::vector<std::vector<int>> colour_groups;
+  void build_colour_groups() {
+    if (!colour_groups.empty())
+      return;
+    std::vector<uint32_t> used(size_t(nglob), 0u);
     for (int e : order) {
+      uint32_t m = 0;
+      for (int p = 0; p < NGLL * NGLL; ++p)
+        m |= used[ibool[idx(e, p / NGLL, p % NGLL)]];
+      int c = 0;
+      while (c < 32 && ((m >> c) & 1u))
+        ++c;
+      if (c >= 32)
+        throw std::runtime_error("colouring needs more than 32 colours");
+      if (int(colour_groups.size()) <= c)
+        colour_groups.resize(c + 1);
+      colour_groups[c].push_back(e);
+      for (int p = 0; p < NGLL * NGLL; ++p)
+        used[ibool[idx(e, p / NGLL, p % NGLL)]] |= 1u << c;
+    }
+  }
+  void stiffness_parallel(const std::vector<real> &U, const std::vector<real> &V,
+                          std::vector<real> &A) {
+    build_colour_groups();
+    for (const auto &grp : colour_groups) {
+      const int n = int(grp.size());
+#pragma omp parallel for schedule(static)
+      for (int k = 0; k < n; ++k)
+        element_force(grp[k], U, V, A, true);
+    }
+  }
+  void forward_step_parallel(int istep) {
+    const long n2 = long(u.size());
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n2; ++i) {
+      u[i] += deltat * v[i] + deltasquareover2 * a[i];
+      v[i] += deltatover2 * a[i];
+      a[i] = 0;
+    }
+    add_sources(istep, a);
+    stiffness_parallel(u, v, a);
+#pragma omp parallel for schedule(static)
+    for (long i = 0; i < n2; ++i) {
+      a[i] = a[i] * minv[i];
+      v[i] += deltatover2 * a[i];
+    }
+  }
+
+  void element_force(int e, const std::vector<real> &U, const std::vector<real> &V,
+                     std::vector<real> &A, bool apply_stacey) {
+    {
       real du[NGLL][NGLL][2][2];
       element_gradient(e, U, du);
       real F[NGLL][NGLL][2][2];
@@ -1007,6 +1067,38 @@ int oracle2d_run_forward(void *hv, int istep0, int nstep, int nsbs, int ntypes,
   };
   try {
     return h->f ? run(h->f) : run(h->d);
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// CPU-baseline timing: nsteps of the colour-parallel forward step on
+// `nthreads` OpenMP threads from the current state; returns the threads
+// used and the wall seconds of the loop.
+int oracle2d_time_parallel(void *hv, int istep0, int nsteps, int nthreads,
+                           double *seconds) {
+  auto *h = static_cast<OracleHandle *>(hv);
+  try {
+    omp_set_num_threads(nthreads);
+    int used = 1;
+#pragma omp parallel
+    {
+#pragma omp single
+      used = omp_get_num_threads();
+    }
+    auto go = [&](auto *o) {
+      o->build_colour_groups();
+      const auto t0 = std::chrono::steady_clock::now();
+      for (int k = 0; k < nsteps; ++k)
+        o->forward_step_parallel(istep0 + k);
+      *seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    };
+    if (h->f)
+      go(h->f);
+    else
+      go(h->d);
+    return used;
   } catch (const std::exception &e) {
     g_err = e.what();
     return -1;
