@@ -1,7 +1,8 @@
 /*
  * specfempp_amd — MI355X (gfx950) implementation of SPECFEM++'s elastic
  * isotropic time-stepping path (2D P-SV; 3D hex8), its discrete adjoint and
- * batched multi-shot runs, exposed through a plain C ABI.
+ * batched multi-shot runs, and the 2D acoustic isotropic time step, exposed
+ * through a plain C ABI.
  *
  * Every entry point returns 0 on success and a negative status on error;
  * smx_last_error() returns the message of the last failure on the calling
@@ -34,6 +35,11 @@ extern "C" {
 #define SMX_SEIS_DISPLACEMENT 0
 #define SMX_SEIS_VELOCITY 1
 #define SMX_SEIS_ACCELERATION 2
+#define SMX_SEIS_PRESSURE 3 /* acoustic elements only */
+
+/* Source time functions (core/specfem/source_time_functions). */
+#define SMX_STF_RICKER 0
+#define SMX_STF_DIRAC 1 /* Gaussian of f0 = 1 / (10 dt) (dirac.cpp:23-35) */
 
 /* Field components selectable by smx_get_field / smx_set_field. */
 #define SMX_U 0
@@ -47,7 +53,15 @@ typedef struct smx_ctx smx_ctx;
  * (src/io/mesh/impl/fortran/dim2/mesh.cpp:26-215) plus the per-GLL-point
  * material of assembly::properties (core/specfem/assembly/properties).
  * Edge types follow include/enumerations/dim2/mesh_entities.hpp:29-38
- * (1 bottom, 2 right, 3 top, 4 left). */
+ * (1 bottom, 2 right, 3 top, 4 left).
+ *
+ * Elements whose 25 points all have mu == 0 are acoustic (a material with
+ * cs = 0, src/io/mesh/impl/fortran/dim2/read_material_properties.cpp:
+ * 178-204); a mesh is either all elastic or all acoustic in this build (the
+ * elastic-acoustic coupling is not built).  Acoustic free-surface
+ * (Dirichlet) entities come from the database's acoustic-surface records
+ * (read_boundaries.cpp:276-300): edge types 1..4 as above, corners 5
+ * bottom-left, 6 bottom-right, 7 top-right, 8 top-left. */
 typedef struct {
   int nspec;            /* number of spectral elements */
   int ngnod;            /* control nodes per element: 4 or 9 */
@@ -60,17 +74,23 @@ typedef struct {
   int nabs;             /* absorbing (Stacey) edges */
   const int *abs_ispec; /* [nabs] element of each edge */
   const int *abs_type;  /* [nabs] edge type 1..4 */
+  int nfree;             /* acoustic free-surface entities (0 if none) */
+  const int *free_ispec; /* [nfree] element */
+  const int *free_type;  /* [nfree] entity type 1..8 */
 } smx_mesh2d;
 
-/* Ricker point forces (core/specfem/source/dim2/vector_source/force_source.cpp,
- * core/specfem/source_time_functions/ricker.cpp). */
+/* Point forces (core/specfem/source/dim2/vector_source/force_source.cpp)
+ * with a Ricker or Dirac time function (core/specfem/source_time_functions/
+ * ricker.cpp, dirac.cpp).  In acoustic elements the force has one component
+ * and the angle is unused (force_source.cpp:36-40). */
 typedef struct {
   int nsrc;
   const double *x, *z;  /* [nsrc] location */
   const float *angle;   /* [nsrc] degrees clockwise from +z */
-  const float *f0;      /* [nsrc] dominant frequency */
+  const float *f0;      /* [nsrc] dominant frequency (unused for Dirac) */
   const float *tshift;  /* [nsrc] */
   const float *factor;  /* [nsrc] amplitude */
+  const int *stf;       /* [nsrc] SMX_STF_*, or NULL for all Ricker */
 } smx_sources2d;
 
 /* Stations and the seismograms recorded at them
@@ -209,6 +229,13 @@ int smx_ctx_destroy(smx_ctx *ctx);
 int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
                  int *nrec);
 
+/* Field components per GLL point: 2 (2D elastic P-SV), 1 (2D acoustic: the
+ * potential), 3 (3D elastic) — the medium's `components` attribute
+ * (include/enumerations/medium.hpp:98,162, attributes<dim, medium>::components).
+ * smx_get_field / smx_set_field / smx_field_device_ptr arrays are
+ * [nglob][ncomp]. */
+int smx_ctx_components(const smx_ctx *ctx, int *ncomp);
+
 /* Device layout of the assembly (2D or 3D): element tiles, GLL points
  * exclusive to one tile, points shared between tiles, and per-tile partial
  * slots of shared points.  No reference counterpart (the reference's layout is Kokkos
@@ -274,7 +301,7 @@ int smx_synchronize(smx_ctx *ctx);
  * seismogram_components host sync (newmark_tests.cpp:185). */
 int smx_get_seismograms(smx_ctx *ctx, void *host_out);
 
-/* Copies one field component array [nglob][2] of `field` to / from host
+/* Copies one field array [nglob][ncomp] of `field` to / from host
  * (element size = the context's precision). */
 int smx_get_field(smx_ctx *ctx, int field, int which, void *host_out);
 int smx_set_field(smx_ctx *ctx, int field, int which, const void *host_in);
@@ -283,7 +310,7 @@ int smx_set_field(smx_ctx *ctx, int field, int which, const void *host_in);
  * (assembly::mesh index_mapping, core/specfem/assembly/mesh/dim2/impl/assemble.tpp). */
 int smx_get_ibool(smx_ctx *ctx, int *host_out);
 
-/* Device pointer of a field array [nglob][2] (for zero-copy interop). */
+/* Device pointer of a field array [nglob][ncomp] (for zero-copy interop). */
 int smx_field_device_ptr(smx_ctx *ctx, int field, int which, void **dptr);
 
 /* Elapsed milliseconds of the last smx_run_forward, measured with HIP events

@@ -19,7 +19,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "build", "liboracle2d.so")
 _lib = None
 
-SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
+SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2, "pressure": 3}
 
 
 def build(force: bool = False) -> str:
@@ -41,6 +41,9 @@ def _load():
     i, f, d = ctypes.c_int, ctypes.c_float, ctypes.c_double
     lib.oracle2d_create.restype = P
     lib.oracle2d_create.argtypes = [i, i, i, i, P, P, P, P, P, i, P, P, f, i, i, P, P, P, P, P, P, i, P, P]
+    lib.oracle2d_create_ex.restype = P
+    lib.oracle2d_create_ex.argtypes = [i, i, i, i, P, P, P, P, P, i, P, P, i, P, P, f, i, i, P, P,
+                                       P, P, P, P, P, i, P, P]
     lib.oracle2d_destroy.argtypes = [P]
     lib.oracle2d_nglob.argtypes = [P]
     lib.oracle2d_t0.argtypes = [P]
@@ -91,14 +94,20 @@ class Oracle2D:
         ang, f0 = arr(sources.angle, np.float32), arr(sources.f0, np.float32)
         tsh, fac = arr(sources.tshift, np.float32), arr(sources.factor, np.float32)
         rx, rz = arr(receivers.x, np.float64), arr(receivers.z, np.float64)
+        fi = arr(getattr(mesh, "free_ispec", np.zeros(0)), np.int32)
+        ft = arr(getattr(mesh, "free_type", np.zeros(0)), np.int32)
+        kinds = arr(sources.kinds() if hasattr(sources, "kinds") else np.zeros(len(sx)),
+                    np.int32)
         self.nrec = len(rx)
         self.nstep = int(nstep)
         self.nloc = mesh.nspec * 25
-        self.h = lib.oracle2d_create(precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
-                                     _ptr(coord), _ptr(knods), _ptr(kappa), _ptr(mu), _ptr(rho),
-                                     len(ai), _ptr(ai), _ptr(at), float(dt), int(nstep),
-                                     len(sx), _ptr(sx), _ptr(sz), _ptr(ang), _ptr(f0), _ptr(tsh),
-                                     _ptr(fac), self.nrec, _ptr(rx), _ptr(rz))
+        self.acoustic = bool((np.asarray(mesh.mu) == 0).all())
+        self.h = lib.oracle2d_create_ex(precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
+                                        _ptr(coord), _ptr(knods), _ptr(kappa), _ptr(mu),
+                                        _ptr(rho), len(ai), _ptr(ai), _ptr(at), len(fi),
+                                        _ptr(fi), _ptr(ft), float(dt), int(nstep), len(sx),
+                                        _ptr(sx), _ptr(sz), _ptr(ang), _ptr(f0), _ptr(tsh),
+                                        _ptr(fac), _ptr(kinds), self.nrec, _ptr(rx), _ptr(rz))
         if not self.h:
             raise RuntimeError("oracle2d_create failed: " + lib.oracle2d_last_error().decode())
         self.lib = lib

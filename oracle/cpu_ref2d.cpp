@@ -1,7 +1,8 @@
 // ORACLE — test infrastructure only.
 //
-// CPU restatement of SPECFEM++'s 2D elastic isotropic P-SV time-stepping path,
-// written to check the HIP implementation in specfempp_amd/.  Only tests/,
+// CPU restatement of SPECFEM++'s 2D elastic isotropic P-SV time-stepping path
+// and of its acoustic isotropic (potential) counterpart, written to check the
+// HIP implementation in specfempp_amd/.  Only tests/,
 // __graft_entry__.smoke() and bench.py's cpu_baseline leg may load this code,
 // and only as the checker / baseline timer.  The product never links it.
 //
@@ -18,6 +19,13 @@
 //                                reference (HomogeneousElasticDomain and
 //                                HomogeneousElasticDomainStaceyDirichletBC),
 //                                reference metric <= 1e-3
+//   * acoustic forward path      SPECFEM2D golden traces shipped with the
+//                                reference (HomogeneousAcousticDomain: Dirac
+//                                source, free surface on all sides, d/v/a/p;
+//                                ...DirichletBC and ...CompositeStaceyDirichletBC:
+//                                four Ricker sources), reference metric; plus
+//                                closed forms (mass = area / kappa, Dirichlet
+//                                points stay at rest without sources there)
 //   * Frechet kernels            "parity unpinned" by reference data (no
 //                                reference fixture holds kernel values); pinned
 //                                by finite differences of the misfit instead.
@@ -162,6 +170,10 @@ template <typename real> struct Oracle2D {
   std::vector<char> pt_stacey;
   std::vector<real> pt_n0, pt_n1, pt_w;
   std::vector<int> order; // element processing order: none group, then stacey
+  // acoustic medium (every element has mu == 0): potential in component 0 of
+  // the [nglob][2] fields; per-point acoustic free-surface (Dirichlet) tag
+  bool acoustic = false;
+  std::vector<char> pt_free;
   // fields [nglob][2]
   std::vector<real> u, v, a, minv;
   // Frechet kernels (rho, mu, kappa) per local point
@@ -186,7 +198,9 @@ template <typename real> struct Oracle2D {
   void build_mesh(int nspec_, int ngnod_, int nnodes, const double *coord,
                   const int *knods, const float *kappa_, const float *mu_,
                   const float *rho_, int nabs, const int *abs_ispec,
-                  const int *abs_type) {
+                  const int *abs_type, int nfree = 0,
+                  const int *free_ispec = nullptr,
+                  const int *free_type = nullptr) {
     nspec = nspec_;
     ngnod = ngnod_;
     if (ngnod != 4 && ngnod != 9)
@@ -284,6 +298,38 @@ template <typename real> struct Oracle2D {
       mu[i] = mu_[i];
       rho[i] = rho_[i];
     }
+    // acoustic elements: mu == 0 at every point (a material with cs = 0,
+    // src/io/mesh/impl/fortran/dim2/read_material_properties.cpp:178-204)
+    {
+      int nac = 0;
+      for (int e = 0; e < nspec; ++e) {
+        bool fluid = true;
+        for (int p = 0; p < NGLL * NGLL; ++p)
+          fluid = fluid && mu[size_t(e) * NGLL * NGLL + p] == 0;
+        nac += fluid;
+      }
+      if (nac != 0 && nac != nspec)
+        throw std::runtime_error("mixed acoustic/elastic meshes are not restated");
+      acoustic = nac == nspec;
+    }
+    // acoustic free surface: per-point tags from the (element, entity) list
+    // (core/specfem/assembly/boundaries/dim2/impl/acoustic_free_surface.tpp:
+    // 60-94, utilities.hpp:6-20)
+    pt_free.assign(nloc, 0);
+    for (int k = 0; k < nfree; ++k) {
+      const int e = free_ispec[k], t = free_type[k];
+      if (e < 0 || e >= nspec || t < 1 || t > 8)
+        throw std::runtime_error("free-surface entity out of range");
+      for (int iz = 0; iz < NGLL; ++iz)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const bool bz = iz == 0, tz = iz == NGLL - 1, lx = ix == 0,
+                     rx = ix == NGLL - 1;
+          if ((t <= 4 && on_edge(t, iz, ix)) || (t == 5 && bz && lx) ||
+              (t == 6 && bz && rx) || (t == 7 && tz && rx) ||
+              (t == 8 && tz && lx))
+            pt_free[idx(e, iz, ix)] = 1;
+        }
+    }
 
     // Stacey data: one (normal, weight) per GLL point; the last listed edge
     // covering a point wins
@@ -369,6 +415,20 @@ template <typename real> struct Oracle2D {
   }
   real rho_vs(int i) const { return std::sqrt(rho[i] * mu[i]); }
 
+  // include/medium/dim2/acoustic/isotropic/properties.hpp:55-70,
+  // material.hpp:88-91
+  real rho_inverse(int i) const { return static_cast<real>(1.0) / rho[i]; }
+  real rho_vpinverse(int i) const {
+    const real kappa_inverse = static_cast<real>(1.0) / kappa[i];
+    return std::sqrt(rho_inverse(i) * kappa_inverse);
+  }
+  // include/boundary_conditions/stacey/stacey.tpp:293-321 (acoustic)
+  void stacey_traction_acoustic(int i, real vel, real &traction) const {
+    const real nrm = std::sqrt(pt_n0[i] * pt_n0[i] + pt_n1[i] * pt_n1[i]);
+    const real factor = pt_w[i] * nrm;
+    traction += static_cast<real>(-1.0) * factor * rho_vpinverse(i) * vel;
+  }
+
   // include/boundary_conditions/stacey/stacey.tpp:46-80
   void stacey_traction(int i, const real *vel, real *traction) const {
     const real vn = vel[0] * pt_n0[i] + vel[1] * pt_n1[i];
@@ -394,6 +454,17 @@ template <typename real> struct Oracle2D {
         for (int ix = 0; ix < NGLL; ++ix) {
           const int i = idx(e, iz, ix);
           const real prod = gll.w[iz] * gll.w[ix];
+          if (acoustic) {
+            // mass_inverse component 1/kappa (acoustic/isotropic/mass_matrix.tpp)
+            real m = static_cast<real>(1.0) / kappa[i];
+            m *= prod * jac[i].jacobian;
+            if (elem_stacey[e] && pt_stacey[i])
+              stacey_traction_acoustic(
+                  i, static_cast<real>(-1.0) * dt * static_cast<real>(0.5), m);
+            minv[2 * ibool[i]] += m;
+            minv[2 * ibool[i] + 1] += static_cast<real>(1.0);
+            continue;
+          }
           real m[2] = {rho[i], rho[i]};
           const real f = prod * jac[i].jacobian;
           m[0] *= f;
@@ -537,8 +608,50 @@ template <typename real> struct Oracle2D {
     }
   }
 
+  // Acoustic element: T = rho^-1 grad chi (acoustic/isotropic/stress.hpp:
+  // 15-56), F = J T . dxi/dx, weighted divergence, Stacey, then the free
+  // surface zeroes the tagged points (dirichlet.hpp:20-45; composite points
+  // take the Stacey term first, composite_stacey_dirichlet.tpp:11-45).
+  void element_force_acoustic(int e, const std::vector<real> &U,
+                              const std::vector<real> &V, std::vector<real> &A,
+                              bool apply_stacey) {
+    real du[NGLL][NGLL][2][2];
+    element_gradient(e, U, du);
+    real F[NGLL][NGLL][2];
+    for (int iz = 0; iz < NGLL; ++iz)
+      for (int ix = 0; ix < NGLL; ++ix) {
+        const int i = idx(e, iz, ix);
+        const real ri = rho_inverse(i);
+        const real T[2] = {ri * du[iz][ix][0][0], ri * du[iz][ix][0][1]};
+        const Jac<real> &J = jac[i];
+        F[iz][ix][0] = J.jacobian * (T[0] * J.xix + T[1] * J.xiz);
+        F[iz][ix][1] = J.jacobian * (T[0] * J.gammax + T[1] * J.gammaz);
+      }
+    for (int iz = 0; iz < NGLL; ++iz)
+      for (int ix = 0; ix < NGLL; ++ix) {
+        real t1 = 0, t2 = 0;
+        for (int l = 0; l < NGLL; ++l) {
+          t1 += F[iz][l][0] * gll.hprime[l][ix] * gll.w[l];
+          t2 += F[l][ix][1] * gll.hprime[l][iz] * gll.w[l];
+        }
+        real acc = gll.w[iz] * t1 + gll.w[ix] * t2;
+        acc *= static_cast<real>(-1.0);
+        const int i = idx(e, iz, ix);
+        const int g = ibool[i];
+        if (apply_stacey && elem_stacey[e] && pt_stacey[i])
+          stacey_traction_acoustic(i, V[2 * g], acc);
+        if (pt_free[i])
+          acc = 0;
+        A[2 * g] += acc;
+      }
+  }
+
   void element_force(int e, const std::vector<real> &U, const std::vector<real> &V,
                      std::vector<real> &A, bool apply_stacey) {
+    if (acoustic) {
+      element_force_acoustic(e, U, V, A, apply_stacey);
+      return;
+    }
     {
       real du[NGLL][NGLL][2][2];
       element_gradient(e, U, du);
@@ -592,6 +705,19 @@ template <typename real> struct Oracle2D {
   // include/kokkos_kernels/impl/compute_source_interaction.tpp:84-101,
   // include/medium/dim2/elastic/isotropic/source.hpp:36-59
   void add_sources(int istep, std::vector<real> &A, real scale = 1) {
+    if (acoustic) { // -stf L / kappa (acoustic/isotropic/source.hpp:36-70)
+      for (int s = 0; s < nsrc; ++s)
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const int i = idx(src_ispec[s], iz, ix);
+            const int g = ibool[i];
+            A[2 * g] += scale * (static_cast<real>(-1.0) *
+                                 stf[(size_t(istep) * nsrc + s) * 2] *
+                                 src_array[((s * 2) * NGLL + iz) * NGLL + ix]) /
+                        kappa[i];
+          }
+      return;
+    }
     for (int s = 0; s < nsrc; ++s)
       for (int iz = 0; iz < NGLL; ++iz)
         for (int ix = 0; ix < NGLL; ++ix) {
@@ -680,17 +806,30 @@ template <typename real> struct Oracle2D {
   // impl/time_functions.hpp:77-83), t0 rule (src/io/sources/dim2/read_sources.cpp:142-172),
   // source array (core/specfem/assembly/compute_source_array/dim2/impl/compute_source_array_from_vector.cpp:14-60,
   // core/specfem/source/dim2/vector_source/force_source.cpp:22-73).
+  // Dirac (kind 1): factor * gaussian(t - tshift, f0) with f0 = 1 / (10 dt)
+  // (core/specfem/source_time_functions/dirac.cpp:7-52,
+  //  impl/time_functions.hpp:9-14).
+  std::vector<int> stf_kind;
+  std::vector<real> f0_used;
   void set_sources(int nsrc_, const double *x, const double *z,
                    const float *angle, const float *f0, const float *tshift,
-                   const float *factor, real dt, int nstep_) {
+                   const float *factor, real dt, int nstep_,
+                   const int *kinds = nullptr) {
     nsrc = nsrc_;
     nstep = nstep_;
     src_ispec.assign(nsrc, 0);
     src_array.assign(size_t(nsrc) * 2 * NGLL * NGLL, 0);
+    stf_kind.assign(nsrc, 0);
+    f0_used.assign(nsrc, 0);
     std::vector<real> t0s(nsrc), tsh(nsrc);
     real min_t0 = std::numeric_limits<real>::max();
     for (int s = 0; s < nsrc; ++s) {
-      const real f = f0[s];
+      stf_kind[s] = kinds ? kinds[s] : 0;
+      if (stf_kind[s] != 0 && stf_kind[s] != 1)
+        throw std::runtime_error("unknown source time function");
+      f0_used[s] = stf_kind[s] == 1 ? static_cast<real>(1.0 / (10.0 * dt))
+                                    : static_cast<real>(f0[s]);
+      const real f = f0_used[s];
       const real hdur = 1.0 / f;
       t0s[s] = -static_cast<real>(1.2) * hdur + static_cast<real>(tshift[s]);
       tsh[s] = tshift[s];
@@ -706,10 +845,16 @@ template <typename real> struct Oracle2D {
       for (int i = 0; i < nstep; ++i) {
         const real t = t0 + i * dt;
         const real tt = t - tsh[s];
-        const real fs = f0[s];
+        const real fs = f0_used[s];
         const real aa = pi * pi * fs * fs;
-        const real d2 = (1.0 - 2.0 * aa * tt * tt) * std::exp(-aa * tt * tt);
-        const real val = static_cast<real>(factor[s]) * d2;
+        real val;
+        if (stf_kind[s] == 1) {
+          const real gs = -1.0 * std::exp(-aa * tt * tt) / (2.0 * aa);
+          val = static_cast<real>(factor[s]) * gs;
+        } else {
+          const real d2 = (1.0 - 2.0 * aa * tt * tt) * std::exp(-aa * tt * tt);
+          val = static_cast<real>(factor[s]) * d2;
+        }
         stf[(size_t(i) * nsrc + s) * 2 + 0] = val;
         stf[(size_t(i) * nsrc + s) * 2 + 1] = val;
       }
@@ -723,7 +868,11 @@ template <typename real> struct Oracle2D {
       lagrange_interpolants<real>(ga, gll.xi, hga, nullptr);
       const real ang = static_cast<real>(angle[s]) * pi /
                        static_cast<real>(180.0);
-      const real fv[2] = {std::sin(ang), std::cos(ang)};
+      real fv[2] = {std::sin(ang), std::cos(ang)};
+      if (acoustic) { // force vector (1) (force_source.cpp:36-40)
+        fv[0] = 1;
+        fv[1] = 0;
+      }
       for (int iz = 0; iz < NGLL; ++iz)
         for (int ix = 0; ix < NGLL; ++ix) {
           const real hl = hxi[ix] * hga[iz];
@@ -762,6 +911,10 @@ template <typename real> struct Oracle2D {
   void seismograms(const std::vector<real> &U, const std::vector<real> &V,
                    const std::vector<real> &A, int ntypes, const int *types,
                    real *out /* [ntypes][nrec][2] */) const {
+    if (acoustic) {
+      seismograms_acoustic(U, V, A, ntypes, types, out);
+      return;
+    }
     for (int t = 0; t < ntypes; ++t) {
       const std::vector<real> &F = types[t] == 0 ? U : (types[t] == 1 ? V : A);
       for (int r = 0; r < nrec; ++r) {
@@ -772,6 +925,39 @@ template <typename real> struct Oracle2D {
             const real h = rec_h[(size_t(r) * NGLL + iz) * NGLL + ix];
             s[0] += h * F[2 * g];
             s[1] += h * F[2 * g + 1];
+          }
+        out[(size_t(t) * nrec + r) * 2 + 0] = s[0];
+        out[(size_t(t) * nrec + r) * 2 + 1] = s[1];
+      }
+    }
+  }
+
+  // Acoustic receivers: rho^-1 grad of the potential (d), its velocity (v)
+  // or acceleration (a), or -acceleration for pressure (type 3), at the
+  // element's points, then interpolated (acoustic/isotropic/wavefield.hpp:
+  // 12-97, compute_seismogram.tpp:117-166).
+  void seismograms_acoustic(const std::vector<real> &U, const std::vector<real> &V,
+                            const std::vector<real> &A, int ntypes,
+                            const int *types, real *out) const {
+    for (int t = 0; t < ntypes; ++t) {
+      const std::vector<real> &F = types[t] == 0 ? U : (types[t] == 1 ? V : A);
+      for (int r = 0; r < nrec; ++r) {
+        const int e = rec_ispec[r];
+        real du[NGLL][NGLL][2][2];
+        if (types[t] != 3)
+          element_gradient(e, F, du);
+        real s[2] = {0, 0};
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const int i = idx(e, iz, ix);
+            const real h = rec_h[(size_t(r) * NGLL + iz) * NGLL + ix];
+            if (types[t] == 3) {
+              s[0] += h * (static_cast<real>(-1.0) * F[2 * ibool[i]]);
+            } else {
+              const real ri = rho_inverse(i);
+              s[0] += h * (ri * du[iz][ix][0][0]);
+              s[1] += h * (ri * du[iz][ix][0][1]);
+            }
           }
         out[(size_t(t) * nrec + r) * 2 + 0] = s[0];
         out[(size_t(t) * nrec + r) * 2 + 1] = s[1];
@@ -994,23 +1180,27 @@ extern "C" {
 
 const char *oracle2d_last_error() { return g_err.c_str(); }
 
-void *oracle2d_create(int precision, int nspec, int ngnod, int nnodes,
-                      const double *coord, const int *knods,
-                      const float *kappa, const float *mu, const float *rho,
-                      int nabs, const int *abs_ispec, const int *abs_type,
-                      float dt, int nstep, int nsrc, const double *sx,
-                      const double *sz, const float *angle, const float *f0,
-                      const float *tshift, const float *factor, int nrec,
-                      const double *rx, const double *rz) {
+// oracle2d_create with acoustic free-surface entities (nfree, element,
+// type 1..8) and source time-function kinds (0 Ricker, 1 Dirac; NULL: Ricker).
+void *oracle2d_create_ex(int precision, int nspec, int ngnod, int nnodes,
+                         const double *coord, const int *knods,
+                         const float *kappa, const float *mu, const float *rho,
+                         int nabs, const int *abs_ispec, const int *abs_type,
+                         int nfree, const int *free_ispec, const int *free_type,
+                         float dt, int nstep, int nsrc, const double *sx,
+                         const double *sz, const float *angle, const float *f0,
+                         const float *tshift, const float *factor,
+                         const int *stf_kind, int nrec, const double *rx,
+                         const double *rz) {
   try {
     auto *h = new OracleHandle{precision, nullptr, nullptr};
     auto setup = [&](auto *o) {
       using real = std::remove_reference_t<decltype(o->deltat)>;
       o->build_mesh(nspec, ngnod, nnodes, coord, knods, kappa, mu, rho, nabs,
-                    abs_ispec, abs_type);
+                    abs_ispec, abs_type, nfree, free_ispec, free_type);
       o->set_timescheme(static_cast<real>(dt));
       o->set_sources(nsrc, sx, sz, angle, f0, tshift, factor,
-                     static_cast<real>(dt), nstep);
+                     static_cast<real>(dt), nstep, stf_kind);
       o->set_receivers(nrec, rx, rz);
       o->initialize(static_cast<real>(dt));
     };
@@ -1026,6 +1216,20 @@ void *oracle2d_create(int precision, int nspec, int ngnod, int nnodes,
     g_err = e.what();
     return nullptr;
   }
+}
+
+void *oracle2d_create(int precision, int nspec, int ngnod, int nnodes,
+                      const double *coord, const int *knods,
+                      const float *kappa, const float *mu, const float *rho,
+                      int nabs, const int *abs_ispec, const int *abs_type,
+                      float dt, int nstep, int nsrc, const double *sx,
+                      const double *sz, const float *angle, const float *f0,
+                      const float *tshift, const float *factor, int nrec,
+                      const double *rx, const double *rz) {
+  return oracle2d_create_ex(precision, nspec, ngnod, nnodes, coord, knods,
+                            kappa, mu, rho, nabs, abs_ispec, abs_type, 0,
+                            nullptr, nullptr, dt, nstep, nsrc, sx, sz, angle,
+                            f0, tshift, factor, nullptr, nrec, rx, rz);
 }
 
 void oracle2d_destroy(void *hv) {

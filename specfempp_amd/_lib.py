@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(HERE, "libspecfempp_amd.so")
 SMX_FIELD_FORWARD, SMX_FIELD_ADJOINT, SMX_FIELD_BACKWARD = 0, 1, 2
 SMX_U, SMX_V, SMX_A, SMX_MASS_INVERSE = 0, 1, 2, 3
 SMX_OK, SMX_ERR_ARG, SMX_ERR_HIP, SMX_ERR_STATE, SMX_ERR_MESH = 0, -1, -2, -3, -4
-SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2}
+SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2, "pressure": 3}
+SMX_STF_RICKER, SMX_STF_DIRAC = 0, 1
 
 # Every symbol include/specfempp_amd.h declares.
 EXPORTS = [
@@ -27,7 +28,7 @@ EXPORTS = [
     "smx_time_stiffness", "smx_set_model", "smx_run_forward_checkpointed",
     "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
     "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots", "smx_ctx_compact",
-    "smx_ctx_create3d", "smx_ctx_create3d_opts", "smx_ctx_geometry",
+    "smx_ctx_create3d", "smx_ctx_create3d_opts", "smx_ctx_geometry", "smx_ctx_components",
 ]
 
 
@@ -36,13 +37,15 @@ class SmxMesh2D(ctypes.Structure):
                 ("coord", ctypes.c_void_p), ("knods", ctypes.c_void_p),
                 ("kappa", ctypes.c_void_p), ("mu", ctypes.c_void_p), ("rho", ctypes.c_void_p),
                 ("nabs", ctypes.c_int), ("abs_ispec", ctypes.c_void_p),
-                ("abs_type", ctypes.c_void_p)]
+                ("abs_type", ctypes.c_void_p), ("nfree", ctypes.c_int),
+                ("free_ispec", ctypes.c_void_p), ("free_type", ctypes.c_void_p)]
 
 
 class SmxSources2D(ctypes.Structure):
     _fields_ = [("nsrc", ctypes.c_int), ("x", ctypes.c_void_p), ("z", ctypes.c_void_p),
                 ("angle", ctypes.c_void_p), ("f0", ctypes.c_void_p),
-                ("tshift", ctypes.c_void_p), ("factor", ctypes.c_void_p)]
+                ("tshift", ctypes.c_void_p), ("factor", ctypes.c_void_p),
+                ("stf", ctypes.c_void_p)]
 
 
 class SmxReceivers2D(ctypes.Structure):
@@ -104,6 +107,7 @@ def load():
                                  ctypes.POINTER(SmxReceivers2D), d, i, i, ctypes.POINTER(P)]),
         "smx_ctx_destroy": (i, [P]),
         "smx_ctx_info": (i, [P, IP, IP, IP, IP]),
+        "smx_ctx_components": (i, [P, IP]),
         "smx_ctx_layout": (i, [P, IP, IP, IP, IP]),
         "smx_ctx_t0": (d, [P]),
         "smx_initialize": (i, [P]),

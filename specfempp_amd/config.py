@@ -1,8 +1,9 @@
 """Sources, receivers and the seeded benchmark configurations.
 
-Sources are vertical/oblique point forces with a Ricker time function, the
-reference's ``force`` source (core/specfem/source/dim2/vector_source/force_source.cpp:22-73,
-core/specfem/source_time_functions/ricker.cpp:9-66).  Receivers are station
+Sources are vertical/oblique point forces with a Ricker or Dirac time
+function, the reference's ``force`` source
+(core/specfem/source/dim2/vector_source/force_source.cpp:22-73,
+core/specfem/source_time_functions/ricker.cpp:9-66, dirac.cpp:7-52).  Receivers are station
 coordinates (core/specfem/receivers/dim2/receiver.hpp).  The seeded
 configurations follow the measured workloads: an 80x60 box of 50 m quads
 (single homogeneous medium, free top, Stacey elsewhere) and an 800x600 box with
@@ -18,6 +19,9 @@ import numpy as np
 from .mesh import Mesh2D, box_mesh, elastic_moduli, gll_coordinates
 
 
+STF_RICKER, STF_DIRAC = 0, 1
+
+
 @dataclass
 class Sources2D:
     x: np.ndarray
@@ -26,6 +30,22 @@ class Sources2D:
     f0: np.ndarray
     tshift: np.ndarray
     factor: np.ndarray
+    # time function per source: STF_RICKER or STF_DIRAC (None: all Ricker)
+    stf: np.ndarray = None
+
+    def kinds(self):
+        n = len(self.x)
+        return (np.zeros(n, np.int32) if self.stf is None
+                else np.asarray(self.stf, dtype=np.int32).reshape(n))
+
+    @staticmethod
+    def dirac(x, z, factor=1e10, angle=0.0, tshift=0.0):
+        """Dirac sources: the reference approximates the impulse by a
+        Gaussian of f0 = 1 / (10 dt) (dirac.cpp:23-35), so f0 is set from dt
+        when the context is built (the f0 given here is ignored)."""
+        s = Sources2D.ricker(x, z, f0=1.0, factor=factor, angle=angle, tshift=tshift)
+        s.stf = np.full(len(s.x), STF_DIRAC, np.int32)
+        return s
 
     @staticmethod
     def ricker(x, z, f0, factor=1e10, angle=0.0, tshift=0.0):

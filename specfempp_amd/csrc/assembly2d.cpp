@@ -613,6 +613,23 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
   auto l2m_of = [](float kappa, float mu) {
     return kappa + (static_cast<float>(4.0 / 3.0)) * mu;
   };
+  // Medium: an element is acoustic when mu == 0 at all its points
+  // (src/io/mesh/impl/fortran/dim2/read_material_properties.cpp:178-204).
+  {
+    int nac = 0;
+    for (int e = 0; e < nspec; ++e) {
+      bool fluid = true;
+      for (int p = 0; p < NPT; ++p)
+        fluid = fluid && m.mu[size_t(e) * NPT + p] == 0.0f;
+      nac += fluid;
+    }
+    if (nac != 0 && nac != nspec)
+      throw SmxError(SMX_ERR_MESH,
+                     "mixed acoustic / elastic meshes need the elastic-acoustic "
+                     "coupling, which is not built");
+    A.medium = nac == nspec ? 1 : 0;
+  }
+  const bool acoustic = A.medium == 1;
   for (int e = 0; e < nspec; ++e)
     for (int p = 0; p < NPT; ++p) {
       const size_t src_i = size_t(e) * NPT + p, o = size_t(p) * ld + slot_of[e];
@@ -620,12 +637,47 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       if (!(rho > 0) || !(mu >= 0) || !(kappa > 0))
         throw SmxError(SMX_ERR_ARG, "non-physical material at element " +
                                         std::to_string(e));
-      A.lam2mu_mu[o] = l2m_of(kappa, mu);
-      A.lam2mu_mu[size_t(NPT) * ld + o] = mu;
+      if (acoustic) {
+        // include/medium/dim2/acoustic/isotropic/material.hpp:88-91
+        A.lam2mu_mu[o] = static_cast<float>(1.0) / rho;
+        A.lam2mu_mu[size_t(NPT) * ld + o] = kappa;
+      } else {
+        A.lam2mu_mu[o] = l2m_of(kappa, mu);
+        A.lam2mu_mu[size_t(NPT) * ld + o] = mu;
+      }
       A.kappa_mu_rho[o] = kappa;
       A.kappa_mu_rho[size_t(NPT) * ld + o] = mu;
       A.kappa_mu_rho[2 * size_t(NPT) * ld + o] = rho;
     }
+
+  // ---- acoustic free surface (core/specfem/assembly/boundaries/dim2/impl/
+  // acoustic_free_surface.tpp:8-96; is_on_boundary, utilities.hpp:6-20)
+  if (m.nfree < 0 || (m.nfree > 0 && (!m.free_ispec || !m.free_type)))
+    throw SmxError(SMX_ERR_ARG, "invalid free-surface list");
+  if (m.nfree > 0 && !acoustic)
+    throw SmxError(SMX_ERR_MESH,
+                   "acoustic free-surface entities on an elastic mesh");
+  if (acoustic) {
+    A.dmask.assign(ld, 0u);
+    for (int i = 0; i < m.nfree; ++i) {
+      const int e = m.free_ispec[i], t = m.free_type[i];
+      if (e < 0 || e >= nspec || t < 1 || t > 8)
+        throw SmxError(SMX_ERR_ARG, "invalid free-surface entity");
+      uint32_t mask = 0;
+      for (int iz = 0; iz < NGLL; ++iz)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const bool bz = iz == 0, tz = iz == NGLL - 1, lx = ix == 0,
+                     rx = ix == NGLL - 1;
+          const bool on = (t == 1 && bz) || (t == 2 && rx) || (t == 3 && tz) ||
+                          (t == 4 && lx) || (t == 5 && bz && lx) ||
+                          (t == 6 && bz && rx) || (t == 7 && tz && rx) ||
+                          (t == 8 && tz && lx);
+          if (on)
+            mask |= 1u << (iz * NGLL + ix);
+        }
+      A.dmask[slot_of[e]] |= mask;
+    }
+  }
 
   // ---- Stacey (core/specfem/assembly/boundaries/dim2/impl/stacey.tpp:15-176):
   // elements in ascending order, each element's edges in list order, the last
@@ -689,9 +741,19 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
             const float kappa = A.kappa_mu_rho[o],
                         mu = A.kappa_mu_rho[size_t(NPT) * ld + o],
                         rho = A.kappa_mu_rho[2 * size_t(NPT) * ld + o];
-            const float l2m = l2m_of(kappa, mu);
-            const float rvp = std::sqrt(l2m / rho) * rho;
-            const float rvs = std::sqrt(rho * mu);
+            float rvp, rvs;
+            if (acoustic) {
+              // rho_vpinverse (acoustic/isotropic/properties.hpp:62-70)
+              const float rho_inv = static_cast<float>(1.0) / rho;
+              const float kappa_inv = static_cast<float>(1.0) / kappa;
+              rvp = std::sqrt(rho_inv * kappa_inv);
+              rvs = 0.0f;
+              (void)mu;
+            } else {
+              const float l2m = l2m_of(kappa, mu);
+              rvp = std::sqrt(l2m / rho) * rho;
+              rvs = std::sqrt(rho * mu);
+            }
             const float nrm = std::sqrt(n0 * n0 + n1 * n1);
             A.stacey[(0 * size_t(NPT) + p) * sld + slot] = n0;
             A.stacey[(1 * size_t(NPT) + p) * sld + slot] = n1;
@@ -782,10 +844,20 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
   {
     std::vector<float> t0s(src.nsrc);
     float min_t0 = std::numeric_limits<float>::max();
+    // Dirac: f0 = 1 / (10 dt) in float (dirac.cpp:23-35); both kinds use
+    // t0 = -1.2 hdur + tshift, hdur = 1 / f0 (ricker.cpp:17-20, dirac.cpp:13-16)
+    std::vector<int> kind(src.nsrc, SMX_STF_RICKER);
+    std::vector<float> f0v(src.nsrc);
+    const float dtf = static_cast<float>(dt_in);
     for (int s = 0; s < src.nsrc; ++s) {
-      if (!(src.f0[s] > 0))
+      kind[s] = src.stf ? src.stf[s] : SMX_STF_RICKER;
+      if (kind[s] != SMX_STF_RICKER && kind[s] != SMX_STF_DIRAC)
+        throw SmxError(SMX_ERR_ARG, "unknown source time function");
+      f0v[s] = kind[s] == SMX_STF_DIRAC ? static_cast<float>(1.0 / (10.0 * dtf))
+                                        : src.f0[s];
+      if (!(f0v[s] > 0))
         throw SmxError(SMX_ERR_ARG, "source f0 must be positive");
-      const float hdur = 1.0 / src.f0[s];
+      const float hdur = 1.0 / f0v[s];
       t0s[s] = -static_cast<float>(1.2) * hdur + src.tshift[s];
       min_t0 = std::min(min_t0, t0s[s]);
     }
@@ -794,13 +866,20 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
     const float pi = static_cast<float>(M_PI);
     for (int s = 0; s < src.nsrc; ++s) {
       const float tshift = t0s[s] - min_t0;
-      const float f0 = src.f0[s];
+      const float f0 = f0v[s];
       for (int i = 0; i < nstep; ++i) {
         const float t = t0 + i * dt;
         const float tt = t - tshift;
         const float a = pi * pi * f0 * f0;
-        const float d2 = (1.0 - 2.0 * a * tt * tt) * std::exp(-a * tt * tt);
-        const float val = src.factor[s] * d2;
+        float val;
+        if (kind[s] == SMX_STF_DIRAC) {
+          // factor * gaussian(t - tshift, f0) (impl/time_functions.hpp:9-14)
+          const float g = -1.0 * std::exp(-a * tt * tt) / (2.0 * a);
+          val = src.factor[s] * g;
+        } else {
+          const float d2 = (1.0 - 2.0 * a * tt * tt) * std::exp(-a * tt * tt);
+          val = src.factor[s] * d2;
+        }
         A.stf[(size_t(i) * src.nsrc + s) * 2 + 0] = val;
         A.stf[(size_t(i) * src.nsrc + s) * 2 + 1] = val;
       }
@@ -813,7 +892,11 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       interpolants(xi, q.xi, hx);
       interpolants(ga, q.xi, hg);
       const float ang = src.angle[s] * pi / static_cast<float>(180.0);
-      const float f[2] = {std::sin(ang), std::cos(ang)};
+      float f[2] = {std::sin(ang), std::cos(ang)};
+      if (acoustic) { // force vector (1) (force_source.cpp:36-40)
+        f[0] = 1.0f;
+        f[1] = 0.0f;
+      }
       for (int iz = 0; iz < NGLL; ++iz)
         for (int ix = 0; ix < NGLL; ++ix) {
           const float hl = hx[ix] * hg[iz];
@@ -829,8 +912,11 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
   A.nsig = (nstep + A.nsbs - 1) / A.nsbs;
   A.seis_types.assign(rec.types, rec.types + rec.ntypes);
   for (int t : A.seis_types)
-    if (t < 0 || t > 2)
-      throw SmxError(SMX_ERR_ARG, "unsupported seismogram type");
+    if (t < 0 || t > 3 || (t == SMX_SEIS_PRESSURE && !acoustic))
+      throw SmxError(SMX_ERR_ARG,
+                     t == SMX_SEIS_PRESSURE
+                         ? "pressure seismograms need acoustic elements"
+                         : "unsupported seismogram type");
   A.rec_ispec.assign(std::max(1, rec.nrec), 0);
   A.rec_elem.assign(std::max(1, rec.nrec), 0);
   A.rec_xi.assign(std::max(1, rec.nrec), 0.0f);
@@ -874,7 +960,10 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       A.src_li.push_back(A.lidx[size_t(p) * ld + A.src_ispec[s]]);
       A.src_idx.push_back(s);
       A.src_L.push_back(A.src_array[(size_t(s) * 2 + 0) * NPT + p]);
-      A.src_L.push_back(A.src_array[(size_t(s) * 2 + 1) * NPT + p]);
+      if (acoustic) // -stf L / kappa (acoustic/isotropic/source.hpp:60-66)
+        A.src_L.push_back(A.kappa_mu_rho[size_t(p) * ld + A.src_ispec[s]]);
+      else
+        A.src_L.push_back(A.src_array[(size_t(s) * 2 + 1) * NPT + p]);
     }
   }
 
@@ -912,6 +1001,42 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       t.nrec++;
       A.trec_li.push_back(ent[i][1]);
       A.trec_entry.push_back(ent[i][2]);
+    }
+  }
+
+  // ---- acoustic receiver weights: the reference interpolates
+  // rho^-1 grad f at the element's points (acoustic/isotropic/
+  // wavefield.hpp:74-93, compute_seismogram.tpp:150-160); that is linear in
+  // the 25 nodal values, w_q = sum_p h_p rho^-1_p dgrad_p / df_q.
+  if (acoustic) {
+    const int nent = A.nrec * NPT;
+    A.rent_w.assign(2 * size_t(std::max(1, nent)), 0.0f);
+    A.rent_w64.assign(2 * size_t(std::max(1, nent)), 0.0);
+    const size_t pld = size_t(NPT) * ld;
+    for (int r = 0; r < A.nrec; ++r) {
+      const int sl = A.rec_ispec[r];
+      double w[NPT][2] = {};
+      for (int iz = 0; iz < NGLL; ++iz)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const int p = iz * NGLL + ix;
+          const size_t o = size_t(p) * ld + sl;
+          const double h = A.rec_h[size_t(r) * NPT + p];
+          const double rinv = 1.0 / double(A.kappa_mu_rho[2 * pld + o]);
+          const double xix = A.geom[o], xiz = A.geom[pld + o],
+                       gax = A.geom[2 * pld + o], gaz = A.geom[3 * pld + o];
+          for (int l = 0; l < NGLL; ++l) {
+            const double dxi = q.hprime[ix][l], dga = q.hprime[iz][l];
+            w[iz * NGLL + l][0] += h * rinv * xix * dxi;
+            w[iz * NGLL + l][1] += h * rinv * xiz * dxi;
+            w[l * NGLL + ix][0] += h * rinv * gax * dga;
+            w[l * NGLL + ix][1] += h * rinv * gaz * dga;
+          }
+        }
+      for (int p = 0; p < NPT; ++p)
+        for (int c = 0; c < 2; ++c) {
+          A.rent_w64[2 * (size_t(r) * NPT + p) + c] = w[p][c];
+          A.rent_w[2 * (size_t(r) * NPT + p) + c] = static_cast<float>(w[p][c]);
+        }
     }
   }
 
@@ -1008,10 +1133,23 @@ static void refresh_fp64(HostAssembly2D &A) {
   const int ld = A.nspec_pad, sld = A.nstacey_pad;
   const size_t pld = size_t(NPT) * ld, sp = size_t(NPT) * sld;
   A.lam2mu64.assign(pld, 0.0);
+  A.imp64.assign(2 * sp, 0.0);
+  if (A.medium == 1) { // acoustic: 1/rho and 1/(rho vp) in double
+    for (size_t o = 0; o < pld; ++o)
+      A.lam2mu64[o] = A.kappa_mu_rho[2 * pld + o] > 0
+                          ? 1.0 / double(A.kappa_mu_rho[2 * pld + o])
+                          : 0.0;
+    for (size_t k = 0; k < A.bpt_g.size(); ++k) {
+      const size_t o = size_t(A.bpt_p[k]) * ld + A.bpt_slot[k];
+      const size_t so = size_t(A.bpt_p[k]) * sld + A.bpt_bslot[k];
+      const double kappa = A.kappa_mu_rho[o], rho = A.kappa_mu_rho[2 * pld + o];
+      A.imp64[so] = std::sqrt(1.0 / (rho * kappa));
+    }
+    return;
+  }
   for (size_t o = 0; o < pld; ++o)
     A.lam2mu64[o] = double(A.kappa_mu_rho[o]) +
                     4.0 / 3.0 * double(A.kappa_mu_rho[pld + o]);
-  A.imp64.assign(2 * sp, 0.0);
   for (size_t k = 0; k < A.bpt_g.size(); ++k) {
     const size_t o = size_t(A.bpt_p[k]) * ld + A.bpt_slot[k];
     const size_t so = size_t(A.bpt_p[k]) * sld + A.bpt_bslot[k];
@@ -1122,6 +1260,9 @@ void refresh_compact(HostAssembly2D &A) {
 void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
   if (shots.empty())
     throw SmxError(SMX_ERR_ARG, "no shots");
+  for (const HostAssembly2D &S : shots)
+    if (S.medium != 0)
+      throw SmxError(SMX_ERR_ARG, "batched shots are built for elastic meshes");
   const HostAssembly2D &S0 = shots[0];
   for (const auto &S : shots)
     if (S.nstep != S0.nstep || S.dt != S0.dt || S.nsbs != S0.nsbs ||

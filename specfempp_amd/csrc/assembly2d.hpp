@@ -87,6 +87,20 @@ struct HostAssembly2D {
   std::vector<uint8_t> ecolor;  // [nspec_pad], 0xff for padding slots
   std::vector<int> elem_slot;   // mesh element -> slot
 
+  // Medium shared by every element: 0 elastic P-SV, 1 acoustic.  Acoustic
+  // meshes reuse the elastic arrays with these meanings: lam2mu_mu =
+  // (1/rho, kappa), Stacey slot 4 = 1/(rho vp) (slot 5 zero), src_L pairs =
+  // (L, kappa at the point), stf pairs equal.
+  int medium = 0;
+  // Acoustic: 25-bit mask per slot of the points tagged acoustic free
+  // surface (Dirichlet) in that element (empty for elastic meshes).
+  std::vector<uint32_t> dmask;
+  // Acoustic receiver entries: (wx, wz) per entry id such that the x / z
+  // seismogram of a potential f is sum_q w_q f_q, i.e. the interpolated
+  // rho^-1 grad f; [nrec*NPT][2] in FP32 and FP64.
+  std::vector<float> rent_w;
+  std::vector<double> rent_w64;
+
   std::vector<int32_t> ibool;   // [NPT][nspec_pad] global ids
   std::vector<uint16_t> lidx;   // [NPT][nspec_pad] tile-local ids
   // [5][NPT][nspec_pad]: xix, xiz, gammax, gammaz, jacobian

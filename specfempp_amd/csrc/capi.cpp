@@ -203,6 +203,17 @@ int smx_ctx_destroy(smx_ctx *ctx) {
   return guarded([&] { delete ctx; });
 }
 
+int smx_ctx_components(const smx_ctx *ctx, int *ncomp) {
+  return guarded([&] {
+    if (!ncomp) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    if (S3(const_cast<smx_ctx *>(ctx))) {
+      *ncomp = 3;
+      return;
+    }
+    *ncomp = S(const_cast<smx_ctx *>(ctx)).assembly().medium == 1 ? 1 : 2;
+  });
+}
+
 int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
                  int *nrec) {
   return guarded([&] {

@@ -2028,6 +2028,8 @@ std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
                                             bool nodes_geometry) {
   if (graph_steps < 1 || graph_steps > 256)
     throw SmxError(SMX_ERR_ARG, "graph_steps must lie in [1, 256]");
+  if (a->medium == 1)
+    return make_acoustic_solver2d(std::move(a), precision, graph_steps);
   if (precision == 4)
     return std::make_unique<Solver2D<float>>(std::move(a), graph_steps,
                                              nodes_geometry);

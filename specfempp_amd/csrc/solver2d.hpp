@@ -58,4 +58,10 @@ std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
                                             int precision, int graph_steps = 8,
                                             bool nodes_geometry = false);
 
+// The acoustic solver (acoustic2d.hip); make_solver2d dispatches to it for
+// assemblies whose medium is acoustic.
+std::unique_ptr<Solver2DBase>
+make_acoustic_solver2d(std::unique_ptr<HostAssembly2D> a, int precision,
+                       int graph_steps);
+
 } // namespace smx

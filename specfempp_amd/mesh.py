@@ -8,6 +8,13 @@ and the list of absorbing edges (include/enumerations/dim2/mesh_entities.hpp:29-
 numbering: bottom=1, right=2, top=3, left=4).  The structured box generator
 below replaces the Fortran mesher for the rectangular domains this path is
 measured on.
+
+A GLL point with zero shear modulus belongs to an acoustic (fluid) element:
+the reference's material reader makes an isotropic material with cs = 0
+acoustic (src/io/mesh/impl/fortran/dim2/read_material_properties.cpp:178-204).
+Acoustic free-surface (Dirichlet) entities are edges 1..4 or corners 5..8
+(bottom-left, bottom-right, top-right, top-left; the mesh-entity types of
+src/io/mesh/impl/fortran/dim2/read_boundaries.cpp:236-275).
 """
 
 from __future__ import annotations
@@ -19,7 +26,17 @@ import numpy as np
 NGLL = 5
 
 BOTTOM, RIGHT, TOP, LEFT = 1, 2, 3, 4
+BOTTOM_LEFT, BOTTOM_RIGHT, TOP_RIGHT, TOP_LEFT = 5, 6, 7, 8
 EDGE_TYPES = {"bottom": BOTTOM, "right": RIGHT, "top": TOP, "left": LEFT}
+
+
+def acoustic_moduli(rho, vp):
+    """(kappa, mu = 0, rho) of a fluid in FP32: ``kappa = density*cp*cp`` in
+    float (include/medium/dim2/acoustic/isotropic/material.hpp:49-53)."""
+    rho = np.asarray(rho, dtype=np.float32)
+    vp = np.asarray(vp, dtype=np.float32)
+    kappa = (rho * vp) * vp
+    return kappa.astype(np.float32), np.zeros_like(kappa, dtype=np.float32), rho.copy()
 
 
 def elastic_moduli(rho, vp, vs):
@@ -53,6 +70,14 @@ class Mesh2D:
     nz: int = 0
     extent: tuple = (0.0, 0.0, 0.0, 0.0)   # xmin, xmax, zmin, zmax
     meta: dict = field(default_factory=dict)
+    # acoustic free-surface entities (element, type 1..8), database order
+    free_ispec: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+    free_type: np.ndarray = field(default_factory=lambda: np.zeros(0, np.int32))
+
+    @property
+    def acoustic(self) -> np.ndarray:
+        """(nspec,) bool: elements whose every GLL point has mu == 0."""
+        return (self.mu.reshape(self.nspec, -1) == 0).all(axis=1)
 
     @property
     def nspec(self) -> int:
@@ -87,6 +112,14 @@ class Mesh2D:
             raise ValueError("absorbing edge element out of range")
         if len(self.abs_type) and not np.isin(self.abs_type, [1, 2, 3, 4]).all():
             raise ValueError("absorbing edge type must be 1..4")
+        self.free_ispec = np.asarray(self.free_ispec, dtype=np.int32)
+        self.free_type = np.asarray(self.free_type, dtype=np.int32)
+        if self.free_ispec.shape != self.free_type.shape:
+            raise ValueError("free_ispec/free_type size mismatch")
+        if len(self.free_ispec) and (self.free_ispec.min() < 0 or self.free_ispec.max() >= ns):
+            raise ValueError("free-surface element out of range")
+        if len(self.free_type) and not np.isin(self.free_type, range(1, 9)).all():
+            raise ValueError("free-surface type must be 1..8")
         return self
 
 
@@ -96,10 +129,12 @@ def gll_points():
 
 
 def box_mesh(xmin, xmax, zmin, zmax, nx, nz, *, ngnod=4, absorbing=("bottom", "right", "left"),
-             rho=2700.0, vp=3000.0, vs=1732.0) -> Mesh2D:
+             rho=2700.0, vp=3000.0, vs=1732.0, free=()) -> Mesh2D:
     """Structured nx-by-nz box; elements row-major from the bottom-left
     (ispec = iz*nx + ix).  ``rho``, ``vp``, ``vs`` are scalars or callables
-    f(ix_elem, iz_elem) -> value (per element)."""
+    f(ix_elem, iz_elem) -> value (per element); elements with vs == 0 are
+    acoustic.  ``free`` names the sides that are acoustic free surfaces
+    (listed per side, bottom/right/top/left, elements in order)."""
     if ngnod == 4:
         nxn, nzn = nx + 1, nz + 1
         xs = np.linspace(xmin, xmax, nxn)
@@ -142,6 +177,11 @@ def box_mesh(xmin, xmax, zmin, zmax, nx, nz, *, ngnod=4, absorbing=("bottom", "r
     pp = np.repeat(p.astype(np.float32), NGLL * NGLL).reshape(-1, NGLL, NGLL)
     ss = np.repeat(s.astype(np.float32), NGLL * NGLL).reshape(-1, NGLL, NGLL)
     kappa, mu, rho_pt = elastic_moduli(rr, pp, ss)
+    fluid = ss == 0
+    if fluid.any():
+        ka, _, _ = acoustic_moduli(rr, pp)
+        kappa = np.where(fluid, ka, kappa).astype(np.float32)
+        mu = np.where(fluid, np.float32(0), mu).astype(np.float32)
 
     # Absorbing edges in element order; within an element bottom, right, top,
     # left (so at a corner GLL point the later edge's normal is kept, as in the
@@ -161,10 +201,15 @@ def box_mesh(xmin, xmax, zmin, zmax, nx, nz, *, ngnod=4, absorbing=("bottom", "r
         ai, at = ai[key], at[key]
     else:
         ai, at = np.zeros(0, np.int64), np.zeros(0, np.int64)
+    fparts = [(ids[on[EDGE_TYPES[a]]], np.full(int(on[EDGE_TYPES[a]].sum()), EDGE_TYPES[a]))
+              for a in ("bottom", "right", "top", "left") if a in free]
+    fi = np.concatenate([p[0] for p in fparts]) if fparts else np.zeros(0, np.int64)
+    ft = np.concatenate([p[1] for p in fparts]) if fparts else np.zeros(0, np.int64)
     return Mesh2D(coord=coord.astype(np.float64), knods=knods, kappa=kappa, mu=mu,
                   rho=rho_pt, abs_ispec=ai.astype(np.int32),
                   abs_type=at.astype(np.int32), nx=nx, nz=nz,
-                  extent=(xmin, xmax, zmin, zmax)).validate()
+                  extent=(xmin, xmax, zmin, zmax), free_ispec=fi.astype(np.int32),
+                  free_type=ft.astype(np.int32)).validate()
 
 
 def gll_coordinates(mesh: Mesh2D) -> np.ndarray:

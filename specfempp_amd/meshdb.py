@@ -15,16 +15,19 @@ order the reference reads them (src/io/mesh/impl/fortran/dim2/mesh.cpp:26-215):
 3. mesh parameters (numat, ngnod, nspec, pointsdisp, plot flag) and the
    boundary counts (read_parameters.cpp:10-60);
 4. attenuation (n_sls, f0, flag) (read_mesh_database.cpp:180-204);
-5. materials, numat records (n, indic, 13 doubles); elastic isotropic
-   materials are indic 1 with a non-zero shear speed: density, cp, cs
-   (read_material_properties.cpp:80-160);
+5. materials, numat records (n, indic, 13 doubles); isotropic materials
+   are indic 1 (density, cp, cs): elastic with a non-zero shear speed,
+   acoustic with cs = 0 (read_material_properties.cpp:160-250);
 6. per element (n, material, ngnod control nodes, pml flag)
    (read_material_properties.cpp:360-410);
 7. (ninterfaces, max_interface_size), the absorbing edges in database order
    (element, bottom/right/top/left flags, type, 8 ints)
-   (read_boundaries.cpp:150-200), acoustic free surface, acoustic forcing,
-   coupled interfaces, tangential nodes, axial elements and the adjacency
-   graph, which are skipped (elastic isotropic meshes only).
+   (read_boundaries.cpp:150-200), the acoustic free surface (element, 1 for a
+   corner node or 2 for an edge, two control-node ids; the entity type comes
+   from comparing those ids with the element's corner nodes,
+   read_boundaries.cpp:236-300), then tangential nodes and the adjacency
+   graph, which are skipped.  Acoustic forcing, coupled interfaces and axial
+   elements are rejected (this build has no coupling).
 
 The reference adds corner entries to the absorbing list only when one
 element carries both a horizontal and a vertical absorbing edge; its test
@@ -39,7 +42,7 @@ import struct
 
 import numpy as np
 
-from .mesh import Mesh2D, elastic_moduli
+from .mesh import Mesh2D, acoustic_moduli, elastic_moduli
 
 NGLL = 5
 # header records after the optional one (AXISYM ... ADD_RANDOM_PERTURBATION)
@@ -85,7 +88,8 @@ def read_mesh_database_2d(path_or_bytes):
     """Reads a 2D ``database.bin`` (optionally gzip-compressed) into a
     ``Mesh2D`` with per-GLL-point (kappa, mu, rho) from each element's
     material; returns (mesh, info) where info holds the database's sizes.
-    Only elastic isotropic materials are accepted."""
+    Isotropic elastic and acoustic materials are accepted (mu = 0 marks an
+    acoustic element)."""
     if isinstance(path_or_bytes, (bytes, bytearray)):
         data = bytes(path_or_bytes)
     else:
@@ -134,10 +138,10 @@ def read_mesh_database_2d(path_or_bytes):
         if n != i + 1:
             raise ValueError("material sets out of order")
         v = vals[2:]
-        if indic != 1 or v[2] == 0:
-            raise ValueError(f"material {n}: only elastic isotropic materials are supported "
-                             f"(indic {indic}, cs {v[2]})")
-        mats.append((v[0], v[1], v[2]))  # density, cp, cs
+        if indic != 1:
+            raise ValueError(f"material {n}: only isotropic materials are supported "
+                             f"(indic {indic})")
+        mats.append((v[0], v[1], v[2]))  # density, cp, cs (0: acoustic)
 
     # 6. elements
     knods = np.full((nspec, ngnod), -1, dtype=np.int32)
@@ -168,8 +172,25 @@ def read_mesh_database_2d(path_or_bytes):
             raise ValueError("an absorbing line must cite exactly one edge")
         abs_ispec[k] = num - 1
         abs_type[k] = 1 + [bool(c) for c in codes].index(True)  # bottom, right, top, left
-    if nelem_acoustic_surface or nelem_acforcing or n_fs or n_fp or n_sp or n_axis:
-        raise ValueError("acoustic, coupled or axisymmetric meshes are not supported")
+    free_ispec = np.zeros(max(nelem_acoustic_surface, 0), dtype=np.int32)
+    free_type = np.zeros(max(nelem_acoustic_surface, 0), dtype=np.int32)
+    for k in range(max(nelem_acoustic_surface, 0)):
+        num, kind, e1, e2 = cur.next("4i")
+        if not 1 <= num <= nspec:
+            raise ValueError("acoustic free-surface element out of range")
+        c = knods[num - 1, :4]
+        e1, e2 = e1 - 1, e2 - 1
+        if kind == 1:  # a corner node: bottom-left, bottom-right, top-right, top-left
+            hit = [t for t, node in zip((5, 6, 7, 8), c) if node == e1]
+        else:          # an edge: bottom, left, right, top by its two corners
+            pairs = ((1, c[0], c[1]), (4, c[0], c[3]), (2, c[1], c[2]), (3, c[2], c[3]))
+            hit = [t for t, a, b in pairs if {e1, e2} == {a, b}]
+        if not hit:
+            raise ValueError("could not type an acoustic free-surface entity")
+        free_ispec[k] = num - 1
+        free_type[k] = hit[0]
+    if nelem_acforcing or n_fs or n_fp or n_sp or n_axis:
+        raise ValueError("acoustic forcing, coupled or axisymmetric meshes are not supported")
     cur.next("ii")  # tangential-curve flags
     for _ in range(n_tangential):
         cur.next("dd")
@@ -184,12 +205,18 @@ def read_mesh_database_2d(path_or_bytes):
     vp = np.array([m[1] for m in mats], dtype=np.float64)[kmat].astype(np.float32)
     vs = np.array([m[2] for m in mats], dtype=np.float64)[kmat].astype(np.float32)
     shape = (nspec, NGLL, NGLL)
-    kappa, mu, rho_p = elastic_moduli(np.broadcast_to(rho[:, None, None], shape),
-                                      np.broadcast_to(vp[:, None, None], shape),
-                                      np.broadcast_to(vs[:, None, None], shape))
+    rb, pb, sb = (np.broadcast_to(x[:, None, None], shape) for x in (rho, vp, vs))
+    kappa, mu, rho_p = elastic_moduli(rb, pb, sb)
+    fluid = sb == 0
+    if fluid.any():
+        ka, _, _ = acoustic_moduli(rb, pb)
+        kappa = np.where(fluid, ka, kappa).astype(np.float32)
+        mu = np.where(fluid, np.float32(0), mu).astype(np.float32)
     mesh = Mesh2D(coord=coord, knods=knods, kappa=kappa, mu=mu, rho=np.ascontiguousarray(rho_p),
                   abs_ispec=abs_ispec, abs_type=abs_type,
-                  meta={"source": "database", "numat": numat, "nproc": nproc})
+                  meta={"source": "database", "numat": numat, "nproc": nproc},
+                  free_ispec=free_ispec, free_type=free_type)
     info = {"nspec": nspec, "npgeo": npgeo, "ngnod": ngnod, "numat": numat,
-            "nelemabs": nelemabs, "materials": mats, "nadjacency": nadj}
+            "nelemabs": nelemabs, "nelem_acoustic_surface": nelem_acoustic_surface,
+            "materials": mats, "nadjacency": nadj}
     return mesh, info

@@ -19,13 +19,14 @@ Keys follow the reference's parameter files (include/parameter_parser/*):
     sources: <path to a sources YAML>
 
 Sources YAML: ``sources: [{force: {x, z, angle, Ricker: {factor, tshift,
-f0}}}, ...]`` (include/parameter_parser/sources.hpp; force sources with a
-Ricker time function, the elastic path built here).  Relative paths are
+f0}} or Dirac: {factor, tshift}}, ...]`` (include/parameter_parser/
+sources.hpp, core/specfem/source/source.tpp:7-60).  Relative paths are
 resolved against ``base_dir`` (the reference resolves them against the
 working directory).  Only what this build implements is accepted (elastic
-isotropic P-SV, GLL4, Newmark, forward mode, force sources with Ricker time
-functions, ASCII seismograms); anything else raises ``ValueError`` naming
-the key.
+isotropic P-SV or acoustic isotropic meshes, GLL4, Newmark, forward mode,
+force sources with Ricker or Dirac time functions, ASCII seismograms of
+displacement, velocity, acceleration and, in acoustic meshes, pressure);
+anything else raises ``ValueError`` naming the key.
 """
 
 from __future__ import annotations
@@ -36,11 +37,11 @@ from dataclasses import dataclass, field
 import numpy as np
 import yaml
 
-from .config import Config2D, Receivers2D, Sources2D
+from .config import STF_DIRAC, STF_RICKER, Config2D, Receivers2D, Sources2D
 from .meshdb import read_mesh_database_2d
 from .seismo_io import write_seismograms
 
-_SEISMO = {"displacement", "velocity", "acceleration"}
+_SEISMO = {"displacement", "velocity", "acceleration", "pressure"}
 
 
 @dataclass
@@ -90,28 +91,35 @@ def _get(d, *keys, default=None, required=True):
 
 
 def read_sources(node):
-    """Force sources with Ricker time functions from a sources YAML node."""
+    """Force sources with Ricker or Dirac time functions from a sources YAML
+    node (a Dirac's f0 is 1 / (10 dt), set by the context)."""
     items = node.get("sources", [])
     n_decl = node.get("number-of-sources")
     if n_decl is not None and int(n_decl) != len(items):
         raise ValueError(f"number-of-sources is {n_decl} but {len(items)} are listed")
-    x, z, ang, f0, ts, fac = [], [], [], [], [], []
+    x, z, ang, f0, ts, fac, kind = [], [], [], [], [], [], []
     for k, item in enumerate(items):
         if "force" not in item:
             raise ValueError(f"source {k}: only force sources are supported ({list(item)})")
         f = item["force"]
-        if "Ricker" not in f:
-            raise ValueError(f"source {k}: only Ricker time functions are supported")
-        r = f["Ricker"]
+        if "Ricker" in f:
+            r = f["Ricker"]
+            kind.append(STF_RICKER)
+            f0.append(float(r["f0"]))
+        elif "Dirac" in f:
+            r = f["Dirac"]
+            kind.append(STF_DIRAC)
+            f0.append(1.0)
+        else:
+            raise ValueError(f"source {k}: only Ricker and Dirac time functions are supported")
         x.append(float(f["x"]))
         z.append(float(f["z"]))
         ang.append(float(f.get("angle", 0.0)))
-        f0.append(float(r["f0"]))
         ts.append(float(r.get("tshift", 0.0)))
-        fac.append(float(r.get("factor", 1.0)))
+        fac.append(float(r["factor"]))
     return Sources2D(x=np.array(x), z=np.array(z), angle=np.array(ang, np.float32),
                      f0=np.array(f0, np.float32), tshift=np.array(ts, np.float32),
-                     factor=np.array(fac, np.float32))
+                     factor=np.array(fac, np.float32), stf=np.array(kind, np.int32))
 
 
 def parse_config(params, base_dir="."):
@@ -145,7 +153,8 @@ def parse_config(params, base_dir="."):
     types = tuple(rec.get("seismogram-type", ["displacement"]))
     bad = [t for t in types if t not in _SEISMO]
     if bad:
-        raise ValueError(f"seismogram-type {bad}: only displacement, velocity, acceleration")
+        raise ValueError(f"seismogram-type {bad}: only displacement, velocity, acceleration, "
+                         "pressure")
     nsbs = int(rec.get("nstep_between_samples", 1))
     if float(rec.get("angle", 0.0)) != 0.0:
         raise ValueError("receiver angle: only 0 is supported")

@@ -7,8 +7,8 @@ value), each written like C++'s ``std::scientific`` at its default precision
 t0 + i * dt * nsbs (include/io/seismogram/seismogram_iterator.hpp:38).
 
 File names:
-* 2D P-SV: ``NET.STA.S2.BX{X,Z}.sem{d,v,a}`` (src/io/seismogram/writer.cpp:
-  20-60);
+* 2D P-SV: ``NET.STA.S2.BX{X,Z}.sem{d,v,a}`` and, for pressure, the single
+  file ``NET.STA.S2.PRE.semp`` (src/io/seismogram/writer.cpp:20-60);
 * 3D: ``NET.STA.S3.{band}X{X,Y,Z}.sem{d,v,a}``, with the band code from the
   sampling interval (src/io/seismogram/impl/channel_generator.cpp:3-80).
 """
@@ -19,7 +19,8 @@ import os
 
 import numpy as np
 
-EXT = {"displacement": "semd", "velocity": "semv", "acceleration": "sema"}
+EXT = {"displacement": "semd", "velocity": "semv", "acceleration": "sema",
+       "pressure": "semp"}
 
 
 def band_code(dt):
@@ -40,6 +41,8 @@ def band_code(dt):
 def station_filenames(network, station, seismo_type, dim=2, dt=None):
     """File names (without folder) of one station's components."""
     ext = EXT[seismo_type]
+    if dim == 2 and seismo_type == "pressure":
+        return [f"{network}.{station}.S2.PRE.{ext}"]
     if dim == 2:
         return [f"{network}.{station}.S2.BX{c}.{ext}" for c in "XZ"]
     band = band_code(dt)
@@ -52,7 +55,8 @@ def sample_times(nsig, dt, t0, nsbs=1):
 
 def write_seismograms(folder, seis, stations, seismo_types, dt, t0, nsbs=1):
     """Writes seis (nsig, ntypes, nrec, ncomp), ncomp 2 (2D) or 3 (3D), for
-    the stations [(network, name), ...]; returns the written paths."""
+    the stations [(network, name), ...]; returns the written paths.  A 2D
+    pressure type writes component 0 only (one file per station)."""
     seis = np.asarray(seis)
     nsig, ntypes, nrec, ncomp = seis.shape
     if ncomp not in (2, 3) or nrec != len(stations) or ntypes != len(seismo_types):
@@ -80,7 +84,8 @@ def read_trace(path):
 
 def read_seismograms(folder, stations, seismo_types, dim=2, dt=None):
     """Reads traces written by the reference or by ``write_seismograms``:
-    returns (times, seis (nsig, ntypes, nrec, ncomp))."""
+    returns (times, seis (nsig, ntypes, nrec, ncomp)); a pressure type fills
+    component 0."""
     out, times = None, None
     for it, st in enumerate(seismo_types):
         for r, (net, sta) in enumerate(stations):

@@ -42,11 +42,13 @@ def _structs(mesh, sources, receivers, seismo_types, nsbs, keep):
                        _p(arr(mesh.coord, np.float64)), _p(arr(mesh.knods, np.int32)),
                        _p(arr(mesh.kappa, np.float32)), _p(arr(mesh.mu, np.float32)),
                        _p(arr(mesh.rho, np.float32)), len(mesh.abs_ispec),
-                       _p(arr(mesh.abs_ispec, np.int32)), _p(arr(mesh.abs_type, np.int32)))
+                       _p(arr(mesh.abs_ispec, np.int32)), _p(arr(mesh.abs_type, np.int32)),
+                       len(mesh.free_ispec), _p(arr(mesh.free_ispec, np.int32)),
+                       _p(arr(mesh.free_type, np.int32)))
     s = _lib.SmxSources2D(len(sources), _p(arr(sources.x, np.float64)),
                           _p(arr(sources.z, np.float64)), _p(arr(sources.angle, np.float32)),
                           _p(arr(sources.f0, np.float32)), _p(arr(sources.tshift, np.float32)),
-                          _p(arr(sources.factor, np.float32)))
+                          _p(arr(sources.factor, np.float32)), _p(arr(sources.kinds(), np.int32)))
     types = arr([_lib.SEIS_TYPES[t] for t in seismo_types], np.int32)
     r = _lib.SmxReceivers2D(len(receivers), _p(arr(receivers.x, np.float64)),
                             _p(arr(receivers.z, np.float64)), len(types), _p(types), int(nsbs))
@@ -101,6 +103,9 @@ class Context:
         check(lib.smx_ctx_info(h, ctypes.byref(ns), ctypes.byref(ng), ctypes.byref(nsig),
                                ctypes.byref(nr)))
         self.nspec, self.nglob, self.nsig, self.nrec = ns.value, ng.value, nsig.value, nr.value
+        nc = ctypes.c_int()
+        check(lib.smx_ctx_components(h, ctypes.byref(nc)))
+        self.ncomp = nc.value  # 2 elastic P-SV, 1 acoustic potential
         self.t0 = lib.smx_ctx_t0(h)
         nt, nx, nsh, npart = (ctypes.c_int() for _ in range(4))
         check(lib.smx_ctx_layout(h, ctypes.byref(nt), ctypes.byref(nx), ctypes.byref(nsh),
@@ -167,13 +172,14 @@ class Context:
 
     def get_field(self, which, field=SMX_FIELD_FORWARD):
         idx = {"u": SMX_U, "v": SMX_V, "a": SMX_A, "minv": SMX_MASS_INVERSE}[which]
-        out = np.empty((self.nglob, 2), dtype=self.dtype)
+        out = np.empty((self.nglob, self.ncomp), dtype=self.dtype)
         check(self.lib.smx_get_field(self.h, field, idx, _p(out)))
-        return out
+        return out if self.ncomp > 1 else out[:, 0]
 
     def set_field(self, which, values, field=SMX_FIELD_FORWARD):
         idx = {"u": SMX_U, "v": SMX_V, "a": SMX_A}[which]
-        v = np.ascontiguousarray(np.asarray(values, dtype=self.dtype).reshape(self.nglob, 2))
+        v = np.ascontiguousarray(np.asarray(values, dtype=self.dtype)
+                                 .reshape(self.nglob, self.ncomp))
         check(self.lib.smx_set_field(self.h, field, idx, _p(v)))
 
     def ibool(self):
@@ -194,7 +200,6 @@ class Context:
         return t.value
 
     # ---- adjoint / Frechet path ------------------------------------------
-    ncomp = 2
 
     def set_model(self, kappa, mu, rho):
         """Replaces the material (each (nspec, 5, 5)) and rebuilds the mass."""
@@ -306,7 +311,7 @@ class TimeMarching:
             ctx.run_forward(0, ctx.nstep)
             ctx.synchronize()
             return
-        total_dofs = 2 * (2 * ctx.nglob)
+        total_dofs = 2 * (ctx.ncomp * ctx.nglob)
         for istep, _ in self.time_scheme.iterate_forward():
             dofs = self.time_scheme.apply_predictor_phase_forward()
             nelem = self.kernels.update_wavefields(istep)

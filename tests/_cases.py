@@ -37,6 +37,72 @@ def fixture_case(name):
     return cfg, g
 
 
+# The three acoustic SPECFEM2D fixtures of the same test list: the 80x60 Q9
+# box with one acoustic material (rho=2700, cp=3000), dt=1.1e-3, 300 steps,
+# read from the reference mesher's databases; free surface on all four sides
+# (first two) or on the top with Stacey elsewhere (third); sources from each
+# fixture's sources.yaml.
+ACOUSTIC_FIXTURES = {
+    "HomogeneousAcousticDomain": dict(dirac=True, sources=[(2500.0, 2500.0)]),
+    "HomogeneousAcousticDomainDirichletBC": dict(
+        dirac=False,
+        sources=[(500.0, 500.0), (3500.0, 500.0), (500.0, 2500.0), (3500.0, 2500.0)]),
+    "HomogeneousAcousticDomainCompositeStaceyDirichletBC": dict(
+        dirac=False,
+        sources=[(500.0, 500.0), (3500.0, 500.0), (500.0, 2500.0), (3500.0, 2500.0)]),
+}
+
+
+def acoustic_fixture_case(name):
+    """(Config2D, traces (ntypes, nsta, 2, nt) including pressure in
+    component 0 of a last 'pressure' type when the fixture has it)."""
+    from specfempp_amd.meshdb import read_mesh_database_2d
+    g = np.load(os.path.join(GOLDEN, f"specfem2d_{name}.npz"))
+    spec = ACOUSTIC_FIXTURES[name]
+    mesh, _ = read_mesh_database_2d(os.path.join(GOLDEN, f"specfem2d_{name}.database.bin.gz"))
+    xs, zs = [s[0] for s in spec["sources"]], [s[1] for s in spec["sources"]]
+    src = (Sources2D.dirac(xs, zs, factor=1e10) if spec["dirac"]
+           else Sources2D.ricker(xs, zs, f0=10.0, factor=1e10))
+    rec = Receivers2D(x=g["x"], z=g["z"], names=tuple(g["names"]))
+    types = tuple(TYPE_NAMES[t] for t in g["types"])
+    traces = g["traces"]
+    if "pressure" in g.files:
+        types = types + ("pressure",)
+        p = np.zeros((1,) + traces.shape[1:])
+        p[0, :, 0, :] = g["pressure"]
+        traces = np.concatenate([traces, p], axis=0)
+    cfg = Config2D(name, mesh, src, rec, 1.1e-3, 300, seismo_types=types)
+    return cfg, traces
+
+
+def acoustic_box(nx=24, nz=18, h=50.0, nstep=300, dt=1e-3, nrec=8, seed=3,
+                 free=("top",), absorbing=("bottom", "right", "left"), ngnod=4):
+    """Heterogeneous acoustic box: smooth seeded +-5% rho and vp per GLL
+    point (vs = 0), free surface and Stacey sides as given, two Ricker
+    sources and one Dirac, receivers on a slanted line; records d, v, a, p."""
+    from specfempp_amd.mesh import acoustic_moduli, gll_coordinates
+    mesh = box_mesh(0.0, nx * h, 0.0, nz * h, nx, nz, ngnod=ngnod, absorbing=absorbing,
+                    rho=2000.0, vp=2500.0, vs=0.0, free=free)
+    rng = np.random.default_rng(seed)
+    xz = gll_coordinates(mesh)
+    fields = []
+    for _ in range(2):
+        kx, kz, ph = rng.uniform(1.0, 3.0), rng.uniform(1.0, 3.0), rng.uniform(0, 2 * np.pi)
+        fields.append(np.sin(kx * np.pi * xz[..., 0] / (nx * h) + ph)
+                      * np.cos(kz * np.pi * xz[..., 1] / (nz * h)))
+    rho = (2000.0 * (1.0 + 0.05 * fields[0])).astype(np.float32)
+    vp = (2500.0 * (1.0 + 0.05 * fields[1])).astype(np.float32)
+    mesh.kappa, mesh.mu, mesh.rho = acoustic_moduli(rho, vp)
+    L, H = nx * h, nz * h
+    src = Sources2D.ricker([0.3 * L, 0.7 * L, 0.52 * L], [0.6 * H, 0.35 * H, 0.47 * H],
+                           f0=12.0, factor=1e10)
+    src.stf = np.array([0, 0, 1], np.int32)
+    src.tshift = np.array([0.0, 0.01, 0.0], np.float32)
+    rec = Receivers2D(x=np.linspace(0.1 * L, 0.9 * L, nrec), z=np.linspace(0.9 * H, 0.2 * H, nrec))
+    return Config2D(f"acoustic_box_{nx}x{nz}", mesh.validate(), src, rec, dt, nstep,
+                    seismo_types=("displacement", "velocity", "acceleration", "pressure"))
+
+
 def reference_metric(ours, traces):
     """Worst per-station sum|s - ref| / sum|ref| over components and types
     (tests/unit-tests/displacement_tests/Newmark/dim2/newmark_tests.cpp:337-362).

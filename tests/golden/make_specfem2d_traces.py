@@ -4,6 +4,9 @@ compare against into small .npz fixtures (data only).
 Source directories (read as text, in this container only):
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomain
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomainStaceyDirichletBC
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousAcousticDomain
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousAcousticDomainDirichletBC
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousAcousticDomainCompositeStaceyDirichletBC
 
 Each case's mesh database (database.bin, written by the reference's mesher)
 is also copied, gzip-compressed, as specfem2d_<case>.database.bin.gz.
@@ -11,8 +14,9 @@ is also copied, gzip-compressed, as specfem2d_<case>.database.bin.gz.
 Each fixture holds the station table (STATIONS: name, network, x, z), the
 sample times, and traces[type][station][component][t] for the seismogram types
 present (d, v, a; pressure 'p' traces are stored separately).  The problem set
-up (80x60 Q9 box, dt=1.1e-3, 300 steps, Ricker sources) is restated in
-tests/test_oracle_golden.py from the fixtures' Par_file / sources.yaml.
+up (80x60 Q9 box, dt=1.1e-3, 300 steps, Ricker or Dirac sources) is restated
+in tests/test_oracle_golden.py and tests/test_oracle_acoustic.py from the
+fixtures' Par_file / sources.yaml.
 
 Usage: python tests/golden/make_specfem2d_traces.py [/root/reference]
 """
@@ -24,7 +28,9 @@ import sys
 import numpy as np
 
 BASE = "tests/unit-tests/displacement_tests/Newmark/serial/dim2"
-CASES = ["HomogeneousElasticDomain", "HomogeneousElasticDomainStaceyDirichletBC"]
+CASES = ["HomogeneousElasticDomain", "HomogeneousElasticDomainStaceyDirichletBC",
+         "HomogeneousAcousticDomain", "HomogeneousAcousticDomainDirichletBC",
+         "HomogeneousAcousticDomainCompositeStaceyDirichletBC"]
 
 
 def main(ref="/root/reference"):

@@ -1,0 +1,146 @@
+"""GPU parity of the 2D acoustic path (through the C ABI) against the CPU
+oracle and the reference's acoustic SPECFEM2D golden traces.
+
+Cases: the three acoustic fixtures (80x60 Q9 box from the reference mesher's
+databases; Dirac source with free surfaces on all sides; four Ricker sources
+with free surfaces on all sides; the same with Stacey on three sides and a
+free top) and a heterogeneous acoustic box (smooth +-5% rho and vp, free top,
+Stacey on three sides, two Ricker sources and one Dirac, d/v/a/p).
+
+Tolerances (relative L2 per seismogram type):
+* FP32 GPU vs FP32 oracle: 1e-5 for displacement and velocity; the
+  acceleration and pressure are M^-1 (f - K chi - B chi_dot), a cancellation
+  whose FP32 floor (oracle FP32 vs oracle FP64) is measured in the test, and
+  the bound is twice that floor (at least 1e-5).
+* FP64 GPU vs FP64 oracle: 1e-9 (the GPU forms 1/rho and 1/(rho vp) in
+  double from the same FP32 model, as the oracle's FP64 instantiation does;
+  the receiver weights are a linear rewrite of the reference's interpolated
+  gradient).
+* Against SPECFEM2D: the reference's criterion with the same bounds as the
+  oracle's own test.
+"""
+
+import dataclasses
+import sys
+
+import numpy as np
+import pytest
+
+from specfempp_amd.solver import Context, simulate
+from tests._cases import (ACOUSTIC_FIXTURES, acoustic_box, acoustic_fixture_case,
+                          reference_metric, rel_l2)
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN_BOUNDS = {"HomogeneousAcousticDomain": 5e-4,
+                 "HomogeneousAcousticDomainDirichletBC": 1e-4,
+                 "HomogeneousAcousticDomainCompositeStaceyDirichletBC": 1e-4}
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _oracle(oracle_lib, cfg, precision):
+    o = oracle_lib.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep,
+                            precision=precision)
+    return o.run(nsbs=cfg.nsbs, types=cfg.seismo_types)
+
+
+def _check_types(cfg, ours, ref32, ref64):
+    for it, t in enumerate(cfg.seismo_types):
+        e = rel_l2(ours[:, it], ref32[:, it])
+        floor = rel_l2(ref32[:, it], ref64[:, it])
+        bound = 1e-5 if t in ("displacement", "velocity") else max(1e-5, 2.0 * floor)
+        _say(f"  {t}: rel L2 vs FP32 oracle {e:.3e} (oracle FP32 floor {floor:.3e}, "
+             f"bound {bound:.1e})")
+        assert np.isfinite(ours[:, it]).all()
+        assert e <= bound, (t, e, bound)
+
+
+@pytest.mark.parametrize("name", sorted(ACOUSTIC_FIXTURES))
+def test_acoustic_fixtures_match_oracle_and_specfem2d(oracle_lib, name):
+    cfg, traces = acoustic_fixture_case(name)
+    _say(f"acoustic fixture {name}: 80x60 Q9, 300 steps, types {cfg.seismo_types}")
+    ours, ctx = simulate(cfg)
+    assert ctx.ncomp == 1
+    ref32, ref64 = _oracle(oracle_lib, cfg, 4), _oracle(oracle_lib, cfg, 8)
+    _check_types(cfg, ours, ref32, ref64)
+    for it, t in enumerate(cfg.seismo_types):
+        e = reference_metric(ours[:, it:it + 1], traces[it:it + 1])
+        _say(f"  {t}: reference metric vs SPECFEM2D {e:.3e}")
+        assert e < GOLDEN_BOUNDS[name]
+
+
+@pytest.mark.parametrize("ngnod", [4, 9])
+def test_acoustic_box_matches_oracle(oracle_lib, ngnod):
+    cfg = acoustic_box(nstep=400, ngnod=ngnod)
+    _say(f"acoustic heterogeneous box 24x18 (ngnod {ngnod}), 400 steps, d/v/a/p")
+    ours, _ = simulate(cfg)
+    ref32, ref64 = _oracle(oracle_lib, cfg, 4), _oracle(oracle_lib, cfg, 8)
+    _check_types(cfg, ours, ref32, ref64)
+    ours64, _ = simulate(cfg, precision=8)
+    for it, t in enumerate(cfg.seismo_types):
+        e = rel_l2(ours64[:, it], ref64[:, it])
+        _say(f"  FP64 {t}: rel L2 vs FP64 oracle {e:.3e}")
+        assert e < 1e-9
+
+
+def test_acoustic_operator_path_equals_fused():
+    cfg = acoustic_box(nstep=120)
+    _say("acoustic operator-by-operator loop vs fused graph (bitwise)")
+    fused, cf = simulate(cfg, fused=True)
+    ops, co = simulate(cfg, fused=False)
+    assert np.array_equal(fused, ops)
+    for w in ("u", "v", "a"):
+        assert np.array_equal(cf.get_field(w), co.get_field(w))
+
+
+def test_acoustic_mass_matches_oracle(oracle_lib):
+    cfg, _ = acoustic_fixture_case("HomogeneousAcousticDomainCompositeStaceyDirichletBC")
+    _say("acoustic mass (with the Stacey term) vs oracle")
+    ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    ctx.initialize()
+    minv = ctx.get_field("minv").astype(np.float64)
+    assert minv.shape == (ctx.nglob,)
+    ib = ctx.ibool().reshape(-1)
+    o = oracle_lib.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    ominv = o.get("minv").reshape(-1, 2)[:, 0]
+    oib = o.get("ibool").astype(np.int64)
+    np.testing.assert_allclose(minv[ib], ominv[oib], rtol=2e-6)
+
+
+def test_acoustic_tile_sizes_and_reruns_agree():
+    cfg = acoustic_box(nstep=200)
+    _say("acoustic tile layouts 64, 16, 1 agree; reruns are bitwise equal")
+    base, ctx = simulate(cfg)
+    ctx.run_forward(0, cfg.nstep)  # a second run of the same context
+    ctx.synchronize()
+    again, _ = simulate(cfg)
+    assert np.array_equal(base, again)
+    for te in (16, 1):
+        s, _ = simulate(cfg, options={"tile_elems": te})
+        for it in range(len(cfg.seismo_types)):
+            assert rel_l2(s[:, it], base[:, it]) < 1e-5
+
+
+def test_acoustic_free_surface_points_stay_at_rest():
+    cfg = acoustic_box(nx=10, nz=8, nstep=200, free=("top", "left"))
+    _say("acoustic free-surface points stay exactly at rest")
+    _, ctx = simulate(cfg, precision=8)
+    u = ctx.get_field("u")
+    ib = ctx.ibool()
+    m = cfg.mesh
+    top = ib[m.free_ispec[m.free_type == 3], 4, :].ravel()
+    left = ib[m.free_ispec[m.free_type == 4], :, 0].ravel()
+    assert np.abs(u).max() > 0
+    assert np.all(u[top] == 0) and np.all(u[left] == 0)
+
+
+def test_acoustic_adjoint_is_rejected():
+    cfg = acoustic_box(nstep=20)
+    ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    ctx.initialize()
+    with pytest.raises(RuntimeError, match="elastic"):
+        ctx.run_forward_checkpointed(10)

@@ -103,3 +103,51 @@ def test_homogeneous_config():
     assert c.mesh.nspec == 4800 and c.mesh.expected_nglob() == 77361
     assert len(c.receivers) == 50 and len(c.sources) == 1
     assert c.mesh.kappa.shape == (4800, NGLL, NGLL)
+
+
+def _acoustic_cfg(**kw):
+    from tests._cases import acoustic_box
+    return acoustic_box(nx=6, nz=5, nstep=10, **kw)
+
+
+def test_acoustic_mesh_model():
+    from specfempp_amd.mesh import acoustic_moduli
+    c = _acoustic_cfg()
+    assert c.mesh.acoustic.all() and (c.mesh.mu == 0).all()
+    # kappa = density * cp * cp in float (acoustic/isotropic/material.hpp:49-53)
+    k, mu, rho = acoustic_moduli(np.float32(2700.0), np.float32(3000.0))
+    assert k == np.float32(np.float32(2700.0 * 3000.0) * np.float32(3000.0)) and mu == 0
+    # free surface: the top side's elements, type 3
+    assert len(c.mesh.free_ispec) == 6 and set(c.mesh.free_type.tolist()) == {3}
+
+
+@pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="library not built")
+def test_acoustic_set_up_errors_are_host_side():
+    """Invalid acoustic set-ups are rejected by the host assembly, before any
+    device call, with the reference's wording of what is missing."""
+    import dataclasses
+    from specfempp_amd.config import Receivers2D, Sources2D
+    from specfempp_amd.solver import Context
+    c = _acoustic_cfg()
+    # mixed media need the coupling, which is not built
+    mu = c.mesh.mu.copy()
+    mu[0] = 1e9
+    mixed = dataclasses.replace(c.mesh, mu=mu)
+    with pytest.raises(_lib.SmxError, match="coupling"):
+        Context(mixed, c.sources, c.receivers, c.dt, c.nstep)
+    # pressure seismograms need acoustic elements
+    e = homogeneous_box(nstep=5)
+    with pytest.raises(_lib.SmxError, match="pressure"):
+        Context(e.mesh, e.sources, e.receivers, e.dt, e.nstep, seismo_types=("pressure",))
+    # a free surface on an elastic mesh
+    fe = dataclasses.replace(e.mesh, free_ispec=np.array([0], np.int32),
+                             free_type=np.array([1], np.int32))
+    with pytest.raises(_lib.SmxError, match="free-surface"):
+        Context(fe, e.sources, e.receivers, e.dt, e.nstep)
+    # unknown time function
+    bad = dataclasses.replace(c.sources, stf=np.array([0, 7, 1], np.int32))
+    with pytest.raises(_lib.SmxError, match="time function"):
+        Context(c.mesh, bad, c.receivers, c.dt, c.nstep)
+    # batched shots are elastic only
+    with pytest.raises(_lib.SmxError, match="batched"):
+        Context.batch([c, c])

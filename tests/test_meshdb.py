@@ -80,3 +80,28 @@ def test_reader_rejects_damaged_files():
     bad[4 + 512] ^= 0xFF                             # first record's closing marker
     with pytest.raises(ValueError):
         read_mesh_database_2d(bytes(bad))
+
+
+ACOUSTIC = ("HomogeneousAcousticDomain", "HomogeneousAcousticDomainDirichletBC",
+            "HomogeneousAcousticDomainCompositeStaceyDirichletBC")
+
+
+@pytest.mark.parametrize("name", ACOUSTIC)
+def test_acoustic_databases(name):
+    """The acoustic fixtures: one acoustic material (cs = 0; kappa =
+    density cp^2 in float), free-surface edges typed from their control
+    nodes, each lying on its side of the box."""
+    mesh, info = read_mesh_database_2d(_db(name))
+    mesh.validate()
+    assert info["materials"] == [(2700.0, 3000.0, 0.0)]
+    assert mesh.acoustic.all()
+    assert np.unique(mesh.kappa).tolist() == [np.float32(np.float32(2700.0 * 3000.0) * 3000.0)]
+    composite = name.endswith("CompositeStaceyDirichletBC")
+    want = [0, 0, 80, 0] if composite else [80, 60, 80, 60]
+    assert np.bincount(mesh.free_type, minlength=9)[1:5].tolist() == want
+    assert len(mesh.abs_ispec) == (200 if composite else 0)
+    corners = mesh.coord[mesh.knods[mesh.free_ispec, :4]]
+    side = {1: (0, 1, 1, 0.0), 2: (1, 2, 0, 4000.0), 3: (2, 3, 1, 3000.0), 4: (3, 0, 0, 0.0)}
+    for k, t in enumerate(mesh.free_type):
+        a, b, axis, val = side[int(t)]
+        assert abs(corners[k, a, axis] - val) < 1e-6 and abs(corners[k, b, axis] - val) < 1e-6

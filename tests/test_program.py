@@ -119,3 +119,66 @@ def test_execute_writes_reference_seismograms(run_dir):
     err = reference_error(s, np.transpose(g["traces"], (3, 0, 1, 2)))
     print("YAML run vs SPECFEM2D (reference metric):", err)
     assert err < 1e-3
+
+
+ACOUSTIC = "HomogeneousAcousticDomain"
+ACOUSTIC_TYPES = ("displacement", "velocity", "acceleration", "pressure")
+
+
+@pytest.fixture()
+def acoustic_run_dir(tmp_path):
+    """The Dirac acoustic fixture laid out as the reference's run: one force
+    source with a Dirac time function, all four seismogram types."""
+    g = np.load(os.path.join(GOLDEN, f"specfem2d_{ACOUSTIC}.npz"))
+    d = tmp_path / "fixture"
+    d.mkdir()
+    with open(d / "STATIONS", "w") as fh:
+        for n, x, z in zip(g["names"], g["x"], g["z"]):
+            fh.write(f"{n}    AA    {x:.7f}    {z:.7f}    0.0    0.0\n")
+    src = {"number-of-sources": 1,
+           "sources": [{"force": {"x": 2500.0, "z": 2500.0, "source_surf": False,
+                                  "angle": 0.0, "vx": 0.0, "vz": 0.0,
+                                  "Dirac": {"factor": 1e10, "tshift": 0.0}}}]}
+    with open(d / "sources.yaml", "w") as fh:
+        yaml.safe_dump(src, fh)
+    shutil.copy(os.path.join(GOLDEN, f"specfem2d_{ACOUSTIC}.database.bin.gz"),
+                d / "database.bin.gz")
+    types = "\n".join(f"      - {t}" for t in ACOUSTIC_TYPES)
+    cfg = CONFIG.replace("      - displacement", types)
+    return tmp_path, g, cfg
+
+
+def _acoustic_traces(g):
+    tr = np.transpose(g["traces"], (3, 0, 1, 2))  # (nt, 3 types, nsta, 2)
+    p = np.zeros(tr.shape[:1] + (1,) + tr.shape[2:])
+    p[:, 0, :, 0] = g["pressure"].T
+    return np.concatenate([tr, p], axis=1)
+
+
+def test_acoustic_yaml_run_is_the_fixture_case(oracle_lib, acoustic_run_dir):
+    base, g, text = acoustic_run_dir
+    prog = program.parse_config(yaml.safe_load(text), base_dir=str(base))
+    cfg = prog.config
+    assert cfg.seismo_types == ACOUSTIC_TYPES and cfg.mesh.acoustic.all()
+    assert cfg.sources.kinds().tolist() == [1]
+    o = oracle_lib.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    np.testing.assert_allclose(o.t0 + cfg.dt * np.arange(cfg.nstep), g["times"], atol=1e-6)
+    s = o.run(types=cfg.seismo_types)
+    assert reference_error(s, _acoustic_traces(g)) < 5e-4
+
+
+@pytest.mark.gpu
+def test_acoustic_execute_writes_pressure_seismograms(acoustic_run_dir):
+    base, g, text = acoustic_run_dir
+    assert program._initialize([]) is True
+    try:
+        assert program._execute(text, "", base_dir=str(base)) is True
+    finally:
+        assert program._finalize() is True
+    st = [("AA", n) for n in g["names"]]
+    assert (base / "OUTPUT_FILES" / f"AA.{g['names'][0]}.S2.PRE.semp").exists()
+    t, s = read_seismograms(base / "OUTPUT_FILES", st, ACOUSTIC_TYPES)
+    np.testing.assert_allclose(t, g["times"], atol=1e-6)
+    err = reference_error(s, _acoustic_traces(g))
+    print("acoustic YAML run vs SPECFEM2D (reference metric):", err)
+    assert err < 5e-4

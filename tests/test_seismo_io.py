@@ -46,3 +46,19 @@ def test_golden_traces_through_files(oracle_lib, tmp_path):
     np.testing.assert_allclose(t, g["times"], atol=1e-6)
     theirs = np.transpose(g["traces"], (3, 0, 1, 2))  # (nt, ntypes, nsta, 2)
     assert reference_error(back, theirs) < 1e-3
+
+
+def test_pressure_files(tmp_path):
+    """2D pressure: one NET.STA.S2.PRE.semp file per station (writer.cpp:
+    73-84), component 0 of the seismogram array."""
+    assert station_filenames("AA", "S0001", "pressure") == ["AA.S0001.S2.PRE.semp"]
+    rng = np.random.default_rng(1)
+    seis = rng.standard_normal((11, 2, 2, 2)).astype(np.float32)
+    seis[:, 1, :, 1] = 0.0
+    st = [("AA", "S0001"), ("AA", "S0002")]
+    paths = write_seismograms(tmp_path, seis, st, ("displacement", "pressure"), dt=1e-3, t0=0.0)
+    assert sorted(p.split("/")[-1] for p in paths if p.endswith(".semp")) == [
+        "AA.S0001.S2.PRE.semp", "AA.S0002.S2.PRE.semp"]
+    assert len(paths) == 2 * 2 + 2
+    _, back = read_seismograms(tmp_path, st, ("displacement", "pressure"))
+    np.testing.assert_allclose(back, seis, rtol=1e-6, atol=1e-12)
