@@ -47,6 +47,81 @@ def tile_kernel_bytes(nspec, layout, nstacey):
             + 8 * layout["npartials"] + 800 * nstacey)
 
 
+def ac_tile_kernel_bytes(nspec, layout, nstacey):
+    """Algorithmic bytes of one k_ac_tile launch (acoustic time step, FP32):
+    * per element: 2-byte tile-local index per GLL point (50 B), the Stacey
+      slot, colour and free-surface mask (9 B);
+    * geometry: 5 terms per GLL point (500 B/element), or 5 per element
+      (20 B) in tiles of affine elements;
+    * 1/rho per GLL point (100 B/element), or per element (4 B) in tiles of
+      element-constant material;
+    * per point exclusive to a tile: chi, chi_dot, 1/M read, chi, chi_dot
+      written: 20 B;
+    * per point shared between tiles: chi read once (4 B); per partial slot:
+      4 B written;
+    * Stacey elements: |n|, edge weight, 1/(rho vp) and the velocity per
+      point, 16 B x 25 = 400 B per element."""
+    ge = layout.get("geom_elem_elements", 0)
+    me = layout.get("mat_elem_elements", 0)
+    per_elem = 59 * nspec + 500 * (nspec - ge) + 20 * ge + 100 * (nspec - me) + 4 * me
+    return (per_elem + 20 * layout["nexclusive"] + 4 * layout["nshared"]
+            + 4 * layout["npartials"] + 400 * nstacey)
+
+
+def acoustic_leg(nx=800, nz=600, nstep=2000, warmup=100, cpu_steps=20):
+    """2D acoustic forward on the layered workload's geometry: the same six
+    layers as fluids (vs = 0), acoustic free surface on top, Stacey on the
+    other sides, 8 Ricker sources, 500 receivers 100 m below the surface
+    (displacement = rho^-1 grad chi and pressure), dt = 4e-4, FP32."""
+    import numpy as np
+    from specfempp_amd.config import Receivers2D, layered_box
+    from specfempp_amd.solver import Context
+
+    cfg = layered_box(nx=nx, nz=nz, nstep=warmup + nstep, acoustic=True)
+    cfg.receivers = Receivers2D(x=cfg.receivers.x, z=cfg.receivers.z - 100.0)
+    cfg.seismo_types = ("displacement", "pressure")
+    ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, warmup + nstep,
+                  seismo_types=cfg.seismo_types)
+    ctx.initialize()
+    ctx.run_forward(0, warmup)
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.run_forward(warmup, nstep)
+    ctx.synchronize()
+    wall = time.perf_counter() - t
+    seis = ctx.seismograms()
+    stiff_ms = ctx.time_stiffness(50)
+    nstacey = len(set(cfg.mesh.abs_ispec.tolist()))
+    tb = ac_tile_kernel_bytes(ctx.nspec, ctx.layout, nstacey)
+    out = {"workload": f"2D acoustic forward, {nx}x{nz} quads of 50 m, 6 seeded fluid layers, "
+                       f"free top + Stacey on 3 sides, 8 Ricker sources, 500 receivers "
+                       f"(displacement, pressure), dt=4e-4, {nstep} steps",
+           "nspec": ctx.nspec, "nglob": ctx.nglob, "wall_s": wall,
+           "ms_per_step": wall * 1e3 / nstep,
+           "point_updates_per_s": ctx.nglob * nstep / wall,
+           "roofline": {"bound": "hbm", "kernel": "k_ac_tile<float, AC_STEP>",
+                        "achieved": tb / (stiff_ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": tb / (stiff_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "algorithmic_bytes_per_launch": tb, "mean_launch_ms": stiff_ms,
+                        "layout": dict(ctx.layout)},
+           "seismograms_finite": bool(np.isfinite(seis).all())}
+    del ctx
+    if cpu_steps:
+        import oracle
+        o = oracle.Oracle2D(cfg.mesh, cfg.sources, Receivers2D(x=cfg.receivers.x[:1],
+                                                               z=cfg.receivers.z[:1]),
+                            cfg.dt, cpu_steps)
+        t = time.perf_counter()
+        o.run(cpu_steps)
+        sec = time.perf_counter() - t
+        out["cpu_baseline"] = {"value": o.nglob * cpu_steps / sec, "unit": "GLL point-updates/s",
+                               "cores": 1, "kind": "port",
+                               "sample": f"{cpu_steps} time steps of the same acoustic workload "
+                                         f"(all 8 sources, 1 receiver), FP32 oracle "
+                                         f"restatement, set-up excluded, {sec:.1f} s"}
+    return out
+
+
 def tile3d_kernel_bytes(nspec, layout, nstacey, assemble=False):
     """Algorithmic bytes of one 3D tile pass (k3_tile, one time step, FP32;
     `assemble`: the timing variant that stores a = (f - Ku)/M instead of
@@ -457,7 +532,10 @@ def main():
                     help="skip the batched-shots leg (64 shots, 3000 steps)")
     ap.add_argument("--no-3d", action="store_true",
                     help="skip the 3D leg (64^3 hex, 4000 steps)")
-    ap.add_argument("--only-leg", choices=("adjoint", "batch", "hex3d", "errors"), default=None,
+    ap.add_argument("--no-acoustic", action="store_true",
+                    help="skip the acoustic leg (800x600 fluid layers, 2000 steps)")
+    ap.add_argument("--only-leg", choices=("adjoint", "batch", "hex3d", "errors", "acoustic"),
+                    default=None,
                     help="profiling aid: run only this leg and print its JSON")
     ap.add_argument("--hex-steps", type=int, default=4000,
                     help="time steps of the 3D leg (4000 is the workload; fewer only "
@@ -485,7 +563,9 @@ def main():
     if args.only_leg:
         leg = {"adjoint": adjoint_leg, "batch": batch_leg,
                "hex3d": lambda: hex_leg(nstep=args.hex_steps),
-               "errors": errors_leg}[args.only_leg]
+               "errors": errors_leg,
+               "acoustic": lambda: acoustic_leg(cpu_steps=0 if args.no_cpu_baseline
+                                                else 20)}[args.only_leg]
         print(json.dumps({args.only_leg: leg()}), flush=True)
         return
     if args.gpus != 1:
@@ -568,6 +648,8 @@ def main():
         out["batch"] = batch_leg()
     if not args.no_3d:
         out["hex3d"] = hex_leg(nstep=args.hex_steps)
+    if not args.no_acoustic:
+        out["acoustic"] = acoustic_leg(cpu_steps=0 if args.no_cpu_baseline else 20)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(cfg)

@@ -104,11 +104,12 @@ def homogeneous_box(nx=80, nz=60, h=50.0, nstep=2000, dt=5e-4, nrec=50) -> Confi
 
 
 def layered_box(nx=800, nz=600, h=50.0, nstep=5000, dt=4e-4, nsrc=8, nrec=500,
-                nlayers=6, seed=1001, f0=5.0) -> Config2D:
+                nlayers=6, seed=1001, f0=5.0, acoustic=False) -> Config2D:
     """Layered box: ``nlayers`` equal horizontal layers with vp ~ U[2000, 6000]
     m/s, vs = vp/sqrt(3), rho = 310 vp^0.25 (Gardner), ``nsrc`` vertical Ricker
     sources (f0 = 5 Hz) at seeded interior positions and ``nrec`` surface
-    receivers; free top, Stacey on the other three sides."""
+    receivers; free top, Stacey on the other three sides.  ``acoustic``: the
+    same layers as fluids (vs = 0) with an acoustic free surface on top."""
     rng = np.random.default_rng(seed)
     vps = rng.uniform(2000.0, 6000.0, size=nlayers)
     xmax, zmax = nx * h, nz * h
@@ -121,15 +122,16 @@ def layered_box(nx=800, nz=600, h=50.0, nstep=5000, dt=4e-4, nsrc=8, nrec=500,
                     absorbing=("bottom", "right", "left"),
                     rho=lambda jx, jz: 310.0 * vps[layer(jz)] ** 0.25,
                     vp=lambda jx, jz: vps[layer(jz)],
-                    vs=lambda jx, jz: vps[layer(jz)] / np.sqrt(3.0))
+                    vs=(0.0 if acoustic else lambda jx, jz: vps[layer(jz)] / np.sqrt(3.0)),
+                    free=("top",) if acoustic else ())
     sx = rng.uniform(0.05 * xmax, 0.95 * xmax, size=nsrc)
     sz = rng.uniform(0.067 * zmax, 0.933 * zmax, size=nsrc)
     src = Sources2D.ricker(sx, sz, f0=f0, factor=1e10)
     rx = 40.0 + (xmax - 80.0) / max(nrec - 1, 1) * np.arange(nrec)
     rec = Receivers2D(x=rx, z=np.full(nrec, zmax))
-    return Config2D(f"layered_{nx}x{nz}", mesh, src, rec, dt, nstep,
-                    meta={"nx": nx, "nz": nz, "h": h, "vp_layers": vps.tolist(),
-                          "seed": seed})
+    return Config2D(f"layered_{'acoustic_' if acoustic else ''}{nx}x{nz}", mesh, src, rec,
+                    dt, nstep, meta={"nx": nx, "nz": nz, "h": h, "vp_layers": vps.tolist(),
+                                     "seed": seed})
 
 
 def gaussian_vs_anomaly(mesh: Mesh2D, x0=2000.0, z0=1500.0, sigma=300.0, amp=0.05,
