@@ -57,11 +57,14 @@ typedef struct smx_ctx smx_ctx;
  *
  * Elements whose 25 points all have mu == 0 are acoustic (a material with
  * cs = 0, src/io/mesh/impl/fortran/dim2/read_material_properties.cpp:
- * 178-204); a mesh is either all elastic or all acoustic in this build (the
- * elastic-acoustic coupling is not built).  Acoustic free-surface
- * (Dirichlet) entities come from the database's acoustic-surface records
- * (read_boundaries.cpp:276-300): edge types 1..4 as above, corners 5
- * bottom-left, 6 bottom-right, 7 top-right, 8 top-left. */
+ * 178-204).  A mesh with both media is coupled on its conforming
+ * elastic-acoustic edges (include/kokkos_kernels/impl/compute_coupling.tpp,
+ * include/medium/dim2/coupling_terms): such a context runs in FP32 through
+ * smx_run_forward only and exposes no single field (smx_ctx_components
+ * reports 0).  Acoustic free-surface (Dirichlet) entities come from the
+ * database's acoustic-surface records (read_boundaries.cpp:276-300): edge
+ * types 1..4 as above, corners 5 bottom-left, 6 bottom-right, 7 top-right,
+ * 8 top-left. */
 typedef struct {
   int nspec;            /* number of spectral elements */
   int ngnod;            /* control nodes per element: 4 or 9 */
@@ -230,7 +233,8 @@ int smx_ctx_info(const smx_ctx *ctx, int *nspec, int *nglob, int *nsig,
                  int *nrec);
 
 /* Field components per GLL point: 2 (2D elastic P-SV), 1 (2D acoustic: the
- * potential), 3 (3D elastic) — the medium's `components` attribute
+ * potential), 3 (3D elastic), 0 for a coupled mesh (one field per medium,
+ * not exposed) — the medium's `components` attribute
  * (include/enumerations/medium.hpp:98,162, attributes<dim, medium>::components).
  * smx_get_field / smx_set_field / smx_field_device_ptr arrays are
  * [nglob][ncomp]. */

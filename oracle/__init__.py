@@ -57,6 +57,13 @@ def _load():
     lib.oracle2d_adjoint.argtypes = [P, i, P, P]
     lib.oracle2d_ref_kernels.argtypes = [P, i, P, P, P]
     lib.oracle2d_time_parallel.argtypes = [P, i, i, i, P]
+    lib.oracle2d_coupled_create.restype = P
+    lib.oracle2d_coupled_create.argtypes = lib.oracle2d_create_ex.argtypes
+    lib.oracle2d_coupled_t0.restype = d
+    lib.oracle2d_coupled_t0.argtypes = [P]
+    lib.oracle2d_coupled_run.argtypes = [P, i, i, i, i, P, P]
+    lib.oracle2d_coupled_destroy.argtypes = [P]
+    lib.oracle2d_coupled_scale.argtypes = [P, d, d]
     lib.oracle2d_last_error.restype = ctypes.c_char_p
     _lib = lib
     return lib
@@ -74,7 +81,9 @@ GET = {"minv": 0, "u": 1, "v": 2, "a": 3, "ibool": 4, "jacobian": 5, "xix": 6, "
 
 
 class Oracle2D:
-    """One forward simulation on the CPU (precision 4 = FP32, 8 = FP64)."""
+    """One forward simulation on the CPU (precision 4 = FP32, 8 = FP64).
+    A mesh with both elastic and acoustic elements runs as the coupled
+    restatement (run() and t0 only)."""
 
     def __init__(self, mesh, sources, receivers, dt, nstep, precision=4):
         lib = _load()
@@ -101,8 +110,11 @@ class Oracle2D:
         self.nrec = len(rx)
         self.nstep = int(nstep)
         self.nloc = mesh.nspec * 25
-        self.acoustic = bool((np.asarray(mesh.mu) == 0).all())
-        self.h = lib.oracle2d_create_ex(precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
+        fluid = (np.asarray(mesh.mu).reshape(mesh.nspec, -1) == 0).all(axis=1)
+        self.acoustic = bool(fluid.all())
+        self.coupled = bool(fluid.any() and not fluid.all())
+        create = lib.oracle2d_coupled_create if self.coupled else lib.oracle2d_create_ex
+        self.h = create(precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
                                         _ptr(coord), _ptr(knods), _ptr(kappa), _ptr(mu),
                                         _ptr(rho), len(ai), _ptr(ai), _ptr(at), len(fi),
                                         _ptr(fi), _ptr(ft), float(dt), int(nstep), len(sx),
@@ -115,8 +127,16 @@ class Oracle2D:
 
     def __del__(self):
         if getattr(self, "h", None):
-            self.lib.oracle2d_destroy(self.h)
+            if self.coupled:
+                self.lib.oracle2d_coupled_destroy(self.h)
+            else:
+                self.lib.oracle2d_destroy(self.h)
             self.h = None
+
+    def plant_coupling_scales(self, fluid_side, solid_side):
+        """Tests only: scale the two coupling terms of a coupled run (a
+        planted mistake on the CPU)."""
+        self.lib.oracle2d_coupled_scale(self.h, float(fluid_side), float(solid_side))
 
     @property
     def nglob(self):
@@ -124,6 +144,8 @@ class Oracle2D:
 
     @property
     def t0(self):
+        if self.coupled:
+            return self.lib.oracle2d_coupled_t0(self.h)
         return self.lib.oracle2d_t0(self.h)
 
     def get(self, what):
@@ -190,7 +212,8 @@ class Oracle2D:
         t = np.array([SEIS_TYPES[s] for s in types], dtype=np.int32)
         nsig = (nstep + nsbs - 1) // nsbs
         out = np.zeros((nsig, len(t), self.nrec, 2), dtype=np.float64)
-        got = self.lib.oracle2d_run_forward(self.h, self.istep, nstep, nsbs, len(t), _ptr(t), _ptr(out))
+        run = self.lib.oracle2d_coupled_run if self.coupled else self.lib.oracle2d_run_forward
+        got = run(self.h, self.istep, nstep, nsbs, len(t), _ptr(t), _ptr(out))
         if got < 0:
             raise RuntimeError(self.lib.oracle2d_last_error().decode())
         self.istep += nstep

@@ -26,6 +26,10 @@
 //                                four Ricker sources), reference metric; plus
 //                                closed forms (mass = area / kappa, Dirichlet
 //                                points stay at rest without sources there)
+//   * coupled elastic-acoustic   SPECFEM2D golden traces of the reference's
+//                                two coupled fixtures (water above the solid,
+//                                restated from its Par_file; water below, from
+//                                the mesher's database), reference metric
 //   * Frechet kernels            "parity unpinned" by reference data (no
 //                                reference fixture holds kernel values); pinned
 //                                by finite differences of the misfit instead.
@@ -814,7 +818,7 @@ template <typename real> struct Oracle2D {
   void set_sources(int nsrc_, const double *x, const double *z,
                    const float *angle, const float *f0, const float *tshift,
                    const float *factor, real dt, int nstep_,
-                   const int *kinds = nullptr) {
+                   const int *kinds = nullptr, const real *t0_all = nullptr) {
     nsrc = nsrc_;
     nstep = nstep_;
     src_ispec.assign(nsrc, 0);
@@ -836,6 +840,10 @@ template <typename real> struct Oracle2D {
       if (t0s[s] < min_t0)
         min_t0 = t0s[s];
     }
+    // the start time is the minimum over every source of the run, which a
+    // sub-mesh of a coupled run receives from outside
+    if (t0_all)
+      min_t0 = *t0_all;
     for (int s = 0; s < nsrc; ++s)
       tsh[s] = t0s[s] - min_t0;
     t0 = min_t0;
@@ -1167,6 +1175,275 @@ template <typename real> struct Oracle2D {
   }
 };
 
+// ================================================ coupled (Coupled2D) ====
+// Elastic-acoustic coupling on a conforming interface, restated as two
+// single-medium oracles (the reference keeps one field per medium) plus the
+// interface terms of include/medium/dim2/coupling_terms/
+// {acoustic_elastic,elastic_acoustic}.hpp:
+//   acoustic self point:  chi_dd += w (n . u_elastic)        (u after the predictor)
+//   elastic self point:   a_c    += (w n_c) chi_dd_acoustic   (chi_dd of this step)
+// with n the self element's edge normal (compute_normal, not unit length)
+// and w the GLL weight along the edge (core/specfem/assembly/
+// conforming_interfaces/dim2/impl/interface_container.tpp:20-80); the
+// acoustic term is zeroed at free-surface points (compute_coupling.tpp:
+// 70-90).  The step order is time_marching.tpp:33-77: both predictors, the
+// acoustic update (coupling, sources, stiffness, divide) and corrector, then
+// the elastic update and corrector.
+template <typename real> struct Coupled2D {
+  Oracle2D<real> el, ac;
+  struct Ent {
+    int i_self;  // local point of the self sub-oracle
+    int g_other; // global point of the other sub-oracle
+    real factor, n0, n1;
+  };
+  std::vector<Ent> ac_ent, el_ent;
+  int nrec = 0;
+  std::vector<int> rec_med, rec_sub; // station -> (0 elastic / 1 acoustic, index)
+  real t0 = 0;
+  // test knob (planted mistakes on the CPU): scales of the two coupling
+  // terms, 1 for the restatement
+  real ac_scale = 1, el_scale = 1;
+
+  void build(int nspec, int ngnod, int nnodes, const double *coord,
+             const int *knods, const float *kappa, const float *mu,
+             const float *rho, int nabs, const int *abs_ispec,
+             const int *abs_type, int nfree, const int *free_ispec,
+             const int *free_type, real dt, int nstep, int nsrc,
+             const double *sx, const double *sz, const float *angle,
+             const float *f0, const float *tshift, const float *factor,
+             const int *kinds, int nrec_, const double *rx, const double *rz) {
+    const int NP = NGLL * NGLL;
+    std::vector<int> medium(nspec), sub(nspec);
+    std::vector<int> knods_m[2];
+    std::vector<float> mat_m[2][3];
+    int count[2] = {0, 0};
+    for (int e = 0; e < nspec; ++e) {
+      bool fluid = true;
+      for (int p = 0; p < NP; ++p)
+        fluid = fluid && mu[size_t(e) * NP + p] == 0.0f;
+      const int m = fluid ? 1 : 0;
+      medium[e] = m;
+      sub[e] = count[m]++;
+      for (int n = 0; n < ngnod; ++n)
+        knods_m[m].push_back(knods[size_t(e) * ngnod + n]);
+      for (int p = 0; p < NP; ++p) {
+        mat_m[m][0].push_back(kappa[size_t(e) * NP + p]);
+        mat_m[m][1].push_back(mu[size_t(e) * NP + p]);
+        mat_m[m][2].push_back(rho[size_t(e) * NP + p]);
+      }
+    }
+    if (count[0] == 0 || count[1] == 0)
+      throw std::runtime_error("a coupled run needs both media");
+    std::vector<int> abs_e[2], abs_t[2], fr_e, fr_t;
+    for (int i = 0; i < nabs; ++i) {
+      const int m = medium[abs_ispec[i]];
+      abs_e[m].push_back(sub[abs_ispec[i]]);
+      abs_t[m].push_back(abs_type[i]);
+    }
+    for (int i = 0; i < nfree; ++i) {
+      if (medium[free_ispec[i]] != 1)
+        throw std::runtime_error("free-surface entity on an elastic element");
+      fr_e.push_back(sub[free_ispec[i]]);
+      fr_t.push_back(free_type[i]);
+    }
+    el.build_mesh(count[0], ngnod, nnodes, coord, knods_m[0].data(),
+                  mat_m[0][0].data(), mat_m[0][1].data(), mat_m[0][2].data(),
+                  int(abs_e[0].size()), abs_e[0].data(), abs_t[0].data());
+    ac.build_mesh(count[1], ngnod, nnodes, coord, knods_m[1].data(),
+                  mat_m[1][0].data(), mat_m[1][1].data(), mat_m[1][2].data(),
+                  int(abs_e[1].size()), abs_e[1].data(), abs_t[1].data(),
+                  int(fr_e.size()), fr_e.data(), fr_t.data());
+    el.set_timescheme(dt);
+    ac.set_timescheme(dt);
+
+    // medium of a point: the sub-mesh whose best candidate lies closest
+    // (a point inside one medium falls outside the other sub-mesh)
+    auto medium_of = [&](double x, double z) {
+      real d[2];
+      Oracle2D<real> *o[2] = {&el, &ac};
+      for (int m = 0; m < 2; ++m) {
+        try {
+          int e;
+          real xi, ga;
+          o[m]->locate(static_cast<real>(static_cast<float>(x)),
+                       static_cast<real>(static_cast<float>(z)), e, xi, ga);
+          real lx, lz;
+          compute_location<real>(&o[m]->cx[e * ngnod], &o[m]->cz[e * ngnod],
+                                 ngnod, xi, ga, lx, lz);
+          d[m] = std::hypot(lx - static_cast<real>(static_cast<float>(x)),
+                            lz - static_cast<real>(static_cast<float>(z)));
+        } catch (const std::exception &) {
+          d[m] = std::numeric_limits<real>::max();
+        }
+      }
+      if (d[0] == std::numeric_limits<real>::max() &&
+          d[1] == std::numeric_limits<real>::max())
+        throw std::runtime_error("point lies outside the mesh");
+      return d[1] < d[0] ? 1 : 0;
+    };
+    // sources per medium; one start time for the whole run
+    std::vector<int> src_m[2];
+    real t0_all = std::numeric_limits<real>::max();
+    for (int k = 0; k < nsrc; ++k) {
+      const int kind = kinds ? kinds[k] : 0;
+      const real f = kind == 1 ? static_cast<real>(1.0 / (10.0 * dt))
+                               : static_cast<real>(f0[k]);
+      const real hdur = 1.0 / f;
+      const real t0k =
+          -static_cast<real>(1.2) * hdur + static_cast<real>(tshift[k]);
+      t0_all = std::min(t0_all, t0k);
+      src_m[medium_of(sx[k], sz[k])].push_back(k);
+    }
+    if (nsrc == 0)
+      t0_all = 0;
+    t0 = t0_all;
+    Oracle2D<real> *o[2] = {&el, &ac};
+    for (int m = 0; m < 2; ++m) {
+      std::vector<double> x, z;
+      std::vector<float> an, ff, ts, fa;
+      std::vector<int> kd;
+      for (int k : src_m[m]) {
+        x.push_back(sx[k]);
+        z.push_back(sz[k]);
+        an.push_back(angle[k]);
+        ff.push_back(f0[k]);
+        ts.push_back(tshift[k]);
+        fa.push_back(factor[k]);
+        kd.push_back(kinds ? kinds[k] : 0);
+      }
+      o[m]->set_sources(int(x.size()), x.data(), z.data(), an.data(),
+                        ff.data(), ts.data(), fa.data(), dt, nstep, kd.data(),
+                        &t0_all);
+    }
+    // receivers per medium
+    nrec = nrec_;
+    rec_med.assign(nrec, 0);
+    rec_sub.assign(nrec, 0);
+    std::vector<double> rxm[2], rzm[2];
+    for (int r = 0; r < nrec; ++r) {
+      const int m = medium_of(rx[r], rz[r]);
+      rec_med[r] = m;
+      rec_sub[r] = int(rxm[m].size());
+      rxm[m].push_back(rx[r]);
+      rzm[m].push_back(rz[r]);
+    }
+    for (int m = 0; m < 2; ++m)
+      o[m]->set_receivers(int(rxm[m].size()), rxm[m].data(), rzm[m].data());
+    el.initialize(dt);
+    ac.initialize(dt);
+    build_interface(nspec, ngnod, knods, medium, sub);
+  }
+
+  // Interface edges: an acoustic and an elastic element sharing an edge
+  // (same two corner nodes); the points pair by position along the edge.
+  void build_interface(int nspec, int ngnod, const int *knods,
+                       const std::vector<int> &medium,
+                       const std::vector<int> &sub) {
+    // edge types 1 bottom, 2 right, 3 top, 4 left; corners (start, end)
+    // in the direction the edge's GLL index increases
+    const int cs[5][2] = {{0, 0}, {0, 1}, {1, 2}, {3, 2}, {0, 3}};
+    auto pt = [](int type, int k, int &iz, int &ix) {
+      iz = type == 1 ? 0 : (type == 3 ? NGLL - 1 : k);
+      ix = type == 4 ? 0 : (type == 2 ? NGLL - 1 : k);
+    };
+    struct Side {
+      int e, type, start;
+    };
+    std::map<std::pair<int, int>, std::vector<Side>> edges;
+    for (int e = 0; e < nspec; ++e)
+      for (int t = 1; t <= 4; ++t) {
+        const int a = knods[size_t(e) * ngnod + cs[t][0]],
+                  b = knods[size_t(e) * ngnod + cs[t][1]];
+        edges[{std::min(a, b), std::max(a, b)}].push_back({e, t, a});
+      }
+    for (auto &kv : edges) {
+      if (kv.second.size() != 2 ||
+          medium[kv.second[0].e] == medium[kv.second[1].e])
+        continue;
+      for (int self = 0; self < 2; ++self) {
+        const Side &S = kv.second[self], &O = kv.second[1 - self];
+        const bool self_ac = medium[S.e] == 1;
+        Oracle2D<real> &so = self_ac ? ac : el;
+        Oracle2D<real> &oo = self_ac ? el : ac;
+        for (int k = 0; k < NGLL; ++k) {
+          int iz, ix, jz, jx;
+          pt(S.type, k, iz, ix);
+          pt(O.type, O.start == S.start ? k : NGLL - 1 - k, jz, jx);
+          const int i = so.idx(sub[S.e], iz, ix);
+          const Jac<real> &J = so.jac[i];
+          real n0, n1, w;
+          // core/specfem/point/jacobian_matrix.hpp:424-449 (compute_normal)
+          if (S.type == 4) {
+            n0 = static_cast<real>(-1.0) * J.xix * J.jacobian;
+            n1 = static_cast<real>(-1.0) * J.xiz * J.jacobian;
+            w = so.gll.w[iz];
+          } else if (S.type == 2) {
+            n0 = J.xix * J.jacobian;
+            n1 = J.xiz * J.jacobian;
+            w = so.gll.w[iz];
+          } else if (S.type == 3) {
+            n0 = J.gammax * J.jacobian;
+            n1 = J.gammaz * J.jacobian;
+            w = so.gll.w[ix];
+          } else {
+            n0 = static_cast<real>(-1.0) * J.gammax * J.jacobian;
+            n1 = static_cast<real>(-1.0) * J.gammaz * J.jacobian;
+            w = so.gll.w[ix];
+          }
+          const int g = oo.ibool[oo.idx(sub[O.e], jz, jx)];
+          (self_ac ? ac_ent : el_ent).push_back({i, g, w, n0, n1});
+        }
+      }
+    }
+  }
+
+  void step(int istep) {
+    el.predictor(el.u, el.v, el.a, el.deltat, el.deltatover2,
+                 el.deltasquareover2);
+    ac.predictor(ac.u, ac.v, ac.a, ac.deltat, ac.deltatover2,
+                 ac.deltasquareover2);
+    // acoustic: coupling (elastic displacement), sources, stiffness, divide
+    for (const Ent &t : ac_ent) {
+      if (ac.pt_free[t.i_self])
+        continue;
+      const real c = t.factor * (t.n0 * el.u[2 * t.g_other] +
+                                 t.n1 * el.u[2 * t.g_other + 1]);
+      ac.a[2 * ac.ibool[t.i_self]] += ac_scale * c;
+    }
+    ac.add_sources(istep, ac.a);
+    ac.stiffness(ac.u, ac.v, ac.a);
+    ac.divide_mass(ac.a);
+    ac.corrector(ac.v, ac.a, ac.deltatover2);
+    // elastic: coupling (acoustic acceleration of this step), sources, ...
+    for (const Ent &t : el_ent) {
+      const int g = el.ibool[t.i_self];
+      const real chi = el_scale * ac.a[2 * t.g_other];
+      el.a[2 * g] += t.factor * t.n0 * chi;
+      el.a[2 * g + 1] += t.factor * t.n1 * chi;
+    }
+    el.add_sources(istep, el.a);
+    el.stiffness(el.u, el.v, el.a);
+    el.divide_mass(el.a);
+    el.corrector(el.v, el.a, el.deltatover2);
+  }
+
+  void seismograms(int ntypes, const int *types, real *out) const {
+    std::vector<real> b[2];
+    const Oracle2D<real> *o[2] = {&el, &ac};
+    for (int m = 0; m < 2; ++m) {
+      b[m].assign(size_t(ntypes) * std::max(1, o[m]->nrec) * 2, 0);
+      o[m]->seismograms(o[m]->u, o[m]->v, o[m]->a, ntypes, types, b[m].data());
+    }
+    for (int t = 0; t < ntypes; ++t)
+      for (int r = 0; r < nrec; ++r)
+        for (int c = 0; c < 2; ++c) {
+          const int m = rec_med[r];
+          out[(size_t(t) * nrec + r) * 2 + c] =
+              b[m][(size_t(t) * o[m]->nrec + rec_sub[r]) * 2 + c];
+        }
+  }
+};
+
 // ===================================================== C interface ======
 struct OracleHandle {
   int precision; // 4 or 8
@@ -1230,6 +1507,97 @@ void *oracle2d_create(int precision, int nspec, int ngnod, int nnodes,
                             kappa, mu, rho, nabs, abs_ispec, abs_type, 0,
                             nullptr, nullptr, dt, nstep, nsrc, sx, sz, angle,
                             f0, tshift, factor, nullptr, nrec, rx, rz);
+}
+
+// Coupled elastic-acoustic run (every argument as oracle2d_create_ex).
+void *oracle2d_coupled_create(int precision, int nspec, int ngnod, int nnodes,
+                              const double *coord, const int *knods,
+                              const float *kappa, const float *mu,
+                              const float *rho, int nabs, const int *abs_ispec,
+                              const int *abs_type, int nfree,
+                              const int *free_ispec, const int *free_type,
+                              float dt, int nstep, int nsrc, const double *sx,
+                              const double *sz, const float *angle,
+                              const float *f0, const float *tshift,
+                              const float *factor, const int *stf_kind,
+                              int nrec, const double *rx, const double *rz) {
+  try {
+    if (precision == 8) {
+      auto *c = new Coupled2D<double>();
+      c->build(nspec, ngnod, nnodes, coord, knods, kappa, mu, rho, nabs,
+               abs_ispec, abs_type, nfree, free_ispec, free_type, dt, nstep,
+               nsrc, sx, sz, angle, f0, tshift, factor, stf_kind, nrec, rx,
+               rz);
+      return new std::pair<int, void *>(8, c);
+    }
+    auto *c = new Coupled2D<float>();
+    c->build(nspec, ngnod, nnodes, coord, knods, kappa, mu, rho, nabs,
+             abs_ispec, abs_type, nfree, free_ispec, free_type, dt, nstep, nsrc,
+             sx, sz, angle, f0, tshift, factor, stf_kind, nrec, rx, rz);
+    return new std::pair<int, void *>(4, c);
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return nullptr;
+  }
+}
+
+double oracle2d_coupled_t0(void *hv) {
+  auto *h = static_cast<std::pair<int, void *> *>(hv);
+  return h->first == 8 ? static_cast<Coupled2D<double> *>(h->second)->t0
+                       : static_cast<Coupled2D<float> *>(h->second)->t0;
+}
+
+// Runs nstep steps from istep0, seismograms every nsbs steps into
+// out[nsig][ntypes][nrec][2]; returns the number of samples.
+int oracle2d_coupled_run(void *hv, int istep0, int nstep, int nsbs, int ntypes,
+                         const int *types, double *out) {
+  auto *h = static_cast<std::pair<int, void *> *>(hv);
+  auto run = [&](auto *c) {
+    using real = std::remove_reference_t<decltype(c->t0)>;
+    std::vector<real> buf(size_t(ntypes) * std::max(1, c->nrec) * 2);
+    int isig = 0;
+    for (int istep = istep0; istep < istep0 + nstep; ++istep) {
+      c->step(istep);
+      if (istep % nsbs == 0) {
+        c->seismograms(ntypes, types, buf.data());
+        for (size_t k = 0; k < size_t(ntypes) * c->nrec * 2; ++k)
+          out[size_t(isig) * ntypes * c->nrec * 2 + k] = buf[k];
+        ++isig;
+      }
+    }
+    return isig;
+  };
+  try {
+    return h->first == 8 ? run(static_cast<Coupled2D<double> *>(h->second))
+                         : run(static_cast<Coupled2D<float> *>(h->second));
+  } catch (const std::exception &e) {
+    g_err = e.what();
+    return -1;
+  }
+}
+
+// Planted mistakes (tests only): scale the fluid-side and solid-side
+// coupling terms.
+void oracle2d_coupled_scale(void *hv, double ac_scale, double el_scale) {
+  auto *h = static_cast<std::pair<int, void *> *>(hv);
+  if (h->first == 8) {
+    auto *c = static_cast<Coupled2D<double> *>(h->second);
+    c->ac_scale = ac_scale;
+    c->el_scale = el_scale;
+  } else {
+    auto *c = static_cast<Coupled2D<float> *>(h->second);
+    c->ac_scale = static_cast<float>(ac_scale);
+    c->el_scale = static_cast<float>(el_scale);
+  }
+}
+
+void oracle2d_coupled_destroy(void *hv) {
+  auto *h = static_cast<std::pair<int, void *> *>(hv);
+  if (h->first == 8)
+    delete static_cast<Coupled2D<double> *>(h->second);
+  else
+    delete static_cast<Coupled2D<float> *>(h->second);
+  delete h;
 }
 
 void oracle2d_destroy(void *hv) {

@@ -33,9 +33,11 @@ HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", f"-I{INCLUDE}",
 SOURCES = [
     ("solver2d.hip", "device"),
     ("acoustic2d.hip", "device"),
+    ("coupled2d.hip", "device"),
     ("solver3d.hip", "device"),
     ("assembly2d.cpp", "host"),
     ("assembly3d.cpp", "host"),
+    ("coupled2d.cpp", "host"),
     ("capi.cpp", "host"),
 ]
 

@@ -33,6 +33,8 @@
 
 namespace smx {
 
+using AcStreamSwap = StreamSwapT<hipStream_t>;
+
 #define AC_HIP(call)                                                           \
   do {                                                                         \
     hipError_t err_ = (call);                                                  \
@@ -120,7 +122,10 @@ __global__ void __launch_bounds__(AC_TILE_THREADS) __attribute__((
               real *__restrict__ seis_terms, int nentries, AcTypes types,
               int ntypes, int nsbs, const int *__restrict__ step_ptr,
               int istep, int ld, int sld, real dt, real dto2, real dt2o2,
-              AcQuad<real> q) {
+              AcQuad<real> q, const int *__restrict__ cpl_li,
+              const int *__restrict__ cpl_g, const real *__restrict__ cpl_w,
+              const real *__restrict__ Ue, const int *__restrict__ cout_li,
+              const int *__restrict__ cout_k, real *__restrict__ chi_dd) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   real *s_u = reinterpret_cast<real *>(lds_raw);
   real *s_a = s_u + nloc_max;
@@ -156,6 +161,15 @@ __global__ void __launch_bounds__(AC_TILE_THREADS) __attribute__((
       const int l = src_li[k], s = src_idx[k];
       const real f = stf[(size_t(step) * nsrc_total + s) * 2];
       s_a[l] += (real(-1.0) * f * src_L[2 * k]) / src_L[2 * k + 1];
+    }
+  // coupling with the solid of a coupled mesh: chi_dd += w (n . u), u the
+  // elastic displacement after this step's predictor (coupling_terms/
+  // acoustic_elastic.hpp:13-26), entries in tile order
+  if (tid == 0)
+    for (int k = T.cpl_off; k < T.cpl_off + T.ncpl; ++k) {
+      const int g = cpl_g[k];
+      s_a[cpl_li[k]] += cpl_w[3 * k] * (cpl_w[3 * k + 1] * Ue[2 * size_t(g)] +
+                                        cpl_w[3 * k + 2] * Ue[2 * size_t(g) + 1]);
     }
 
   // ---- element potential (column ix, rows 0..4)
@@ -269,10 +283,10 @@ __global__ void __launch_bounds__(AC_TILE_THREADS) __attribute__((
     real v = k < kAcPrefetch ? pre_v[k] : V[g];
     real a = s_a[i] * m;
     v += dto2 * a;
-    if (sample) {
+    if (sample || T.ncout > 0)
       s_a[i] = a;
+    if (sample)
       s_v[i] = v;
-    }
     if (MODE == AC_STEP) {
       real u = s_u[i];
       u += dt * v + dt2o2 * a;
@@ -286,6 +300,13 @@ __global__ void __launch_bounds__(AC_TILE_THREADS) __attribute__((
   }
   for (int i = T.nex + tid; i < nloc; i += AC_TILE_THREADS)
     part[T.sh_off + i - T.nex] = s_a[i];
+  // new accelerations of the tile's exclusive interface points, read by the
+  // elastic update of this step
+  if (MODE != AC_ASSEMBLE && T.ncout > 0) {
+    __syncthreads();
+    for (int k = T.cout_off + tid; k < T.cout_off + T.ncout; k += AC_TILE_THREADS)
+      chi_dd[cout_k[k]] = s_a[cout_li[k]];
+  }
 
   // ---- receiver terms on exclusive points (state of step n)
   if (MODE != AC_ASSEMBLE && sample && T.nrec > 0) {
@@ -314,7 +335,8 @@ __global__ void __launch_bounds__(AC_NODE_THREADS)
                 const real *__restrict__ Minv, const real *__restrict__ part,
                 const int *__restrict__ shp_off, const int *__restrict__ shp_pos,
                 int nshared, int nex_total, real dt, real dto2, real dt2o2,
-                int *__restrict__ step_ptr) {
+                int *__restrict__ step_ptr, const int *__restrict__ sh_iface,
+                real *__restrict__ chi_dd) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j == 0 && step_ptr && MODE != AC_ASSEMBLE)
     *step_ptr += 1;
@@ -330,6 +352,11 @@ __global__ void __launch_bounds__(AC_NODE_THREADS)
   real v = V[g];
   real a = acc * m;
   v += dto2 * a;
+  if (sh_iface) {
+    const int k = sh_iface[j];
+    if (k >= 0)
+      chi_dd[k] = a;
+  }
   if (MODE == AC_STEP) {
     real u = U[g];
     u += dt * v + dt2o2 * a;
@@ -564,6 +591,12 @@ public:
     copy_to(stacey_, st);
     minv_ = alloc<real>(n_);
     part_ = alloc<real>(std::max<size_t>(1, A.sh_list.size()));
+    cpl_li_ = upload_raw<int>(A.cpl_li);
+    cpl_g_ = upload_raw<int>(A.cpl_idx);
+    cpl_w_ = upload_real(A.cpl_w);
+    cout_li_ = upload_raw<int>(A.cout_li);
+    cout_k_ = upload_raw<int>(A.cout_k);
+    sh_iface_ = upload_raw<int>(A.sh_iface);
     lds_bytes_ = ac_lds_bytes<real>(std::max(1, A.nloc_max));
     for (const void *f : {reinterpret_cast<const void *>(&k_ac_tile<real, AC_STEP>),
                           reinterpret_cast<const void *>(&k_ac_tile<real, AC_FINAL>),
@@ -690,6 +723,30 @@ public:
   }
 
   void synchronize() override { AC_HIP(hipStreamSynchronize(stream_)); }
+
+  // ---- coupled meshes (the driver enqueues both media on its stream)
+  void bind_coupling(void *elastic_u, void *chi_dd) override {
+    if (sizeof(real) != 4)
+      throw SmxError(SMX_ERR_ARG, "coupled meshes run in FP32");
+    ue_ = static_cast<const real *>(elastic_u);
+    chi_dd_ = static_cast<real *>(chi_dd);
+  }
+  void *coupling_field() override { return field(SMX_FIELD_FORWARD).u; }
+  void enqueue_begin(void *stream, int istep0) override {
+    require_init();
+    AcStreamSwap sw(stream_, static_cast<hipStream_t>(stream));
+    k_ac_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_, istep0);
+    AC_HIP(hipGetLastError());
+    launch_predictor(field(SMX_FIELD_FORWARD));
+  }
+  void enqueue_step(void *stream, bool final) override {
+    AcStreamSwap sw(stream_, static_cast<hipStream_t>(stream));
+    Field &F = field(SMX_FIELD_FORWARD);
+    if (final)
+      fused_step<AC_FINAL>(F);
+    else
+      fused_step<AC_STEP>(F);
+  }
 
   void get_seismograms(void *host) override {
     const HostAssembly2D &A = *asm_;
@@ -863,7 +920,8 @@ private:
         part ? part : part_, src_li_, src_idx_, src_L_, stf_,
         std::max(1, A.nsrc), trec_li_, trec_entry_, rent_w_, rent_h_, terms,
         A.nrec * NPT, types_, int(A.seis_types.size()), A.nsbs, step_ptr, istep,
-        A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_);
+        A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, cpl_li_, cpl_g_,
+        cpl_w_, ue_, cout_li_, cout_k_, chi_dd_);
     AC_HIP(hipGetLastError());
   }
   template <int MODE> void launch_shared(Field &F, int *step_ptr) {
@@ -872,7 +930,7 @@ private:
     k_ac_shared<real, MODE><<<dim3((n + AC_NODE_THREADS - 1) / AC_NODE_THREADS),
                               dim3(AC_NODE_THREADS), 0, stream_>>>(
         F.u, F.v, F.a, minv_, part_, shp_off_, shp_pos_, A.nshared, A.nex_total,
-        dt_, dto2_, dt2o2_, step_ptr);
+        dt_, dto2_, dt2o2_, step_ptr, chi_dd_ ? sh_iface_ : nullptr, chi_dd_);
     AC_HIP(hipGetLastError());
   }
   template <int MODE> void fused_step(Field &F) {
@@ -925,6 +983,12 @@ private:
   uint16_t *lidx_ = nullptr;
   uint8_t *ecolor_ = nullptr;
   uint32_t *dmask_ = nullptr;
+  // coupled meshes: coupling entries, the elastic displacement they read,
+  // and the interface accelerations this solver writes for the elastic one
+  int *cpl_li_ = nullptr, *cpl_g_ = nullptr, *cout_li_ = nullptr,
+      *cout_k_ = nullptr, *sh_iface_ = nullptr;
+  real *cpl_w_ = nullptr, *chi_dd_ = nullptr;
+  const real *ue_ = nullptr;
   real *geom_ = nullptr, *geom_e_ = nullptr, *mat_ = nullptr, *mat_e_ = nullptr,
        *kmr_ = nullptr, *stacey_ = nullptr, *src_L_ = nullptr, *stf_ = nullptr,
        *rent_h_ = nullptr, *rent_w_ = nullptr, *minv_ = nullptr,

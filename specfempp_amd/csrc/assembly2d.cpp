@@ -861,8 +861,10 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       t0s[s] = -static_cast<float>(1.2) * hdur + src.tshift[s];
       min_t0 = std::min(min_t0, t0s[s]);
     }
-    A.t0 = src.nsrc ? min_t0 : 0.0;
-    const float t0 = src.nsrc ? min_t0 : 0.0f;
+    if (opt.has_t0) // one medium of a coupled run: the run's start time
+      min_t0 = static_cast<float>(opt.t0);
+    A.t0 = (src.nsrc || opt.has_t0) ? min_t0 : 0.0;
+    const float t0 = (src.nsrc || opt.has_t0) ? min_t0 : 0.0f;
     const float pi = static_cast<float>(M_PI);
     for (int s = 0; s < src.nsrc; ++s) {
       const float tshift = t0s[s] - min_t0;

@@ -55,6 +55,10 @@ struct TileInfo {
   int32_t rec_off; // receiver entries on exclusive points
   int32_t nrec;
   int32_t flags;   // TILE_GEOM_ELEM | TILE_MAT_ELEM
+  int32_t cpl_off; // coupling entries (coupled meshes; 0 otherwise)
+  int32_t ncpl;
+  int32_t cout_off; // acoustic tiles: interface points whose new
+  int32_t ncout;    // acceleration the elastic update reads
 };
 
 // Tile flags: every element of the tile has element-constant geometry
@@ -100,6 +104,21 @@ struct HostAssembly2D {
   // rho^-1 grad f; [nrec*NPT][2] in FP32 and FP64.
   std::vector<float> rent_w;
   std::vector<double> rent_w64;
+
+  // Coupling with the other medium of a coupled mesh (empty otherwise),
+  // entries in tile order (TileInfo cpl_off / ncpl), added to the tile's
+  // accumulator with the sources:
+  //   elastic tile:  (li, k, w n0, w n1):   a += (w n) chi_dd[k]
+  //   acoustic tile: (li, g, w, n0, n1):    chi_dd += w (n . u_elastic[g])
+  // cpl_w holds 3 values per entry ((w n0, w n1, 0) or (w, n0, n1)).
+  // Acoustic tiles also store the new acceleration of their exclusive
+  // interface points (cout_li, cout_k; TileInfo cout_off / ncout) and the
+  // shared-point pass that of shared ones (sh_iface[j] = k or -1); k indexes
+  // the n_iface acoustic interface points.
+  std::vector<int32_t> cpl_li, cpl_idx;
+  std::vector<float> cpl_w;
+  std::vector<int32_t> cout_li, cout_k, sh_iface;
+  int n_iface = 0;
 
   std::vector<int32_t> ibool;   // [NPT][nspec_pad] global ids
   std::vector<uint16_t> lidx;   // [NPT][nspec_pad] tile-local ids
@@ -180,6 +199,10 @@ struct HostAssembly2D {
 // Assembly choices that change the layout, not the results' meaning.
 struct AssemblyOptions {
   int tile_elems = ELEM_TILE; // elements per tile (1..64)
+  // Start time of the run when the sources given are only part of it (one
+  // medium of a coupled mesh): the minimum t0 over every source.
+  bool has_t0 = false;
+  double t0 = 0.0;
 };
 
 // Builds the assembly; throws SmxError on invalid input.

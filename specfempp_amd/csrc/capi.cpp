@@ -8,6 +8,7 @@
 #include <thread>
 #include <vector>
 
+#include "coupled2d.hpp"
 #include "solver2d.hpp"
 #include "solver3d.hpp"
 #include "specfempp_amd.h"
@@ -67,10 +68,16 @@ int smx_ctx_create2d(const smx_mesh2d *mesh, const smx_sources2d *sources,
     if (!mesh || !sources || !receivers || !out)
       throw smx::SmxError(SMX_ERR_ARG, "null argument");
     *out = nullptr;
-    auto a = std::make_unique<smx::HostAssembly2D>();
-    smx::assemble2d(*mesh, *sources, *receivers, dt, nstep, *a);
     auto ctx = std::make_unique<smx_ctx>();
-    ctx->solver = smx::make_solver2d(std::move(a), precision);
+    if (smx::mesh_is_coupled2d(*mesh)) {
+      auto c = std::make_unique<smx::CoupledAssembly2D>();
+      smx::assemble_coupled2d(*mesh, *sources, *receivers, dt, nstep, {}, *c);
+      ctx->solver = smx::make_coupled_solver2d(std::move(c), precision, 8);
+    } else {
+      auto a = std::make_unique<smx::HostAssembly2D>();
+      smx::assemble2d(*mesh, *sources, *receivers, dt, nstep, *a);
+      ctx->solver = smx::make_solver2d(std::move(a), precision);
+    }
     *out = ctx.release();
   });
 }
@@ -98,12 +105,21 @@ int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
     if (!mesh || !sources || !receivers || !out)
       throw smx::SmxError(SMX_ERR_ARG, "null argument");
     *out = nullptr;
-    auto a = std::make_unique<smx::HostAssembly2D>();
-    smx::assemble2d(*mesh, *sources, *receivers, dt, nstep, *a,
-                    assembly_options(opts));
     auto ctx = std::make_unique<smx_ctx>();
-    ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts),
-                                      geometry_from_nodes(opts));
+    if (smx::mesh_is_coupled2d(*mesh)) {
+      auto c = std::make_unique<smx::CoupledAssembly2D>();
+      smx::assemble_coupled2d(*mesh, *sources, *receivers, dt, nstep,
+                              assembly_options(opts), *c);
+      ctx->solver = smx::make_coupled_solver2d(std::move(c), precision,
+                                               graph_steps(opts));
+    } else {
+      auto a = std::make_unique<smx::HostAssembly2D>();
+      smx::assemble2d(*mesh, *sources, *receivers, dt, nstep, *a,
+                      assembly_options(opts));
+      ctx->solver = smx::make_solver2d(std::move(a), precision,
+                                       graph_steps(opts),
+                                       geometry_from_nodes(opts));
+    }
     *out = ctx.release();
   });
 }
@@ -210,7 +226,8 @@ int smx_ctx_components(const smx_ctx *ctx, int *ncomp) {
       *ncomp = 3;
       return;
     }
-    *ncomp = S(const_cast<smx_ctx *>(ctx)).assembly().medium == 1 ? 1 : 2;
+    const int md = S(const_cast<smx_ctx *>(ctx)).assembly().medium;
+    *ncomp = md == 1 ? 1 : (md == 2 ? 0 : 2); // 0: one field per medium
   });
 }
 
@@ -374,6 +391,9 @@ int smx_get_ibool(smx_ctx *ctx, int *host_out) {
   return guarded([&] {
     if (!host_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
     const auto &A = S(ctx).assembly();
+    if (A.medium == 2)
+      throw smx::SmxError(SMX_ERR_STATE,
+                          "a coupled mesh numbers each medium separately");
     for (int e = 0; e < A.nspec; ++e)
       for (int p = 0; p < smx::NPT; ++p)
         host_out[size_t(e) * smx::NPT + p] =

@@ -41,6 +41,8 @@
 
 namespace smx {
 
+using StreamSwap = StreamSwapT<hipStream_t>;
+
 #define SMX_HIP(call)                                                          \
   do {                                                                         \
     hipError_t err_ = (call);                                                  \
@@ -229,7 +231,9 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
            real *__restrict__ seis_terms, int nentries, SeisTypes types,
            int ntypes, int nsbs, int *__restrict__ step_ptr, int istep,
            int ld, int sld, real dt, real dto2, real dt2o2, QuadArgs<real> q,
-           void *__restrict__ rec, long rec_stride, int every) {
+           void *__restrict__ rec, long rec_stride, int every,
+           const int *__restrict__ cpl_li, const int *__restrict__ cpl_k,
+           const real *__restrict__ cpl_w, const real *__restrict__ chi_dd) {
   using V2 = typename Vec2<real>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V2 *s_u = reinterpret_cast<V2 *>(lds_raw);
@@ -291,6 +295,18 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
       V2 acc = s_a[l];
       acc.x += stf[(size_t(step) * nsrc_total + s) * 2 + 0] * src_L[2 * k];
       acc.y += stf[(size_t(step) * nsrc_total + s) * 2 + 1] * src_L[2 * k + 1];
+      s_a[l] = acc;
+    }
+  // coupling with the fluid of a coupled mesh: a += (w n) chi_dd, the
+  // acoustic acceleration of this step (include/medium/dim2/coupling_terms/
+  // elastic_acoustic.hpp:13-26), entries in tile order
+  if (tid == 0)
+    for (int k = T.cpl_off; k < T.cpl_off + T.ncpl; ++k) {
+      const int l = cpl_li[k];
+      const real chi = chi_dd[cpl_k[k]];
+      V2 acc = s_a[l];
+      acc.x += cpl_w[3 * k] * chi;
+      acc.y += cpl_w[3 * k + 1] * chi;
       s_a[l] = acc;
     }
 
@@ -1197,6 +1213,9 @@ public:
     rent_h_ = upload_real(A.rent_h);
     minv_ = alloc<real>(n2_);
     part_ = alloc<real>(2 * std::max<size_t>(1, A.sh_list.size()));
+    cpl_li_ = upload_raw<int>(A.cpl_li);
+    cpl_k_ = upload_raw<int>(A.cpl_idx);
+    cpl_w_ = upload_real(A.cpl_w);
     lds_bytes_ = tile_lds_bytes<real>(std::max(1, A.nloc_max));
     SMX_HIP(hipFuncSetAttribute(
         reinterpret_cast<const void *>(&k_tile<real, MODE_STEP>),
@@ -1360,6 +1379,29 @@ public:
   }
 
   void synchronize() override { SMX_HIP(hipStreamSynchronize(stream_)); }
+
+  // ---- coupled meshes (the driver enqueues both media on its stream)
+  void bind_coupling(void *, void *chi_dd) override {
+    if (sizeof(real) != 4)
+      throw SmxError(SMX_ERR_ARG, "coupled meshes run in FP32");
+    chi_dd_ = static_cast<const real *>(chi_dd);
+  }
+  void *coupling_field() override { return field(SMX_FIELD_FORWARD).u; }
+  void enqueue_begin(void *stream, int istep0) override {
+    require_init();
+    StreamSwap sw(stream_, static_cast<hipStream_t>(stream));
+    k_set_step<<<dim3(1), dim3(64), 0, stream_>>>(step_, istep0);
+    SMX_HIP(hipGetLastError());
+    launch_predictor(field(SMX_FIELD_FORWARD));
+  }
+  void enqueue_step(void *stream, bool final) override {
+    StreamSwap sw(stream_, static_cast<hipStream_t>(stream));
+    Field &F = field(SMX_FIELD_FORWARD);
+    if (final)
+      fused_step<MODE_FINAL>(F);
+    else
+      fused_step<MODE_STEP>(F);
+  }
 
   void get_seismograms(void *host) override {
     const HostAssembly2D &A = *asm_;
@@ -1735,7 +1777,8 @@ private:
         src_idx_, src_L_, stf_, std::max(1, A.nsrc), trec_li_, trec_entry_,
         rent_h_, terms, A.nrec * NPT, stypes_, ntypes, A.nsbs, step_ptr, istep,
         A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, rec,
-        rec_stride(), std::max(1, rec_every_));
+        rec_stride(), std::max(1, rec_every_), cpl_li_, cpl_k_, cpl_w_,
+        chi_dd_);
     SMX_HIP(hipGetLastError());
   }
   template <int MODE>
@@ -1999,6 +2042,11 @@ private:
        *seis_terms_ = nullptr, *scratch_u_ = nullptr, *scratch_v_ = nullptr,
        *scratch_part_ = nullptr;
   int *shp_off_ = nullptr, *shp_pos_ = nullptr;
+  // coupled meshes: entries of the fluid coupling and the fluid's
+  // interface accelerations (owned by the acoustic solver)
+  int *cpl_li_ = nullptr, *cpl_k_ = nullptr;
+  real *cpl_w_ = nullptr;
+  const real *chi_dd_ = nullptr;
   size_t lds_bytes_ = 0;
   Field fields_[3];
   QuadArgs<real> quad_;

@@ -50,6 +50,30 @@ public:
   // Geometry source of the tile kernels: 0 per point, 1 per element (some
   // tiles), 2 re-formed from the control nodes; mismatches as in 3D.
   virtual void geometry_mode(int *mode, long long *mismatches) const = 0;
+
+  // Coupled meshes: one solver per medium, driven step by step on the
+  // driver's HIP stream (passed as void*).  bind_coupling gives the elastic
+  // displacement (acoustic solver) and the buffer of the fluid's interface
+  // accelerations (both); coupling_field is this solver's displacement.
+  virtual void bind_coupling(void *, void *) {
+    throw SmxError(SMX_ERR_ARG, "this context does not take part in coupling");
+  }
+  virtual void *coupling_field() { return nullptr; }
+  virtual void enqueue_begin(void *, int) {
+    throw SmxError(SMX_ERR_ARG, "this context does not take part in coupling");
+  }
+  virtual void enqueue_step(void *, bool) {
+    throw SmxError(SMX_ERR_ARG, "this context does not take part in coupling");
+  }
+};
+
+// Swaps a solver's stream for the duration of a scope (coupled drivers
+// enqueue a medium's kernels on their own stream).
+template <typename S> struct StreamSwapT {
+  S &slot;
+  S saved;
+  StreamSwapT(S &s, S with) : slot(s), saved(s) { slot = with; }
+  ~StreamSwapT() { slot = saved; }
 };
 
 // graph_steps: time steps per replayed HIP graph; nodes_geometry: re-form

@@ -25,9 +25,11 @@ order the reference reads them (src/io/mesh/impl/fortran/dim2/mesh.cpp:26-215):
    (read_boundaries.cpp:150-200), the acoustic free surface (element, 1 for a
    corner node or 2 for an edge, two control-node ids; the entity type comes
    from comparing those ids with the element's corner nodes,
-   read_boundaries.cpp:236-300), then tangential nodes and the adjacency
-   graph, which are skipped.  Acoustic forcing, coupled interfaces and axial
-   elements are rejected (this build has no coupling).
+   read_boundaries.cpp:236-300), the fluid-solid edges (acoustic element,
+   elastic element; mesh.cpp:112-127, kept in ``info``: the coupling is
+   rebuilt from the elements' shared edges), then tangential nodes and the
+   adjacency graph, which are skipped.  Acoustic forcing, poroelastic
+   interfaces and axial elements are rejected.
 
 The reference adds corner entries to the absorbing list only when one
 element carries both a horizontal and a vertical absorbing edge; its test
@@ -189,8 +191,17 @@ def read_mesh_database_2d(path_or_bytes):
             raise ValueError("could not type an acoustic free-surface entity")
         free_ispec[k] = num - 1
         free_type[k] = hit[0]
-    if nelem_acforcing or n_fs or n_fp or n_sp or n_axis:
-        raise ValueError("acoustic forcing, coupled or axisymmetric meshes are not supported")
+    if nelem_acforcing or n_fp or n_sp or n_axis:
+        raise ValueError("acoustic forcing, poroelastic or axisymmetric meshes are not "
+                         "supported")
+    # fluid-solid edges (mesh.cpp:112-127): (acoustic element, elastic element)
+    # pairs; the coupling itself is rebuilt from the shared edges
+    fluid_solid = []
+    for _ in range(max(n_fs, 0)):
+        a, b = cur.next("ii")
+        if not (1 <= a <= nspec and 1 <= b <= nspec):
+            raise ValueError("fluid-solid element out of range")
+        fluid_solid.append((a - 1, b - 1))
     cur.next("ii")  # tangential-curve flags
     for _ in range(n_tangential):
         cur.next("dd")
@@ -218,5 +229,6 @@ def read_mesh_database_2d(path_or_bytes):
                   free_ispec=free_ispec, free_type=free_type)
     info = {"nspec": nspec, "npgeo": npgeo, "ngnod": ngnod, "numat": numat,
             "nelemabs": nelemabs, "nelem_acoustic_surface": nelem_acoustic_surface,
+            "fluid_solid": fluid_solid,
             "materials": mats, "nadjacency": nadj}
     return mesh, info

@@ -172,7 +172,7 @@ class Context:
 
     def get_field(self, which, field=SMX_FIELD_FORWARD):
         idx = {"u": SMX_U, "v": SMX_V, "a": SMX_A, "minv": SMX_MASS_INVERSE}[which]
-        out = np.empty((self.nglob, self.ncomp), dtype=self.dtype)
+        out = np.empty((self.nglob, max(self.ncomp, 1)), dtype=self.dtype)
         check(self.lib.smx_get_field(self.h, field, idx, _p(out)))
         return out if self.ncomp > 1 else out[:, 0]
 

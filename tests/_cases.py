@@ -103,6 +103,107 @@ def acoustic_box(nx=24, nz=18, h=50.0, nstep=300, dt=1e-3, nrec=8, seed=3,
                     seismo_types=("displacement", "velocity", "acceleration", "pressure"))
 
 
+def coupled_fixture_case():
+    """AcousticElasticCoupledDomain restated from its Par_file and
+    topography.dat (the fixture ships no database): a 6400x4800 m box of
+    144x108 nine-node quads, rows 1-54 elastic (rho 2500, vp 3400, vs 1963),
+    rows 55-108 acoustic (rho 1020, vp 1500), flat interface at z = 2400 m;
+    no absorbing edge, so the acoustic elements' outer edges (top, left,
+    right) are acoustic free surfaces (as the mesher writes them, cf. the
+    acoustic fixtures' databases); one Ricker force (f0 10 Hz, factor 1e9) at
+    (1575, 2900) in the water, dt = 0.85e-3, 600 steps; 10 stations."""
+    from specfempp_amd.mesh import BOTTOM, LEFT, RIGHT, TOP
+    g = np.load(os.path.join(GOLDEN, "specfem2d_AcousticElasticCoupledDomain.npz"))
+    nx, nz = 144, 108
+    water = (lambda jx, jz: np.asarray(jz) >= 54)
+
+    def pick(a, b):
+        return lambda jx, jz: np.where(water(jx, jz), b, a)
+
+    mesh = box_mesh(0.0, 6400.0, 0.0, 4800.0, nx, nz, ngnod=9, absorbing=(),
+                    rho=pick(2500.0, 1020.0), vp=pick(3400.0, 1500.0), vs=pick(1963.0, 0.0))
+    ids = np.arange(nx * nz)
+    jz, jx = ids // nx, ids % nx
+    fi, ft = [], []
+    for t, on in ((TOP, jz == nz - 1), (LEFT, (jx == 0) & (jz >= 54)),
+                  (RIGHT, (jx == nx - 1) & (jz >= 54))):
+        fi.append(ids[on])
+        ft.append(np.full(int(on.sum()), t))
+    mesh.free_ispec = np.concatenate(fi).astype(np.int32)
+    mesh.free_type = np.concatenate(ft).astype(np.int32)
+    mesh.validate()
+    src = Sources2D.ricker([1575.0], [2900.0], f0=10.0, factor=1e9)
+    rec = Receivers2D(x=g["x"], z=g["z"], names=tuple(g["names"]))
+    cfg = Config2D("AcousticElasticCoupledDomain", mesh, src, rec, 0.85e-3, 600,
+                   seismo_types=("displacement",))
+    return cfg, g["traces"]
+
+
+def coupled_variant2_case():
+    """AcousticElasticCoupledDomainVariant2 from the reference mesher's
+    database: the same box with the water below (rows 1-54 acoustic) and the
+    solid above, free surfaces on the water's outer edges, the source (force,
+    f0 10 Hz, factor 1e9) at (1575, 2900) in the solid, 11 stations."""
+    from specfempp_amd.meshdb import read_mesh_database_2d
+    name = "AcousticElasticCoupledDomainVariant2"
+    g = np.load(os.path.join(GOLDEN, f"specfem2d_{name}.npz"))
+    mesh, info = read_mesh_database_2d(os.path.join(GOLDEN, f"specfem2d_{name}.database.bin.gz"))
+    src = Sources2D.ricker([1575.0], [2900.0], f0=10.0, factor=1e9)
+    rec = Receivers2D(x=g["x"], z=g["z"], names=tuple(g["names"]))
+    cfg = Config2D(name, mesh, src, rec, 0.85e-3, 600, seismo_types=("displacement",),
+                   meta={"database": info})
+    return cfg, g["traces"]
+
+
+def coupled_box(nx=20, nz=16, h=50.0, nstep=400, dt=1e-3, vertical=True, nrec=8, seed=11,
+                ngnod=4):
+    """Small coupled box with heterogeneous media: water (rho 1020, vp 1500,
+    +-3 % seeded) on the left half (vertical interface) or the top half,
+    solid (rho 2400, vp 3000, vs 1700, +-3 %) elsewhere; the water's top
+    edges are an acoustic free surface; Stacey on the bottom, left and right
+    sides; a Ricker source in each medium (the solid one oblique); receivers
+    on a line crossing the interface."""
+    from specfempp_amd.mesh import TOP, acoustic_moduli, elastic_moduli, gll_coordinates
+    L, H = nx * h, nz * h
+
+    def wet(jx, jz):
+        return (np.asarray(jx) < nx // 2) if vertical else (np.asarray(jz) >= nz // 2)
+
+    mesh = box_mesh(0.0, L, 0.0, H, nx, nz, ngnod=ngnod,
+                    absorbing=("bottom", "right", "left"),
+                    rho=lambda jx, jz: np.where(wet(jx, jz), 1020.0, 2400.0),
+                    vp=lambda jx, jz: np.where(wet(jx, jz), 1500.0, 3000.0),
+                    vs=lambda jx, jz: np.where(wet(jx, jz), 0.0, 1700.0))
+    rng = np.random.default_rng(seed)
+    xz = gll_coordinates(mesh)
+    f = np.sin(rng.uniform(1, 3) * np.pi * xz[..., 0] / L + rng.uniform(0, 6)) * \
+        np.cos(rng.uniform(1, 3) * np.pi * xz[..., 1] / H)
+    fluid = mesh.acoustic[:, None, None]
+    rho = (np.where(fluid, 1020.0, 2400.0) * (1 + 0.03 * f)).astype(np.float32)
+    vp = (np.where(fluid, 1500.0, 3000.0) * (1 - 0.03 * f)).astype(np.float32)
+    vs = np.where(fluid, 0.0, 1700.0 * (1 + 0.02 * f)).astype(np.float32)
+    ke, me, _ = elastic_moduli(rho, vp, vs)
+    ka, _, _ = acoustic_moduli(rho, vp)
+    mesh.kappa = np.where(fluid, ka, ke).astype(np.float32)
+    mesh.mu = np.where(fluid, np.float32(0), me).astype(np.float32)
+    mesh.rho = rho
+    ids = np.arange(nx * nz)
+    top_wet = (ids // nx == nz - 1) & mesh.acoustic
+    mesh.free_ispec = ids[top_wet].astype(np.int32)
+    mesh.free_type = np.full(int(top_wet.sum()), TOP, np.int32)
+    mesh.validate()
+    if vertical:
+        src = Sources2D.ricker([0.3 * L, 0.72 * L], [0.55 * H, 0.4 * H], f0=15.0, factor=1e9,
+                               angle=np.array([0.0, 30.0]))
+        rec = Receivers2D(x=np.linspace(0.1 * L, 0.9 * L, nrec), z=np.full(nrec, 0.62 * H))
+    else:
+        src = Sources2D.ricker([0.3 * L, 0.7 * L], [0.7 * H, 0.3 * H], f0=15.0, factor=1e9,
+                               angle=np.array([0.0, 30.0]))
+        rec = Receivers2D(x=np.full(nrec, 0.45 * L), z=np.linspace(0.1 * H, 0.9 * H, nrec))
+    return Config2D(f"coupled_box_{'v' if vertical else 'h'}", mesh, src, rec, dt, nstep,
+                    seismo_types=("displacement", "velocity", "acceleration"))
+
+
 def reference_metric(ours, traces):
     """Worst per-station sum|s - ref| / sum|ref| over components and types
     (tests/unit-tests/displacement_tests/Newmark/dim2/newmark_tests.cpp:337-362).

@@ -7,9 +7,12 @@ Source directories (read as text, in this container only):
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousAcousticDomain
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousAcousticDomainDirichletBC
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousAcousticDomainCompositeStaceyDirichletBC
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/AcousticElasticCoupledDomain
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/AcousticElasticCoupledDomainVariant2
 
 Each case's mesh database (database.bin, written by the reference's mesher)
-is also copied, gzip-compressed, as specfem2d_<case>.database.bin.gz.
+is also copied, gzip-compressed, as specfem2d_<case>.database.bin.gz (the
+coupled case ships no database; its mesh is restated from its Par_file).
 
 Each fixture holds the station table (STATIONS: name, network, x, z), the
 sample times, and traces[type][station][component][t] for the seismogram types
@@ -30,17 +33,19 @@ import numpy as np
 BASE = "tests/unit-tests/displacement_tests/Newmark/serial/dim2"
 CASES = ["HomogeneousElasticDomain", "HomogeneousElasticDomainStaceyDirichletBC",
          "HomogeneousAcousticDomain", "HomogeneousAcousticDomainDirichletBC",
-         "HomogeneousAcousticDomainCompositeStaceyDirichletBC"]
+         "HomogeneousAcousticDomainCompositeStaceyDirichletBC", "AcousticElasticCoupledDomain",
+         "AcousticElasticCoupledDomainVariant2"]
 
 
 def main(ref="/root/reference"):
     out_dir = os.path.dirname(os.path.abspath(__file__))
     for case in CASES:
         d = os.path.join(ref, BASE, case)
-        with open(os.path.join(d, "database.bin"), "rb") as fh:
-            db = fh.read()
-        with open(os.path.join(out_dir, f"specfem2d_{case}.database.bin.gz"), "wb") as fh:
-            fh.write(gzip.compress(db, 9, mtime=0))
+        if os.path.exists(os.path.join(d, "database.bin")):  # the coupled case ships none
+            with open(os.path.join(d, "database.bin"), "rb") as fh:
+                db = fh.read()
+            with open(os.path.join(out_dir, f"specfem2d_{case}.database.bin.gz"), "wb") as fh:
+                fh.write(gzip.compress(db, 9, mtime=0))
         names, nets, xs, zs = [], [], [], []
         with open(os.path.join(d, "STATIONS")) as fh:
             for line in fh:

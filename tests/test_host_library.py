@@ -129,12 +129,20 @@ def test_acoustic_set_up_errors_are_host_side():
     from specfempp_amd.config import Receivers2D, Sources2D
     from specfempp_amd.solver import Context
     c = _acoustic_cfg()
-    # mixed media need the coupling, which is not built
+    # a coupled mesh (one solid element at the bottom-left corner): a
+    # free-surface entity on the solid element, or pressure asked of a
+    # station in the solid, are rejected before any device call
     mu = c.mesh.mu.copy()
     mu[0] = 1e9
-    mixed = dataclasses.replace(c.mesh, mu=mu)
-    with pytest.raises(_lib.SmxError, match="coupling"):
+    mixed = dataclasses.replace(c.mesh, mu=mu, free_ispec=np.array([0], np.int32),
+                                free_type=np.array([1], np.int32))
+    with pytest.raises(_lib.SmxError, match="elastic element"):
         Context(mixed, c.sources, c.receivers, c.dt, c.nstep)
+    mixed = dataclasses.replace(mixed, free_ispec=np.zeros(0, np.int32),
+                                free_type=np.zeros(0, np.int32))
+    solid_station = Receivers2D(x=np.array([20.0]), z=np.array([20.0]))
+    with pytest.raises(_lib.SmxError, match="pressure"):
+        Context(mixed, c.sources, solid_station, c.dt, c.nstep, seismo_types=("pressure",))
     # pressure seismograms need acoustic elements
     e = homogeneous_box(nstep=5)
     with pytest.raises(_lib.SmxError, match="pressure"):
