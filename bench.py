@@ -122,6 +122,57 @@ def acoustic_leg(nx=800, nz=600, nstep=2000, warmup=100, cpu_steps=20):
     return out
 
 
+def coupled_leg(nx=800, nz=600, nstep=2000, warmup=100):
+    """Coupled elastic-acoustic forward: the 800x600 layered workload with its
+    top 100 element rows turned into water (rho 1020, vp 1500: a 5 km ocean
+    over the layered solid), the water's top edges an acoustic free
+    surface, Stacey on the other sides, the 8 Ricker sources moved 0.5-1.5
+    km below the seabed, 500 receivers 200 m above it (displacement =
+    rho^-1 grad chi there), dt = 4e-4."""
+    import numpy as np
+    from specfempp_amd.config import Receivers2D, layered_box
+    from specfempp_amd.mesh import TOP, acoustic_moduli
+    from specfempp_amd.solver import Context
+
+    cfg = layered_box(nx=nx, nz=nz, nstep=warmup + nstep)
+    m = cfg.mesh
+    water_rows = nz // 6
+    ids = np.arange(m.nspec)
+    wet = ids // nx >= nz - water_rows
+    kw, _, rw = acoustic_moduli(np.float32(1020.0), np.float32(1500.0))
+    m.kappa[wet], m.mu[wet], m.rho[wet] = kw, 0.0, rw
+    top = wet & (ids // nx == nz - 1)
+    m.free_ispec = ids[top].astype(np.int32)
+    m.free_type = np.full(int(top.sum()), TOP, np.int32)
+    seabed = (nz - water_rows) * 50.0
+    # the 8 seeded sources moved 0.5-1.5 km below the seabed, receivers
+    # 200 m above it, so that waves cross the interface within the run
+    cfg.sources.z = seabed - 500.0 - (cfg.sources.z % 1000.0)
+    cfg.receivers = Receivers2D(x=cfg.receivers.x, z=np.full(len(cfg.receivers.x),
+                                                              seabed + 200.0))
+    ctx = Context(m, cfg.sources, cfg.receivers, cfg.dt, warmup + nstep)
+    ctx.initialize()
+    ctx.run_forward(0, warmup)
+    ctx.synchronize()
+    t = time.perf_counter()
+    ctx.run_forward(warmup, nstep)
+    ctx.synchronize()
+    wall = time.perf_counter() - t
+    seis = ctx.seismograms()
+    out = {"workload": f"2D coupled forward, {nx}x{nz} quads of 50 m: top {water_rows} rows "
+                       f"water over the 6 seeded layers, free sea surface, Stacey on the "
+                       f"other sides, 8 Ricker sources 0.5-1.5 km below the seabed, 500 "
+                       f"receivers 200 m above it, dt=4e-4, {nstep} steps",
+           "nspec": ctx.nspec, "nglob_both_media": ctx.nglob, "layout": dict(ctx.layout),
+           "wall_s": wall, "ms_per_step": wall * 1e3 / nstep,
+           "point_updates_per_s": ctx.nglob * nstep / wall,
+           "ms_per_step_event": ctx.last_run_ms() / nstep,
+           "seismograms_finite": bool(np.isfinite(seis).all()),
+           "max_abs_seismogram": float(np.abs(seis).max())}
+    del ctx
+    return out
+
+
 def tile3d_kernel_bytes(nspec, layout, nstacey, assemble=False):
     """Algorithmic bytes of one 3D tile pass (k3_tile, one time step, FP32;
     `assemble`: the timing variant that stores a = (f - Ku)/M instead of
@@ -534,7 +585,10 @@ def main():
                     help="skip the 3D leg (64^3 hex, 4000 steps)")
     ap.add_argument("--no-acoustic", action="store_true",
                     help="skip the acoustic leg (800x600 fluid layers, 2000 steps)")
-    ap.add_argument("--only-leg", choices=("adjoint", "batch", "hex3d", "errors", "acoustic"),
+    ap.add_argument("--no-coupled", action="store_true",
+                    help="skip the coupled elastic-acoustic leg (800x600, 2000 steps)")
+    ap.add_argument("--only-leg", choices=("adjoint", "batch", "hex3d", "errors", "acoustic",
+                                           "coupled"),
                     default=None,
                     help="profiling aid: run only this leg and print its JSON")
     ap.add_argument("--hex-steps", type=int, default=4000,
@@ -565,7 +619,8 @@ def main():
                "hex3d": lambda: hex_leg(nstep=args.hex_steps),
                "errors": errors_leg,
                "acoustic": lambda: acoustic_leg(cpu_steps=0 if args.no_cpu_baseline
-                                                else 20)}[args.only_leg]
+                                                else 20),
+               "coupled": coupled_leg}[args.only_leg]
         print(json.dumps({args.only_leg: leg()}), flush=True)
         return
     if args.gpus != 1:
@@ -650,6 +705,8 @@ def main():
         out["hex3d"] = hex_leg(nstep=args.hex_steps)
     if not args.no_acoustic:
         out["acoustic"] = acoustic_leg(cpu_steps=0 if args.no_cpu_baseline else 20)
+    if not args.no_coupled:
+        out["coupled"] = coupled_leg()
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(cfg, args.cpu_steps)
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(cfg)

@@ -182,3 +182,39 @@ def test_acoustic_execute_writes_pressure_seismograms(acoustic_run_dir):
     err = reference_error(s, _acoustic_traces(g))
     print("acoustic YAML run vs SPECFEM2D (reference metric):", err)
     assert err < 5e-4
+
+
+COUPLED = "AcousticElasticCoupledDomainVariant2"
+
+
+@pytest.mark.gpu
+def test_coupled_yaml_run_from_the_reference_database(tmp_path):
+    """The coupled fixture laid out as the reference's run (its mesher's
+    database, sources.yaml, STATIONS): _execute writes seismograms that meet
+    the SPECFEM2D traces."""
+    g = np.load(os.path.join(GOLDEN, f"specfem2d_{COUPLED}.npz"))
+    d = tmp_path / "fixture"
+    d.mkdir()
+    with open(d / "STATIONS", "w") as fh:
+        for n, x, z in zip(g["names"], g["x"], g["z"]):
+            fh.write(f"{n}    AA    {x:.7f}    {z:.7f}    0.0    0.0\n")
+    src = {"number-of-sources": 1,
+           "sources": [{"force": {"x": 1575.0, "z": 2900.0, "source_surf": False,
+                                  "angle": 0.0, "vx": 0.0, "vz": 0.0,
+                                  "Ricker": {"factor": 1e9, "tshift": 0.0, "f0": 10.0}}}]}
+    with open(d / "sources.yaml", "w") as fh:
+        yaml.safe_dump(src, fh)
+    shutil.copy(os.path.join(GOLDEN, f"specfem2d_{COUPLED}.database.bin.gz"),
+                d / "database.bin.gz")
+    text = CONFIG.replace("dt: 1.1e-3", "dt: 0.85e-3").replace("nstep: 300", "nstep: 600")
+    assert program._initialize([]) is True
+    try:
+        assert program._execute(text, "", base_dir=str(tmp_path)) is True
+    finally:
+        assert program._finalize() is True
+    st = [("AA", n) for n in g["names"]]
+    t, s = read_seismograms(tmp_path / "OUTPUT_FILES", st, ("displacement",))
+    np.testing.assert_allclose(t, g["times"], atol=1e-6)
+    err = reference_error(s, np.transpose(g["traces"], (3, 0, 1, 2)))
+    print("coupled YAML run vs SPECFEM2D (reference metric):", err)
+    assert err < 1e-4
