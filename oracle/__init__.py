@@ -253,7 +253,7 @@ def _load3():
 
 
 GET3 = {"minv": 0, "ibool": 1, "jacobian": 2, "source_array": 3, "receiver_h": 4,
-        "u": 5, "v": 6, "a": 7}
+        "u": 5, "v": 6, "a": 7, "inverse_jacobian": 8}
 
 
 class Oracle3D:

@@ -927,7 +927,8 @@ int oracle3d_run(void *hv, int nsteps, int ntypes, const int *types,
 
 // Arrays for closed-form checks: 0 minv [nglob][3], 1 ibool [nspec][125],
 // 2 jacobian [nspec][125], 3 source array [nsrc][3][125], 4 receiver
-// interpolants [nrec][125], 5 u, 6 v, 7 a (current fields).  Returns the
+// interpolants [nrec][125], 5 u, 6 v, 7 a (current fields), 8 inverse
+// Jacobian [nloc][9].  Returns the
 // length; copies when out is non-null.
 long oracle3d_get(void *hv, int what, double *out) {
   auto *h = static_cast<Handle3 *>(hv);
@@ -944,6 +945,12 @@ long oracle3d_get(void *hv, int what, double *out) {
     case 5: v.assign(o->u.begin(), o->u.end()); break;
     case 6: v.assign(o->v.begin(), o->v.end()); break;
     case 7: v.assign(o->a.begin(), o->a.end()); break;
+    case 8: // inverse-Jacobian terms [nloc][9] (xi, eta, gamma) x (x, y, z)
+      for (auto &j : o->jac)
+        for (double t : {j.xix, j.xiy, j.xiz, j.etax, j.etay, j.etaz, j.gax,
+                         j.gay, j.gaz})
+          v.push_back(t);
+      break;
     default: return -1;
     }
     if (out)

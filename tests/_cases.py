@@ -204,6 +204,75 @@ def coupled_box(nx=20, nz=16, h=50.0, nstep=400, dt=1e-3, vertical=True, nrec=8,
                     seismo_types=("displacement", "velocity", "acceleration"))
 
 
+def warped(cfg, amp=0.18, topo=0.25):
+    """cfg with its mesh's control nodes moved smoothly: interior nodes by up
+    to ``amp`` element sizes along a sine pattern (edges stay on the box
+    sides) and the top surface by a ``topo``-element-size sinusoidal
+    topography (the interior follows with depth); elements become curved
+    (Q9) or skewed (Q4) so that the per-point geometry, curved Stacey edges
+    and free surfaces, interface normals and point location are all
+    exercised."""
+    import dataclasses
+    m = cfg.mesh
+    c = m.coord.copy()
+    x0, x1 = c[:, 0].min(), c[:, 0].max()
+    z0, z1 = c[:, 1].min(), c[:, 1].max()
+    L, H = x1 - x0, z1 - z0
+    h = L / max(m.nx, 1) if m.nx else L / 20
+    u, w = (c[:, 0] - x0) / L, (c[:, 1] - z0) / H
+    dx = amp * h * np.sin(np.pi * u) * np.sin(2 * np.pi * w)
+    dz = amp * h * np.sin(2 * np.pi * u) * np.sin(np.pi * w) \
+        + topo * h * np.sin(3 * np.pi * u) * w
+    c[:, 0] += dx
+    c[:, 1] += dz
+    mesh = dataclasses.replace(m, coord=c)
+    # stations and sources follow the warp: each keeps its element and its
+    # (xi, gamma) there, so that no point lands on an element edge, where the
+    # located element (and the gradient-based fluid seismograms) would hinge
+    # on rounding
+    def follow(x, z):
+        return map_points(mesh, *locate_reference(m, np.asarray(x), np.asarray(z)))
+    rx, rz = follow(cfg.receivers.x, cfg.receivers.z)
+    sx, sz = follow(cfg.sources.x, cfg.sources.z)
+    rec = dataclasses.replace(cfg.receivers, x=rx, z=rz)
+    src = dataclasses.replace(cfg.sources, x=sx, z=sz)
+    return dataclasses.replace(cfg, mesh=mesh, receivers=rec, sources=src,
+                               name=cfg.name + "_warped")
+
+
+def _shape(xi, ga, ngnod):
+    if ngnod == 4:
+        return np.stack([0.25 * (xi - 1) * (ga - 1), -0.25 * (xi + 1) * (ga - 1),
+                         0.25 * (xi + 1) * (ga + 1), -0.25 * (xi - 1) * (ga + 1)], axis=-1)
+    return np.stack([0.25 * (xi - 1) * xi * (ga - 1) * ga, 0.25 * (xi + 1) * xi * (ga - 1) * ga,
+                     0.25 * (xi + 1) * xi * (ga + 1) * ga, 0.25 * (xi - 1) * xi * (ga + 1) * ga,
+                     0.5 * (1 - xi * xi) * (ga - 1) * ga, 0.5 * (xi + 1) * xi * (1 - ga * ga),
+                     0.5 * (1 - xi * xi) * (ga + 1) * ga, 0.5 * (xi - 1) * xi * (1 - ga * ga),
+                     (1 - xi * xi) * (1 - ga * ga)], axis=-1)
+
+
+def locate_reference(mesh, x, z):
+    """(element, xi, gamma) of points in an axis-aligned box mesh (test set-up
+    only), nudged 10 % of an element inside so that no point sits on an
+    edge."""
+    c = mesh.coord[mesh.knods[:, :4]]
+    lo, hi = c.min(axis=1), c.max(axis=1)
+    e = np.array([int(np.nonzero((lo[:, 0] <= xx) & (xx <= hi[:, 0]) & (lo[:, 1] <= zz)
+                                 & (zz <= hi[:, 1]))[0][0]) for xx, zz in zip(x, z)])
+    xi = np.clip(2 * (x - lo[e, 0]) / (hi[e, 0] - lo[e, 0]) - 1, -0.9, 0.9)
+    ga = np.clip(2 * (z - lo[e, 1]) / (hi[e, 1] - lo[e, 1]) - 1, -0.9, 0.9)
+    return e, xi, ga
+
+
+def map_points(mesh, e, xi, ga):
+    """Physical coordinates of (element, xi, gamma) through the element's
+    shape functions (double precision)."""
+    N = _shape(np.asarray(xi), np.asarray(ga), mesh.ngnod)  # (n, ngnod)
+    c = mesh.coord[mesh.knods[e]]                           # (n, ngnod, 2)
+    p = np.einsum("nk,nkd->nd", N, c)
+    return p[:, 0], p[:, 1]
+
+
 def reference_metric(ours, traces):
     """Worst per-station sum|s - ref| / sum|ref| over components and types
     (tests/unit-tests/displacement_tests/Newmark/dim2/newmark_tests.cpp:337-362).
