@@ -273,6 +273,54 @@ def map_points(mesh, e, xi, ga):
     return p[:, 0], p[:, 1]
 
 
+_HEX_SIGNS = np.array([[-1, -1, -1], [1, -1, -1], [1, 1, -1], [-1, 1, -1],
+                       [-1, -1, 1], [1, -1, 1], [1, 1, 1], [-1, 1, 1]], dtype=float)
+
+
+def warped3d(cfg, amp=0.15, topo=0.2):
+    """3D counterpart of ``warped``: hex8 control nodes moved by a smooth
+    interior pattern (up to ``amp`` element sizes, box faces kept planar
+    except the top) and a ``topo``-element-size topography on top; sources
+    and stations keep their element and (xi, eta, gamma), nudged 10 % of an
+    element inside, and are mapped through the trilinear shape functions."""
+    import dataclasses
+    m = cfg.mesh
+    c0 = m.coord
+    lo, hi = c0.min(axis=0), c0.max(axis=0)
+    ext = hi - lo
+    n = np.array(m.shape, dtype=float) if m.shape and m.shape[0] else None
+    h = float(np.min(ext / n)) if n is not None else float(np.min(ext)) / 4
+    u = (c0 - lo) / ext
+    s = np.sin
+    d = np.zeros_like(c0)
+    d[:, 0] = amp * h * s(np.pi * u[:, 0]) * s(np.pi * u[:, 1]) * s(2 * np.pi * u[:, 2])
+    d[:, 1] = amp * h * s(2 * np.pi * u[:, 0]) * s(np.pi * u[:, 1]) * s(np.pi * u[:, 2])
+    d[:, 2] = amp * h * s(np.pi * u[:, 0]) * s(2 * np.pi * u[:, 1]) * s(np.pi * u[:, 2]) \
+        + topo * h * s(3 * np.pi * u[:, 0]) * np.cos(2 * np.pi * u[:, 1]) * u[:, 2]
+    c = c0 + d
+    corners = c0[m.knods]                   # (nspec, 8, 3) of the box
+    elo, ehi = corners.min(axis=1), corners.max(axis=1)
+
+    def follow(x, y, z):
+        p = np.stack([x, y, z], axis=1)
+        out = []
+        for q in p:
+            e = int(np.nonzero(np.all((elo <= q) & (q <= ehi), axis=1))[0][0])
+            r = np.clip(2 * (q - elo[e]) / (ehi[e] - elo[e]) - 1, -0.9, 0.9)
+            N = np.prod(1 + _HEX_SIGNS * r, axis=1) / 8.0
+            out.append(N @ c[m.knods[e]])
+        out = np.array(out)
+        return out[:, 0], out[:, 1], out[:, 2]
+
+    mesh = dataclasses.replace(m, coord=c)
+    rx, ry, rz = follow(cfg.receivers.x, cfg.receivers.y, cfg.receivers.z)
+    sx, sy, sz = follow(cfg.sources.x, cfg.sources.y, cfg.sources.z)
+    return dataclasses.replace(cfg, mesh=mesh,
+                               receivers=dataclasses.replace(cfg.receivers, x=rx, y=ry, z=rz),
+                               sources=dataclasses.replace(cfg.sources, x=sx, y=sy, z=sz),
+                               name=cfg.name + "_warped")
+
+
 def reference_metric(ours, traces):
     """Worst per-station sum|s - ref| / sum|ref| over components and types
     (tests/unit-tests/displacement_tests/Newmark/dim2/newmark_tests.cpp:337-362).
