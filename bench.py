@@ -624,8 +624,10 @@ def main():
         print(json.dumps({args.only_leg: leg()}), flush=True)
         return
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is "
-                                   "design-only (see DESIGN.md)", "n_gpus": args.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU: sharding the mesh over "
+                                   "several GPUs (tiles per rank, halo exchange of the "
+                                   "shared points) is designed but not built",
+                          "n_gpus": args.gpus}))
         sys.exit(2)
 
     import torch
