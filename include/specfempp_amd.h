@@ -40,6 +40,13 @@ extern "C" {
 /* Source time functions (core/specfem/source_time_functions). */
 #define SMX_STF_RICKER 0
 #define SMX_STF_DIRAC 1 /* Gaussian of f0 = 1 / (10 dt) (dirac.cpp:23-35) */
+/* factor (1 + erf(t / (hdur / 1.628))) / 2, hdur raised to 5 dt, start time
+ * -2 hdur + tshift (heaviside.cpp:10-78) */
+#define SMX_STF_HEAVISIDE 2
+
+/* 2D source kinds. */
+#define SMX_SRC_FORCE 0
+#define SMX_SRC_MOMENT 1
 
 /* Field components selectable by smx_get_field / smx_set_field. */
 #define SMX_U 0
@@ -83,17 +90,23 @@ typedef struct {
 } smx_mesh2d;
 
 /* Point forces (core/specfem/source/dim2/vector_source/force_source.cpp)
- * with a Ricker or Dirac time function (core/specfem/source_time_functions/
- * ricker.cpp, dirac.cpp).  In acoustic elements the force has one component
- * and the angle is unused (force_source.cpp:36-40). */
+ * and moment tensors (core/specfem/source/dim2/tensor_source/
+ * moment_tensor_source.cpp, elastic elements only) with a Ricker, Dirac or
+ * Heaviside time function (core/specfem/source_time_functions/ricker.cpp,
+ * dirac.cpp, heaviside.cpp).  In acoustic elements the force has one
+ * component and the angle is unused (force_source.cpp:36-40). */
 typedef struct {
   int nsrc;
   const double *x, *z;  /* [nsrc] location */
-  const float *angle;   /* [nsrc] degrees clockwise from +z */
-  const float *f0;      /* [nsrc] dominant frequency (unused for Dirac) */
+  const float *angle;   /* [nsrc] degrees clockwise from +z (forces) */
+  const float *f0;      /* [nsrc] dominant frequency (Ricker); half duration
+                           hdur in seconds (Heaviside); unused (Dirac) */
   const float *tshift;  /* [nsrc] */
   const float *factor;  /* [nsrc] amplitude */
   const int *stf;       /* [nsrc] SMX_STF_*, or NULL for all Ricker */
+  const int *kind;      /* [nsrc] SMX_SRC_*, or NULL for all point forces */
+  const float *moment;  /* [nsrc][3] Mxx, Mzz, Mxz of SMX_SRC_MOMENT sources
+                           (may be NULL when there are none) */
 } smx_sources2d;
 
 /* Stations and the seismograms recorded at them

@@ -44,6 +44,8 @@ def _load():
     lib.oracle2d_create_ex.restype = P
     lib.oracle2d_create_ex.argtypes = [i, i, i, i, P, P, P, P, P, i, P, P, i, P, P, f, i, i, P, P,
                                        P, P, P, P, P, i, P, P]
+    lib.oracle2d_create_mt.restype = P
+    lib.oracle2d_create_mt.argtypes = lib.oracle2d_create_ex.argtypes + [P, P]
     lib.oracle2d_destroy.argtypes = [P]
     lib.oracle2d_nglob.argtypes = [P]
     lib.oracle2d_t0.argtypes = [P]
@@ -113,13 +115,21 @@ class Oracle2D:
         fluid = (np.asarray(mesh.mu).reshape(mesh.nspec, -1) == 0).all(axis=1)
         self.acoustic = bool(fluid.all())
         self.coupled = bool(fluid.any() and not fluid.all())
-        create = lib.oracle2d_coupled_create if self.coupled else lib.oracle2d_create_ex
-        self.h = create(precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
-                                        _ptr(coord), _ptr(knods), _ptr(kappa), _ptr(mu),
-                                        _ptr(rho), len(ai), _ptr(ai), _ptr(at), len(fi),
-                                        _ptr(fi), _ptr(ft), float(dt), int(nstep), len(sx),
-                                        _ptr(sx), _ptr(sz), _ptr(ang), _ptr(f0), _ptr(tsh),
-                                        _ptr(fac), _ptr(kinds), self.nrec, _ptr(rx), _ptr(rz))
+        args = (precision, mesh.nspec, mesh.ngnod, mesh.nnodes,
+                _ptr(coord), _ptr(knods), _ptr(kappa), _ptr(mu),
+                _ptr(rho), len(ai), _ptr(ai), _ptr(at), len(fi),
+                _ptr(fi), _ptr(ft), float(dt), int(nstep), len(sx),
+                _ptr(sx), _ptr(sz), _ptr(ang), _ptr(f0), _ptr(tsh),
+                _ptr(fac), _ptr(kinds), self.nrec, _ptr(rx), _ptr(rz))
+        stype = getattr(sources, "source_kinds", lambda: None)()
+        if stype is not None and stype.any():
+            if self.coupled:
+                raise ValueError("the coupled oracle takes point forces only")
+            self.h = lib.oracle2d_create_mt(*args, _ptr(arr(stype, np.int32)),
+                                            _ptr(arr(sources.moments(), np.float32)))
+        else:
+            create = lib.oracle2d_coupled_create if self.coupled else lib.oracle2d_create_ex
+            self.h = create(*args)
         if not self.h:
             raise RuntimeError("oracle2d_create failed: " + lib.oracle2d_last_error().decode())
         self.lib = lib

@@ -813,12 +813,24 @@ template <typename real> struct Oracle2D {
   // Dirac (kind 1): factor * gaussian(t - tshift, f0) with f0 = 1 / (10 dt)
   // (core/specfem/source_time_functions/dirac.cpp:7-52,
   //  impl/time_functions.hpp:9-14).
+  // Heaviside (kind 2, f0 holds the half duration): hdur raised to 5 dt,
+  // t0 = -2 hdur + tshift, then hdur / 1.628 and factor * (1 + erf(t / hdur))
+  // / 2 (core/specfem/source_time_functions/heaviside.cpp:10-78,
+  //  impl/time_functions.hpp:132-136, setup/constants.hpp.in:52).
+  // Moment tensors (src_type 1, moment = Mxx, Mzz, Mxz): the element's
+  // Jacobian interpolated at the source, then per point
+  // [Mxx Mxz; Mxz Mzz] (d/dx, d/dz)(h_xi h_gamma)
+  // (core/specfem/assembly/compute_source_array/dim2/impl/
+  //  compute_source_array_from_tensor.cpp:15-77,
+  //  core/specfem/source/dim2/tensor_source/moment_tensor_source.cpp:21-34,
+  //  include/algorithms/interpolate.hpp:73-94).
   std::vector<int> stf_kind;
   std::vector<real> f0_used;
   void set_sources(int nsrc_, const double *x, const double *z,
                    const float *angle, const float *f0, const float *tshift,
                    const float *factor, real dt, int nstep_,
-                   const int *kinds = nullptr, const real *t0_all = nullptr) {
+                   const int *kinds = nullptr, const real *t0_all = nullptr,
+                   const int *src_type = nullptr, const float *moment = nullptr) {
     nsrc = nsrc_;
     nstep = nstep_;
     src_ispec.assign(nsrc, 0);
@@ -829,13 +841,21 @@ template <typename real> struct Oracle2D {
     real min_t0 = std::numeric_limits<real>::max();
     for (int s = 0; s < nsrc; ++s) {
       stf_kind[s] = kinds ? kinds[s] : 0;
-      if (stf_kind[s] != 0 && stf_kind[s] != 1)
+      if (stf_kind[s] < 0 || stf_kind[s] > 2)
         throw std::runtime_error("unknown source time function");
-      f0_used[s] = stf_kind[s] == 1 ? static_cast<real>(1.0 / (10.0 * dt))
-                                    : static_cast<real>(f0[s]);
-      const real f = f0_used[s];
-      const real hdur = 1.0 / f;
-      t0s[s] = -static_cast<real>(1.2) * hdur + static_cast<real>(tshift[s]);
+      if (stf_kind[s] == 2) {
+        real hd = static_cast<real>(f0[s]);
+        if (hd < 5.0 * dt)
+          hd = 5.0 * dt;
+        t0s[s] = -static_cast<real>(2.0) * hd + static_cast<real>(tshift[s]);
+        f0_used[s] = hd / static_cast<real>(1.628); // the adjusted half duration
+      } else {
+        f0_used[s] = stf_kind[s] == 1 ? static_cast<real>(1.0 / (10.0 * dt))
+                                      : static_cast<real>(f0[s]);
+        const real f = f0_used[s];
+        const real hdur = 1.0 / f;
+        t0s[s] = -static_cast<real>(1.2) * hdur + static_cast<real>(tshift[s]);
+      }
       tsh[s] = tshift[s];
       if (t0s[s] < min_t0)
         min_t0 = t0s[s];
@@ -856,7 +876,10 @@ template <typename real> struct Oracle2D {
         const real fs = f0_used[s];
         const real aa = pi * pi * fs * fs;
         real val;
-        if (stf_kind[s] == 1) {
+        if (stf_kind[s] == 2) {
+          const real hv = 0.5 * (1.0 + std::erf(tt / fs));
+          val = static_cast<real>(factor[s]) * hv;
+        } else if (stf_kind[s] == 1) {
           const real gs = -1.0 * std::exp(-aa * tt * tt) / (2.0 * aa);
           val = static_cast<real>(factor[s]) * gs;
         } else {
@@ -871,9 +894,33 @@ template <typename real> struct Oracle2D {
       locate(static_cast<real>(static_cast<float>(x[s])),
              static_cast<real>(static_cast<float>(z[s])), ispec, xi, ga);
       src_ispec[s] = ispec;
-      real hxi[NGLL], hga[NGLL];
-      lagrange_interpolants<real>(xi, gll.xi, hxi, nullptr);
-      lagrange_interpolants<real>(ga, gll.xi, hga, nullptr);
+      real hxi[NGLL], hga[NGLL], hpxi[NGLL], hpga[NGLL];
+      lagrange_interpolants<real>(xi, gll.xi, hxi, hpxi);
+      lagrange_interpolants<real>(ga, gll.xi, hga, hpga);
+      if (src_type && src_type[s] == 1) {
+        if (acoustic)
+          throw std::runtime_error("moment-tensor sources need elastic elements");
+        Jac<real> d{0, 0, 0, 0, 0};
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const real hl = hxi[ix] * hga[iz];
+            const Jac<real> &J = jac[idx(ispec, iz, ix)];
+            d.xix += hl * J.xix;
+            d.xiz += hl * J.xiz;
+            d.gammax += hl * J.gammax;
+            d.gammaz += hl * J.gammaz;
+          }
+        const real mxx = moment[3 * s], mzz = moment[3 * s + 1],
+                   mxz = moment[3 * s + 2];
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const real dx = (hpxi[ix] * d.xix) * hga[iz] + hxi[ix] * (hpga[iz] * d.gammax);
+            const real dz = (hpxi[ix] * d.xiz) * hga[iz] + hxi[ix] * (hpga[iz] * d.gammaz);
+            src_array[((s * 2 + 0) * NGLL + iz) * NGLL + ix] = mxx * dx + mxz * dz;
+            src_array[((s * 2 + 1) * NGLL + iz) * NGLL + ix] = mxz * dx + mzz * dz;
+          }
+        continue;
+      }
       const real ang = static_cast<real>(angle[s]) * pi /
                        static_cast<real>(180.0);
       real fv[2] = {std::sin(ang), std::cos(ang)};
@@ -1459,6 +1506,20 @@ const char *oracle2d_last_error() { return g_err.c_str(); }
 
 // oracle2d_create with acoustic free-surface entities (nfree, element,
 // type 1..8) and source time-function kinds (0 Ricker, 1 Dirac; NULL: Ricker).
+// As oracle2d_create_ex, with source kinds (0 point force, 1 moment tensor;
+// NULL: all forces) and moment tensors [nsrc][3] (Mxx, Mzz, Mxz).
+void *oracle2d_create_mt(int precision, int nspec, int ngnod, int nnodes,
+                         const double *coord, const int *knods,
+                         const float *kappa, const float *mu, const float *rho,
+                         int nabs, const int *abs_ispec, const int *abs_type,
+                         int nfree, const int *free_ispec, const int *free_type,
+                         float dt, int nstep, int nsrc, const double *sx,
+                         const double *sz, const float *angle, const float *f0,
+                         const float *tshift, const float *factor,
+                         const int *stf_kind, int nrec, const double *rx,
+                         const double *rz, const int *src_type,
+                         const float *moment);
+
 void *oracle2d_create_ex(int precision, int nspec, int ngnod, int nnodes,
                          const double *coord, const int *knods,
                          const float *kappa, const float *mu, const float *rho,
@@ -1469,6 +1530,24 @@ void *oracle2d_create_ex(int precision, int nspec, int ngnod, int nnodes,
                          const float *tshift, const float *factor,
                          const int *stf_kind, int nrec, const double *rx,
                          const double *rz) {
+  return oracle2d_create_mt(precision, nspec, ngnod, nnodes, coord, knods,
+                            kappa, mu, rho, nabs, abs_ispec, abs_type, nfree,
+                            free_ispec, free_type, dt, nstep, nsrc, sx, sz,
+                            angle, f0, tshift, factor, stf_kind, nrec, rx, rz,
+                            nullptr, nullptr);
+}
+
+void *oracle2d_create_mt(int precision, int nspec, int ngnod, int nnodes,
+                         const double *coord, const int *knods,
+                         const float *kappa, const float *mu, const float *rho,
+                         int nabs, const int *abs_ispec, const int *abs_type,
+                         int nfree, const int *free_ispec, const int *free_type,
+                         float dt, int nstep, int nsrc, const double *sx,
+                         const double *sz, const float *angle, const float *f0,
+                         const float *tshift, const float *factor,
+                         const int *stf_kind, int nrec, const double *rx,
+                         const double *rz, const int *src_type,
+                         const float *moment) {
   try {
     auto *h = new OracleHandle{precision, nullptr, nullptr};
     auto setup = [&](auto *o) {
@@ -1477,7 +1556,8 @@ void *oracle2d_create_ex(int precision, int nspec, int ngnod, int nnodes,
                     abs_ispec, abs_type, nfree, free_ispec, free_type);
       o->set_timescheme(static_cast<real>(dt));
       o->set_sources(nsrc, sx, sz, angle, f0, tshift, factor,
-                     static_cast<real>(dt), nstep, stf_kind);
+                     static_cast<real>(dt), nstep, stf_kind, nullptr, src_type,
+                     moment);
       o->set_receivers(nrec, rx, rz);
       o->initialize(static_cast<real>(dt));
     };

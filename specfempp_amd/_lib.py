@@ -16,7 +16,8 @@ SMX_FIELD_FORWARD, SMX_FIELD_ADJOINT, SMX_FIELD_BACKWARD = 0, 1, 2
 SMX_U, SMX_V, SMX_A, SMX_MASS_INVERSE = 0, 1, 2, 3
 SMX_OK, SMX_ERR_ARG, SMX_ERR_HIP, SMX_ERR_STATE, SMX_ERR_MESH = 0, -1, -2, -3, -4
 SEIS_TYPES = {"displacement": 0, "velocity": 1, "acceleration": 2, "pressure": 3}
-SMX_STF_RICKER, SMX_STF_DIRAC = 0, 1
+SMX_STF_RICKER, SMX_STF_DIRAC, SMX_STF_HEAVISIDE = 0, 1, 2
+SMX_SRC_FORCE, SMX_SRC_MOMENT = 0, 1
 
 # Every symbol include/specfempp_amd.h declares.
 EXPORTS = [
@@ -45,7 +46,8 @@ class SmxSources2D(ctypes.Structure):
     _fields_ = [("nsrc", ctypes.c_int), ("x", ctypes.c_void_p), ("z", ctypes.c_void_p),
                 ("angle", ctypes.c_void_p), ("f0", ctypes.c_void_p),
                 ("tshift", ctypes.c_void_p), ("factor", ctypes.c_void_p),
-                ("stf", ctypes.c_void_p)]
+                ("stf", ctypes.c_void_p), ("kind", ctypes.c_void_p),
+                ("moment", ctypes.c_void_p)]
 
 
 class SmxReceivers2D(ctypes.Structure):

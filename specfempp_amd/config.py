@@ -19,7 +19,8 @@ import numpy as np
 from .mesh import Mesh2D, box_mesh, elastic_moduli, gll_coordinates
 
 
-STF_RICKER, STF_DIRAC = 0, 1
+STF_RICKER, STF_DIRAC, STF_HEAVISIDE = 0, 1, 2
+SRC_FORCE, SRC_MOMENT = 0, 1
 
 
 @dataclass
@@ -30,13 +31,50 @@ class Sources2D:
     f0: np.ndarray
     tshift: np.ndarray
     factor: np.ndarray
-    # time function per source: STF_RICKER or STF_DIRAC (None: all Ricker)
+    # time function per source: STF_RICKER, STF_DIRAC or STF_HEAVISIDE (None:
+    # all Ricker); for a Heaviside source f0 holds the half duration hdur (s)
     stf: np.ndarray = None
+    # source kind per source: SRC_FORCE or SRC_MOMENT (None: all forces), and
+    # the moment tensors [n][3] = (Mxx, Mzz, Mxz) of the SRC_MOMENT sources
+    kind: np.ndarray = None
+    moment: np.ndarray = None
 
     def kinds(self):
+        """Time-function kind per source."""
         n = len(self.x)
         return (np.zeros(n, np.int32) if self.stf is None
                 else np.asarray(self.stf, dtype=np.int32).reshape(n))
+
+    def source_kinds(self):
+        """SRC_FORCE / SRC_MOMENT per source."""
+        n = len(self.x)
+        return (np.zeros(n, np.int32) if self.kind is None
+                else np.asarray(self.kind, dtype=np.int32).reshape(n))
+
+    def moments(self):
+        n = len(self.x)
+        return (np.zeros((n, 3), np.float32) if self.moment is None
+                else np.asarray(self.moment, dtype=np.float32).reshape(n, 3))
+
+    @staticmethod
+    def moment_tensor(x, z, Mxx, Mzz, Mxz, f0=None, factor=1.0, tshift=0.0,
+                      hdur=None):
+        """Moment-tensor sources (core/specfem/source/dim2/tensor_source/
+        moment_tensor_source.cpp) with a Ricker time function of dominant
+        frequency f0, or a Heaviside (error-function step) of half duration
+        hdur (core/specfem/source_time_functions/heaviside.cpp)."""
+        if (f0 is None) == (hdur is None):
+            raise ValueError("give exactly one of f0 (Ricker) and hdur (Heaviside)")
+        s = Sources2D.ricker(x, z, f0=hdur if f0 is None else f0, factor=factor,
+                             tshift=tshift)
+        n = len(s.x)
+        if hdur is not None:
+            s.stf = np.full(n, STF_HEAVISIDE, np.int32)
+        s.kind = np.full(n, SRC_MOMENT, np.int32)
+        m = np.stack([np.broadcast_to(np.asarray(v, np.float32), (n,))
+                      for v in (Mxx, Mzz, Mxz)], axis=1)
+        s.moment = np.ascontiguousarray(m)
+        return s
 
     @staticmethod
     def dirac(x, z, factor=1e10, angle=0.0, tshift=0.0):

@@ -44,6 +44,29 @@ float legendre_deriv(float z, int n) {
   return p3d;
 }
 
+// Lagrange interpolants and their derivatives through the GLL nodes at xi
+// (src/quadrature/gll/lagrange_poly.cpp:11-43: h = prod / den, h' = sum of
+// the partial products / den).
+void interpolants_d(float xi, const float *nodes, float *h, float *hp) {
+  for (int d = 0; d < NGLL; d++) {
+    float num = 1.0, den = 1.0, sum = 0.0;
+    for (int i = 0; i < NGLL; i++) {
+      if (i == d)
+        continue;
+      num = num * (xi - nodes[i]);
+      den = den * (nodes[d] - nodes[i]);
+      float part = 1.0;
+      for (int j = 0; j < NGLL; j++)
+        if (j != d && j != i)
+          part = part * (xi - nodes[j]);
+      sum = sum + part;
+    }
+    const float inv = 1.0 / den;
+    h[d] = num * inv;
+    hp[d] = sum * inv;
+  }
+}
+
 // Lagrange interpolants through the GLL nodes at xi
 // (src/quadrature/gll/lagrange_poly.cpp:11-43).
 void interpolants(float xi, const float *nodes, float *h) {
@@ -277,6 +300,25 @@ Quadrature::Quadrature() {
 void refresh_boundary_matrices(HostAssembly2D &A);
 static void build_shared_csr(HostAssembly2D &A);
 static void finish_assembly(HostAssembly2D &A);
+
+float source_start_time(int stf_kind, float f0, float tshift, double dt) {
+  const float dtf = static_cast<float>(dt);
+  if (stf_kind == SMX_STF_HEAVISIDE) {
+    float hd = f0;
+    if (!(hd > 0))
+      throw SmxError(SMX_ERR_ARG, "Heaviside half duration must be positive");
+    if (hd < 5.0 * dtf)
+      hd = 5.0 * dtf;
+    return -static_cast<float>(2.0) * hd + tshift;
+  }
+  if (stf_kind != SMX_STF_RICKER && stf_kind != SMX_STF_DIRAC)
+    throw SmxError(SMX_ERR_ARG, "unknown source time function");
+  const float fv = stf_kind == SMX_STF_DIRAC ? static_cast<float>(1.0 / (10.0 * dtf)) : f0;
+  if (!(fv > 0))
+    throw SmxError(SMX_ERR_ARG, "source f0 must be positive");
+  const float hdur = 1.0 / fv;
+  return -static_cast<float>(1.2) * hdur + tshift;
+}
 
 void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
                 const smx_receivers2d &rec, double dt_in, int nstep,
@@ -851,14 +893,16 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
     const float dtf = static_cast<float>(dt_in);
     for (int s = 0; s < src.nsrc; ++s) {
       kind[s] = src.stf ? src.stf[s] : SMX_STF_RICKER;
-      if (kind[s] != SMX_STF_RICKER && kind[s] != SMX_STF_DIRAC)
-        throw SmxError(SMX_ERR_ARG, "unknown source time function");
-      f0v[s] = kind[s] == SMX_STF_DIRAC ? static_cast<float>(1.0 / (10.0 * dtf))
-                                        : src.f0[s];
-      if (!(f0v[s] > 0))
-        throw SmxError(SMX_ERR_ARG, "source f0 must be positive");
-      const float hdur = 1.0 / f0v[s];
-      t0s[s] = -static_cast<float>(1.2) * hdur + src.tshift[s];
+      t0s[s] = source_start_time(kind[s], src.f0[s], src.tshift[s], dt_in);
+      if (kind[s] == SMX_STF_HEAVISIDE) { // the adjusted half duration
+        float hd = src.f0[s];
+        if (hd < 5.0 * dtf)
+          hd = 5.0 * dtf;
+        f0v[s] = hd / static_cast<float>(1.628);
+      } else {
+        f0v[s] = kind[s] == SMX_STF_DIRAC ? static_cast<float>(1.0 / (10.0 * dtf))
+                                          : src.f0[s];
+      }
       min_t0 = std::min(min_t0, t0s[s]);
     }
     if (opt.has_t0) // one medium of a coupled run: the run's start time
@@ -874,7 +918,11 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
         const float tt = t - tshift;
         const float a = pi * pi * f0 * f0;
         float val;
-        if (kind[s] == SMX_STF_DIRAC) {
+        if (kind[s] == SMX_STF_HEAVISIDE) {
+          // factor * (1 + erf(t / hdur)) / 2 (impl/time_functions.hpp:132-136)
+          const float hv = 0.5 * (1.0 + std::erf(tt / f0));
+          val = src.factor[s] * hv;
+        } else if (kind[s] == SMX_STF_DIRAC) {
           // factor * gaussian(t - tshift, f0) (impl/time_functions.hpp:9-14)
           const float g = -1.0 * std::exp(-a * tt * tt) / (2.0 * a);
           val = src.factor[s] * g;
@@ -893,6 +941,44 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
       float hx[NGLL], hg[NGLL];
       interpolants(xi, q.xi, hx);
       interpolants(ga, q.xi, hg);
+      const int skind = src.kind ? src.kind[s] : SMX_SRC_FORCE;
+      if (skind != SMX_SRC_FORCE && skind != SMX_SRC_MOMENT)
+        throw SmxError(SMX_ERR_ARG, "unknown source kind");
+      if (skind == SMX_SRC_MOMENT) {
+        // moment tensor (core/specfem/assembly/compute_source_array/dim2/
+        // impl/compute_source_array_from_tensor.cpp:15-77): the element's
+        // Jacobian interpolated at the source (include/algorithms/
+        // interpolate.hpp:73-94), then [Mxx Mxz; Mxz Mzz] grad(h_xi h_gamma)
+        if (acoustic)
+          throw SmxError(SMX_ERR_ARG, "moment-tensor sources need elastic elements");
+        if (!src.moment)
+          throw SmxError(SMX_ERR_ARG, "moment-tensor source without a moment tensor");
+        float hpx[NGLL], hpg[NGLL];
+        interpolants_d(xi, q.xi, hx, hpx);
+        interpolants_d(ga, q.xi, hg, hpg);
+        const int sl = slot_of[e];
+        const size_t pl = size_t(NPT) * ld;
+        float dxix = 0, dxiz = 0, dgax = 0, dgaz = 0;
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const float hl = hx[ix] * hg[iz];
+            const size_t o = size_t(iz * NGLL + ix) * ld + sl;
+            dxix += hl * A.geom[o];
+            dxiz += hl * A.geom[pl + o];
+            dgax += hl * A.geom[2 * pl + o];
+            dgaz += hl * A.geom[3 * pl + o];
+          }
+        const float mxx = src.moment[3 * s], mzz = src.moment[3 * s + 1],
+                    mxz = src.moment[3 * s + 2];
+        for (int iz = 0; iz < NGLL; ++iz)
+          for (int ix = 0; ix < NGLL; ++ix) {
+            const float dsx = (hpx[ix] * dxix) * hg[iz] + hx[ix] * (hpg[iz] * dgax);
+            const float dsz = (hpx[ix] * dxiz) * hg[iz] + hx[ix] * (hpg[iz] * dgaz);
+            A.src_array[(size_t(s) * 2 + 0) * NPT + iz * NGLL + ix] = mxx * dsx + mxz * dsz;
+            A.src_array[(size_t(s) * 2 + 1) * NPT + iz * NGLL + ix] = mxz * dsx + mzz * dsz;
+          }
+        continue;
+      }
       const float ang = src.angle[s] * pi / static_cast<float>(180.0);
       float f[2] = {std::sin(ang), std::cos(ang)};
       if (acoustic) { // force vector (1) (force_source.cpp:36-40)

@@ -197,6 +197,12 @@ struct HostAssembly2D {
 };
 
 // Assembly choices that change the layout, not the results' meaning.
+// Start time of one source's time function before the run's shift
+// (src/io/sources/dim2/read_sources.cpp:142-172): -1.2 / f0 + tshift for
+// Ricker and Dirac (f0 = 1 / (10 dt)), -2 max(hdur, 5 dt) + tshift for
+// Heaviside; throws on an unknown kind or a non-positive f0 / hdur.
+float source_start_time(int stf_kind, float f0, float tshift, double dt);
+
 struct AssemblyOptions {
   int tile_elems = ELEM_TILE; // elements per tile (1..64)
   // Start time of the run when the sources given are only part of it (one

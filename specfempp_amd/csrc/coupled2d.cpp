@@ -59,8 +59,8 @@ struct Part {
   std::vector<int> knods, abs_e, abs_t, fr_e, fr_t;
   std::vector<float> kappa, mu, rho;
   std::vector<double> sx, sz, rx, rz;
-  std::vector<float> ang, f0, tsh, fac;
-  std::vector<int> kind;
+  std::vector<float> ang, f0, tsh, fac, mom;
+  std::vector<int> kind, skind;
 };
 
 } // namespace
@@ -132,18 +132,15 @@ void assemble_coupled2d(const smx_mesh2d &m, const smx_sources2d &src,
   };
 
   // one start time for the whole run (src/io/sources/dim2/read_sources.cpp:142-172)
-  const float dtf = static_cast<float>(dt);
   float t0_all = std::numeric_limits<float>::max();
   for (int s = 0; s < src.nsrc; ++s) {
     const int kind = src.stf ? src.stf[s] : SMX_STF_RICKER;
-    const float f0 = kind == SMX_STF_DIRAC
-                         ? static_cast<float>(1.0 / (10.0 * dtf))
-                         : src.f0[s];
-    if (!(f0 > 0))
-      throw SmxError(SMX_ERR_ARG, "source f0 must be positive");
-    const float hdur = 1.0 / f0;
-    t0_all = std::min(t0_all, -static_cast<float>(1.2) * hdur + src.tshift[s]);
+    t0_all = std::min(t0_all, source_start_time(kind, src.f0[s], src.tshift[s], dt));
     Part &P = part[medium_of(src.x[s], src.z[s])];
+    const int sk = src.kind ? src.kind[s] : SMX_SRC_FORCE;
+    P.skind.push_back(sk);
+    for (int c = 0; c < 3; ++c)
+      P.mom.push_back(sk == SMX_SRC_MOMENT && src.moment ? src.moment[3 * s + c] : 0.0f);
     P.sx.push_back(src.x[s]);
     P.sz.push_back(src.z[s]);
     P.ang.push_back(src.angle[s]);
@@ -200,6 +197,8 @@ void assemble_coupled2d(const smx_mesh2d &m, const smx_sources2d &src,
     ss.tshift = P.tsh.data();
     ss.factor = P.fac.data();
     ss.stf = P.kind.data();
+    ss.kind = P.skind.data();
+    ss.moment = P.mom.data();
     smx_receivers2d rr{};
     rr.nrec = static_cast<int>(P.rx.size());
     rr.x = P.rx.data();

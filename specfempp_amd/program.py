@@ -19,12 +19,14 @@ Keys follow the reference's parameter files (include/parameter_parser/*):
     sources: <path to a sources YAML>
 
 Sources YAML: ``sources: [{force: {x, z, angle, Ricker: {factor, tshift,
-f0}} or Dirac: {factor, tshift}}, ...]`` (include/parameter_parser/
+f0}} or Dirac: {factor, tshift}}, {moment-tensor: {x, z, Mxx, Mzz, Mxz,
+Heaviside: {hdur, factor, tshift}}}, ...]`` (include/parameter_parser/
 sources.hpp, core/specfem/source/source.tpp:7-60).  Relative paths are
 resolved against ``base_dir`` (the reference resolves them against the
 working directory).  Only what this build implements is accepted (elastic
 isotropic P-SV or acoustic isotropic meshes, GLL4, Newmark, forward mode,
-force sources with Ricker or Dirac time functions, ASCII seismograms of
+force and moment-tensor sources with Ricker, Dirac or Heaviside time
+functions, ASCII seismograms of
 displacement, velocity, acceleration and, in acoustic meshes, pressure);
 anything else raises ``ValueError`` naming the key.
 """
@@ -37,7 +39,8 @@ from dataclasses import dataclass, field
 import numpy as np
 import yaml
 
-from .config import STF_DIRAC, STF_RICKER, Config2D, Receivers2D, Sources2D
+from .config import (SRC_FORCE, SRC_MOMENT, STF_DIRAC, STF_HEAVISIDE, STF_RICKER, Config2D,
+                     Receivers2D, Sources2D)
 from .meshdb import read_mesh_database_2d
 from .seismo_io import write_seismograms
 
@@ -91,17 +94,27 @@ def _get(d, *keys, default=None, required=True):
 
 
 def read_sources(node):
-    """Force sources with Ricker or Dirac time functions from a sources YAML
-    node (a Dirac's f0 is 1 / (10 dt), set by the context)."""
+    """Force and moment-tensor sources with Ricker, Dirac or Heaviside time
+    functions from a sources YAML node (src/io/sources/dim2/read_sources.cpp;
+    a Dirac's f0 is 1 / (10 dt), set by the context; a Heaviside's half
+    duration travels in f0)."""
     items = node.get("sources", [])
     n_decl = node.get("number-of-sources")
     if n_decl is not None and int(n_decl) != len(items):
         raise ValueError(f"number-of-sources is {n_decl} but {len(items)} are listed")
-    x, z, ang, f0, ts, fac, kind = [], [], [], [], [], [], []
+    x, z, ang, f0, ts, fac, kind, skind, mom = [], [], [], [], [], [], [], [], []
     for k, item in enumerate(items):
-        if "force" not in item:
-            raise ValueError(f"source {k}: only force sources are supported ({list(item)})")
-        f = item["force"]
+        if "force" in item:
+            f = item["force"]
+            skind.append(SRC_FORCE)
+            mom.append((0.0, 0.0, 0.0))
+        elif "moment-tensor" in item:
+            f = item["moment-tensor"]
+            skind.append(SRC_MOMENT)
+            mom.append((float(f["Mxx"]), float(f["Mzz"]), float(f["Mxz"])))
+        else:
+            raise ValueError(f"source {k}: only force and moment-tensor sources are "
+                             f"supported ({list(item)})")
         if "Ricker" in f:
             r = f["Ricker"]
             kind.append(STF_RICKER)
@@ -110,8 +123,13 @@ def read_sources(node):
             r = f["Dirac"]
             kind.append(STF_DIRAC)
             f0.append(1.0)
+        elif "Heaviside" in f:
+            r = f["Heaviside"]
+            kind.append(STF_HEAVISIDE)
+            f0.append(float(r["hdur"]))
         else:
-            raise ValueError(f"source {k}: only Ricker and Dirac time functions are supported")
+            raise ValueError(f"source {k}: only Ricker, Dirac and Heaviside time functions "
+                             "are supported")
         x.append(float(f["x"]))
         z.append(float(f["z"]))
         ang.append(float(f.get("angle", 0.0)))
@@ -119,7 +137,9 @@ def read_sources(node):
         fac.append(float(r["factor"]))
     return Sources2D(x=np.array(x), z=np.array(z), angle=np.array(ang, np.float32),
                      f0=np.array(f0, np.float32), tshift=np.array(ts, np.float32),
-                     factor=np.array(fac, np.float32), stf=np.array(kind, np.int32))
+                     factor=np.array(fac, np.float32), stf=np.array(kind, np.int32),
+                     kind=np.array(skind, np.int32),
+                     moment=np.array(mom, np.float32).reshape(-1, 3))
 
 
 def parse_config(params, base_dir="."):

@@ -48,11 +48,23 @@ def _structs(mesh, sources, receivers, seismo_types, nsbs, keep):
     s = _lib.SmxSources2D(len(sources), _p(arr(sources.x, np.float64)),
                           _p(arr(sources.z, np.float64)), _p(arr(sources.angle, np.float32)),
                           _p(arr(sources.f0, np.float32)), _p(arr(sources.tshift, np.float32)),
-                          _p(arr(sources.factor, np.float32)), _p(arr(sources.kinds(), np.int32)))
+                          _p(arr(sources.factor, np.float32)), _p(arr(sources.kinds(), np.int32)),
+                          _p(arr(_source_kinds(sources), np.int32)),
+                          _p(arr(_moments(sources), np.float32)))
     types = arr([_lib.SEIS_TYPES[t] for t in seismo_types], np.int32)
     r = _lib.SmxReceivers2D(len(receivers), _p(arr(receivers.x, np.float64)),
                             _p(arr(receivers.z, np.float64)), len(types), _p(types), int(nsbs))
     return m, s, r
+
+
+def _source_kinds(sources):
+    f = getattr(sources, "source_kinds", None)
+    return f() if f else np.zeros(len(sources.x), np.int32)
+
+
+def _moments(sources):
+    f = getattr(sources, "moments", None)
+    return f() if f else np.zeros((len(sources.x), 3), np.float32)
 
 
 def _options(options):

@@ -9,6 +9,10 @@ Source directories (read as text, in this container only):
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousAcousticDomainCompositeStaceyDirichletBC
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/AcousticElasticCoupledDomain
   tests/unit-tests/displacement_tests/Newmark/serial/dim2/AcousticElasticCoupledDomainVariant2
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomainStaceyBC
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomainWAdjacencyMap
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomainMomentTensor
+  tests/unit-tests/displacement_tests/Newmark/serial/dim2/HomogeneousElasticDomainMomentTensorHeaviside
 
 Each case's mesh database (database.bin, written by the reference's mesher)
 is also copied, gzip-compressed, as specfem2d_<case>.database.bin.gz (the
@@ -17,8 +21,8 @@ coupled case ships no database; its mesh is restated from its Par_file).
 Each fixture holds the station table (STATIONS: name, network, x, z), the
 sample times, and traces[type][station][component][t] for the seismogram types
 present (d, v, a; pressure 'p' traces are stored separately).  The problem set
-up (80x60 Q9 box, dt=1.1e-3, 300 steps, Ricker or Dirac sources) is restated
-in tests/test_oracle_golden.py and tests/test_oracle_acoustic.py from the
+up (80x60 Q9 box, dt=1.1e-3, 300 or 500 steps, Ricker, Dirac or Heaviside
+force and moment-tensor sources) is restated in tests/_cases.py from the
 fixtures' Par_file / sources.yaml.
 
 Usage: python tests/golden/make_specfem2d_traces.py [/root/reference]
@@ -34,7 +38,9 @@ BASE = "tests/unit-tests/displacement_tests/Newmark/serial/dim2"
 CASES = ["HomogeneousElasticDomain", "HomogeneousElasticDomainStaceyDirichletBC",
          "HomogeneousAcousticDomain", "HomogeneousAcousticDomainDirichletBC",
          "HomogeneousAcousticDomainCompositeStaceyDirichletBC", "AcousticElasticCoupledDomain",
-         "AcousticElasticCoupledDomainVariant2"]
+         "AcousticElasticCoupledDomainVariant2", "HomogeneousElasticDomainStaceyBC",
+         "HomogeneousElasticDomainWAdjacencyMap", "HomogeneousElasticDomainMomentTensor",
+         "HomogeneousElasticDomainMomentTensorHeaviside"]
 
 
 def main(ref="/root/reference"):
