@@ -26,6 +26,13 @@
 //                                four Ricker sources), reference metric; plus
 //                                closed forms (mass = area / kappa, Dirichlet
 //                                points stay at rest without sources there)
+//   * moment tensors, Heaviside  SPECFEM2D golden traces of the reference's
+//                                HomogeneousElasticDomainMomentTensor (d, v, a)
+//                                and ...MomentTensorHeaviside (d, v; its
+//                                acceleration is not pinned by data at 1e-3),
+//                                planted sign / transposition of the tensor
+//                                fail them; the Heaviside table against
+//                                scipy's erf
 //   * coupled elastic-acoustic   SPECFEM2D golden traces of the reference's
 //                                two coupled fixtures (water above the solid,
 //                                restated from its Par_file; water below, from
