@@ -218,3 +218,43 @@ def test_coupled_yaml_run_from_the_reference_database(tmp_path):
     err = reference_error(s, np.transpose(g["traces"], (3, 0, 1, 2)))
     print("coupled YAML run vs SPECFEM2D (reference metric):", err)
     assert err < 1e-4
+
+
+def test_moment_tensor_and_heaviside_sources_are_read():
+    """A reference sources.yaml with a moment tensor and a Heaviside time
+    function (the HomogeneousElasticDomainMomentTensorHeaviside fixture's
+    layout) reads into the moment-tensor source of that fixture."""
+    import yaml
+    from specfempp_amd.config import SRC_FORCE, SRC_MOMENT, STF_HEAVISIDE, STF_RICKER
+    from specfempp_amd.program import read_sources
+    node = yaml.safe_load("""
+number-of-sources: 2
+sources:
+  - moment-tensor:
+      x: 2500.0
+      z: 2500.0
+      source_surf: false
+      Mxx: 0.5
+      Mzz: -0.25
+      Mxz: 1.0
+      Heaviside:
+        hdur: 0.1
+        factor: 1e8
+        tshift: 0.0
+  - force:
+      x: 100.0
+      z: 200.0
+      angle: 30.0
+      Ricker:
+        f0: 10.0
+        factor: 1e10
+""")
+    s = read_sources(node)
+    assert list(s.source_kinds()) == [SRC_MOMENT, SRC_FORCE]
+    assert list(s.kinds()) == [STF_HEAVISIDE, STF_RICKER]
+    np.testing.assert_allclose(s.moments()[0], [0.5, -0.25, 1.0])
+    np.testing.assert_allclose(s.f0, [0.1, 10.0], rtol=1e-6)
+    np.testing.assert_allclose(s.factor, [1e8, 1e10], rtol=1e-6)
+    with pytest.raises(ValueError, match="time functions"):
+        read_sources({"sources": [{"moment-tensor": {"x": 0, "z": 0, "Mxx": 1, "Mzz": 1,
+                                                     "Mxz": 0, "External": {}}}]})
