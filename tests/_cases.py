@@ -370,8 +370,11 @@ def warped3d(cfg, amp=0.15, topo=0.2):
 
 
 def reference_metric(ours, traces):
-    """Worst per-station sum|s - ref| / sum|ref| over components and types
+    """Worst sum|s - ref| / sum|ref| over both components, taken per station
+    and per seismogram type
     (tests/unit-tests/displacement_tests/Newmark/dim2/newmark_tests.cpp:337-362).
+    The reference pools all types of a station into one ratio
+    (newmark_tests.cpp:195-198), so this per-type figure is the stricter one.
     ours: (nt, ntypes, nsta, 2); traces: (ntypes, nsta, 2, nt)."""
     worst = 0.0
     for it in range(traces.shape[0]):
