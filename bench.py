@@ -252,7 +252,7 @@ def cpu_baseline_all_cores(cfg, nsteps=120):
 
 def adjoint_leg(nfd=20, every=100):
     """Adjoint-gradient workload: the 80x60 homogeneous box (4000x3000 m,
-    96,681 points), its Ricker source at (2000 m, 2000 m) and 50 receivers,
+    77,361 points), its Ricker source at (2000 m, 2000 m) and 50 receivers,
     2000 steps of 5e-4 s; observed data from the same box with a 5% Gaussian
     vs anomaly (sigma 300 m) at (2000 m, 1500 m); L2 waveform misfit from the
     homogeneous starting model; checkpoints every 100 steps.  Reports the
@@ -318,7 +318,7 @@ def adjoint_leg(nfd=20, every=100):
         fd = (8.0 * (chi64(at(1)) - chi64(at(-1))) - (chi64(at(2)) - chi64(at(-2)))) / 12.0
         an = sum(float(np.sum(gg * d)) for gg, d in zip(g32, dirs))
         errs.append(abs(an - fd) / abs(fd))
-    return {"workload": "80x60 homogeneous box, 96,681 points, 2000 steps of 5e-4 s, "
+    return {"workload": "80x60 homogeneous box, 77,361 points, 2000 steps of 5e-4 s, "
                         "50 receivers, L2 misfit vs a 5% Gaussian vs anomaly, "
                         f"checkpoints every {every} steps",
             "forward_ms": fwd_ms, "gradient_ms": best[0], "checkpointed_forward_ms": best[1],
@@ -348,7 +348,7 @@ def batch_leg(nshots=64, nstep=3000, baseline_steps=300):
     shots = batched_shots(nshots=nshots, nstep=nstep)
     gen_s = time.perf_counter() - t
     t = time.perf_counter()
-    best, timings = autotune(shots, steps=30, warmup=5, use_cache=False)
+    best, timings = autotune(shots, use_cache=False)
     tune_s = time.perf_counter() - t
     t = time.perf_counter()
     ctx = Context.batch(shots, options=best)

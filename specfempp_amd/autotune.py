@@ -11,8 +11,12 @@ compute_stiffness_interaction.tpp).  Here the choice matters in two regimes:
 * large meshes and batches are HBM-bound: 64-element tiles (8x8 squares)
   minimise the points shared between tiles and win.
 
-Candidates are timed with HIP events on the context's own stream over a
-short run; results are cached per workload signature and device.
+Candidates are timed with HIP events on the context's own stream.  Every
+context captures and instantiates its step graph when it is initialised, and
+every candidate is timed over the same steps: one predictor, a body whose
+length is a multiple of every graph length (so each candidate replays only
+its graph), and the final step.  Results are cached per workload signature
+and device.
 """
 
 from __future__ import annotations
@@ -26,9 +30,8 @@ from .solver import Context
 
 TILE_CHOICES = (64, 32, 16)
 GRAPH_CHOICES = (8, 32)
-# below this many 64-element tiles a step does not fill the GPU and smaller
-# tiles are worth trying (256 CUs x 8 resident workgroups)
-SMALL_TILE_LIMIT = 2048
+# timed body steps: a common multiple of every graph length
+BODY_STEPS = 64
 
 
 def _cache_path():
@@ -48,9 +51,8 @@ def signature(configs, precision):
 
 
 def candidates(configs):
-    n64 = sum(-(-c.mesh.nspec // 64) for c in configs)
-    tiles = TILE_CHOICES if n64 < SMALL_TILE_LIMIT else (64,)
-    return [{"tile_elems": t, "graph_steps": g} for t in tiles for g in GRAPH_CHOICES]
+    """Every tile size x every graph length, at every mesh size."""
+    return [{"tile_elems": t, "graph_steps": g} for t in TILE_CHOICES for g in GRAPH_CHOICES]
 
 
 def _time(configs, options, precision, steps, warmup):
@@ -79,7 +81,8 @@ def _with_nstep(cfg, n):
     return c
 
 
-def autotune(configs, *, precision=4, steps=60, warmup=10, use_cache=True, verbose=False):
+def autotune(configs, *, precision=4, steps=BODY_STEPS + 1, warmup=BODY_STEPS // 2 + 1,
+             use_cache=True, verbose=False):
     """Returns (best options dict, {label: ms per step}) for a Config2D or a
     list of Config2D run as one batch."""
     if not isinstance(configs, (list, tuple)):

@@ -213,7 +213,7 @@ def smooth_perturbation(mesh: Mesh2D, rng, nmodes=6, kmax=4):
     return f / np.sqrt(np.mean(f * f))
 
 
-def batched_shots(nshots=64, seed=2025, nstep=3000, dt=4e-4, nrec=100, nmin=40, nmax=400,
+def batched_shots(nshots=64, seed=1003, nstep=3000, dt=4e-4, nrec=100, nmin=40, nmax=400,
                   h=50.0):
     """Independent shots on meshes of uneven size: shot k has nx ~ U[nmin, nmax]
     quads, nz = round(0.75 nx), six seeded layers (as layered_box), one Ricker

@@ -654,6 +654,12 @@ public:
     k_ac_invert<real><<<dim3((A.nglob + 255) / 256), dim3(256), 0, stream_>>>(
         minv_, A.nglob);
     AC_HIP(hipGetLastError());
+    // Step graph captured, instantiated and uploaded once, outside every
+    // run_forward call; a coupled driver captures both media itself.
+    if (!chi_dd_) {
+      build_graph(field(SMX_FIELD_FORWARD));
+      AC_HIP(hipGraphUpload(graph_exec_, stream_));
+    }
     AC_HIP(hipStreamSynchronize(stream_));
     initialized_ = true;
   }

@@ -61,6 +61,9 @@ public:
   void initialize() override {
     ac_->initialize();
     el_->initialize();
+    build_graph(); // outside every run_forward call
+    CP_HIP(hipGraphUpload(graph_exec_, stream_));
+    CP_HIP(hipStreamSynchronize(stream_));
     initialized_ = true;
   }
 

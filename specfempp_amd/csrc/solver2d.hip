@@ -1310,6 +1310,13 @@ public:
     const long m = 2L * A.nglob;
     k_invert<real><<<dim3((m + 255) / 256), dim3(256), 0, stream_>>>(M, m);
     SMX_HIP(hipGetLastError());
+    // The step graph is captured, instantiated and uploaded here, so no
+    // run_forward call pays for it.  Under a coupled driver the driver
+    // captures both media itself.
+    if (!chi_dd_) {
+      build_graph(field(SMX_FIELD_FORWARD));
+      SMX_HIP(hipGraphUpload(graph_exec_, stream_));
+    }
     SMX_HIP(hipStreamSynchronize(stream_));
     initialized_ = true;
   }

@@ -1025,7 +1025,17 @@ public:
   int precision() const override { return sizeof(real); }
   const HostAssembly3D &assembly() const override { return *asm_; }
 
-  void initialize() override { initialized_ = true; }
+  // The step graph is captured, instantiated and uploaded here (and again
+  // after set_model), so no run_forward call pays for it.
+  void initialize() override {
+    prepare_graph();
+    initialized_ = true;
+  }
+  void prepare_graph() {
+    build_graph();
+    SMX3_HIP(hipGraphUpload(graph_exec_, stream_));
+    SMX3_HIP(hipStreamSynchronize(stream_));
+  }
 
   void geometry_mode(int *mode, long long *mismatches) const override {
     if (mode)
@@ -1142,6 +1152,8 @@ public:
     ckpt_every_ = 0;
     have_gradient_ = false;
     SMX3_HIP(hipStreamSynchronize(stream_));
+    if (initialized_)
+      prepare_graph();
   }
 
   void run_forward_checkpointed(int every) override {
