@@ -35,7 +35,10 @@ extern "C" {
 #define SMX_SEIS_DISPLACEMENT 0
 #define SMX_SEIS_VELOCITY 1
 #define SMX_SEIS_ACCELERATION 2
-#define SMX_SEIS_PRESSURE 3 /* acoustic elements only */
+/* 2D only: -chi_dd in acoustic elements (acoustic/isotropic/wavefield.hpp);
+ * -(lambda + 2/3 mu)(du_x/dx + du_z/dz) in elastic P-SV elements
+ * (elastic/isotropic/wavefield.hpp:57-94); component 0 holds it, 1 is 0 */
+#define SMX_SEIS_PRESSURE 3
 
 /* Source time functions (core/specfem/source_time_functions). */
 #define SMX_STF_RICKER 0

@@ -166,7 +166,7 @@ class Oracle2D:
 
     def set_field(self, what, values):
         v = np.ascontiguousarray(np.asarray(values, dtype=np.float64).ravel())
-        self.lib.oracle2d_set_field(self.h, {"u": 1, "v": 2, "a": 3}[what], _ptr(v))
+        self.lib.oracle2d_set_field(self.h, {"u": 1, "v": 2, "a": 3, "stf": 13}[what], _ptr(v))
 
     def apply_stiffness(self, with_stacey=True):
         self.lib.oracle2d_apply_stiffness(self.h, int(with_stacey))

@@ -978,6 +978,10 @@ template <typename real> struct Oracle2D {
       return;
     }
     for (int t = 0; t < ntypes; ++t) {
+      if (types[t] == 3) {
+        seismograms_elastic_pressure(U, out + size_t(t) * nrec * 2);
+        continue;
+      }
       const std::vector<real> &F = types[t] == 0 ? U : (types[t] == 1 ? V : A);
       for (int r = 0; r < nrec; ++r) {
         real s[2] = {0, 0};
@@ -991,6 +995,34 @@ template <typename real> struct Oracle2D {
         out[(size_t(t) * nrec + r) * 2 + 0] = s[0];
         out[(size_t(t) * nrec + r) * 2 + 1] = s[1];
       }
+    }
+  }
+
+  // Elastic P-SV pressure: p = -(lambda + 2/3 mu)(du_x/dx + du_z/dz) at the
+  // receiver element's points, formed in double from the FP32 lambda, mu and
+  // gradient, stored in FP32, then interpolated
+  // (medium/dim2/elastic/isotropic/wavefield.hpp:57-94,
+  // compute_seismogram.tpp:117-166).  Only the first component is a
+  // seismogram (one PRE trace); the second is written 0.
+  void seismograms_elastic_pressure(const std::vector<real> &U,
+                                    real *out /* [nrec][2] */) const {
+    for (int r = 0; r < nrec; ++r) {
+      const int e = rec_ispec[r];
+      real du[NGLL][NGLL][2][2];
+      element_gradient(e, U, du);
+      real s = 0;
+      for (int iz = 0; iz < NGLL; ++iz)
+        for (int ix = 0; ix < NGLL; ++ix) {
+          const int i = idx(e, iz, ix);
+          const double coef = static_cast<double>(lambda(i)) +
+                              (2.0 / 3.0) * static_cast<double>(mu[i]);
+          const real p = static_cast<real>(
+              -1.0 * (coef * static_cast<double>(du[iz][ix][0][0] + du[iz][ix][1][1])));
+          const real h = rec_h[(size_t(r) * NGLL + iz) * NGLL + ix];
+          s += h * p;
+        }
+      out[size_t(r) * 2 + 0] = s;
+      out[size_t(r) * 2 + 1] = 0;
     }
   }
 
@@ -1821,7 +1853,9 @@ long oracle2d_get(void *hv, int what, double *out) {
 int oracle2d_set_field(void *hv, int what, const double *in) {
   auto *h = static_cast<OracleHandle *>(hv);
   auto set = [&](auto *o) {
-    auto &F = what == 1 ? o->u : (what == 2 ? o->v : o->a);
+    // 13: the source time function table [nstep][nsrc][2] (checks of the
+    // table's own rounding)
+    auto &F = what == 13 ? o->stf : (what == 1 ? o->u : (what == 2 ? o->v : o->a));
     for (size_t k = 0; k < F.size(); ++k)
       F[k] = in[k];
     return 0;

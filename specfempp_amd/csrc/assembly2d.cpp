@@ -1000,11 +1000,8 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
   A.nsig = (nstep + A.nsbs - 1) / A.nsbs;
   A.seis_types.assign(rec.types, rec.types + rec.ntypes);
   for (int t : A.seis_types)
-    if (t < 0 || t > 3 || (t == SMX_SEIS_PRESSURE && !acoustic))
-      throw SmxError(SMX_ERR_ARG,
-                     t == SMX_SEIS_PRESSURE
-                         ? "pressure seismograms need acoustic elements"
-                         : "unsupported seismogram type");
+    if (t < 0 || t > 3)
+      throw SmxError(SMX_ERR_ARG, "unsupported seismogram type");
   A.rec_ispec.assign(std::max(1, rec.nrec), 0);
   A.rec_elem.assign(std::max(1, rec.nrec), 0);
   A.rec_xi.assign(std::max(1, rec.nrec), 0.0f);

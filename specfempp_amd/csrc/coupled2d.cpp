@@ -138,9 +138,11 @@ void assemble_coupled2d(const smx_mesh2d &m, const smx_sources2d &src,
     t0_all = std::min(t0_all, source_start_time(kind, src.f0[s], src.tshift[s], dt));
     Part &P = part[medium_of(src.x[s], src.z[s])];
     const int sk = src.kind ? src.kind[s] : SMX_SRC_FORCE;
+    if (sk == SMX_SRC_MOMENT && !src.moment) // as assemble2d
+      throw SmxError(SMX_ERR_ARG, "moment-tensor source without moment components");
     P.skind.push_back(sk);
     for (int c = 0; c < 3; ++c)
-      P.mom.push_back(sk == SMX_SRC_MOMENT && src.moment ? src.moment[3 * s + c] : 0.0f);
+      P.mom.push_back(sk == SMX_SRC_MOMENT ? src.moment[3 * s + c] : 0.0f);
     P.sx.push_back(src.x[s]);
     P.sz.push_back(src.z[s]);
     P.ang.push_back(src.angle[s]);
@@ -160,12 +162,6 @@ void assemble_coupled2d(const smx_mesh2d &m, const smx_sources2d &src,
     part[md].rx.push_back(rec.x[r]);
     part[md].rz.push_back(rec.z[r]);
   }
-  for (int t = 0; t < rec.ntypes; ++t)
-    if (rec.types[t] == SMX_SEIS_PRESSURE)
-      for (int r = 0; r < rec.nrec; ++r)
-        if (out.rec_med[r] == 0)
-          throw SmxError(SMX_ERR_ARG,
-                         "pressure seismograms need acoustic elements");
 
   AssemblyOptions o = opt;
   o.has_t0 = true;
@@ -204,12 +200,7 @@ void assemble_coupled2d(const smx_mesh2d &m, const smx_sources2d &src,
     rr.x = P.rx.data();
     rr.z = P.rz.data();
     rr.ntypes = rec.ntypes;
-    std::vector<int> types(rec.types, rec.types + rec.ntypes);
-    if (md == 0) // the solid records no pressure (no elastic station asks)
-      for (int &t : types)
-        if (t == SMX_SEIS_PRESSURE)
-          t = SMX_SEIS_ACCELERATION;
-    rr.types = types.data();
+    rr.types = rec.types;
     rr.nstep_between_samples = rec.nstep_between_samples;
     A[md] = std::make_unique<HostAssembly2D>();
     assemble2d(sm, ss, rr, dt, nstep, *A[md], o);

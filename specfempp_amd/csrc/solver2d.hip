@@ -502,6 +502,8 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((
       const int l = trec_li[k], id = trec_entry[k];
       const real h = rent_h[id];
       for (int it = 0; it < ntypes; ++it) {
+        if (types.t[it] == 3) // pressure: k_seis_pressure
+          continue;
         const V2 f = types.t[it] == 0 ? s_u[l] : (types.t[it] == 1 ? s_v[l] : s_a[l]);
         real *o = seis_terms + (size_t(it) * nentries + id) * 2;
         o[0] = h * f.x;
@@ -602,7 +604,8 @@ __global__ void k_seis_fused(const real *__restrict__ U,
   const int grp = threadIdx.x >> 5, p = threadIdx.x & 31;
   const int pair = blockIdx.x * SEIS_GROUPS + grp;
   const int r = pair % nrec, it = pair / nrec;
-  const bool active = it < ntypes && p < NPT;
+  // pressure types are written by k_seis_pressure
+  const bool active = it < ntypes && p < NPT && types.t[it] != 3;
   if (active) {
     const int type = types.t[it];
     const int nentries = nrec * NPT;
@@ -647,6 +650,79 @@ __global__ void k_seis_fused(const real *__restrict__ U,
   }
 }
 
+// Pressure seismograms of elastic P-SV receivers
+// (medium/dim2/elastic/isotropic/wavefield.hpp:57-94): at every point of the
+// receiver's element p = -(lambda + 2/3 mu)(du_x/dx + du_z/dz), formed in
+// double from `real` lambda, mu and gradient and stored as `real`, then
+// s = sum_p h_p p in point order (compute_seismogram.tpp:117-166).  The
+// displacement read is the one at the end of the step: on the fused path
+// this kernel runs before the step's tile pass, when U holds the predicted
+// displacement that the corrector leaves unchanged; on the operator path it
+// runs after the corrector.  One 32-lane group per receiver; lane p < 25
+// evaluates point p.  Writes component 0 of every pressure type, 0 in
+// component 1.
+template <typename real>
+__global__ void k_seis_pressure(const real *__restrict__ U,
+                                const int *__restrict__ rent_g,
+                                const real *__restrict__ rent_h,
+                                const int *__restrict__ rec_slot,
+                                const real *__restrict__ geom,
+                                const real *__restrict__ mat, int ld,
+                                QuadArgs<real> q, int nrec, SeisTypes types,
+                                int ntypes, int nsbs,
+                                const int *__restrict__ step_ptr, int isig,
+                                real *__restrict__ out) {
+  using V2 = typename Vec2<real>::T;
+  __shared__ real s_t[SEIS_GROUPS][NPT];
+  if (step_ptr) {
+    const int step = *step_ptr;
+    if (step % nsbs != 0)
+      return;
+    isig = step / nsbs;
+  }
+  const int grp = threadIdx.x >> 5, p = threadIdx.x & 31;
+  const int r = blockIdx.x * SEIS_GROUPS + grp;
+  const bool active = r < nrec && p < NPT;
+  const V2 *__restrict__ U2 = reinterpret_cast<const V2 *>(U);
+  if (active) {
+    const int iz = p / NGLL, ix = p - iz * NGLL;
+    const int base = r * NPT;
+    real dxi_x = 0, dxi_z = 0, dga_x = 0, dga_z = 0;
+#pragma unroll
+    for (int l = 0; l < NGLL; ++l) {
+      const V2 a = U2[rent_g[base + iz * NGLL + l]]; // point (iz, l)
+      const V2 b = U2[rent_g[base + l * NGLL + ix]]; // point (l, ix)
+      dxi_x += q.hp[ix][l] * a.x;
+      dxi_z += q.hp[ix][l] * a.y;
+      dga_x += q.hp[iz][l] * b.x;
+      dga_z += q.hp[iz][l] * b.y;
+    }
+    const size_t pld = size_t(NPT) * ld;
+    const size_t o = size_t(p) * ld + rec_slot[r];
+    const real xix = geom[o], xiz = geom[pld + o], gax = geom[2 * pld + o],
+               gaz = geom[3 * pld + o];
+    const real l2m = mat[o], mu = mat[pld + o];
+    const real lam = l2m - real(2.0) * mu;
+    const real du_xx = xix * dxi_x + gax * dga_x;
+    const real du_zz = xiz * dxi_z + gaz * dga_z;
+    const double coef = double(lam) + (2.0 / 3.0) * double(mu);
+    const real pr = real(-1.0 * (coef * double(du_xx + du_zz)));
+    s_t[grp][p] = rent_h[base + p] * pr;
+  }
+  __syncthreads();
+  if (active && p == 0) {
+    real s = 0;
+    for (int k = 0; k < NPT; ++k)
+      s += s_t[grp][k];
+    for (int it = 0; it < ntypes; ++it)
+      if (types.t[it] == 3) {
+        const size_t o = ((size_t(isig) * ntypes + it) * nrec + r) * 2;
+        out[o] = s;
+        out[o + 1] = 0;
+      }
+  }
+}
+
 // Operator-path seismograms from the finished fields: s = sum_p h_p F[ibool].
 template <typename real>
 __global__ void k_seismo(const real *__restrict__ U, const real *__restrict__ V,
@@ -657,7 +733,7 @@ __global__ void k_seismo(const real *__restrict__ U, const real *__restrict__ V,
                          real *__restrict__ out) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = t % nrec, it = t / nrec;
-  if (it >= ntypes)
+  if (it >= ntypes || types.t[it] == 3) // pressure: k_seis_pressure
     return;
   const int type = types.t[it];
   const real *F = type == 0 ? U : (type == 1 ? V : A);
@@ -1211,6 +1287,7 @@ public:
     rent_g_ = upload_raw<int>(A.rent_g);
     rent_shared_ = upload_raw<int>(A.rent_shared);
     rent_h_ = upload_real(A.rent_h);
+    rec_slot_ = upload_raw<int>(A.rec_ispec);
     minv_ = alloc<real>(n2_);
     part_ = alloc<real>(2 * std::max<size_t>(1, A.sh_list.size()));
     cpl_li_ = upload_raw<int>(A.cpl_li);
@@ -1258,6 +1335,9 @@ public:
       throw SmxError(SMX_ERR_ARG, "at most 4 seismogram types");
     for (int k = 0; k < 4; ++k)
       stypes_.t[k] = k < ntypes ? A.seis_types[k] : 0;
+    has_pressure_ = A.nrec > 0 &&
+                    std::find(A.seis_types.begin(), A.seis_types.end(),
+                              int(SMX_SEIS_PRESSURE)) != A.seis_types.end();
     dt_ = A.dt;
     dto2_ = A.dt_over2;
     dt2o2_ = A.dt2_over2;
@@ -1358,6 +1438,8 @@ public:
     k_seismo<real><<<dim3((n + 127) / 128), dim3(128), 0, stream_>>>(
         F.u, F.v, F.a, rent_g_, rent_h_, A.nrec, stypes_, ntypes, isig, seis_);
     SMX_HIP(hipGetLastError());
+    if (has_pressure_)
+      launch_pressure(F.u, nullptr, isig);
   }
 
   void run_forward(int istep0, int nstep) override {
@@ -1788,6 +1870,15 @@ private:
         chi_dd_);
     SMX_HIP(hipGetLastError());
   }
+  void launch_pressure(const real *U, const int *step_ptr, int isig) {
+    const HostAssembly2D &A = *asm_;
+    k_seis_pressure<real><<<dim3((A.nrec + SEIS_GROUPS - 1) / SEIS_GROUPS),
+                            dim3(32 * SEIS_GROUPS), 0, stream_>>>(
+        U, rent_g_, rent_h_, rec_slot_, geom_, mat_, A.nspec_pad, quad_,
+        A.nrec, stypes_, int(A.seis_types.size()), A.nsbs, step_ptr, isig,
+        seis_);
+    SMX_HIP(hipGetLastError());
+  }
   template <int MODE>
   void launch_shared(real *U, real *V, real *Aout, int *step_ptr,
                      real *rec = nullptr) {
@@ -2003,6 +2094,8 @@ private:
     const HostAssembly2D &A = *asm_;
     const int ntypes = int(A.seis_types.size());
     const bool seis = A.nrec > 0 && ntypes > 0;
+    if (has_pressure_) // reads the displacement before the tiles advance it
+      launch_pressure(F.u, step_, 0);
     launch_tiles<MODE>(F.u, F.v, F.a, nullptr, 0, step_,
                        seis ? seis_terms_ : nullptr);
     if (seis) {
@@ -2036,7 +2129,8 @@ private:
   int *sh_list_ = nullptr, *ibool_ = nullptr, *sslot_ = nullptr,
       *src_li_ = nullptr, *src_idx_ = nullptr, *trec_li_ = nullptr,
       *trec_entry_ = nullptr, *rent_g_ = nullptr, *rent_shared_ = nullptr,
-      *step_ = nullptr;
+      *step_ = nullptr, *rec_slot_ = nullptr;
+  bool has_pressure_ = false; // elastic pressure seismograms requested
   uint16_t *lidx_ = nullptr;
   uint8_t *ecolor_ = nullptr;
   real *geom_e_ = nullptr, *mat_e_ = nullptr;

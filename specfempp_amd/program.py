@@ -12,9 +12,18 @@ Keys follow the reference's parameter files (include/parameter_parser/*):
     simulation-setup:
       quadrature: {quadrature-type: GLL4}
       solver: {time-marching: {time-scheme: {type: Newmark, dt, nstep}}}
-      simulation-mode: {forward: {writer: {seismogram: {output-format: ascii,
-                                                        output-folder}}}}
+      simulation-mode: {forward: {writer: {seismogram: {format: ascii,
+                                                        directory}}}}
     receivers: {stations, angle, seismogram-type: [...], nstep_between_samples}
+
+Enumerated strings compare case-insensitively with the reference's aliases
+(core/specfem/macros/config_strings.hpp): ``ascii``/``txt``,
+``displacement``/``disp``/``displ``/``d`` and the like.  The writer's
+``directory`` defaults to ``base_dir`` and must exist, its ``format`` to
+ASCII (src/parameter_parser/writer/seismogram.cpp:11-56); ``output-format``
+and ``output-folder`` are read as aliases.  The receiver ``angle`` is kept
+and not applied, as the reference stores it without using it
+(core/specfem/assembly/receivers/dim2/receivers.cpp:97-101).
     databases: {mesh-database}
     sources: <path to a sources YAML>
 
@@ -26,9 +35,8 @@ resolved against ``base_dir`` (the reference resolves them against the
 working directory).  Only what this build implements is accepted (elastic
 isotropic P-SV or acoustic isotropic meshes, GLL4, Newmark, forward mode,
 force and moment-tensor sources with Ricker, Dirac or Heaviside time
-functions, ASCII seismograms of
-displacement, velocity, acceleration and, in acoustic meshes, pressure);
-anything else raises ``ValueError`` naming the key.
+functions, ASCII seismograms of displacement, velocity, acceleration and
+pressure); anything else raises ``ValueError`` naming the key.
 """
 
 from __future__ import annotations
@@ -44,7 +52,49 @@ from .config import (SRC_FORCE, SRC_MOMENT, STF_DIRAC, STF_HEAVISIDE, STF_RICKER
 from .meshdb import read_mesh_database_2d
 from .seismo_io import write_seismograms
 
-_SEISMO = {"displacement", "velocity", "acceleration", "pressure"}
+# seismogram types and their aliases (config_strings.hpp:25-58), lower case
+_SEISMO_ALIASES = {
+    "displacement": ("displacement", "disp", "displ", "d"),
+    "velocity": ("velocity", "vel", "v", "veloc"),
+    "acceleration": ("acceleration", "acc", "a", "accel"),
+    "pressure": ("pressure", "p", "pres"),
+}
+_UNBUILT_SEISMO = ("rotation", "r", "rot", "intrinsic_rotation", "intrinsic_rot",
+                   "intrinsic_r", "intrinsic_rotat", "intrinsic-rotation", "ir", "curl", "crl")
+_ASCII = ("ascii", "txt")
+_SU = ("su", "seismic_unix", "seismic-unix")
+_GLL4 = ("gll4", "gll-4", "gll_4")
+
+
+def seismogram_type(name):
+    """The wavefield type a seismogram-type string names
+    (src/parameter_parser/receivers.cpp:14-48)."""
+    low = str(name).lower()
+    for t, aliases in _SEISMO_ALIASES.items():
+        if low in aliases:
+            return t
+    if low in _UNBUILT_SEISMO:
+        raise ValueError(f"seismogram-type {name!r}: rotation and curl seismograms belong to "
+                         "media this build does not implement")
+    raise ValueError(f"seismogram-type {name!r}: unknown seismogram type")
+
+
+def writer_settings(node, base_dir="."):
+    """(format, directory) of a seismogram writer node as the reference
+    reads it: ``directory`` (default: the working directory, here
+    ``base_dir``) must exist; ``format`` (default ASCII) must name ASCII —
+    Seismic Unix is declared but not implemented there either."""
+    node = node or {}
+    fmt = str(node.get("format", node.get("output-format", "ASCII")))
+    folder = str(node.get("directory", node.get("output-folder", ".")))
+    folder = folder if os.path.isabs(folder) else os.path.join(base_dir, folder)
+    if not os.path.isdir(folder):
+        raise ValueError(f"Output folder : {folder} does not exist.")
+    if fmt.lower() in _SU:
+        raise ValueError("Seismic Unix format not implemented yet")
+    if fmt.lower() not in _ASCII:
+        raise ValueError(f"seismogram format {fmt!r}: unknown seismogram format")
+    return "ascii", folder
 
 
 @dataclass
@@ -62,6 +112,7 @@ class Program:
     stations: Stations
     output_folder: str
     title: str = ""
+    receiver_angle: float = 0.0
     extra: dict = field(default_factory=dict)
 
 
@@ -152,7 +203,7 @@ def parse_config(params, base_dir="."):
 
     setup = _get(p, "simulation-setup")
     qt = _get(setup, "quadrature", "quadrature-type", required=False, default="GLL4")
-    if qt != "GLL4":
+    if str(qt).lower() not in _GLL4:
         raise ValueError(f"quadrature-type {qt!r}: only GLL4 (NGLL = 5) is built")
     ts = _get(setup, "solver", "time-marching", "time-scheme")
     if str(ts.get("type", "Newmark")).lower() != "newmark":
@@ -163,24 +214,18 @@ def parse_config(params, base_dir="."):
         raise ValueError(f"simulation-mode {list(mode)}: only forward runs are driven from YAML "
                          "(gradients: specfempp_amd.adjoint)")
     seis_w = _get(mode, "forward", "writer", "seismogram", required=False, default={}) or {}
-    fmt = seis_w.get("output-format", "ascii")
-    if fmt != "ascii":
-        raise ValueError(f"seismogram output-format {fmt!r}: only ascii is written")
-    out = path(str(seis_w.get("output-folder", ".")))
+    _, out = writer_settings(seis_w, base_dir)
 
     rec = _get(p, "receivers")
     st = read_stations(path(rec["stations"]))
-    types = tuple(rec.get("seismogram-type", ["displacement"]))
-    bad = [t for t in types if t not in _SEISMO]
-    if bad:
-        raise ValueError(f"seismogram-type {bad}: only displacement, velocity, acceleration, "
-                         "pressure")
+    types = tuple(seismogram_type(t) for t in rec.get("seismogram-type", ["displacement"]))
     nsbs = int(rec.get("nstep_between_samples", 1))
-    if float(rec.get("angle", 0.0)) != 0.0:
-        raise ValueError("receiver angle: only 0 is supported")
+    angle = float(rec.get("angle", 0.0))  # stored, not applied (receivers.cpp:97-101)
 
     mesh, info = read_mesh_database_2d(path(_get(p, "databases", "mesh-database")))
     src_node = p.get("sources")
+    if src_node is None:  # setup.cpp:50-55
+        raise ValueError("Error reading specfem source configuration: no parameters/sources")
     if isinstance(src_node, str):
         with open(path(src_node)) as fh:
             src_node = yaml.safe_load(fh)
@@ -191,7 +236,7 @@ def parse_config(params, base_dir="."):
     title = str(_get(p, "header", "title", required=False, default=""))
     cfg = Config2D(title or "yaml", mesh, sources, receivers, dt, nstep, nsbs=nsbs,
                    seismo_types=types, meta={"database": info})
-    return Program(cfg, st, out, title)
+    return Program(cfg, st, out, title, receiver_angle=angle)
 
 
 class _State:

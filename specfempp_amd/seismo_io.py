@@ -98,13 +98,16 @@ def read_seismograms(folder, stations, seismo_types, dim=2, dt=None):
     return times, out
 
 
-def reference_error(ours, theirs):
-    """The reference's criterion: the worst, over types and stations, of
-    sum|ours - theirs| / sum|theirs| taken over time and components
-    (tests/unit-tests/displacement_tests/Newmark/dim2/newmark_tests.cpp:
-    337-362); both (nsig, ntypes, nrec, ncomp)."""
+def reference_error(ours, theirs, per_type=False):
+    """The reference's criterion: per station, sum|ours - theirs| over every
+    seismogram type, component and sample divided by sum|theirs| over the
+    same; the worst station (tests/unit-tests/displacement_tests/Newmark/
+    dim2/newmark_tests.cpp:189-198,337-362).  ``per_type=True`` takes each
+    (type, station) pair on its own instead, which is never looser.  Both
+    arrays (nsig, ntypes, nrec, ncomp)."""
     a = np.asarray(ours, dtype=np.float64)
     b = np.asarray(theirs, dtype=np.float64)
-    num = np.abs(a - b).sum(axis=(0, 3))
-    den = np.abs(b).sum(axis=(0, 3))
+    axes = (0, 3) if per_type else (0, 1, 3)
+    num = np.abs(a - b).sum(axis=axes)
+    den = np.abs(b).sum(axis=axes)
     return float((num / den).max())

@@ -29,13 +29,17 @@ TYPE_NAMES = {"d": "displacement", "v": "velocity", "a": "acceleration"}
 # four sides with four vertical forces; free edges with a database holding
 # the adjacency map and one force; one moment tensor Mxz = 1 with a Ricker
 # (f0 = 10 Hz, factor 1e8) or a Heaviside (hdur = 0.1 s, factor 1e8, 500
-# steps) time function.  Each entry: sources and the seismogram types whose
-# per-type reference metric is pinned below 1e-3.  The Heaviside fixture's
-# acceleration sits at 1.0e-2 (7.9e-3 with the FP64 oracle, whose
-# displacement agrees to 1.4e-6): the reference's criterion pools d, v, a
-# and the elastic pressure of each station (newmark_tests.cpp:195-362), and
-# the elastic pressure seismogram is not part of this build, so the
-# acceleration of that fixture is checked only against the oracle.
+# steps) time function.  The two moment-tensor fixtures also record the
+# elastic pressure (PRE traces).  Each entry: sources and the seismogram
+# types whose per-type reference metric is pinned below 1e-3; every type of
+# every fixture is pinned by the reference's own pooled per-station metric.
+# The Heaviside fixture's acceleration is the one type left out of the
+# per-type pin: it sits at 7.9e-3 with the FP64 oracle and 1.0e-2 with the
+# FP32 one, because the Heaviside's step drives the modes above the mesh's
+# resolution (100-200 Hz, where the source has no energy) with rounding
+# noise alone, and the acceleration weights them by omega^2.  In the signal
+# band (below 50 Hz) it agrees to 3.6e-4; test_oracle_sources.py holds the
+# checks that show this.
 DB_FIXTURES = {
     "HomogeneousElasticDomainStaceyBC": dict(
         sources=lambda: Sources2D.ricker([500.0, 3500.0, 500.0, 3500.0],
@@ -48,12 +52,22 @@ DB_FIXTURES = {
     "HomogeneousElasticDomainMomentTensor": dict(
         sources=lambda: Sources2D.moment_tensor([2500.0], [2500.0], 0.0, 0.0, 1.0,
                                                 f0=10.0, factor=1e8),
-        nstep=300, pinned="dva"),
+        nstep=300, pinned="dvap"),
     "HomogeneousElasticDomainMomentTensorHeaviside": dict(
         sources=lambda: Sources2D.moment_tensor([2500.0], [2500.0], 0.0, 0.0, 1.0,
                                                 hdur=0.1, factor=1e8),
-        nstep=500, pinned="dv"),
+        nstep=500, pinned="dvp"),
 }
+
+
+def _append_pressure(g, types, traces):
+    """Adds a 'pressure' type (component 0 = the PRE trace, component 1 = 0)
+    when the fixture's npz holds one."""
+    if "pressure" not in g.files:
+        return types, traces
+    p = np.zeros((1,) + traces.shape[1:])
+    p[0, :, 0, :] = g["pressure"]
+    return types + ("pressure",), np.concatenate([traces, p], axis=0)
 
 
 def db_fixture_case(name):
@@ -65,11 +79,13 @@ def db_fixture_case(name):
     mesh, _ = read_mesh_database_2d(os.path.join(GOLDEN, f"specfem2d_{name}.database.bin.gz"))
     rec = Receivers2D(x=g["x"], z=g["z"], names=tuple(g["names"]))
     letters = [str(t) for t in g["types"]]
-    types = tuple(TYPE_NAMES[t] for t in letters)
+    types, traces = _append_pressure(g, tuple(TYPE_NAMES[t] for t in letters), g["traces"])
+    if len(types) > len(letters):
+        letters.append("p")
     cfg = Config2D(name, mesh, spec["sources"](), rec, 1.1e-3, spec["nstep"],
                    seismo_types=types)
     pinned = [i for i, t in enumerate(letters) if t in spec["pinned"]]
-    return cfg, g["traces"], pinned
+    return cfg, traces, pinned
 
 
 def fixture_case(name):
@@ -80,9 +96,9 @@ def fixture_case(name):
     src = Sources2D.ricker([s[0] for s in spec["sources"]], [s[1] for s in spec["sources"]],
                            f0=10.0, factor=1e10)
     rec = Receivers2D(x=g["x"], z=g["z"], names=tuple(g["names"]))
-    types = tuple(TYPE_NAMES[t] for t in g["types"])
+    types, traces = _append_pressure(g, tuple(TYPE_NAMES[t] for t in g["types"]), g["traces"])
     cfg = Config2D(name, mesh, src, rec, 1.1e-3, 300, seismo_types=types)
-    return cfg, g
+    return cfg, dict(g, traces=traces)
 
 
 # The three acoustic SPECFEM2D fixtures of the same test list: the 80x60 Q9
@@ -382,6 +398,24 @@ def reference_metric(ours, traces):
             a = np.asarray(ours[:, it, ista, :], dtype=np.float64).T
             b = traces[it, ista]
             worst = max(worst, float(np.abs(a - b).sum() / np.abs(b).sum()))
+    return worst
+
+
+def pooled_metric(ours, traces):
+    """The reference's own criterion: per station, sum|s - ref| over every
+    seismogram type, component and sample divided by sum|ref| over the
+    same, worst station (newmark_tests.cpp:195-198,337-362).  The elastic
+    pressure, being the largest in magnitude, dominates it where recorded.
+    ours: (nt, ntypes, nsta, 2); traces: (ntypes, nsta, 2, nt)."""
+    worst = 0.0
+    for ista in range(traces.shape[1]):
+        err = nrm = 0.0
+        for it in range(traces.shape[0]):
+            a = np.asarray(ours[:, it, ista, :], dtype=np.float64).T
+            b = traces[it, ista]
+            err += float(np.abs(a - b).sum())
+            nrm += float(np.abs(b).sum())
+        worst = max(worst, err / nrm)
     return worst
 
 

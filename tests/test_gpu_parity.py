@@ -90,6 +90,31 @@ def test_specfem2d_fixtures(oracle_lib, name):
     assert metric < 1e-3
 
 
+@pytest.mark.parametrize("fused", [True, False])
+def test_sampling_every_third_step_with_pressure(oracle_lib, fused):
+    """nstep_between_samples = 3 over 301 steps (not a multiple of 3) with all
+    four elastic types (displacement, velocity, acceleration and the elastic
+    pressure), on the fused and the operator-by-operator path, against the
+    FP32 oracle sampled the same way (sample k = the state after step 3k):
+    displacement and pressure within 1e-5, velocity and acceleration within
+    twice the oracle's own FP32-vs-FP64 floor (at least 1e-5)."""
+    import dataclasses
+    cfg0, _ = fixture_case("HomogeneousElasticDomain")
+    cfg = dataclasses.replace(cfg0, nstep=301, nsbs=3)
+    _say(f"nsbs=3, 301 steps, d/v/a/p, fused={fused}")
+    ours, ctx = simulate(cfg, fused=fused)
+    ref, _ = _oracle_run(oracle_lib, cfg)
+    ref64, _ = _oracle_run(oracle_lib, cfg, precision=8)
+    assert ours.shape == ref.shape == (101,) + ref.shape[1:]
+    assert np.all(ours[:, cfg.seismo_types.index("pressure"), :, 1] == 0)
+    for it, t in enumerate(cfg.seismo_types):
+        err = rel_l2(ours[:, it], ref[:, it])
+        floor = rel_l2(ref[:, it], ref64[:, it])
+        _say(f"  {t}: rel L2 vs oracle = {err:.3e} (floor {floor:.3e})")
+        bound = 1e-5 if t in ("displacement", "pressure") else max(1e-5, 2.0 * floor)
+        assert err <= bound, t
+
+
 def test_layered_matches_oracle(oracle_lib):
     cfg = small_layered(nstep=400)
     _say("forward layered 120x90, 400 steps")

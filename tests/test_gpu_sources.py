@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from specfempp_amd.solver import simulate
-from tests._cases import DB_FIXTURES, db_fixture_case, reference_metric, rel_l2
+from tests._cases import DB_FIXTURES, db_fixture_case, pooled_metric, reference_metric, rel_l2
 
 pytestmark = pytest.mark.gpu
 
@@ -36,7 +36,13 @@ def test_database_fixture_on_gpu(oracle_lib, name):
     tol = max(1e-5, 2.0 * floor)
     e_or = rel_l2(ours[:, 0], ref[:, 0])
     e_sp = [reference_metric(ours[:, i:i + 1], traces[i:i + 1]) for i in pinned]
+    pooled = pooled_metric(ours, traces)
     _say(f"  vs oracle {e_or:.3e} (bound {tol:.3e}); vs SPECFEM2D per pinned type "
-         f"{np.round(e_sp, 7)}")
+         f"{np.round(e_sp, 7)}, pooled over {cfg.seismo_types} {pooled:.3e}")
     assert e_or <= tol
     assert max(e_sp) < 1e-3
+    assert pooled < 1e-3
+    if "pressure" in cfg.seismo_types:
+        ip = cfg.seismo_types.index("pressure")
+        assert rel_l2(ours[:, ip], ref[:, ip]) <= max(
+            1e-5, 2.0 * rel_l2(ref[:, ip], ref64[:, ip]))

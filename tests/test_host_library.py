@@ -122,7 +122,7 @@ def test_acoustic_mesh_model():
 
 
 @pytest.mark.skipif(not os.path.exists(_lib.LIB_PATH), reason="library not built")
-def test_acoustic_set_up_errors_are_host_side():
+def test_acoustic_set_up_errors_are_host_side(monkeypatch):
     """Invalid acoustic set-ups are rejected by the host assembly, before any
     device call, with the reference's wording of what is missing."""
     import dataclasses
@@ -130,8 +130,8 @@ def test_acoustic_set_up_errors_are_host_side():
     from specfempp_amd.solver import Context
     c = _acoustic_cfg()
     # a coupled mesh (one solid element at the bottom-left corner): a
-    # free-surface entity on the solid element, or pressure asked of a
-    # station in the solid, are rejected before any device call
+    # free-surface entity on the solid element, or a moment-tensor source
+    # given without its components, are rejected before any device call
     mu = c.mesh.mu.copy()
     mu[0] = 1e9
     mixed = dataclasses.replace(c.mesh, mu=mu, free_ispec=np.array([0], np.int32),
@@ -140,13 +140,13 @@ def test_acoustic_set_up_errors_are_host_side():
         Context(mixed, c.sources, c.receivers, c.dt, c.nstep)
     mixed = dataclasses.replace(mixed, free_ispec=np.zeros(0, np.int32),
                                 free_type=np.zeros(0, np.int32))
-    solid_station = Receivers2D(x=np.array([20.0]), z=np.array([20.0]))
-    with pytest.raises(_lib.SmxError, match="pressure"):
-        Context(mixed, c.sources, solid_station, c.dt, c.nstep, seismo_types=("pressure",))
-    # pressure seismograms need acoustic elements
+    import specfempp_amd.solver as solver_mod
+    with monkeypatch.context() as mp:  # NULL moment array across the C ABI
+        mp.setattr(solver_mod, "_moments", lambda s: np.zeros((0, 3), np.float32))
+        mt = dataclasses.replace(c.sources, kind=np.array([1, 0, 0], np.int32))
+        with pytest.raises(_lib.SmxError, match="moment"):
+            Context(mixed, mt, c.receivers, c.dt, c.nstep)
     e = homogeneous_box(nstep=5)
-    with pytest.raises(_lib.SmxError, match="pressure"):
-        Context(e.mesh, e.sources, e.receivers, e.dt, e.nstep, seismo_types=("pressure",))
     # a free surface on an elastic mesh
     fe = dataclasses.replace(e.mesh, free_ispec=np.array([0], np.int32),
                              free_type=np.array([1], np.int32))

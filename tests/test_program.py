@@ -4,6 +4,7 @@ database.bin): the parsed run equals the restated fixture case, and the
 oracle on it meets the SPECFEM2D traces; on the GPU, _initialize / _execute
 / _finalize write reference-format seismograms that meet them too."""
 
+import gzip
 import os
 import shutil
 
@@ -34,8 +35,8 @@ parameters:
       forward:
         writer:
           seismogram:
-            output-format: ascii
-            output-folder: "OUTPUT_FILES"
+            format: ascii
+            directory: "OUTPUT_FILES"
   receivers:
     stations: "fixture/STATIONS"
     angle: 0.0
@@ -72,6 +73,7 @@ def run_dir(tmp_path):
     with open(d / "sources.yaml", "w") as fh:
         yaml.safe_dump(src, fh)
     shutil.copy(os.path.join(GOLDEN, f"specfem2d_{NAME}.database.bin.gz"), d / "database.bin.gz")
+    (tmp_path / "OUTPUT_FILES").mkdir()
     return tmp_path, g
 
 
@@ -93,7 +95,7 @@ def test_unsupported_settings_are_named(run_dir):
     base, _ = run_dir
     for bad, key in (("quadrature-type: GLL4", "quadrature-type: GLL7"),
                      ("type: Newmark", "type: LeapFrog"),
-                     ("output-format: ascii", "output-format: hdf5")):
+                     ("format: ascii", "format: hdf5")):
         with pytest.raises(ValueError, match=key.split(":")[0]):
             program.parse_config(yaml.safe_load(CONFIG.replace(bad, key)), base_dir=str(base))
 
@@ -143,6 +145,7 @@ def acoustic_run_dir(tmp_path):
         yaml.safe_dump(src, fh)
     shutil.copy(os.path.join(GOLDEN, f"specfem2d_{ACOUSTIC}.database.bin.gz"),
                 d / "database.bin.gz")
+    (tmp_path / "OUTPUT_FILES").mkdir()
     types = "\n".join(f"      - {t}" for t in ACOUSTIC_TYPES)
     cfg = CONFIG.replace("      - displacement", types)
     return tmp_path, g, cfg
@@ -164,7 +167,7 @@ def test_acoustic_yaml_run_is_the_fixture_case(oracle_lib, acoustic_run_dir):
     o = oracle_lib.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
     np.testing.assert_allclose(o.t0 + cfg.dt * np.arange(cfg.nstep), g["times"], atol=1e-6)
     s = o.run(types=cfg.seismo_types)
-    assert reference_error(s, _acoustic_traces(g)) < 5e-4
+    assert reference_error(s, _acoustic_traces(g), per_type=True) < 5e-4
 
 
 @pytest.mark.gpu
@@ -179,8 +182,8 @@ def test_acoustic_execute_writes_pressure_seismograms(acoustic_run_dir):
     assert (base / "OUTPUT_FILES" / f"AA.{g['names'][0]}.S2.PRE.semp").exists()
     t, s = read_seismograms(base / "OUTPUT_FILES", st, ACOUSTIC_TYPES)
     np.testing.assert_allclose(t, g["times"], atol=1e-6)
-    err = reference_error(s, _acoustic_traces(g))
-    print("acoustic YAML run vs SPECFEM2D (reference metric):", err)
+    err = reference_error(s, _acoustic_traces(g), per_type=True)
+    print("acoustic YAML run vs SPECFEM2D (reference metric, per type):", err)
     assert err < 5e-4
 
 
@@ -207,6 +210,7 @@ def test_coupled_yaml_run_from_the_reference_database(tmp_path):
     shutil.copy(os.path.join(GOLDEN, f"specfem2d_{COUPLED}.database.bin.gz"),
                 d / "database.bin.gz")
     text = CONFIG.replace("dt: 1.1e-3", "dt: 0.85e-3").replace("nstep: 300", "nstep: 600")
+    (tmp_path / "OUTPUT_FILES").mkdir()
     assert program._initialize([]) is True
     try:
         assert program._execute(text, "", base_dir=str(tmp_path)) is True
@@ -258,3 +262,126 @@ sources:
     with pytest.raises(ValueError, match="time functions"):
         read_sources({"sources": [{"moment-tensor": {"x": 0, "z": 0, "Mxx": 1, "Mzz": 1,
                                                      "Mxz": 0, "External": {}}}]})
+
+
+# ---- the reference fixtures' own run files, unchanged ----------------------
+# Each case's specfem_config.yaml, sources file and STATIONS as the reference
+# holds them, laid out under the reference tests' working directory
+# (tests/unit-tests, which the configs' relative paths start from), with the
+# case's database.bin.
+REF_CASES = sorted(os.listdir(os.path.join(GOLDEN, "configs")))
+REF_REL = "displacement_tests/Newmark/serial/dim2"
+_LETTERS = {"d": "displacement", "v": "velocity", "a": "acceleration"}
+
+
+def _reference_layout(root, case):
+    d = root / REF_REL / case
+    d.mkdir(parents=True)
+    src = os.path.join(GOLDEN, "configs", case)
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), d / f)
+    with gzip.open(os.path.join(GOLDEN, f"specfem2d_{case}.database.bin.gz")) as fi, \
+            open(d / "database.bin", "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    with open(d / "specfem_config.yaml") as fh:
+        return fh.read()
+
+
+def _expected_types(g):
+    types = tuple(_LETTERS[str(t)] for t in g["types"])
+    return types + (("pressure",) if "pressure" in g.files else ())
+
+
+@pytest.mark.parametrize("case", REF_CASES)
+def test_reference_config_parses_unchanged(tmp_path, case):
+    """Every fixture's own specfem_config.yaml (writer keys format /
+    directory, receiver angle, its sources file and STATIONS) parses
+    unchanged into the run its traces were made with: stations, seismogram
+    types (pressure included), step count, and the output folder '.'."""
+    text = _reference_layout(tmp_path, case)
+    prog = program.parse_config(yaml.safe_load(text), base_dir=str(tmp_path))
+    g = np.load(os.path.join(GOLDEN, f"specfem2d_{case}.npz"))
+    cfg = prog.config
+    np.testing.assert_allclose(cfg.receivers.x, g["x"])
+    np.testing.assert_allclose(cfg.receivers.z, g["z"])
+    assert list(prog.stations.names) == [str(n) for n in g["names"]]
+    assert cfg.seismo_types == _expected_types(g)
+    assert cfg.nstep == len(g["times"])
+    assert os.path.samefile(prog.output_folder, tmp_path)
+    assert prog.receiver_angle == 0.0
+
+
+def test_writer_keys_and_aliases(tmp_path):
+    """The writer as the reference reads it: format compared without case
+    (ASCII / txt), Seismic Unix declared but not implemented, an unknown
+    format and a missing directory refused; seismogram types by their
+    aliases; a nonzero receiver angle kept and not applied."""
+    (tmp_path / "out").mkdir()
+    assert program.writer_settings({"format": "ASCII", "directory": "out"}, str(tmp_path)) == (
+        "ascii", str(tmp_path / "out"))
+    assert os.path.samefile(program.writer_settings({"format": "txt"}, str(tmp_path))[1],
+                            tmp_path)
+    assert program.writer_settings({"output-folder": "out"}, str(tmp_path))[1].endswith("out")
+    with pytest.raises(ValueError, match="Seismic Unix"):
+        program.writer_settings({"format": "SU"}, str(tmp_path))
+    with pytest.raises(ValueError, match="unknown seismogram format"):
+        program.writer_settings({"format": "HDF5"}, str(tmp_path))
+    with pytest.raises(ValueError, match="does not exist"):
+        program.writer_settings({"directory": "missing"}, str(tmp_path))
+    assert [program.seismogram_type(t) for t in ("Displ", "VEL", "a", "PRES")] == [
+        "displacement", "velocity", "acceleration", "pressure"]
+    with pytest.raises(ValueError, match="rotation and curl"):
+        program.seismogram_type("rotation")
+    text = _reference_layout(tmp_path, "HomogeneousElasticDomainMomentTensor")
+    prog = program.parse_config(yaml.safe_load(text.replace("angle: 0.0", "angle: 30.0")),
+                                base_dir=str(tmp_path))
+    assert prog.receiver_angle == 30.0
+
+
+def test_reference_config_run_meets_specfem2d(oracle_lib, tmp_path):
+    """The moment-tensor fixture's own files, unchanged, run on the oracle:
+    every type (d, v, a and the elastic pressure) meets the SPECFEM2D traces
+    by the reference's pooled per-station criterion."""
+    case = "HomogeneousElasticDomainMomentTensor"
+    text = _reference_layout(tmp_path, case)
+    prog = program.parse_config(yaml.safe_load(text), base_dir=str(tmp_path))
+    cfg = prog.config
+    g = np.load(os.path.join(GOLDEN, f"specfem2d_{case}.npz"))
+    o = oracle_lib.Oracle2D(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    s = o.run(types=cfg.seismo_types)
+    theirs = np.transpose(g["traces"], (3, 0, 1, 2))
+    p = np.zeros(theirs.shape[:1] + (1,) + theirs.shape[2:])
+    p[:, 0, :, 0] = g["pressure"].T
+    theirs = np.concatenate([theirs, p], axis=1)
+    assert reference_error(s, theirs) < 1e-3
+    assert reference_error(s, theirs, per_type=True) < 1e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("case", ["HomogeneousElasticDomain",
+                                  "HomogeneousElasticDomainMomentTensor",
+                                  "HomogeneousElasticDomainMomentTensorHeaviside"])
+def test_reference_config_execute_with_elastic_pressure(tmp_path, case):
+    """The three elastic fixtures that record pressure, run through
+    _initialize / _execute / _finalize from their own files unchanged: the
+    written d/v/a/PRE files meet the SPECFEM2D traces by the reference's
+    pooled criterion, and the pressure alone by 1e-3."""
+    text = _reference_layout(tmp_path, case)
+    g = np.load(os.path.join(GOLDEN, f"specfem2d_{case}.npz"))
+    assert program._initialize([]) is True
+    try:
+        assert program._execute(text, "", base_dir=str(tmp_path)) is True
+    finally:
+        assert program._finalize() is True
+    types = _expected_types(g)
+    st = [("AA", str(n)) for n in g["names"]]
+    t, s = read_seismograms(tmp_path, st, types)
+    np.testing.assert_allclose(t, g["times"], atol=1e-6)
+    theirs = np.transpose(g["traces"], (3, 0, 1, 2))
+    p = np.zeros(theirs.shape[:1] + (1,) + theirs.shape[2:])
+    p[:, 0, :, 0] = g["pressure"].T
+    theirs = np.concatenate([theirs, p], axis=1)
+    pooled = reference_error(s, theirs)
+    pres = reference_error(s[:, -1:], theirs[:, -1:], per_type=True)
+    print(f"{case}: pooled {pooled:.3e}, pressure {pres:.3e}", flush=True)
+    assert pooled < 1e-3 and pres < 1e-3

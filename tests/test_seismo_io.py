@@ -62,3 +62,15 @@ def test_pressure_files(tmp_path):
     assert len(paths) == 2 * 2 + 2
     _, back = read_seismograms(tmp_path, st, ("displacement", "pressure"))
     np.testing.assert_allclose(back, seis, rtol=1e-6, atol=1e-12)
+
+
+def test_reference_error_pools_types_per_station():
+    """The reference's criterion sums errors and norms over a station's types
+    before dividing; per_type divides per (type, station) pair."""
+    theirs = np.zeros((4, 2, 1, 2))
+    theirs[:, 0, 0, 0] = 1.0     # type 0: norm 4
+    theirs[:, 1, 0, 0] = 100.0   # type 1: norm 400
+    ours = theirs.copy()
+    ours[:, 0, 0, 0] += 0.5      # type 0 off by 2 in total
+    assert np.isclose(reference_error(ours, theirs), 2.0 / 404.0)
+    assert np.isclose(reference_error(ours, theirs, per_type=True), 0.5)
