@@ -300,6 +300,21 @@ int smx_corrector(smx_ctx *ctx, int field, int *dofs);
  * (include/kokkos_kernels/domain_kernels.hpp:72-141). */
 int smx_update_wavefields(smx_ctx *ctx, int field, int istep, int *nelem);
 
+/* smx_update_wavefields on caller-held device fields instead of the
+ * context's: u and v ([nglob][2], context precision) are read, a is updated
+ * in place (a := (a + f - K u - B v) / M for step istep).  No counterpart in
+ * the reference: a variant for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops).  It calls no
+ * allocator; the partial sums of points shared between tiles live in a
+ * fixed 16 MiB device array of the library, one per GPU (up to 2,097,152
+ * partial slots in FP32, 1,048,576 in FP64; larger meshes are refused with
+ * SMX_ERR_ARG).  Every call on a GPU uses that GPU's one array, so calls
+ * that overlap on one GPU (two contexts, two threads) race; nothing in the
+ * array needs to start at zero.  2D elastic contexts only (SMX_ERR_STATE
+ * otherwise); the context must be initialized. */
+int smx_update_wavefields_noalloc(smx_ctx *ctx, const void *u, const void *v,
+                                  void *a, int istep, int *nelem);
+
 /* Seismograms of the current fields into sample isig.
  * Replaces domain_kernels::compute_seismograms(isig)
  * (include/kokkos_kernels/domain_kernels.hpp:188-203). */

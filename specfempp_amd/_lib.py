@@ -30,6 +30,7 @@ EXPORTS = [
     "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
     "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots", "smx_ctx_compact",
     "smx_ctx_create3d", "smx_ctx_create3d_opts", "smx_ctx_geometry", "smx_ctx_components",
+    "smx_update_wavefields_noalloc",
 ]
 
 
@@ -116,6 +117,7 @@ def load():
         "smx_predictor": (i, [P, i, IP]),
         "smx_corrector": (i, [P, i, IP]),
         "smx_update_wavefields": (i, [P, i, i, IP]),
+        "smx_update_wavefields_noalloc": (i, [P, P, P, P, i, IP]),
         "smx_compute_seismograms": (i, [P, i, i]),
         "smx_run_forward": (i, [P, i, i]),
         "smx_synchronize": (i, [P]),

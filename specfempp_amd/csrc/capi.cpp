@@ -334,6 +334,14 @@ int smx_update_wavefields(smx_ctx *ctx, int field, int istep, int *nelem) {
   });
 }
 
+int smx_update_wavefields_noalloc(smx_ctx *ctx, const void *u, const void *v,
+                                  void *a, int istep, int *nelem) {
+  return guarded([&] {
+    const int n = S(ctx).update_wavefields_noalloc(u, v, a, istep);
+    if (nelem) *nelem = n;
+  });
+}
+
 int smx_compute_seismograms(smx_ctx *ctx, int field, int isig) {
   return guarded([&] { S(ctx).compute_seismograms(field, isig); });
 }

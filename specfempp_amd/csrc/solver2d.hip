@@ -33,6 +33,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -1251,6 +1252,37 @@ __global__ void k_adj_inject(real *__restrict__ UB,
 }
 
 // ========================================================================
+// Shared-point partials of smx_update_wavefields_noalloc: a fixed device
+// array (one copy per GPU, held until the process ends), so that entry point
+// calls no allocator.  Its address is looked up once per device, with that
+// device current; host code uses only that address, never the array's name.
+constexpr size_t kNoAllocScratchBytes = size_t(16) << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+void *noalloc_scratch(int device) {
+  static std::mutex mu;
+  static void *ptrs[kMaxDevices] = {};
+  if (device < 0 || device >= kMaxDevices)
+    throw SmxError(SMX_ERR_ARG, "device id outside the supported range");
+  std::lock_guard<std::mutex> lock(mu);
+  if (!ptrs[device]) {
+    int cur = 0;
+    SMX_HIP(hipGetDevice(&cur));
+    if (cur != device)
+      SMX_HIP(hipSetDevice(device));
+    void *p = nullptr;
+    const hipError_t err = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (cur != device)
+      SMX_HIP(hipSetDevice(cur));
+    if (err != hipSuccess || !p)
+      throw SmxError(SMX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") +
+                                      hipGetErrorString(err));
+    ptrs[device] = p;
+  }
+  return ptrs[device];
+}
+
 template <typename real> class Solver2D final : public Solver2DBase {
 public:
   Solver2D(std::unique_ptr<HostAssembly2D> a, int graph_steps,
@@ -1423,6 +1455,36 @@ public:
     Field &F = field(f);
     launch_tiles<MODE_ASSEMBLE>(F.u, F.v, F.a, nullptr, istep, nullptr);
     launch_shared<MODE_ASSEMBLE>(F.u, F.v, F.a, nullptr);
+    return A.nspec;
+  }
+
+  // The same two launches on caller-held fields, partials in `scratch`.
+  // Every partial slot is written by the tile pass before the shared-point
+  // pass reads it, so nothing in the scratch array needs clearing.
+  int update_wavefields_noalloc(const void *u, const void *v, void *a,
+                                int istep) override {
+    require_init();
+    const HostAssembly2D &A = *asm_;
+    if (istep < 0 || istep >= A.nstep)
+      throw SmxError(SMX_ERR_ARG, "istep outside [0, nstep)");
+    if (!u || !v || !a)
+      throw SmxError(SMX_ERR_ARG, "null field pointer");
+    const size_t slots = std::max<size_t>(1, A.sh_list.size());
+    const size_t need = 2 * slots * sizeof(real);
+    if (need > kNoAllocScratchBytes)
+      throw SmxError(SMX_ERR_ARG,
+                     "mesh needs " + std::to_string(need) +
+                         " bytes of shared-point partials; the no-alloc "
+                         "scratch holds " +
+                         std::to_string(kNoAllocScratchBytes) + " (" +
+                         std::to_string(kNoAllocScratchBytes / (2 * sizeof(real))) +
+                         " partial slots at this precision)");
+    real *part = static_cast<real *>(noalloc_scratch(device_));
+    real *U = const_cast<real *>(static_cast<const real *>(u));
+    real *V = const_cast<real *>(static_cast<const real *>(v));
+    real *Aout = static_cast<real *>(a);
+    launch_tiles<MODE_ASSEMBLE>(U, V, Aout, part, istep, nullptr);
+    launch_shared<MODE_ASSEMBLE>(U, V, Aout, nullptr, nullptr, part);
     return A.nspec;
   }
 
@@ -1881,12 +1943,13 @@ private:
   }
   template <int MODE>
   void launch_shared(real *U, real *V, real *Aout, int *step_ptr,
-                     real *rec = nullptr) {
+                     real *rec = nullptr, real *part = nullptr) {
     const HostAssembly2D &A = *asm_;
     const int n = std::max(1, A.nshared);
     k_shared<real, MODE><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS),
                            dim3(NODE_THREADS), 0, stream_>>>(
-        U, V, Aout, minv_, part_, shp_off_, shp_pos_, A.nshared, A.nex_total,
+        U, V, Aout, minv_, part ? part : part_, shp_off_, shp_pos_, A.nshared,
+        A.nex_total,
         dt_, dto2_, dt2o2_, step_ptr, rec, rec_stride(),
         std::max(1, rec_every_));
     SMX_HIP(hipGetLastError());

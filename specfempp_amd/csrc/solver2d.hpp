@@ -19,6 +19,14 @@ public:
   virtual int predictor(int field) = 0;
   virtual int corrector(int field) = 0;
   virtual int update_wavefields(int field, int istep) = 0;
+  // update_wavefields on caller-held device fields [nglob][2] (u, v read;
+  // a := (a + f - K u - B v) / M) with the shared-point partials in the
+  // library's fixed device scratch array: allocates nothing.
+  virtual int update_wavefields_noalloc(const void *, const void *, void *,
+                                        int) {
+    throw SmxError(SMX_ERR_STATE,
+                   "smx_update_wavefields_noalloc: 2D elastic contexts only");
+  }
   virtual void compute_seismograms(int field, int isig) = 0;
   virtual void run_forward(int istep0, int nstep) = 0;
   virtual void synchronize() = 0;
