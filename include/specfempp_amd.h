@@ -31,6 +31,23 @@ extern "C" {
 #define SMX_FIELD_ADJOINT 1
 #define SMX_FIELD_BACKWARD 2
 
+/* Medium tags, in the order of specfem::element::medium_tag
+ * (include/enumerations/medium.hpp:18-28).  This library builds elastic_psv
+ * and acoustic in 2D (and elastic in 3D, through smx_run_forward); the
+ * medium-tagged operators update nothing and return 0 for any other tag and
+ * for a built tag without elements in the context's mesh, as the
+ * reference's loop expects of media a mesh does not hold
+ * (core/specfem/solver/time_marching.tpp:33-84). */
+#define SMX_MEDIUM_ELASTIC_PSV 0
+#define SMX_MEDIUM_ELASTIC_SH 1
+#define SMX_MEDIUM_ELASTIC_PSV_T 2
+#define SMX_MEDIUM_ACOUSTIC 3
+#define SMX_MEDIUM_POROELASTIC 4
+#define SMX_MEDIUM_ELECTROMAGNETIC_TE 5
+#define SMX_MEDIUM_ELASTIC 6
+#define SMX_MEDIUM_ELASTIC_SPIN 7
+#define SMX_MEDIUM_ELECTROMAGNETIC 8
+
 /* Seismogram types (specfem::wavefield::type). */
 #define SMX_SEIS_DISPLACEMENT 0
 #define SMX_SEIS_VELOCITY 1
@@ -315,7 +332,29 @@ int smx_update_wavefields(smx_ctx *ctx, int field, int istep, int *nelem);
 int smx_update_wavefields_noalloc(smx_ctx *ctx, const void *u, const void *v,
                                   void *a, int istep, int *nelem);
 
-/* Seismograms of the current fields into sample isig.
+/* The operators of one medium, as the reference's loop calls them
+ * (core/specfem/solver/time_marching.tpp:33-84):
+ *   newmark::apply_predictor_phase_forward(tag)  (newmark.tpp:152-174)
+ *   newmark::apply_corrector_phase_forward(tag)  (newmark.tpp:130-150)
+ *   domain_kernels::update_wavefields<tag>(istep) (domain_kernels.hpp:72-141)
+ * *dofs / *nelem get the degrees of freedom / elements of that medium
+ * updated (0 for a medium the mesh does not hold).  On a coupled mesh each
+ * call advances one medium's field, and called in the reference's order
+ * (predictors, update<acoustic>, corrector(acoustic), update<elastic_psv>,
+ * corrector(elastic_psv)) they couple the media as the fused step does:
+ * update<acoustic> reads the solid's predicted displacement on the
+ * interface, update<elastic_psv> the fluid's new acceleration. */
+int smx_predictor_medium(smx_ctx *ctx, int field, int medium, int *dofs);
+/* Degrees of freedom of a 2D context: nglob x components summed over its
+ * media (assembly::get_total_degrees_of_freedom, which the reference's loop
+ * checks the operators' counts against, time_marching.tpp:22-23,97-115). */
+int smx_ctx_degrees_of_freedom(const smx_ctx *ctx, long long *dofs);
+int smx_corrector_medium(smx_ctx *ctx, int field, int medium, int *dofs);
+int smx_update_wavefields_medium(smx_ctx *ctx, int field, int medium,
+                                 int istep, int *nelem);
+
+/* Seismograms of the current fields into sample isig (both media of a
+ * coupled mesh).
  * Replaces domain_kernels::compute_seismograms(isig)
  * (include/kokkos_kernels/domain_kernels.hpp:188-203). */
 int smx_compute_seismograms(smx_ctx *ctx, int field, int isig);

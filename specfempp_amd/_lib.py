@@ -30,8 +30,14 @@ EXPORTS = [
     "smx_run_adjoint", "smx_get_gradient", "smx_last_adjoint_stats",
     "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots", "smx_ctx_compact",
     "smx_ctx_create3d", "smx_ctx_create3d_opts", "smx_ctx_geometry", "smx_ctx_components",
-    "smx_update_wavefields_noalloc",
+    "smx_update_wavefields_noalloc", "smx_predictor_medium", "smx_corrector_medium",
+    "smx_update_wavefields_medium", "smx_ctx_degrees_of_freedom",
 ]
+
+# Medium tags (specfem::element::medium_tag, include/enumerations/medium.hpp:18-28).
+MEDIUM_TAGS = {"elastic_psv": 0, "elastic_sh": 1, "elastic_psv_t": 2, "acoustic": 3,
+               "poroelastic": 4, "electromagnetic_te": 5, "elastic": 6, "elastic_spin": 7,
+               "electromagnetic": 8}
 
 
 class SmxMesh2D(ctypes.Structure):
@@ -118,6 +124,10 @@ def load():
         "smx_corrector": (i, [P, i, IP]),
         "smx_update_wavefields": (i, [P, i, i, IP]),
         "smx_update_wavefields_noalloc": (i, [P, P, P, P, i, IP]),
+        "smx_predictor_medium": (i, [P, i, i, IP]),
+        "smx_corrector_medium": (i, [P, i, i, IP]),
+        "smx_update_wavefields_medium": (i, [P, i, i, i, IP]),
+        "smx_ctx_degrees_of_freedom": (i, [P, ctypes.POINTER(ctypes.c_longlong)]),
         "smx_compute_seismograms": (i, [P, i, i]),
         "smx_run_forward": (i, [P, i, i]),
         "smx_synchronize": (i, [P]),

@@ -327,6 +327,17 @@ __global__ void __launch_bounds__(AC_TILE_THREADS) __attribute__((
   }
 }
 
+// Operator path of a coupled mesh: chi_dd[k] = the new acceleration of
+// interface point k (global point iface_g[k]).
+template <typename real>
+__global__ void k_ac_iface(const real *__restrict__ A,
+                           const int *__restrict__ iface_g, int n,
+                           real *__restrict__ chi_dd) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < n)
+    chi_dd[k] = A[iface_g[k]];
+}
+
 // Points shared between tiles: partials summed in tile order, then the same
 // divide / corrector / predictor as the tile's exclusive points.
 template <typename real, int MODE>
@@ -597,6 +608,21 @@ public:
     cout_li_ = upload_raw<int>(A.cout_li);
     cout_k_ = upload_raw<int>(A.cout_k);
     sh_iface_ = upload_raw<int>(A.sh_iface);
+    {
+      // global point of every interface index k: exclusive entries through
+      // their tile, shared ones through sh_iface
+      std::vector<int32_t> ig(std::max(1, A.n_iface), -1);
+      for (const TileInfo &t : A.tiles)
+        for (int k = t.cout_off; k < t.cout_off + t.ncout; ++k)
+          ig[A.cout_k[k]] = t.ex_off + A.cout_li[k];
+      for (size_t j = 0; j < A.sh_iface.size(); ++j)
+        if (A.sh_iface[j] >= 0)
+          ig[A.sh_iface[j]] = A.nex_total + int32_t(j);
+      for (int k = 0; k < A.n_iface; ++k)
+        if (ig[k] < 0 || ig[k] >= A.nglob)
+          throw SmxError(SMX_ERR_MESH, "interface point without a global point");
+      iface_g_ = upload_raw<int>(ig);
+    }
     lds_bytes_ = ac_lds_bytes<real>(std::max(1, A.nloc_max));
     for (const void *f : {reinterpret_cast<const void *>(&k_ac_tile<real, AC_STEP>),
                           reinterpret_cast<const void *>(&k_ac_tile<real, AC_FINAL>),
@@ -640,6 +666,11 @@ public:
 
   int precision() const override { return sizeof(real); }
   const HostAssembly2D &assembly() const override { return *asm_; }
+  int medium() const override { return SMX_MEDIUM_ACOUSTIC; }
+  void on_stream(void *stream, const std::function<void()> &fn) override {
+    AcStreamSwap sw(stream_, static_cast<hipStream_t>(stream));
+    fn();
+  }
 
   void initialize() override {
     const HostAssembly2D &A = *asm_;
@@ -685,6 +716,13 @@ public:
     Field &F = field(f);
     launch_tiles<AC_ASSEMBLE>(F, istep, nullptr, nullptr);
     launch_shared<AC_ASSEMBLE>(F, nullptr);
+    if (chi_dd_ && A.n_iface > 0) {
+      // the new accelerations of the interface points, read by the elastic
+      // update of this step (the fused passes store them themselves)
+      k_ac_iface<real><<<dim3((A.n_iface + 255) / 256), dim3(256), 0,
+                          stream_>>>(F.a, iface_g_, A.n_iface, chi_dd_);
+      AC_HIP(hipGetLastError());
+    }
     return A.nspec;
   }
 
@@ -992,7 +1030,7 @@ private:
   // coupled meshes: coupling entries, the elastic displacement they read,
   // and the interface accelerations this solver writes for the elastic one
   int *cpl_li_ = nullptr, *cpl_g_ = nullptr, *cout_li_ = nullptr,
-      *cout_k_ = nullptr, *sh_iface_ = nullptr;
+      *cout_k_ = nullptr, *sh_iface_ = nullptr, *iface_g_ = nullptr;
   real *cpl_w_ = nullptr, *chi_dd_ = nullptr;
   const real *ue_ = nullptr;
   real *geom_ = nullptr, *geom_e_ = nullptr, *mat_ = nullptr, *mat_e_ = nullptr,

@@ -334,6 +334,38 @@ int smx_update_wavefields(smx_ctx *ctx, int field, int istep, int *nelem) {
   });
 }
 
+int smx_ctx_degrees_of_freedom(const smx_ctx *ctx, long long *dofs) {
+  return guarded([&] {
+    if (!dofs)
+      throw smx::SmxError(SMX_ERR_ARG, "null output");
+    if (S3(const_cast<smx_ctx *>(ctx)))
+      throw smx::SmxError(SMX_ERR_STATE, "operation not available for 3D contexts");
+    *dofs = S(const_cast<smx_ctx *>(ctx)).total_dofs();
+  });
+}
+
+int smx_predictor_medium(smx_ctx *ctx, int field, int medium, int *dofs) {
+  return guarded([&] {
+    const int n = S(ctx).predictor_medium(field, medium);
+    if (dofs) *dofs = n;
+  });
+}
+
+int smx_corrector_medium(smx_ctx *ctx, int field, int medium, int *dofs) {
+  return guarded([&] {
+    const int n = S(ctx).corrector_medium(field, medium);
+    if (dofs) *dofs = n;
+  });
+}
+
+int smx_update_wavefields_medium(smx_ctx *ctx, int field, int medium,
+                                 int istep, int *nelem) {
+  return guarded([&] {
+    const int n = S(ctx).update_wavefields_medium(field, medium, istep);
+    if (nelem) *nelem = n;
+  });
+}
+
 int smx_update_wavefields_noalloc(smx_ctx *ctx, const void *u, const void *v,
                                   void *a, int istep, int *nelem) {
   return guarded([&] {

@@ -67,10 +67,46 @@ public:
     initialized_ = true;
   }
 
-  int predictor(int) override { return fused_only(); }
-  int corrector(int) override { return fused_only(); }
-  int update_wavefields(int, int) override { return fused_only(); }
-  void compute_seismograms(int, int) override { fused_only(); }
+  int medium() const override { return -1; }
+  long total_dofs() const override {
+    return el_->total_dofs() + ac_->total_dofs();
+  }
+  void on_stream(void *, const std::function<void()> &) override {
+    throw SmxError(SMX_ERR_STATE, "a coupled driver runs on its own stream");
+  }
+
+  // The untagged operators would have to advance both media at once, which
+  // the reference's loop never asks: its operators name one medium each.
+  int predictor(int) override { return untagged(); }
+  int corrector(int) override { return untagged(); }
+  int update_wavefields(int, int) override { return untagged(); }
+
+  // Medium-tagged operators (time_marching.tpp:33-84): each runs on the
+  // medium's solver, enqueued on this driver's stream, so the reference's
+  // order predictor(acoustic), predictor(elastic_psv), update<acoustic>,
+  // corrector(acoustic), update<elastic_psv>, corrector(elastic_psv) holds
+  // the coupling: the fluid update reads the solid's predicted
+  // displacement, the solid update the fluid's new acceleration on the
+  // interface.
+  int predictor_medium(int f, int m) override {
+    return on_medium(f, m, [&](Solver2DBase &s) { return s.predictor(f); });
+  }
+  int corrector_medium(int f, int m) override {
+    return on_medium(f, m, [&](Solver2DBase &s) { return s.corrector(f); });
+  }
+  int update_wavefields_medium(int f, int m, int istep) override {
+    if (!initialized_)
+      throw SmxError(SMX_ERR_STATE, "smx_initialize has not been called");
+    return on_medium(f, m,
+                     [&](Solver2DBase &s) { return s.update_wavefields(f, istep); });
+  }
+  // Both media's receivers (compute_seismogram.tpp:19-171 runs per medium).
+  void compute_seismograms(int f, int isig) override {
+    forward_only(f);
+    for (Solver2DBase *s : {ac_.get(), el_.get()})
+      if (s->assembly().nrec > 0)
+        s->on_stream(stream_, [&] { s->compute_seismograms(f, isig); });
+  }
 
   void run_forward(int istep0, int nstep) override {
     if (!initialized_)
@@ -172,10 +208,25 @@ public:
   }
 
 private:
-  int fused_only() {
+  int untagged() {
     throw SmxError(SMX_ERR_STATE,
-                   "coupled meshes run through smx_run_forward (the fused "
-                   "step of both media)");
+                   "a coupled mesh advances one medium per operator: use the "
+                   "medium-tagged operators (smx_*_medium) or smx_run_forward");
+  }
+  static void forward_only(int f) {
+    if (f != SMX_FIELD_FORWARD)
+      throw SmxError(SMX_ERR_ARG, "coupled meshes hold the forward field only");
+  }
+  template <typename Fn> int on_medium(int f, int m, Fn fn) {
+    forward_only(f);
+    Solver2DBase *s = m == SMX_MEDIUM_ELASTIC_PSV
+                          ? el_.get()
+                          : (m == SMX_MEDIUM_ACOUSTIC ? ac_.get() : nullptr);
+    if (!s)
+      return 0;
+    int n = 0;
+    s->on_stream(stream_, [&] { n = fn(*s); });
+    return n;
   }
   void no_fields() {
     throw SmxError(SMX_ERR_STATE,

@@ -1407,6 +1407,11 @@ public:
 
   int precision() const override { return sizeof(real); }
   const HostAssembly2D &assembly() const override { return *asm_; }
+  int medium() const override { return SMX_MEDIUM_ELASTIC_PSV; }
+  void on_stream(void *stream, const std::function<void()> &fn) override {
+    StreamSwap sw(stream_, static_cast<hipStream_t>(stream));
+    fn();
+  }
 
   void initialize() override {
     const HostAssembly2D &A = *asm_;

@@ -1,6 +1,7 @@
 // Device-resident 2D elastic P-SV solver: the object behind an smx_ctx.
 #pragma once
 
+#include <functional>
 #include <memory>
 
 #include "assembly2d.hpp"
@@ -19,6 +20,28 @@ public:
   virtual int predictor(int field) = 0;
   virtual int corrector(int field) = 0;
   virtual int update_wavefields(int field, int istep) = 0;
+  // The medium a single-medium solver advances (SMX_MEDIUM_*); a coupled
+  // driver returns -1 and dispatches the tagged operators per medium.
+  virtual int medium() const = 0;
+  // Medium-tagged operators (time_marching.tpp:33-84): the DOFs / elements
+  // of `medium` updated, 0 when this context has no element of it.
+  virtual int predictor_medium(int field, int m) {
+    return m == medium() ? predictor(field) : 0;
+  }
+  virtual int corrector_medium(int field, int m) {
+    return m == medium() ? corrector(field) : 0;
+  }
+  virtual int update_wavefields_medium(int field, int m, int istep) {
+    return m == medium() ? update_wavefields(field, istep) : 0;
+  }
+  // Degrees of freedom of the context: nglob x components summed over its
+  // media (assembly::get_total_degrees_of_freedom).
+  virtual long total_dofs() const {
+    return long(assembly().nglob) * (medium() == SMX_MEDIUM_ACOUSTIC ? 1 : 2);
+  }
+  // Runs fn with the solver's launches enqueued on `stream` (a coupled
+  // driver orders both media's operators on its own stream).
+  virtual void on_stream(void *stream, const std::function<void()> &fn) = 0;
   // update_wavefields on caller-held device fields [nglob][2] (u, v read;
   // a := (a + f - K u - B v) / M) with the shared-point partials in the
   // library's fixed device scratch array: allocates nothing.

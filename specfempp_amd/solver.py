@@ -243,8 +243,8 @@ class Context:
 
 
 class DomainKernels:
-    """specfem::kokkos_kernels::domain_kernels<forward, dim2, 5> for the
-    elastic P-SV medium."""
+    """specfem::kokkos_kernels::domain_kernels<forward, dim2, 5>
+    (include/kokkos_kernels/domain_kernels.hpp:37-213)."""
 
     def __init__(self, ctx: Context, field=SMX_FIELD_FORWARD):
         self.ctx, self.field = ctx, field
@@ -253,10 +253,17 @@ class DomainKernels:
         # dt is fixed at assembly time (the Stacey mass term uses it)
         self.ctx.initialize()
 
-    def update_wavefields(self, istep) -> int:
+    def update_wavefields(self, istep, medium=None) -> int:
+        """update_wavefields<medium>(istep): the elements of ``medium``
+        updated (0 for a medium the mesh does not hold).  Without a medium:
+        the context's single medium."""
         n = ctypes.c_int()
-        check(self.ctx.lib.smx_update_wavefields(self.ctx.h, self.field, int(istep),
-                                                 ctypes.byref(n)))
+        if medium is None:
+            check(self.ctx.lib.smx_update_wavefields(self.ctx.h, self.field, int(istep),
+                                                     ctypes.byref(n)))
+        else:
+            check(self.ctx.lib.smx_update_wavefields_medium(
+                self.ctx.h, self.field, _lib.MEDIUM_TAGS[medium], int(istep), ctypes.byref(n)))
         return n.value
 
     def compute_seismograms(self, isig):
@@ -264,7 +271,8 @@ class DomainKernels:
 
 
 class Newmark:
-    """Forward Newmark time scheme (core/specfem/timescheme/newmark.hpp)."""
+    """Forward Newmark time scheme (core/specfem/timescheme/newmark.hpp,
+    time_scheme interface timescheme.hpp:16-201)."""
 
     def __init__(self, ctx: Context, field=SMX_FIELD_FORWARD):
         self.ctx, self.field = ctx, field
@@ -275,14 +283,22 @@ class Newmark:
         for istep in range(self.nstep):
             yield istep, self.ctx.dt
 
-    def apply_predictor_phase_forward(self) -> int:
+    def apply_predictor_phase_forward(self, medium=None) -> int:
         n = ctypes.c_int()
-        check(self.ctx.lib.smx_predictor(self.ctx.h, self.field, ctypes.byref(n)))
+        if medium is None:
+            check(self.ctx.lib.smx_predictor(self.ctx.h, self.field, ctypes.byref(n)))
+        else:
+            check(self.ctx.lib.smx_predictor_medium(self.ctx.h, self.field,
+                                                    _lib.MEDIUM_TAGS[medium], ctypes.byref(n)))
         return n.value
 
-    def apply_corrector_phase_forward(self) -> int:
+    def apply_corrector_phase_forward(self, medium=None) -> int:
         n = ctypes.c_int()
-        check(self.ctx.lib.smx_corrector(self.ctx.h, self.field, ctypes.byref(n)))
+        if medium is None:
+            check(self.ctx.lib.smx_corrector(self.ctx.h, self.field, ctypes.byref(n)))
+        else:
+            check(self.ctx.lib.smx_corrector_medium(self.ctx.h, self.field,
+                                                    _lib.MEDIUM_TAGS[medium], ctypes.byref(n)))
         return n.value
 
     def compute_seismogram(self, istep) -> bool:
@@ -307,14 +323,25 @@ class Newmark:
 class TimeMarching:
     """specfem::solver::time_marching<forward, dim2, 5>.
 
-    ``run(fused=False)`` sequences the operators one by one exactly as the
-    reference loop does, with the same DOF / element-count checks;
-    ``run(fused=True)`` (default) enqueues the fused, graph-replayed step."""
+    ``run(fused=False)`` calls the medium-tagged operators one by one in the
+    reference loop's order (core/specfem/solver/time_marching.tpp:33-116),
+    for every medium tag, with the same DOF / element-count checks; it runs
+    single-medium and coupled meshes.  ``run(fused=True)`` (default)
+    enqueues the fused, graph-replayed step."""
+
+    PREDICTED = ("acoustic", "elastic", "elastic_psv", "elastic_sh", "poroelastic",
+                 "elastic_psv_t")
+    SOLID = ("elastic", "elastic_psv", "elastic_sh", "elastic_psv_t")
 
     def __init__(self, ctx: Context):
         self.ctx = ctx
         self.kernels = DomainKernels(ctx)
         self.time_scheme = Newmark(ctx)
+
+    def total_degrees_of_freedom(self) -> int:
+        n = ctypes.c_longlong()
+        check(self.ctx.lib.smx_ctx_degrees_of_freedom(self.ctx.h, ctypes.byref(n)))
+        return n.value
 
     def run(self, fused=True):
         ctx = self.ctx
@@ -323,14 +350,19 @@ class TimeMarching:
             ctx.run_forward(0, ctx.nstep)
             ctx.synchronize()
             return
-        total_dofs = 2 * (ctx.ncomp * ctx.nglob)
-        for istep, _ in self.time_scheme.iterate_forward():
-            dofs = self.time_scheme.apply_predictor_phase_forward()
-            nelem = self.kernels.update_wavefields(istep)
-            dofs += self.time_scheme.apply_corrector_phase_forward()
-            if self.time_scheme.compute_seismogram(istep):
-                self.kernels.compute_seismograms(self.time_scheme.get_seismogram_step())
-                self.time_scheme.increment_seismogram_step()
+        ts, k = self.time_scheme, self.kernels
+        total_dofs = 2 * self.total_degrees_of_freedom()
+        for istep, _ in ts.iterate_forward():
+            dofs = sum(ts.apply_predictor_phase_forward(m) for m in self.PREDICTED)
+            nelem = k.update_wavefields(istep, "acoustic")
+            dofs += ts.apply_corrector_phase_forward("acoustic")
+            nelem += sum(k.update_wavefields(istep, m) for m in self.SOLID)
+            dofs += sum(ts.apply_corrector_phase_forward(m) for m in self.SOLID)
+            nelem += k.update_wavefields(istep, "poroelastic")
+            dofs += ts.apply_corrector_phase_forward("poroelastic")
+            if ts.compute_seismogram(istep):
+                k.compute_seismograms(ts.get_seismogram_step())
+                ts.increment_seismogram_step()
             if dofs != total_dofs:
                 raise RuntimeError(f"The time loop has not updated all the degrees of "
                                    f"freedom. Only {dofs} out of {total_dofs}.")

@@ -10,7 +10,10 @@ the water below (the reference mesher's database).
 
 Tolerances: per seismogram type twice the type's FP32 floor (oracle FP32 vs
 FP64, measured in the test), at least 1e-5, against the FP32 and the FP64
-oracle; against SPECFEM2D the oracle's own bound (reference metric 1e-4).
+oracle; against SPECFEM2D the oracle's own bound (reference metric 1e-4);
+the medium-tagged operator path against the fused path and the oracle the
+same FP32-floor bound (the two paths round the coupled accelerations in a
+different order: 6e-7 in displacement, 2e-6 in velocity).
 """
 
 import sys
@@ -82,6 +85,53 @@ def test_coupled_run_is_reproducible_and_tiles_agree(oracle_lib):
     for it in range(len(cfg.seismo_types)):
         bound = max(1e-6, 2.0 * rel_l2(ref32[:, it], ref64[:, it]))
         assert rel_l2(c[:, it], a[:, it]) <= bound
+
+
+@pytest.mark.parametrize("vertical", [True, False])
+def test_coupled_operator_path(oracle_lib, vertical):
+    """The reference loop's medium-tagged operator sequence on a coupled mesh
+    (predictors, update<acoustic>, corrector(acoustic), update<elastic_psv>,
+    corrector(elastic_psv), seismograms of both media) against the fused
+    path and the oracle, with the loop's DOF and element checks."""
+    cfg = coupled_box(vertical=vertical, nstep=300)
+    _say(f"coupled operator path ({'vertical' if vertical else 'horizontal'}), 300 steps")
+    fused, _ = simulate(cfg)
+    ops, ctx = simulate(cfg, fused=False)
+    ref32, ref64 = _oracle(oracle_lib, cfg, 4), _oracle(oracle_lib, cfg, 8)
+    for it, t in enumerate(cfg.seismo_types):
+        e_f = rel_l2(ops[:, it], fused[:, it])
+        e_o = rel_l2(ops[:, it], ref32[:, it])
+        bound = max(1e-5, 2.0 * rel_l2(ref32[:, it], ref64[:, it]))
+        _say(f"  {t}: operator vs fused {e_f:.3e}; vs FP32 oracle {e_o:.3e} (bound {bound:.1e})")
+        assert e_f <= bound and e_o <= bound
+
+
+def test_coupled_medium_counts():
+    """Each medium-tagged operator counts its own medium: the solid's DOFs
+    are 2 per point, the fluid's 1, unknown media count 0, and together they
+    cover the context's degrees of freedom and elements."""
+    import ctypes
+    from specfempp_amd import _lib
+    cfg = coupled_box(nstep=10)
+    ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    ctx.initialize()
+    n = ctypes.c_int()
+    counts = {}
+    for name, tag in _lib.MEDIUM_TAGS.items():
+        _lib.check(ctx.lib.smx_predictor_medium(ctx.h, 0, tag, ctypes.byref(n)))
+        d = n.value
+        _lib.check(ctx.lib.smx_update_wavefields_medium(ctx.h, 0, tag, 0, ctypes.byref(n)))
+        counts[name] = (d, n.value)
+    total = ctypes.c_longlong()
+    _lib.check(ctx.lib.smx_ctx_degrees_of_freedom(ctx.h, ctypes.byref(total)))
+    acoustic = int(cfg.mesh.acoustic.sum())
+    assert counts["acoustic"][1] == acoustic
+    assert counts["elastic_psv"][1] == cfg.mesh.nspec - acoustic
+    assert counts["elastic_psv"][0] % 2 == 0
+    assert counts["acoustic"][0] + counts["elastic_psv"][0] == total.value
+    assert all(v == (0, 0) for k, v in counts.items() if k not in ("acoustic", "elastic_psv"))
+    # the adjoint field is not held on coupled meshes
+    assert ctx.lib.smx_predictor_medium(ctx.h, 1, 0, ctypes.byref(n)) == _lib.SMX_ERR_ARG
 
 
 def test_coupled_context_limits():

@@ -42,20 +42,38 @@ def test_library_exports_every_declared_symbol():
     assert b"null context" in lib.smx_last_error()
 
 
-def test_cpp_shim_compiles(tmp_path):
-    """The reference-named C++ classes compile against the C header."""
+def test_cpp_shim_compiles_and_iterates(tmp_path):
+    """The reference-named C++ classes compile against the C header, and the
+    time scheme's iteration and seismogram cadence (timescheme.hpp:16-201)
+    run as the reference's do: forward steps 0..nstep-1, backward
+    nstep-1..0, a sample every nstep_between_samples steps."""
     import shutil
     import subprocess
     if shutil.which("g++") is None:
         pytest.skip("no g++")
-    src = tmp_path / "use_shim.cpp"
-    src.write_text('#include "specfempp_amd_shim.hpp"\n'
-                   "int main() { specfempp_amd::domain_kernels k(nullptr);\n"
-                   "  specfempp_amd::newmark n(nullptr, 10, 1, 1e-3f);\n"
-                   "  return n.compute_seismogram(0) ? 0 : 1; }\n")
-    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include", str(src)],
-                       capture_output=True, text=True)
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", f"-I{ROOT}/include",
+                        f"{ROOT}/tests/cpp/reference_loop.cpp"], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+    src = tmp_path / "iterate.cpp"
+    src.write_text(
+        '#include <cstdio>\n#include "specfempp_amd_shim.hpp"\n'
+        "using namespace specfempp_amd;\n"
+        "int main() { newmark n(nullptr, 7, 3, 0.5f);\n"
+        "  for (const auto s : n.iterate_forward()) std::printf(\"f%d:%g \", s.istep, s.dt);\n"
+        "  for (const auto s : n.iterate_backward()) std::printf(\"b%d \", s.istep);\n"
+        "  for (int i = 0; i < 7; ++i) if (n.compute_seismogram(i)) n.increment_seismogram_step();\n"
+        "  std::printf(\"n%d m%d p%d\", n.get_seismogram_step(), n.get_max_seismogram_step(),\n"
+        "              n.apply_predictor_phase_backward(medium_tag::acoustic));\n"
+        "  return 0; }\n")
+    exe = tmp_path / "iterate"
+    r = subprocess.run(["g++", "-std=c++17", f"-I{ROOT}/include", str(src),
+                        f"-L{ROOT}/specfempp_amd", "-lspecfempp_amd", "-L/opt/rocm/lib",
+                        "-Wl,-rpath-link,/opt/rocm/lib", f"-Wl,-rpath,{ROOT}/specfempp_amd",
+                        "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    out = subprocess.run([str(exe)], capture_output=True, text=True, timeout=60).stdout
+    assert out == ("f0:0.5 f1:0.5 f2:0.5 f3:0.5 f4:0.5 f5:0.5 f6:0.5 "
+                   "b6 b5 b4 b3 b2 b1 b0 n3 m2 p0")
 
 
 def test_missing_library_fails_loudly(monkeypatch, tmp_path):
