@@ -557,6 +557,10 @@ def errors_leg(c1_oracle_steps=200):
                        "shots_with_arrivals": len(live),
                        "worst_shot_with_arrivals_fp32_vs_fp64":
                            max(_rel(b32[k], b64[k]) for k in live),
+                       "per_shot": [{"shot": k, "nspec": int(shots[k].mesh.nspec),
+                                     "energy_over_max": float(energy[k] / energy.max()),
+                                     "fp32_vs_fp64": _rel(b32[k], b64[k])}
+                                    for k in range(len(shots))],
                        "smallest_two_shots_fp32_vs_oracle_fp32": orc}
     # 64^3 hex, 4000 steps
     cfg = hex_box_case()
