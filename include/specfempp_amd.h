@@ -411,6 +411,20 @@ int smx_time_stiffness(smx_ctx *ctx, int nlaunch, double *avg_ms);
 int smx_set_model(smx_ctx *ctx, const float *kappa, const float *mu,
                   const float *rho);
 
+/* smx_set_model with kappa, mu, rho in device memory ([nspec][5][5] each,
+ * mesh element order, FP32), for callers whose model already lives on the
+ * GPU (a torch.autograd.Function over CUDA tensors): validated and turned
+ * into every derived array on the device, bitwise as smx_set_model does on
+ * the host; the arrays must stay valid until the call returns.  2D elastic
+ * FP32 single-mesh contexts (SMX_ERR_STATE otherwise).  No reference
+ * counterpart beyond smx_set_model's. */
+int smx_set_model_device(smx_ctx *ctx, const float *kappa, const float *mu,
+                         const float *rho);
+
+/* smx_get_gradient into device memory: [3][nspec][5][5] double, bitwise as
+ * the host path forms it.  2D elastic FP32 contexts. */
+int smx_get_gradient_device(smx_ctx *ctx, double *device_out);
+
 /* Forward run of all nstep steps from rest, with seismograms as
  * smx_run_forward, keeping (u, v) in device memory every `every` steps.
  * Replaces the forward half of a combined simulation, which writes the last

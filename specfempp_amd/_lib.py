@@ -31,7 +31,8 @@ EXPORTS = [
     "smx_ctx_create2d_opts", "smx_ctx_create2d_batch", "smx_ctx_shots", "smx_ctx_compact",
     "smx_ctx_create3d", "smx_ctx_create3d_opts", "smx_ctx_geometry", "smx_ctx_components",
     "smx_update_wavefields_noalloc", "smx_predictor_medium", "smx_corrector_medium",
-    "smx_update_wavefields_medium", "smx_ctx_degrees_of_freedom",
+    "smx_update_wavefields_medium", "smx_ctx_degrees_of_freedom", "smx_set_model_device",
+    "smx_get_gradient_device",
 ]
 
 # Medium tags (specfem::element::medium_tag, include/enumerations/medium.hpp:18-28).
@@ -128,6 +129,8 @@ def load():
         "smx_corrector_medium": (i, [P, i, i, IP]),
         "smx_update_wavefields_medium": (i, [P, i, i, i, IP]),
         "smx_ctx_degrees_of_freedom": (i, [P, ctypes.POINTER(ctypes.c_longlong)]),
+        "smx_set_model_device": (i, [P, P, P, P]),
+        "smx_get_gradient_device": (i, [P, P]),
         "smx_compute_seismograms": (i, [P, i, i]),
         "smx_run_forward": (i, [P, i, i]),
         "smx_synchronize": (i, [P]),

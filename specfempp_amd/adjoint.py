@@ -47,23 +47,31 @@ def forward_seismograms(sim: Context, kappa, mu, rho, every=100):
     """Sets the model, runs the checkpointed forward and returns the
     displacement seismograms (nsig, nrec, ncomp)."""
     sim.set_model(kappa, mu, rho)
+    return _checkpointed_seismograms(sim, every)
+
+
+def _checkpointed_seismograms(sim, every):
     sim.run_forward_checkpointed(every)
     sim.synchronize()
     sim._model_version = getattr(sim, "_model_version", 0) + 1
     return sim.seismograms()[:, _displacement_index(sim)]
 
 
-def adjoint_gradient(sim: Context, dchi_ds):
-    """Gradient (3, nspec, 5, 5[, 5]) for dchi/ds = dchi_ds (nsig, nrec,
-    ncomp), after forward_seismograms on the same context.  Seismogram i was
-    recorded after step i * nsbs; the other steps get no adjoint source."""
+def _adjoint_source(sim, dchi_ds):
     g = np.asarray(dchi_ds, dtype=np.float64)
     nc = sim.ncomp
     if g.shape != (sim.nsig, sim.nrec, nc):
         raise ValueError(f"dchi/ds must have shape {(sim.nsig, sim.nrec, nc)}, got {g.shape}")
     gsrc = np.zeros((sim.nstep, sim.nrec, nc), dtype=np.float64)
     gsrc[::sim.nsbs][:sim.nsig] = g
-    sim.run_adjoint(gsrc)
+    return gsrc
+
+
+def adjoint_gradient(sim: Context, dchi_ds):
+    """Gradient (3, nspec, 5, 5[, 5]) for dchi/ds = dchi_ds (nsig, nrec,
+    ncomp), after forward_seismograms on the same context.  Seismogram i was
+    recorded after step i * nsbs; the other steps get no adjoint source."""
+    sim.run_adjoint(_adjoint_source(sim, dchi_ds))
     return sim.gradient()
 
 
@@ -80,15 +88,30 @@ def misfit_and_gradient(sim: Context, model, observed, every=100):
 try:  # torch is plumbing for the autograd interface, not a compute path
     import torch
 
+    def _on_device(sim, t):
+        """Whether the model and gradient can stay in device memory: CUDA
+        tensors and a 2D elastic FP32 single-mesh context."""
+        return (t.is_cuda and isinstance(sim, Context) and sim.precision == 4
+                and sim.ncomp == 2 and getattr(sim, "nshots", 1) == 1)
+
     class Seismograms(torch.autograd.Function):
         """s = Seismograms.apply(kappa, mu, rho, sim, every): displacement
         seismograms (nsig, nrec, ncomp) of context ``sim`` for the material
-        tensors (nspec, 5, 5[, 5]); differentiable in kappa, mu and rho."""
+        tensors (nspec, 5, 5[, 5]); differentiable in kappa, mu and rho.
+        CUDA tensors on a 2D elastic FP32 context stay on the GPU: the model
+        is read from, and the gradient written to, device memory."""
 
         @staticmethod
         def forward(ctx, kappa, mu, rho, sim, every=100):
-            host = [t.detach().to("cpu", torch.float32).numpy() for t in (kappa, mu, rho)]
-            s = forward_seismograms(sim, *host, every=every)
+            ctx.device_path = _on_device(sim, kappa)
+            if ctx.device_path:
+                model = [t.detach().to(torch.float32).contiguous() for t in (kappa, mu, rho)]
+                torch.cuda.current_stream(kappa.device).synchronize()
+                sim.set_model_device(*[t.data_ptr() for t in model])
+                s = _checkpointed_seismograms(sim, every)
+            else:
+                host = [t.detach().to("cpu", torch.float32).numpy() for t in (kappa, mu, rho)]
+                s = forward_seismograms(sim, *host, every=every)
             ctx.sim = sim
             ctx.version = sim._model_version
             ctx.like = kappa
@@ -100,10 +123,18 @@ try:  # torch is plumbing for the autograd interface, not a compute path
             if sim._model_version != ctx.version:
                 raise RuntimeError("the context ran another model since this forward; "
                                    "its checkpoints are gone")
-            g = adjoint_gradient(sim, grad_s.detach().to("cpu", torch.float64).numpy())
+            gs = grad_s.detach().to("cpu", torch.float64).numpy()
             like = ctx.like
-            out = [torch.from_numpy(g[i]).to(device=like.device, dtype=like.dtype)
-                   for i in range(3)]
+            if ctx.device_path:
+                sim.run_adjoint(_adjoint_source(sim, gs))
+                g = torch.empty((3,) + tuple(like.shape), dtype=torch.float64,
+                                device=like.device)
+                sim.gradient_device(g.data_ptr())
+                out = [g[i].to(like.dtype) for i in range(3)]
+            else:
+                g = adjoint_gradient(sim, gs)
+                out = [torch.from_numpy(g[i]).to(device=like.device, dtype=like.dtype)
+                       for i in range(3)]
             return out[0], out[1], out[2], None, None
 
 except ImportError:  # pragma: no cover - torch is part of the image

@@ -277,6 +277,7 @@ int smx_ctx_compact(const smx_ctx *ctx, int *geom_elements, int *mat_elements) {
       if (mat_elements) *mat_elements = B.mat_elem ? B.nspec : 0;
       return;
     }
+    S(const_cast<smx_ctx *>(ctx)).sync_host_model();
     const auto &A = S(const_cast<smx_ctx *>(ctx)).assembly();
     int g = 0, m = 0;
     for (const auto &t : A.tiles) {
@@ -474,6 +475,21 @@ int smx_set_model(smx_ctx *ctx, const float *kappa, const float *mu,
       s3->set_model(kappa, mu, rho);
     else
       S(ctx).set_model(kappa, mu, rho);
+  });
+}
+
+int smx_set_model_device(smx_ctx *ctx, const float *kappa, const float *mu,
+                         const float *rho) {
+  return guarded([&] {
+    if (!kappa || !mu || !rho) throw smx::SmxError(SMX_ERR_ARG, "null model");
+    S(ctx).set_model_device(kappa, mu, rho);
+  });
+}
+
+int smx_get_gradient_device(smx_ctx *ctx, double *device_out) {
+  return guarded([&] {
+    if (!device_out) throw smx::SmxError(SMX_ERR_ARG, "null output");
+    S(ctx).get_gradient_device(device_out);
   });
 }
 

@@ -33,6 +33,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <unordered_map>
+#include <limits>
 #include <mutex>
 #include <cstring>
 #include <string>
@@ -1251,6 +1253,172 @@ __global__ void k_adj_inject(real *__restrict__ UB,
   UB2[g] = ub;
 }
 
+// ------------------------------------------------------------------------
+// A new model read from device memory (smx_set_model_device, FP32): the
+// device form of set_model2d's arithmetic, in its FP32 / double order with
+// no contraction, so that every derived array is bitwise the host's.
+// Smallest element index holding a non-physical value (INT_MAX if none).
+__global__ void k_model_check(const float *__restrict__ kappa,
+                              const float *__restrict__ mu,
+                              const float *__restrict__ rho, long n,
+                              int *__restrict__ bad) {
+  const long i = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && !(rho[i] > 0 && mu[i] >= 0 && kappa[i] > 0))
+    atomicMin(bad, int(i / NPT));
+}
+
+// The correctly rounded FP32 square root the host's std::sqrt gives: the
+// device's v_sqrt_f32 is within 1 ulp only, while the double square root is
+// correctly rounded, and rounding it to float is then exact (53 >= 2*24+2
+// bits make the double rounding innocuous).
+__device__ __forceinline__ float sqrt_rn(float x) {
+  return static_cast<float>(sqrt(static_cast<double>(x)));
+}
+
+// (kappa, mu, rho) and (lambda+2mu, mu) at every GLL point, slot layout.
+__global__ void k_model_in(const float *__restrict__ kappa,
+                           const float *__restrict__ mu,
+                           const float *__restrict__ rho,
+                           const int *__restrict__ elem_slot, int nspec,
+                           int ld, float *__restrict__ kmr,
+                           float *__restrict__ mat) {
+#pragma clang fp contract(off)
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(nspec) * NPT)
+    return;
+  const int e = int(t / NPT), p = int(t % NPT);
+  const size_t pld = size_t(NPT) * ld, o = size_t(p) * ld + elem_slot[e];
+  const float k = kappa[t], m = mu[t], r = rho[t];
+  mat[o] = k + static_cast<float>(4.0 / 3.0) * m;
+  mat[pld + o] = m;
+  kmr[o] = k;
+  kmr[pld + o] = m;
+  kmr[2 * pld + o] = r;
+}
+
+// Stacey impedances rho vp, rho vs of the absorbing GLL points
+// (elastic/isotropic/properties.hpp:77-87).
+__global__ void k_model_stacey(const int *__restrict__ bpt_p,
+                               const int *__restrict__ bpt_bslot,
+                               const int *__restrict__ bpt_slot, int nbpt,
+                               int ld, int sld, const float *__restrict__ kmr,
+                               float *__restrict__ stacey) {
+#pragma clang fp contract(off)
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= nbpt)
+    return;
+  const int p = bpt_p[k], bs = bpt_bslot[k];
+  const size_t pld = size_t(NPT) * ld, sp = size_t(NPT) * sld;
+  const size_t o = size_t(p) * ld + bpt_slot[k];
+  const float kk = kmr[o], mm = kmr[pld + o], rr = kmr[2 * pld + o];
+  const float l2m = kk + static_cast<float>(4.0 / 3.0) * mm;
+  stacey[4 * sp + size_t(p) * sld + bs] = sqrt_rn(l2m / rr) * rr;
+  stacey[5 * sp + size_t(p) * sld + bs] = sqrt_rn(rr * mm);
+}
+
+// Boundary matrices of the absorbing nodes, summed in double over each
+// node's points in point-list order, stored in FP32.
+__global__ void k_model_bsum(const int *__restrict__ boff,
+                             const int *__restrict__ bk,
+                             const int *__restrict__ bpt_p,
+                             const int *__restrict__ bpt_bslot, int nbnode,
+                             int sld, const float *__restrict__ stacey,
+                             float *__restrict__ bsum) {
+#pragma clang fp contract(off)
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= nbnode)
+    return;
+  const size_t sp = size_t(NPT) * sld;
+  double a0 = 0, a1 = 0, a2 = 0;
+  for (int q = boff[j]; q < boff[j + 1]; ++q) {
+    const int k = bk[q];
+    const size_t so = size_t(bpt_p[k]) * sld + bpt_bslot[k];
+    const double n0 = stacey[so], n1 = stacey[sp + so],
+                 nrm = stacey[2 * sp + so], w = stacey[3 * sp + so];
+    const double h0 = n0 / nrm, h1 = n1 / nrm, f = nrm * w;
+    const double P = stacey[4 * sp + so], S = stacey[5 * sp + so];
+    a0 += f * ((P - S) * h0 * h0 + S);
+    a1 += f * ((P - S) * h0 * h1);
+    a2 += f * ((P - S) * h1 * h1 + S);
+  }
+  bsum[3 * j + 0] = static_cast<float>(a0);
+  bsum[3 * j + 1] = static_cast<float>(a1);
+  bsum[3 * j + 2] = static_cast<float>(a2);
+}
+
+// Per-element copies of the moduli and the tiles' element-constant flag
+// (refresh_compact): one 64-lane block per tile, lane j = element j.
+__global__ void __launch_bounds__(ELEM_TILE)
+    k_model_compact(TileInfo *__restrict__ tiles,
+                    const float *__restrict__ mat,
+                    float *__restrict__ mat_e, int ld) {
+  const int t = blockIdx.x, j = threadIdx.x;
+  const int ecount = tiles[t].ecount;
+  const int sl = t * ELEM_TILE + j;
+  const size_t pld = size_t(NPT) * ld;
+  bool same = true;
+  if (j < ecount) {
+    for (int c = 0; c < 2; ++c)
+      for (int p = 1; p < NPT; ++p)
+        same = same && mat[c * pld + size_t(p) * ld + sl] == mat[c * pld + sl];
+    for (int c = 0; c < 2; ++c)
+      mat_e[c * size_t(ld) + sl] = mat[c * pld + sl];
+  }
+  const bool all = __all(same) != 0;
+  if (j == 0)
+    tiles[t].flags =
+        (tiles[t].flags & ~TILE_MAT_ELEM) | (all ? TILE_MAT_ELEM : 0);
+}
+
+// The gradient in mesh element order, as get_gradient forms it on the host
+// (w_iz w_ix J and the chain rule in double, plus the Stacey impedance
+// terms at absorbing points).
+template <typename real>
+__global__ void k_gradient_out(const int *__restrict__ elem_slot,
+                               const int *__restrict__ sslot,
+                               const real *__restrict__ geom, QuadArgs<real> q,
+                               const int *__restrict__ ibool,
+                               const real *__restrict__ kmr,
+                               const real *__restrict__ stacey,
+                               const real *__restrict__ gd,
+                               const real *__restrict__ ge,
+                               const real *__restrict__ racc,
+                               const real *__restrict__ sP,
+                               const real *__restrict__ sS, int nspec, int ld,
+                               int sld, double *__restrict__ out) {
+#pragma clang fp contract(off)
+  const long t = long(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= long(nspec) * NPT)
+    return;
+  const int e = int(t / NPT), p = int(t % NPT);
+  const int slot = elem_slot[e], bs = sslot[slot];
+  const size_t pld = size_t(NPT) * ld, sp = size_t(NPT) * sld;
+  const size_t o = size_t(p) * ld + slot;
+  const int iz = p / NGLL, ix = p % NGLL;
+  const double wj = double(q.w[iz]) * q.w[ix] * geom[4 * pld + o];
+  const double Gd = gd[o], Ge = ge[o];
+  double gk = -wj * Gd;
+  double gm = -wj * (2.0 * Ge - 2.0 / 3.0 * Gd);
+  double gr = -wj * double(racc[ibool[o]]);
+  if (bs >= 0 && stacey[3 * sp + size_t(p) * sld + bs] != real(0)) {
+    const double k = kmr[o], m = kmr[pld + o], rh = kmr[2 * pld + o];
+    const double l2m = k + 4.0 / 3.0 * m;
+    const double P = sqrt(rh * l2m), S = sqrt(rh * m);
+    const double dP = sP[size_t(p) * sld + bs], dS = sS[size_t(p) * sld + bs];
+    gk += dP * rh / (2.0 * P);
+    gm += dP * (4.0 / 3.0) * rh / (2.0 * P);
+    gr += dP * l2m / (2.0 * P);
+    if (S > 0) {
+      gm += dS * rh / (2.0 * S);
+      gr += dS * m / (2.0 * S);
+    }
+  }
+  const size_t nloc = size_t(nspec) * NPT;
+  out[t] = gk;
+  out[nloc + t] = gm;
+  out[2 * nloc + t] = gr;
+}
+
 // ========================================================================
 // Shared-point partials of smx_update_wavefields_noalloc: a fixed device
 // array (one copy per GPU, held until the process ends), so that entry point
@@ -1632,10 +1800,97 @@ public:
   void set_model(const float *kappa, const float *mu,
                  const float *rho) override {
     HostAssembly2D &A = *asm_;
+    sync_host_model(); // the host arrays a device model left stale
     set_model2d(A, kappa, mu, rho);
     upload_material();
     ckpt_every_ = 0;
     initialize();
+  }
+
+  // The same update from device arrays: validated, then every derived
+  // device array recomputed in place; the host copy is marked stale.
+  void set_model_device(const float *kappa, const float *mu,
+                        const float *rho) override {
+    if (sizeof(real) != 4)
+      return Solver2DBase::set_model_device(kappa, mu, rho);
+    if (asm_->shot_elem_off.size() > 2)
+      throw SmxError(SMX_ERR_STATE, "device-memory models are built for single-mesh contexts");
+    const HostAssembly2D &A = *asm_;
+    ensure_model_maps();
+    ensure_adjoint(); // the boundary matrices are refreshed with the model
+    const long n = long(A.nspec) * NPT;
+    const int ok = std::numeric_limits<int>::max();
+    SMX_HIP(hipMemcpyAsync(model_bad_, &ok, sizeof(int), hipMemcpyHostToDevice,
+                           stream_));
+    k_model_check<<<dim3((n + 255) / 256), dim3(256), 0, stream_>>>(
+        kappa, mu, rho, n, model_bad_);
+    SMX_HIP(hipGetLastError());
+    int bad = ok;
+    SMX_HIP(hipMemcpyAsync(&bad, model_bad_, sizeof(int), hipMemcpyDeviceToHost,
+                           stream_));
+    SMX_HIP(hipStreamSynchronize(stream_));
+    if (bad != ok)
+      throw SmxError(SMX_ERR_ARG,
+                     "non-physical material at element " + std::to_string(bad));
+    float *kmr = reinterpret_cast<float *>(kmr_), *mat = reinterpret_cast<float *>(mat_);
+    float *st = reinterpret_cast<float *>(stacey_);
+    k_model_in<<<dim3((n + 255) / 256), dim3(256), 0, stream_>>>(
+        kappa, mu, rho, elem_slot_, A.nspec, A.nspec_pad, kmr, mat);
+    SMX_HIP(hipGetLastError());
+    const int nbpt = int(A.bpt_g.size()), nbn = int(A.bnode_g.size());
+    if (nbpt > 0) {
+      k_model_stacey<<<dim3((nbpt + 255) / 256), dim3(256), 0, stream_>>>(
+          bpt_p_, bpt_bslot_, bpt_slot_, nbpt, A.nspec_pad, A.nstacey_pad, kmr, st);
+      SMX_HIP(hipGetLastError());
+    }
+    if (nbn > 0) {
+      k_model_bsum<<<dim3((nbn + 255) / 256), dim3(256), 0, stream_>>>(
+          bnode_off_, bnode_pts_, bpt_p_, bpt_bslot_, nbn, A.nstacey_pad, st,
+          reinterpret_cast<float *>(bsum_));
+      SMX_HIP(hipGetLastError());
+    }
+    k_model_compact<<<dim3(A.ntiles), dim3(ELEM_TILE), 0, stream_>>>(
+        tiles_, mat, reinterpret_cast<float *>(mat_e_), A.nspec_pad);
+    SMX_HIP(hipGetLastError());
+    host_model_stale_ = true;
+    ckpt_every_ = 0;
+    initialize();
+  }
+
+  void get_gradient_device(double *out) override {
+    if (sizeof(real) != 4)
+      return Solver2DBase::get_gradient_device(out);
+    if (!have_gradient_)
+      throw SmxError(SMX_ERR_STATE, "no adjoint run");
+    const HostAssembly2D &A = *asm_;
+    ensure_model_maps();
+    const long n = long(A.nspec) * NPT;
+    k_gradient_out<real><<<dim3((n + 255) / 256), dim3(256), 0, stream_>>>(
+        elem_slot_, sslot_, geom_, quad_, ibool_, kmr_, stacey_, gd_, ge_,
+        racc_, sp_, ss_, A.nspec, A.nspec_pad, A.nstacey_pad, out);
+    SMX_HIP(hipGetLastError());
+    SMX_HIP(hipStreamSynchronize(stream_));
+  }
+
+  void sync_host_model() override {
+    if (!host_model_stale_)
+      return;
+    HostAssembly2D &A = *asm_;
+    download(A.kappa_mu_rho.data(), reinterpret_cast<const float *>(kmr_),
+             A.kappa_mu_rho.size());
+    download(A.lam2mu_mu.data(), reinterpret_cast<const float *>(mat_),
+             A.lam2mu_mu.size());
+    download(A.stacey.data(), reinterpret_cast<const float *>(stacey_),
+             A.stacey.size());
+    download(A.mat_e.data(), reinterpret_cast<const float *>(mat_e_),
+             A.mat_e.size());
+    download(A.tiles.data(), tiles_, A.tiles.size());
+    if (bsum_ && !A.bsum.empty()) {
+      std::vector<float> b(A.bsum.size());
+      download(b.data(), reinterpret_cast<const float *>(bsum_), b.size());
+      std::copy(b.begin(), b.end(), A.bsum.begin());
+    }
+    host_model_stale_ = false;
   }
 
   void run_forward_checkpointed(int every) override {
@@ -1782,6 +2037,7 @@ public:
   void get_gradient(double *out) override {
     if (!have_gradient_)
       throw SmxError(SMX_ERR_STATE, "no adjoint run");
+    sync_host_model();
     const HostAssembly2D &A = *asm_;
     const int ld = A.nspec_pad, sld = A.nstacey_pad;
     const size_t pld = size_t(NPT) * ld, sp = size_t(NPT) * sld;
@@ -2126,6 +2382,31 @@ private:
       if (!e)
         SMX_HIP(hipEventCreate(&e));
   }
+  // Slot maps of the device-memory model and gradient, uploaded once: the
+  // element slots, the absorbing points and each absorbing node's points.
+  void ensure_model_maps() {
+    if (elem_slot_)
+      return;
+    const HostAssembly2D &A = *asm_;
+    elem_slot_ = upload_raw<int>(A.elem_slot);
+    bpt_p_ = upload_raw<int>(A.bpt_p);
+    bpt_bslot_ = upload_raw<int>(A.bpt_bslot);
+    bpt_slot_ = upload_raw<int>(A.bpt_slot);
+    std::unordered_map<int, int> index;
+    for (size_t j = 0; j < A.bnode_g.size(); ++j)
+      index[A.bnode_g[j]] = int(j);
+    std::vector<int> count(A.bnode_g.size() + 1, 0), pts(A.bpt_g.size());
+    for (size_t k = 0; k < A.bpt_g.size(); ++k)
+      ++count[index.at(A.bpt_g[k]) + 1];
+    for (size_t j = 0; j < A.bnode_g.size(); ++j)
+      count[j + 1] += count[j];
+    std::vector<int> fill(count.begin(), count.end() - 1);
+    for (size_t k = 0; k < A.bpt_g.size(); ++k) // ascending k per node
+      pts[fill[index.at(A.bpt_g[k])]++] = int(k);
+    bnode_off_ = upload_raw<int>(count);
+    bnode_pts_ = upload_raw<int>(pts);
+    model_bad_ = alloc<int>(1);
+  }
   // Device arrays of the adjoint, allocated on the first adjoint run.
   void ensure_adjoint() {
     if (rb0_)
@@ -2199,6 +2480,11 @@ private:
       *trec_entry_ = nullptr, *rent_g_ = nullptr, *rent_shared_ = nullptr,
       *step_ = nullptr, *rec_slot_ = nullptr;
   bool has_pressure_ = false; // elastic pressure seismograms requested
+  // device-memory model: slot maps, and whether the host copy is stale
+  int *elem_slot_ = nullptr, *bpt_p_ = nullptr, *bpt_bslot_ = nullptr,
+      *bpt_slot_ = nullptr, *bnode_off_ = nullptr, *bnode_pts_ = nullptr,
+      *model_bad_ = nullptr;
+  bool host_model_stale_ = false;
   uint16_t *lidx_ = nullptr;
   uint8_t *ecolor_ = nullptr;
   real *geom_e_ = nullptr, *mat_e_ = nullptr;

@@ -68,6 +68,20 @@ public:
   // and rebuilds the mass matrix; earlier checkpoints are dropped.
   virtual void set_model(const float *kappa, const float *mu,
                          const float *rho) = 0;
+  // set_model / get_gradient with device arrays (2D FP32 contexts): the
+  // model [nspec][5][5] x3 (float) is read, and the gradient [3][nspec][5][5]
+  // (double) written, in device memory; no host copy of either.
+  virtual void set_model_device(const float *, const float *, const float *) {
+    throw SmxError(SMX_ERR_STATE,
+                   "device-memory models are built for 2D elastic FP32 contexts");
+  }
+  virtual void get_gradient_device(double *) {
+    throw SmxError(SMX_ERR_STATE,
+                   "device-memory gradients are built for 2D elastic FP32 contexts");
+  }
+  // Brings the host copy of the model-derived arrays up to date after a
+  // device-memory model (no-op otherwise).
+  virtual void sync_host_model() {}
   // Forward run of all nstep steps from rest (seismograms as run_forward),
   // keeping the state every `every` steps for the adjoint.
   virtual void run_forward_checkpointed(int every) = 0;

@@ -219,6 +219,18 @@ class Context:
                    for x in (kappa, mu, rho))
         check(self.lib.smx_set_model(self.h, _p(k), _p(m), _p(r)))
 
+    def set_model_device(self, kappa_ptr, mu_ptr, rho_ptr):
+        """set_model from device arrays (addresses of contiguous float32
+        [nspec][5][5] arrays in GPU memory, e.g. CUDA tensors' data_ptr());
+        2D elastic FP32 contexts."""
+        check(self.lib.smx_set_model_device(self.h, ctypes.c_void_p(kappa_ptr),
+                                            ctypes.c_void_p(mu_ptr), ctypes.c_void_p(rho_ptr)))
+
+    def gradient_device(self, out_ptr):
+        """The gradient written into device memory at out_ptr: contiguous
+        float64 (3, nspec, 5, 5)."""
+        check(self.lib.smx_get_gradient_device(self.h, ctypes.c_void_p(out_ptr)))
+
     def run_forward_checkpointed(self, every=100):
         """All nstep steps from rest, keeping the state every ``every`` steps."""
         check(self.lib.smx_run_forward_checkpointed(self.h, int(every)))

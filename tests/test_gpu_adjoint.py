@@ -191,3 +191,81 @@ def test_adjoint_error_paths():
     with pytest.raises(RuntimeError, match="non-physical"):
         c.set_model(*bad)
     assert _lib.load().smx_run_adjoint(None, None) != 0
+
+
+def _compact(ctx):
+    import ctypes
+    g, m = ctypes.c_int(), ctypes.c_int()
+    _lib.check(ctx.lib.smx_ctx_compact(ctx.h, ctypes.byref(g), ctypes.byref(m)))
+    return g.value, m.value
+
+
+def test_device_model_and_gradient_match_the_host_path(oracle_grad):
+    """smx_set_model_device and smx_get_gradient_device (CUDA tensors in and
+    out) against smx_set_model and smx_get_gradient, bit for bit: the
+    seismograms, the gradient read either way, and the element-constant
+    moduli flags, for a model varying inside elements and one constant per
+    element; a non-physical value is refused naming its element and leaves
+    the model as it was."""
+    import torch
+    cfg, _, gs, _ = oracle_grad
+    _say("device-memory model and gradient vs the host path, 12x9 box")
+    varying = perturbed_model(cfg)
+    const = tuple(np.ascontiguousarray(np.broadcast_to(x[:, :1, :1], x.shape))
+                  for x in varying)
+    for name, model in (("varying", varying), ("element-constant", const)):
+        ch = _ctx(cfg, 4)
+        ch.set_model(*model)
+        ch.run_forward_checkpointed(every=64)
+        ch.synchronize()
+        s_h = ch.seismograms()
+        ch.run_adjoint(gs.astype(np.float32))
+        g_h = ch.gradient()
+        cd = _ctx(cfg, 4)
+        tm = [torch.from_numpy(np.ascontiguousarray(x)).to("cuda:0") for x in model]
+        torch.cuda.synchronize()
+        cd.set_model_device(*[t.data_ptr() for t in tm])
+        cd.run_forward_checkpointed(every=64)
+        cd.synchronize()
+        s_d = cd.seismograms()
+        cd.run_adjoint(gs.astype(np.float32))
+        gdev = torch.empty((3, cfg.mesh.nspec, 5, 5), dtype=torch.float64, device="cuda:0")
+        cd.gradient_device(gdev.data_ptr())
+        g_d = gdev.cpu().numpy()
+        _say(f"  {name}: seismograms equal {np.array_equal(s_h, s_d)}, gradient equal "
+             f"{np.array_equal(g_h, g_d)}, flags {_compact(ch)} / {_compact(cd)}")
+        assert np.array_equal(s_h, s_d)
+        assert np.array_equal(g_h, g_d)
+        assert np.array_equal(g_h, cd.gradient())  # the host copy brought up to date
+        assert _compact(ch) == _compact(cd)
+    bad = [t.clone() for t in tm]
+    bad[2].view(-1)[7 * 25 + 3] = -1.0
+    torch.cuda.synchronize()
+    with pytest.raises(_lib.SmxError, match="element 7"):
+        cd.set_model_device(*[t.data_ptr() for t in bad])
+    cd.run_forward_checkpointed(every=64)
+    cd.synchronize()
+    assert np.array_equal(cd.seismograms(), s_d)
+
+
+def test_autograd_on_cuda_tensors_matches_host_tensors(oracle_grad):
+    """The autograd function with CUDA tensors (model and gradient kept in
+    device memory) gives the host-tensor result bit for bit."""
+    import torch
+    cfg, _, _, _ = oracle_grad
+    _say("autograd: CUDA tensors vs host tensors")
+    model = perturbed_model(cfg)
+    w = torch.from_numpy(np.random.default_rng(3).standard_normal(
+        (cfg.nstep, len(cfg.receivers), 2)))
+    out = {}
+    for dev in ("cpu", "cuda:0"):
+        c = _ctx(cfg, 4)
+        ts = [torch.from_numpy(np.ascontiguousarray(x)).to(dev).requires_grad_(True)
+              for x in model]
+        s = Seismograms.apply(*ts, c, 64)
+        loss = (s.double() * w.to(dev)).sum()
+        loss.backward()
+        out[dev] = (s.detach().cpu().numpy(), [t.grad.cpu().numpy() for t in ts])
+    assert np.array_equal(out["cpu"][0], out["cuda:0"][0])
+    for a, b in zip(out["cpu"][1], out["cuda:0"][1]):
+        assert np.array_equal(a, b)
