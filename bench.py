@@ -335,8 +335,8 @@ def batch_leg(nshots=64, nstep=3000, baseline_steps=300):
     """Batched multi-shot forward: 64 independent shots, nx ~ U[40, 400] quads
     of 50 m, nz = round(0.75 nx), six seeded layers each, one Ricker source
     (f0 ~ U[5, 15] Hz) and 100 surface receivers per shot, dt = 4e-4 s, 3000
-    steps, FP32.  The layout (tile size, graph length) comes from the
-    autotuner.  All shots advance together, one launch per kernel per step.
+    steps, FP32.  The layout (tile size, graph length, occupancy variant of
+    the tile kernel) comes from the autotuner.  All shots advance together, one launch per kernel per step.
     The plain per-shot GPU path (one context per shot, run one after the
     other) is timed on `baseline_steps` steps of every shot and scaled to
     `nstep`."""

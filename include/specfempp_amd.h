@@ -212,7 +212,22 @@ typedef struct smx_options {
    * the device reproduces every stored value bit for bit (fewer HBM bytes,
    * more registers); 0 (default): read the stored per-point geometry. */
   int geometry_from_nodes;
+  /* Occupancy variant of the FP32 time-step tile kernel of 2D elastic
+   * contexts: the fewest waves per SIMD its register allocation must allow
+   * (6, 7 or 8; 0 = the default, 7).  Results are bitwise the same for
+   * every value; only the speed changes.  Nonzero values are an error for
+   * FP64, acoustic, coupled and 3D contexts. */
+  int waves_per_eu;
 } smx_options;
+
+/* Changes the time steps per replayed HIP graph (1..256) and the
+ * occupancy variant (smx_options.waves_per_eu) of a 2D elastic context,
+ * single-shot or batched, at any point between runs: the step graph is
+ * captured and uploaded again here (on an initialised context), so the next
+ * smx_run_forward pays for neither.  Results are bitwise unchanged.  No
+ * reference counterpart (Kokkos chooses its launch shapes itself); the
+ * layout autotuner uses it to time candidates on one assembly. */
+int smx_ctx_set_launch(smx_ctx *ctx, int graph_steps, int waves_per_eu);
 
 /* smx_ctx_create2d with layout options (opts may be NULL). */
 int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,

@@ -32,7 +32,7 @@ EXPORTS = [
     "smx_ctx_create3d", "smx_ctx_create3d_opts", "smx_ctx_geometry", "smx_ctx_components",
     "smx_update_wavefields_noalloc", "smx_predictor_medium", "smx_corrector_medium",
     "smx_update_wavefields_medium", "smx_ctx_degrees_of_freedom", "smx_set_model_device",
-    "smx_get_gradient_device",
+    "smx_get_gradient_device", "smx_ctx_set_launch",
 ]
 
 # Medium tags (specfem::element::medium_tag, include/enumerations/medium.hpp:18-28).
@@ -85,7 +85,7 @@ class SmxReceivers3D(ctypes.Structure):
 
 class SmxOptions(ctypes.Structure):
     _fields_ = [("tile_elems", ctypes.c_int), ("graph_steps", ctypes.c_int),
-                ("geometry_from_nodes", ctypes.c_int)]
+                ("geometry_from_nodes", ctypes.c_int), ("waves_per_eu", ctypes.c_int)]
 
 
 class SmxError(RuntimeError):
@@ -133,6 +133,7 @@ def load():
         "smx_get_gradient_device": (i, [P, P]),
         "smx_compute_seismograms": (i, [P, i, i]),
         "smx_run_forward": (i, [P, i, i]),
+        "smx_ctx_set_launch": (i, [P, i, i]),
         "smx_synchronize": (i, [P]),
         "smx_get_seismograms": (i, [P, P]),
         "smx_get_field": (i, [P, i, i, P]),

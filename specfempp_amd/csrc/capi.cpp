@@ -95,6 +95,7 @@ int graph_steps(const smx_options *o) {
 bool geometry_from_nodes(const smx_options *o) {
   return o && o->geometry_from_nodes;
 }
+int waves_per_eu(const smx_options *o) { return o ? o->waves_per_eu : 0; }
 } // namespace
 
 int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
@@ -108,6 +109,9 @@ int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
     auto ctx = std::make_unique<smx_ctx>();
     if (smx::mesh_is_coupled2d(*mesh)) {
       auto c = std::make_unique<smx::CoupledAssembly2D>();
+      if (waves_per_eu(opts) != 0)
+        throw smx::SmxError(SMX_ERR_ARG, "waves_per_eu variants exist for "
+                                         "single-medium elastic contexts only");
       smx::assemble_coupled2d(*mesh, *sources, *receivers, dt, nstep,
                               assembly_options(opts), *c);
       ctx->solver = smx::make_coupled_solver2d(std::move(c), precision,
@@ -118,7 +122,8 @@ int smx_ctx_create2d_opts(const smx_mesh2d *mesh, const smx_sources2d *sources,
                       assembly_options(opts));
       ctx->solver = smx::make_solver2d(std::move(a), precision,
                                        graph_steps(opts),
-                                       geometry_from_nodes(opts));
+                                       geometry_from_nodes(opts),
+                                       waves_per_eu(opts));
     }
     *out = ctx.release();
   });
@@ -166,7 +171,8 @@ int smx_ctx_create2d_batch(int nshots, const smx_mesh2d *meshes,
     shots.clear();
     auto ctx = std::make_unique<smx_ctx>();
     ctx->solver = smx::make_solver2d(std::move(a), precision, graph_steps(opts),
-                                      geometry_from_nodes(opts));
+                                      geometry_from_nodes(opts),
+                                      waves_per_eu(opts));
     *out = ctx.release();
   });
 }
@@ -200,6 +206,9 @@ int smx_ctx_create3d_opts(const smx_mesh3d *mesh, const smx_sources3d *sources,
     if (!mesh || !sources || !receivers || !out)
       throw smx::SmxError(SMX_ERR_ARG, "null argument");
     *out = nullptr;
+    if (waves_per_eu(opts) != 0)
+      throw smx::SmxError(SMX_ERR_ARG,
+                          "waves_per_eu variants exist for 2D elastic contexts only");
     // tiles of 32 elements in rounds of 4 (FP32); 8 elements, one per round,
     // in FP64 (twice the LDS per value)
     smx::AssemblyOptions3D ao;
@@ -384,6 +393,15 @@ int smx_run_forward(smx_ctx *ctx, int istep0, int nstep) {
     if (auto *s3 = S3(ctx))
       return s3->run_forward(istep0, nstep);
     S(ctx).run_forward(istep0, nstep);
+  });
+}
+
+int smx_ctx_set_launch(smx_ctx *ctx, int graph_steps, int waves_per_eu) {
+  return guarded([&] {
+    if (S3(ctx))
+      throw smx::SmxError(SMX_ERR_STATE,
+                          "smx_ctx_set_launch: 2D elastic contexts only");
+    S(ctx).set_launch(graph_steps, waves_per_eu);
   });
 }
 

@@ -38,6 +38,7 @@
 #include <mutex>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "solver2d.hpp"
@@ -215,10 +216,13 @@ __host__ __device__ constexpr size_t tile_lds_bytes(int nloc_max) {
 // footprint allows.
 // GN: the tiles not flagged element-constant re-form their geometry from
 // the control nodes (instead of reading it per point).
-template <typename real, int MODE, bool GN = false>
-__global__ void __launch_bounds__(TILE_THREADS) __attribute__((
-    amdgpu_waves_per_eu(sizeof(real) == 4 && MODE == MODE_STEP ? (GN ? 4 : 7) : 1,
-                        8))) k_tile(const TileInfo *__restrict__ tiles, int nloc_max,
+// WPE: if nonzero, the minimum waves per SIMD the compiler must allow (an
+// occupancy variant of the FP32 time-step kernel timed by the autotuner);
+// the arithmetic is the same, only the register allocation changes.
+template <typename real, int MODE, bool GN = false, int WPE = 0>
+__global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_eu(
+    WPE ? WPE : (sizeof(real) == 4 && MODE == MODE_STEP ? (GN ? 4 : 7) : 1),
+    8))) k_tile(const TileInfo *__restrict__ tiles, int nloc_max,
            const int *__restrict__ sh_list, const uint16_t *__restrict__ lidx,
            const uint8_t *__restrict__ ecolor, int ncolors,
            const real *__restrict__ geom, const real *__restrict__ mat,
@@ -1454,9 +1458,10 @@ void *noalloc_scratch(int device) {
 template <typename real> class Solver2D final : public Solver2DBase {
 public:
   Solver2D(std::unique_ptr<HostAssembly2D> a, int graph_steps,
-           bool nodes_geometry)
+           bool nodes_geometry, int waves_per_eu)
       : asm_(std::move(a)) {
     gsteps_ = graph_steps;
+    wpe_ = waves_per_eu;
     const HostAssembly2D &A = *asm_;
     SMX_HIP(hipGetDevice(&device_));
     SMX_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -1516,6 +1521,11 @@ public:
                           reinterpret_cast<const void *>(&k_adj_tile<real, true>)})
       SMX_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   int(lds_bytes_)));
+    if constexpr (std::is_same<real, float>::value)
+      for (const void *f : {reinterpret_cast<const void *>(&k_tile<real, MODE_STEP, false, 6>),
+                            reinterpret_cast<const void *>(&k_tile<real, MODE_STEP, false, 8>)})
+        SMX_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    int(lds_bytes_)));
     // [0] step of the forward / recompute, [1] its copy for k_shared,
     // [2] step of the adjoint, [3] its copy for k_adj_shared
     step_ = alloc<int>(4);
@@ -1604,6 +1614,31 @@ public:
     }
     SMX_HIP(hipStreamSynchronize(stream_));
     initialized_ = true;
+  }
+
+  // New graph length and occupancy variant: the step graphs are dropped and,
+  // on an initialised context, the forward graph is captured and uploaded
+  // again, so the next run_forward replays the new layout at once.
+  void set_launch(int graph_steps, int waves_per_eu) override {
+    check_launch(graph_steps, waves_per_eu, sizeof(real));
+    if (chi_dd_)
+      throw SmxError(SMX_ERR_STATE,
+                     "the launch layout of a coupled medium is its driver's");
+    SMX_HIP(hipStreamSynchronize(stream_));
+    if (adj_stream_)
+      SMX_HIP(hipStreamSynchronize(adj_stream_));
+    drop_graph(graph_, graph_exec_);
+    for (int b = 0; b < 2; ++b) {
+      drop_graph(rec_graph_[b], rec_graph_exec_[b]);
+      drop_graph(adj_graph_[b], adj_graph_exec_[b]);
+    }
+    gsteps_ = graph_steps;
+    wpe_ = waves_per_eu;
+    if (initialized_) {
+      build_graph(field(SMX_FIELD_FORWARD));
+      SMX_HIP(hipGraphUpload(graph_exec_, stream_));
+      SMX_HIP(hipStreamSynchronize(stream_));
+    }
   }
 
   int predictor(int f) override {
@@ -2108,6 +2143,7 @@ private:
     real *u = nullptr, *v = nullptr, *a = nullptr;
   };
   int gsteps_ = 8; // time steps per replayed graph
+  int wpe_ = 0;    // occupancy variant of k_tile<float, MODE_STEP> (0: default)
 
   template <typename T> T *alloc(size_t n) {
     void *p = nullptr;
@@ -2182,6 +2218,12 @@ private:
     const HostAssembly2D &A = *asm_;
     const int ntypes = int(A.seis_types.size());
     auto kern = gmode_ == 2 ? &k_tile<real, MODE, true> : &k_tile<real, MODE, false>;
+    if constexpr (MODE == MODE_STEP && std::is_same<real, float>::value) {
+      if (gmode_ != 2 && wpe_ == 6)
+        kern = &k_tile<real, MODE, false, 6>;
+      else if (gmode_ != 2 && wpe_ == 8)
+        kern = &k_tile<real, MODE, false, 8>;
+    }
     kern<<<dim3(A.ntiles), dim3(TILE_THREADS), lds_bytes_, stream_>>>(
         tiles_, A.nloc_max, sh_list_, lidx_, ecolor_, A.nelem_colors, geom_,
         mat_, geom_e_, mat_e_, nodes_, sslot_, stacey_, U, V, Aout, minv_,
@@ -2526,19 +2568,35 @@ private:
   bool have_adj_stats_ = false, have_gradient_ = false;
 };
 
-std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
-                                            int precision, int graph_steps,
-                                            bool nodes_geometry) {
+void check_launch(int graph_steps, int waves_per_eu, int precision) {
   if (graph_steps < 1 || graph_steps > 256)
     throw SmxError(SMX_ERR_ARG, "graph_steps must lie in [1, 256]");
-  if (a->medium == 1)
+  if (waves_per_eu != 0 && waves_per_eu != 6 && waves_per_eu != 7 &&
+      waves_per_eu != 8)
+    throw SmxError(SMX_ERR_ARG, "waves_per_eu must be 0 (default), 6, 7 or 8");
+  if (waves_per_eu != 0 && precision != 4)
+    throw SmxError(SMX_ERR_ARG,
+                   "waves_per_eu variants exist for FP32 contexts only");
+}
+
+std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
+                                            int precision, int graph_steps,
+                                            bool nodes_geometry,
+                                            int waves_per_eu) {
+  check_launch(graph_steps, 0, precision);
+  if (a->medium == 1) {
+    if (waves_per_eu != 0)
+      throw SmxError(SMX_ERR_ARG,
+                     "waves_per_eu variants exist for 2D elastic contexts only");
     return make_acoustic_solver2d(std::move(a), precision, graph_steps);
+  }
+  check_launch(graph_steps, waves_per_eu, precision);
   if (precision == 4)
     return std::make_unique<Solver2D<float>>(std::move(a), graph_steps,
-                                             nodes_geometry);
+                                             nodes_geometry, waves_per_eu);
   if (precision == 8)
     return std::make_unique<Solver2D<double>>(std::move(a), graph_steps,
-                                              nodes_geometry);
+                                              nodes_geometry, 0);
   throw SmxError(SMX_ERR_ARG, "precision must be 4 or 8");
 }
 

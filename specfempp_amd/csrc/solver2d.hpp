@@ -52,6 +52,12 @@ public:
   }
   virtual void compute_seismograms(int field, int isig) = 0;
   virtual void run_forward(int istep0, int nstep) = 0;
+  // Time steps per replayed graph and occupancy variant of the time-step
+  // tile kernel (0 default, 6, 7, 8 waves per SIMD), for the autotuner.
+  virtual void set_launch(int, int) {
+    throw SmxError(SMX_ERR_STATE,
+                   "smx_ctx_set_launch: 2D elastic contexts only");
+  }
   virtual void synchronize() = 0;
 
   virtual void get_seismograms(void *host) = 0;
@@ -122,10 +128,16 @@ template <typename S> struct StreamSwapT {
 };
 
 // graph_steps: time steps per replayed HIP graph; nodes_geometry: re-form
-// per-point geometry from the control nodes when bitwise reproducible.
+// per-point geometry from the control nodes when bitwise reproducible;
+// waves_per_eu: occupancy variant of the FP32 time-step tile kernel (0 the
+// default, 6, 7 or 8 waves per SIMD; elastic contexts only).
 std::unique_ptr<Solver2DBase> make_solver2d(std::unique_ptr<HostAssembly2D> a,
                                             int precision, int graph_steps = 8,
-                                            bool nodes_geometry = false);
+                                            bool nodes_geometry = false,
+                                            int waves_per_eu = 0);
+// Throws SMX_ERR_ARG unless graph_steps and waves_per_eu are valid for a
+// context of this precision.
+void check_launch(int graph_steps, int waves_per_eu, int precision);
 
 // The acoustic solver (acoustic2d.hip); make_solver2d dispatches to it for
 // assemblies whose medium is acoustic.

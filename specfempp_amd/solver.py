@@ -71,7 +71,8 @@ def _options(options):
     if options is None:
         return None
     o = _lib.SmxOptions(int(options.get("tile_elems", 0)), int(options.get("graph_steps", 0)),
-                        int(bool(options.get("geometry_from_nodes", False))))
+                        int(bool(options.get("geometry_from_nodes", False))),
+                        int(options.get("waves_per_eu", 0)))
     return ctypes.byref(o)
 
 
@@ -79,11 +80,12 @@ class Context:
     """An smx_ctx: the device-resident assembly, fields and seismograms.
 
     ``options`` (dict, optional): ``tile_elems`` (elements per tile, 1..64),
-    ``graph_steps`` (time steps per replayed graph), as chosen by
-    ``specfempp_amd.autotune``, and ``geometry_from_nodes`` (re-form the
-    per-point geometry from the control nodes when the device reproduces
-    it bit for bit; off by default).  ``Context.batch`` builds one context over
-    many independent shots."""
+    ``graph_steps`` (time steps per replayed graph) and ``waves_per_eu``
+    (occupancy variant of the FP32 time-step tile kernel: 0 default, 6, 7,
+    8), as chosen by ``specfempp_amd.autotune``, and ``geometry_from_nodes``
+    (re-form the per-point geometry from the control nodes when the device
+    reproduces it bit for bit; off by default).  ``Context.batch`` builds one
+    context over many independent shots."""
 
     def __init__(self, mesh, sources, receivers, dt, nstep, *, nsbs=1,
                  seismo_types=("displacement",), precision=4, options=None, _batch=None):
@@ -205,6 +207,11 @@ class Context:
         check(self.lib.smx_last_run_stats(self.h, ctypes.byref(ms), ctypes.byref(sms),
                                           ctypes.byref(n)))
         return ms.value
+
+    def set_launch(self, graph_steps, waves_per_eu=0):
+        """New graph length and tile-kernel occupancy variant (smx_ctx_set_launch);
+        the step graph is re-captured here, results are bitwise unchanged."""
+        check(self.lib.smx_ctx_set_launch(self.h, int(graph_steps), int(waves_per_eu)))
 
     def time_stiffness(self, nlaunch=20):
         t = ctypes.c_double()

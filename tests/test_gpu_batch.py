@@ -5,7 +5,8 @@
   its seismograms (and adjoint gradients) equal those of its own context bit
   for bit.
 * Tile sizes 16 / 32 / 64 only change summation order: seismograms stay
-  within 1e-5 of the oracle.  The graph length changes nothing.
+  within 1e-5 of the oracle.  The graph length and the occupancy variant of
+  the tile kernel change nothing, also when switched between runs.
 """
 
 import sys
@@ -13,6 +14,8 @@ import sys
 import numpy as np
 import pytest
 
+from specfempp_amd._lib import SmxError
+from specfempp_amd.autotune import autotune, candidates, label
 from specfempp_amd.config import batched_shots
 from specfempp_amd.solver import Context, simulate, simulate_batch
 from tests._cases import rel_l2
@@ -75,6 +78,66 @@ def test_graph_length_changes_nothing():
     for g in (1, 32):
         s, _ = simulate(cfg, options={"graph_steps": g})
         assert np.array_equal(s, base)
+
+
+@pytest.mark.parametrize("wpe", [6, 8])
+def test_occupancy_variant_changes_nothing(wpe):
+    cfg = _shots(n=1, nstep=203, seed=5)[0]
+    _say(f"waves_per_eu={wpe}: bitwise identical to the default, at creation and "
+         f"switched between runs")
+    base, _ = simulate(cfg)
+    s, _ = simulate(cfg, options={"waves_per_eu": wpe, "graph_steps": 32})
+    assert np.array_equal(s, base)
+
+    # two runs of 100 + 103 steps (the split itself rounds differently from
+    # one run: the first run ends with a separate corrector, the second
+    # starts with a separate predictor), the layout switched in between
+    def split(switch):
+        ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep,
+                      nsbs=cfg.nsbs, seismo_types=cfg.seismo_types)
+        ctx.initialize()
+        ctx.run_forward(0, 100)
+        if switch:
+            ctx.set_launch(32, wpe)
+        ctx.run_forward(100, 103)
+        ctx.synchronize()
+        return ctx.seismograms()
+
+    ref = split(False)
+    assert rel_l2(ref, base) <= 1e-6
+    assert np.array_equal(split(True), ref)
+
+
+def test_set_launch_rejects_bad_values():
+    cfg = _shots(n=1, nstep=20, seed=5)[0]
+    ctx = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep)
+    for g, w in ((0, 0), (257, 0), (8, 5), (8, 9)):
+        with pytest.raises(SmxError):
+            ctx.set_launch(g, w)
+    c8 = Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep, precision=8)
+    with pytest.raises(SmxError):
+        c8.set_launch(8, 8)
+    c8.set_launch(32, 0)
+    with pytest.raises(SmxError):
+        Context(cfg.mesh, cfg.sources, cfg.receivers, cfg.dt, cfg.nstep, precision=8,
+                options={"waves_per_eu": 8})
+
+
+def test_autotune_times_every_candidate():
+    shots = _shots(n=2, nstep=50, seed=23)
+    _say("autotuner on a 2-shot batch: 3 tile sizes x 2 graph lengths x 3 "
+         "occupancy variants")
+    best, timings = autotune(shots, use_cache=False)
+    cands = candidates(shots)
+    assert len(cands) == 18
+    assert sorted(timings) == sorted(label(c) for c in cands)
+    assert all(np.isfinite(v) and v > 0 for v in timings.values())
+    assert best in cands
+    assert timings[label(best)] == min(timings.values())
+    got, _ = simulate_batch(shots, options=best)
+    ref, _ = simulate_batch(shots, options={"tile_elems": best["tile_elems"]})
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
 
 
 def test_batched_adjoint_equals_individual_gradients():
