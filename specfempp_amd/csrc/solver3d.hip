@@ -209,7 +209,9 @@ __device__ __forceinline__ void finish_point(long i, long n3, real acc,
 // comes from (GeomMode); ME: moduli per element instead of per GLL point;
 // RW: elements per round (block = RW x 128 threads).  LDS: the tile's
 // accumulator [3][nloc_max], then per round slot the element displacement
-// [3][125] and stress integrand [3][3][125].
+// [3][125] and stress integrand [3][3][125], then the global ids of the
+// tile's shared points [nsh_max] and its round slots [nround_max][RW]
+// (int), so that no global load of a round waits on another.
 template <typename real, int MODE, int GM, bool ME, int RW>
 __global__ void __launch_bounds__(RW * ELEM_THREADS)
     k3_tile(const TileInfo3 *__restrict__ tiles,
@@ -222,9 +224,9 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
             const real *__restrict__ Minv, real *__restrict__ part, long npart,
             const int *__restrict__ tsrc_li, const int *__restrict__ tsrc_id,
             const real *__restrict__ tsrc_val, const real *__restrict__ stf,
-            int nsrc, int nloc_max, long npad, real dt, real dto2, real dt2o2,
-            Quad3<real> q, const int *__restrict__ step_ptr, Rec3<real> rec,
-            Aux3<real> aux) {
+            int nsrc, int nloc_max, int nsh_max, long npad, real dt, real dto2,
+            real dt2o2, Quad3<real> q, const int *__restrict__ step_ptr,
+            Rec3<real> rec, Aux3<real> aux) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   real *s_a = reinterpret_cast<real *>(lds_raw);
   const TileInfo3 T = tiles[blockIdx.x];
@@ -235,6 +237,9 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
   const int iz = pp / (NGLL * NGLL), iy = (pp / NGLL) % NGLL, ix = pp % NGLL;
   real *se = s_a + 3 * size_t(nloc_max) + size_t(sub) * 12 * NPT3;
   real *sf = se + 3 * NPT3;
+  int *s_sh = reinterpret_cast<int *>(s_a + 3 * size_t(nloc_max) +
+                                      size_t(RW) * 12 * NPT3);
+  int *s_slot = s_sh + nsh_max;
   const int nloc = T.nex + T.nsh;
   const int step = MODE == M3_ASSEMBLE ? 0 : *step_ptr;
   for (int i = tid; i < nloc; i += RW * ELEM_THREADS) {
@@ -242,6 +247,10 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
     s_a[nloc_max + i] = 0;
     s_a[2 * nloc_max + i] = 0;
   }
+  for (int i = tid; i < T.nsh; i += RW * ELEM_THREADS)
+    s_sh[i] = sh_list[T.sh_off + i];
+  for (int i = tid; i < T.nround * RW; i += RW * ELEM_THREADS)
+    s_slot[i] = round_slots[size_t(T.round_off) * RW + i];
   __syncthreads();
   // point sources first, in list order (one thread: fixed order)
   if (MODE != M3_ASSEMBLE && tid == 0)
@@ -254,20 +263,28 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
     }
   __syncthreads();
 
-  // Rounds are software-pipelined: the indices, displacement, geometry and
-  // moduli of round r + 1 are requested while round r computes (no point
-  // of the tile is written before all rounds are done, and shared points
-  // only by the next kernel, so the early reads see this step's values).
+  // Rounds are software-pipelined: the displacement, geometry and moduli of
+  // round r + 1 are requested while round r computes, and the tile-local
+  // point ids of round r + 2 (the global id then comes from LDS), so each
+  // global load depends on nothing fetched in the same round (no point of
+  // the tile is written before all rounds are done, and shared points only
+  // by the next kernel, so the early reads see this step's values).
+  auto slot_of = [&](int r) { return r < T.nround ? s_slot[r * RW + sub] : -1; };
+  int l_pre = 0; // tile-local id of this thread's point in the next fetch
+  auto fetch_l = [&](int r) {
+    const int s = slot_of(r);
+    if (pact && s >= 0)
+      l_pre = lidx[size_t(s) * NPT3 + p];
+  };
   int slot_n = -1, l_n = 0;
   long g_n = 0;
   real u_n[3] = {0, 0, 0}, G_n[10], L_n = 0, M_n = 0;
   auto fetch = [&](int r) {
-    slot_n = r < T.nround ? round_slots[(size_t(T.round_off) + r) * RW + sub] : -1;
+    slot_n = slot_of(r);
     if (!(pact && slot_n >= 0))
       return;
-    l_n = lidx[size_t(slot_n) * NPT3 + p];
-    g_n = l_n < T.nex ? long(T.ex_off) + l_n
-                      : long(sh_list[T.sh_off + l_n - T.nex]);
+    l_n = l_pre;
+    g_n = l_n < T.nex ? long(T.ex_off) + l_n : long(s_sh[l_n - T.nex]);
 #pragma unroll
     for (int c = 0; c < 3; ++c)
       u_n[c] = U[c * npad + g_n];
@@ -280,7 +297,9 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
     L_n = ME ? mat[size_t(slot_n) * 2] : mat[size_t(slot_n) * 2 * NPT3 + p];
     M_n = ME ? mat[size_t(slot_n) * 2 + 1] : mat[(size_t(slot_n) * 2 + 1) * NPT3 + p];
   };
+  fetch_l(0);
   fetch(0);
+  fetch_l(1);
   for (int r = 0; r < T.nround; ++r) {
     const int slot = slot_n;
     const bool act = pact && slot >= 0;
@@ -297,6 +316,7 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
         se[c * NPT3 + p] = u_n[c];
     __syncthreads();
     fetch(r + 1);
+    fetch_l(r + 2);
     if (act) {
       real dxi[3], det[3], dga[3];
 #pragma unroll
@@ -958,8 +978,13 @@ public:
     tsrc_val_ = upload<real>(A.tsrc_val);
     npart_ = std::max(1L, long(A.npart));
     part_ = alloc<real>(3 * size_t(npart_));
+    for (const TileInfo3 &t : A.tiles) {
+      nsh_max_ = std::max(nsh_max_, t.nsh);
+      nround_max_ = std::max(nround_max_, t.nround);
+    }
     tile_lds_ = (size_t(3) * std::max(1, A.nloc_max) + size_t(kRoundWidth) * 12 * NPT3) *
-                sizeof(real);
+                    sizeof(real) +
+                (size_t(nsh_max_) + size_t(nround_max_) * kRoundWidth) * sizeof(int);
     {
       int dev = 0;
       SMX3_HIP(hipGetDevice(&dev));
@@ -1656,7 +1681,7 @@ private:
           tiles_, round_slots_, lidx_, sh_list_, geom_, nodes_, xi3_, mat_,
           sslot_, stacey_, U_, V_, acc_out, minv_, part_, npart_, tsrc_li_,
           tsrc_id_, tsrc_val_, stf_, std::max(1, A.nsrc), std::max(1, A.nloc_max),
-          npad_, dt_, dto2_, dt2o2_, quad_, step_, rec, aux);
+          nsh_max_, npad_, dt_, dto2_, dt2o2_, quad_, step_, rec, aux);
     };
     const bool me = A.mat_elem;
     switch (gmode_) {
@@ -1776,6 +1801,7 @@ private:
        *aux_a_ = nullptr;
   long npart_ = 1;
   size_t tile_lds_ = 0, adj_lds_ = 0;
+  int nsh_max_ = 0, nround_max_ = 0; // over the tiles (k3_tile's LDS tables)
   bool adj_tiles_ = false; // adjoint as tiles (else colour passes)
   bool need_disp_ = false;
   hipStream_t stream_ = nullptr;
