@@ -240,7 +240,8 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
            int ld, int sld, real dt, real dto2, real dt2o2, QuadArgs<real> q,
            void *__restrict__ rec, long rec_stride, int every,
            const int *__restrict__ cpl_li, const int *__restrict__ cpl_k,
-           const real *__restrict__ cpl_w, const real *__restrict__ chi_dd) {
+           const real *__restrict__ cpl_w, const real *__restrict__ chi_dd,
+           const int *__restrict__ sh_pad, int sh_stride) {
   using V2 = typename Vec2<real>::T;
   extern __shared__ __align__(16) unsigned char lds_raw[];
   V2 *s_u = reinterpret_cast<V2 *>(lds_raw);
@@ -283,14 +284,27 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
   for (int iz = 0; iz < NGLL; ++iz)
     li[iz] = valid ? int(lidx[size_t(iz * NGLL + ix) * ld + slot]) : 0;
 
-  // ---- tile points: displacement in, accumulator cleared
-  for (int i = tid; i < nloc; i += TILE_THREADS) {
-    const int g = i < T.nex ? T.ex_off + i : sh_list[T.sh_off + i - T.nex];
+  // ---- tile points: displacement in, accumulator cleared.  The global ids
+  // of the tile's shared points sit at a fixed stride per tile (sh_pad,
+  // padded with id 0), so their displacement loads wait on one load only,
+  // not on the tile record as well.
+  for (int s = tid; s < sh_stride; s += TILE_THREADS) {
+    const V2 u = U2[sh_pad[size_t(tile) * sh_stride + s]];
+    if (s < T.nsh) {
+      V2 z;
+      z.x = 0;
+      z.y = 0;
+      s_u[T.nex + s] = u;
+      s_a[T.nex + s] = z;
+    }
+  }
+  for (int i = tid; i < T.nex; i += TILE_THREADS) {
+    const int g = T.ex_off + i;
     s_u[i] = U2[g];
     V2 z;
     z.x = 0;
     z.y = 0;
-    if (MODE == MODE_ASSEMBLE && i < T.nex)
+    if (MODE == MODE_ASSEMBLE)
       z = A2[g];
     s_a[i] = z;
   }
@@ -1471,6 +1485,16 @@ public:
     n2_ = 2L * nglob_pad_;
     tiles_ = upload_raw<TileInfo>(A.tiles);
     sh_list_ = upload_raw<int>(A.sh_list);
+    {
+      // shared-point ids at a fixed stride per tile for the tile gather
+      for (const TileInfo &t : A.tiles)
+        sh_stride_ = std::max(sh_stride_, t.nsh);
+      std::vector<int> pad(size_t(std::max(1, A.ntiles)) * sh_stride_, 0);
+      for (int t = 0; t < A.ntiles; ++t)
+        for (int k = 0; k < A.tiles[t].nsh; ++k)
+          pad[size_t(t) * sh_stride_ + k] = A.sh_list[A.tiles[t].sh_off + k];
+      sh_pad_ = upload_raw<int>(pad);
+    }
     shp_off_ = upload_raw<int>(A.shp_off);
     shp_pos_ = upload_raw<int>(A.shp_pos);
     lidx_ = upload_raw<uint16_t>(A.lidx);
@@ -2232,7 +2256,7 @@ private:
         rent_h_, terms, A.nrec * NPT, stypes_, ntypes, A.nsbs, step_ptr, istep,
         A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, rec,
         rec_stride(), std::max(1, rec_every_), cpl_li_, cpl_k_, cpl_w_,
-        chi_dd_);
+        chi_dd_, sh_pad_, sh_stride_);
     SMX_HIP(hipGetLastError());
   }
   void launch_pressure(const real *U, const int *step_ptr, int isig) {
@@ -2545,6 +2569,8 @@ private:
   real *cpl_w_ = nullptr;
   const real *chi_dd_ = nullptr;
   size_t lds_bytes_ = 0;
+  int *sh_pad_ = nullptr; // [ntiles][sh_stride_] global ids of shared points
+  int sh_stride_ = 1;
   Field fields_[3];
   QuadArgs<real> quad_;
   SeisTypes stypes_;
