@@ -125,7 +125,8 @@ __global__ void __launch_bounds__(AC_TILE_THREADS) __attribute__((
               AcQuad<real> q, const int *__restrict__ cpl_li,
               const int *__restrict__ cpl_g, const real *__restrict__ cpl_w,
               const real *__restrict__ Ue, const int *__restrict__ cout_li,
-              const int *__restrict__ cout_k, real *__restrict__ chi_dd) {
+              const int *__restrict__ cout_k, real *__restrict__ chi_dd,
+              const int *__restrict__ sh_pad, int sh_stride) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   real *s_u = reinterpret_cast<real *>(lds_raw);
   real *s_a = s_u + nloc_max;
@@ -148,11 +149,20 @@ __global__ void __launch_bounds__(AC_TILE_THREADS) __attribute__((
   for (int iz = 0; iz < NGLL; ++iz)
     li[iz] = valid ? int(lidx[size_t(iz * NGLL + ix) * ld + slot]) : 0;
 
-  // ---- tile points: potential in, accumulator cleared
-  for (int i = tid; i < nloc; i += AC_TILE_THREADS) {
-    const int g = i < T.nex ? T.ex_off + i : sh_list[T.sh_off + i - T.nex];
+  // ---- tile points: potential in, accumulator cleared (shared points'
+  // global ids at a fixed stride per tile, padded with id 0, so that their
+  // loads do not wait on the tile record)
+  for (int s = tid; s < sh_stride; s += AC_TILE_THREADS) {
+    const real u = U[sh_pad[size_t(blockIdx.x) * sh_stride + s]];
+    if (s < T.nsh) {
+      s_u[T.nex + s] = u;
+      s_a[T.nex + s] = real(0);
+    }
+  }
+  for (int i = tid; i < T.nex; i += AC_TILE_THREADS) {
+    const int g = T.ex_off + i;
     s_u[i] = U[g];
-    s_a[i] = (MODE == AC_ASSEMBLE && i < T.nex) ? A[g] : real(0);
+    s_a[i] = MODE == AC_ASSEMBLE ? A[g] : real(0);
   }
   __syncthreads();
   // point sources first, in list order: -stf L / kappa
@@ -559,6 +569,16 @@ public:
     n_ = (A.nglob + 255) / 256 * 256;
     tiles_ = upload_raw<TileInfo>(A.tiles);
     sh_list_ = upload_raw<int>(A.sh_list);
+    {
+      // shared-point ids at a fixed stride per tile for the tile gather
+      for (const TileInfo &t : A.tiles)
+        sh_stride_ = std::max(sh_stride_, t.nsh);
+      std::vector<int> pad(size_t(std::max(1, A.ntiles)) * sh_stride_, 0);
+      for (int t = 0; t < A.ntiles; ++t)
+        for (int k = 0; k < A.tiles[t].nsh; ++k)
+          pad[size_t(t) * sh_stride_ + k] = A.sh_list[A.tiles[t].sh_off + k];
+      sh_pad_ = upload_raw<int>(pad);
+    }
     shp_off_ = upload_raw<int>(A.shp_off);
     shp_pos_ = upload_raw<int>(A.shp_pos);
     lidx_ = upload_raw<uint16_t>(A.lidx);
@@ -965,7 +985,7 @@ private:
         std::max(1, A.nsrc), trec_li_, trec_entry_, rent_w_, rent_h_, terms,
         A.nrec * NPT, types_, int(A.seis_types.size()), A.nsbs, step_ptr, istep,
         A.nspec_pad, A.nstacey_pad, dt_, dto2_, dt2o2_, quad_, cpl_li_, cpl_g_,
-        cpl_w_, ue_, cout_li_, cout_k_, chi_dd_);
+        cpl_w_, ue_, cout_li_, cout_k_, chi_dd_, sh_pad_, sh_stride_);
     AC_HIP(hipGetLastError());
   }
   template <int MODE> void launch_shared(Field &F, int *step_ptr) {
@@ -1039,6 +1059,8 @@ private:
        *part_ = nullptr, *seis_ = nullptr, *seis_terms_ = nullptr,
        *scratch_part_ = nullptr;
   size_t lds_bytes_ = 0;
+  int *sh_pad_ = nullptr; // [ntiles][sh_stride_] global ids of shared points
+  int sh_stride_ = 1;
   Field fields_[3], scratch_;
   AcQuad<real> quad_;
   AcTypes types_;
