@@ -159,25 +159,24 @@ template <typename real> struct Aux3 {
 };
 
 // Finish one (point, component), i = c * npad + g, from its assembled force
-// acc: a = acc / M, v += dt/2 a, then the next predictor
-// u += dt v + dt^2/2 a, v += dt/2 a (divide_mass_matrix.tpp:13-65,
+// acc and its loaded 1/M, v and u: a = acc / M, v += dt/2 a, then the next
+// predictor u += dt v + dt^2/2 a, v += dt/2 a (divide_mass_matrix.tpp:13-65,
 // newmark.tpp:13-128).
 template <typename real, int MODE>
-__device__ __forceinline__ void finish_point(long i, long n3, real acc,
-                                             real *__restrict__ U,
-                                             real *__restrict__ V,
-                                             real *__restrict__ A,
-                                             const real *__restrict__ Minv,
-                                             real dt, real dto2, real dt2o2,
-                                             const Rec3<real> &rec, int j,
-                                             const Aux3<real> &aux,
-                                             bool sample) {
-  const real a = acc * Minv[i];
+__device__ __forceinline__ void finish_loaded(long i, long n3, real acc,
+                                              real minv, real v, real u,
+                                              real *__restrict__ U,
+                                              real *__restrict__ V,
+                                              real *__restrict__ A,
+                                              real dt, real dto2, real dt2o2,
+                                              const Rec3<real> &rec, int j,
+                                              const Aux3<real> &aux,
+                                              bool sample) {
+  const real a = acc * minv;
   if (MODE == M3_ASSEMBLE) {
     A[i] = a;
     return;
   }
-  real v = V[i];
   v += dto2 * a;
   if (sample) {
     aux.v[i] = v;
@@ -190,7 +189,6 @@ __device__ __forceinline__ void finish_point(long i, long n3, real acc,
     A[i] = a;
     return;
   }
-  real u = U[i];
   u += dt * v + dt2o2 * a;
   v += dto2 * a;
   U[i] = u;
@@ -203,6 +201,23 @@ __device__ __forceinline__ void finish_point(long i, long n3, real acc,
     if (b >= 0)
       rec.vb[(size_t(j + 1) * 3 + c) * rec.nb + b] = v;
   }
+}
+// finish_loaded with its loads.
+template <typename real, int MODE>
+__device__ __forceinline__ void finish_point(long i, long n3, real acc,
+                                             real *__restrict__ U,
+                                             real *__restrict__ V,
+                                             real *__restrict__ A,
+                                             const real *__restrict__ Minv,
+                                             real dt, real dto2, real dt2o2,
+                                             const Rec3<real> &rec, int j,
+                                             const Aux3<real> &aux,
+                                             bool sample) {
+  const real minv = Minv[i];
+  const real v = MODE == M3_ASSEMBLE ? real(0) : V[i];
+  const real u = (MODE == M3_STEP || MODE == M3_RECORD) ? U[i] : real(0);
+  finish_loaded<real, MODE>(i, n3, acc, minv, v, u, U, V, A, dt, dto2, dt2o2,
+                            rec, j, aux, sample);
 }
 
 // One tile of elements (see the file comment).  GM: where the geometry
@@ -409,12 +424,37 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
   const long n3 = 3 * npad;
   const bool sample = MODE != M3_ASSEMBLE && aux.v != nullptr && step % aux.nsbs == 0;
   const int j = MODE == M3_RECORD ? step % rec.every : 0;
-  for (int i = tid; i < T.nex; i += RW * ELEM_THREADS) {
-    const long g = long(T.ex_off) + i;
+  // up to kFin exclusive points per thread: all their loads are requested
+  // before the first is used (the round registers are free by now)
+  constexpr int kFin = 4;
+  for (int i0 = tid; i0 < T.nex; i0 += kFin * RW * ELEM_THREADS) {
+    real mv[kFin][3], vv[kFin][3], uv[kFin][3];
 #pragma unroll
-    for (int c = 0; c < 3; ++c)
-      finish_point<real, MODE>(c * npad + g, n3, s_a[c * nloc_max + i], U, V, A,
-                               Minv, dt, dto2, dt2o2, rec, j, aux, sample);
+    for (int k = 0; k < kFin; ++k) {
+      const int i = i0 + k * RW * ELEM_THREADS;
+      if (i < T.nex) {
+        const long g = long(T.ex_off) + i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+          mv[k][c] = Minv[c * npad + g];
+          vv[k][c] = MODE == M3_ASSEMBLE ? real(0) : V[c * npad + g];
+          uv[k][c] = (MODE == M3_STEP || MODE == M3_RECORD) ? U[c * npad + g]
+                                                            : real(0);
+        }
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kFin; ++k) {
+      const int i = i0 + k * RW * ELEM_THREADS;
+      if (i < T.nex) {
+        const long g = long(T.ex_off) + i;
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+          finish_loaded<real, MODE>(c * npad + g, n3, s_a[c * nloc_max + i],
+                                    mv[k][c], vv[k][c], uv[k][c], U, V, A, dt,
+                                    dto2, dt2o2, rec, j, aux, sample);
+      }
+    }
   }
   for (int i = tid; i < T.nsh; i += RW * ELEM_THREADS)
 #pragma unroll
