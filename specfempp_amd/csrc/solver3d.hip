@@ -719,21 +719,41 @@ template <typename real> struct AdjState3 {
 // (rb of step k-1 is formed here; ab is not kept).  Accumulates rb . a'_k
 // per node (density term) and, at the absorbing nodes, rb v~_k^T (9) and
 // rb * a'_k per component (3) for the impedance terms; B at a node is the
-// sum of its face points' symmetric 3x3 Stacey matrices.
+// sum of its face points' symmetric 3x3 Stacey matrices.  adj_load reads
+// what the update needs of node g (all three components), adj_apply
+// applies it, adj_finish does both.
+template <typename real> struct AdjNode {
+  real rb[3], a[3], ub[3], vb[3], minv[3], R;
+  int b;
+};
+
 template <typename real>
-__device__ __forceinline__ void adj_finish(long g, const real kr[3], int j,
-                                           const AdjState3<real> &S) {
-  const long npad = S.npad, nb = S.nb;
+__device__ __forceinline__ void adj_load(long g, int j, const AdjState3<real> &S,
+                                         AdjNode<real> &n) {
+  const long npad = S.npad;
   const real *ap = S.rec_a + size_t(j) * 3 * npad;
-  real rb[3], a[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    rb[c] = S.rb[c * npad + g];
-    a[c] = ap[c * npad + g];
+    const long i = c * npad + g;
+    n.rb[c] = S.rb[i];
+    n.a[c] = ap[i];
+    n.ub[c] = S.ub[i];
+    n.vb[c] = S.vb[i];
+    n.minv[c] = S.minv[i];
   }
-  S.R[g] += rb[0] * a[0] + rb[1] * a[1] + rb[2] * a[2];
+  n.R = S.R[g];
+  n.b = S.bidx[g];
+}
+
+template <typename real>
+__device__ __forceinline__ void adj_apply(long g, const real kr[3], int j,
+                                          const AdjState3<real> &S,
+                                          const AdjNode<real> &n) {
+  const long npad = S.npad, nb = S.nb;
+  const real *rb = n.rb, *a = n.a;
+  S.R[g] = n.R + (rb[0] * a[0] + rb[1] * a[1] + rb[2] * a[2]);
   real br[3] = {0, 0, 0};
-  const int b = S.bidx[g];
+  const int b = n.b;
   if (b >= 0) {
     const real *bm = S.bmat;
     const real mxx = bm[b], myy = bm[nb + b], mzz = bm[2 * nb + b],
@@ -754,14 +774,22 @@ __device__ __forceinline__ void adj_finish(long g, const real kr[3], int j,
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     const long i = c * npad + g;
-    const real uu = S.ub[i] + kr[c];
-    const real vv = S.vb[i] - br[c];
+    const real uu = n.ub[c] + kr[c];
+    const real vv = n.vb[c] - br[c];
     const real vbn = vv + S.dt * uu;
     const real abn = S.dto2 * vv + S.dt2o2 * uu;
     S.ub[i] = uu;
     S.vb[i] = vbn;
-    S.rb[i] = S.minv[i] * (abn + S.dto2 * vbn);
+    S.rb[i] = n.minv[c] * (abn + S.dto2 * vbn);
   }
+}
+
+template <typename real>
+__device__ __forceinline__ void adj_finish(long g, const real kr[3], int j,
+                                           const AdjState3<real> &S) {
+  AdjNode<real> n;
+  adj_load(g, j, S, n);
+  adj_apply(g, kr, j, S, n);
 }
 
 // Adjoint node pass after the colour passes: every node with its assembled
@@ -925,9 +953,25 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
     }
     __syncthreads();
   }
-  for (int i = tid; i < T.nex; i += RW * ELEM_THREADS) {
-    const real kr[3] = {s_a[i], s_a[nloc_max + i], s_a[2 * nloc_max + i]};
-    adj_finish(long(T.ex_off) + i, kr, j, S);
+  // up to two exclusive nodes per thread: their loads are requested before
+  // the first update (the round registers are free by now)
+  constexpr int kAdjFin = 2;
+  for (int i0 = tid; i0 < T.nex; i0 += kAdjFin * RW * ELEM_THREADS) {
+    AdjNode<real> nd[kAdjFin];
+#pragma unroll
+    for (int k = 0; k < kAdjFin; ++k) {
+      const int i = i0 + k * RW * ELEM_THREADS;
+      if (i < T.nex)
+        adj_load(long(T.ex_off) + i, j, S, nd[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kAdjFin; ++k) {
+      const int i = i0 + k * RW * ELEM_THREADS;
+      if (i < T.nex) {
+        const real kr[3] = {s_a[i], s_a[nloc_max + i], s_a[2 * nloc_max + i]};
+        adj_apply(long(T.ex_off) + i, kr, j, S, nd[k]);
+      }
+    }
   }
   for (int i = tid; i < T.nsh; i += RW * ELEM_THREADS)
 #pragma unroll
