@@ -202,23 +202,6 @@ __device__ __forceinline__ void finish_loaded(long i, long n3, real acc,
       rec.vb[(size_t(j + 1) * 3 + c) * rec.nb + b] = v;
   }
 }
-// finish_loaded with its loads.
-template <typename real, int MODE>
-__device__ __forceinline__ void finish_point(long i, long n3, real acc,
-                                             real *__restrict__ U,
-                                             real *__restrict__ V,
-                                             real *__restrict__ A,
-                                             const real *__restrict__ Minv,
-                                             real dt, real dto2, real dt2o2,
-                                             const Rec3<real> &rec, int j,
-                                             const Aux3<real> &aux,
-                                             bool sample) {
-  const real minv = Minv[i];
-  const real v = MODE == M3_ASSEMBLE ? real(0) : V[i];
-  const real u = (MODE == M3_STEP || MODE == M3_RECORD) ? U[i] : real(0);
-  finish_loaded<real, MODE>(i, n3, acc, minv, v, u, U, V, A, dt, dto2, dt2o2,
-                            rec, j, aux, sample);
-}
 
 // One tile of elements (see the file comment).  GM: where the geometry
 // comes from (GeomMode); ME: moduli per element instead of per GLL point;
@@ -480,15 +463,27 @@ __global__ void __launch_bounds__(NODE3_THREADS)
   const int j = MODE == M3_RECORD ? step % rec.every : 0;
   const long g = long(nex_total) + s;
   const long n3 = 3 * npad;
-  const int k0 = sh_poff[s], k1 = sh_poff[s + 1];
+  // the point's own 1/M, v and u first (they do not wait on the partials),
+  // then the partials of the three components in tile order
+  real mv[3], vv[3], uv[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
-    real acc = 0;
-    for (int k = k0; k < k1; ++k)
-      acc += part[c * npart + sh_pidx[k]];
-    finish_point<real, MODE>(c * npad + g, n3, acc, U, V, A, Minv, dt, dto2,
-                             dt2o2, rec, j, aux, sample);
+    mv[c] = Minv[c * npad + g];
+    vv[c] = MODE == M3_ASSEMBLE ? real(0) : V[c * npad + g];
+    uv[c] = (MODE == M3_STEP || MODE == M3_RECORD) ? U[c * npad + g] : real(0);
   }
+  const int k0 = sh_poff[s], k1 = sh_poff[s + 1];
+  real acc[3] = {0, 0, 0};
+  for (int k = k0; k < k1; ++k) {
+    const int q = sh_pidx[k];
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      acc[c] += part[c * npart + q];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c)
+    finish_loaded<real, MODE>(c * npad + g, n3, acc[c], mv[c], vv[c], uv[c], U,
+                              V, A, dt, dto2, dt2o2, rec, j, aux, sample);
 }
 
 // Seismograms of step `*step_ptr` (if sampled).  POST = false, before the
