@@ -986,15 +986,18 @@ __global__ void __launch_bounds__(NODE3_THREADS)
   if (s >= nsh)
     return;
   const int j = *adj_step % S.every;
-  real kr[3];
+  // the node's own state first (it does not wait on the partials), then the
+  // partials of the three components in tile order
+  AdjNode<real> nd;
+  adj_load(long(nex_total) + s, j, S, nd);
+  real kr[3] = {0, 0, 0};
+  for (int k = sh_poff[s]; k < sh_poff[s + 1]; ++k) {
+    const int q = sh_pidx[k];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    real acc = 0;
-    for (int k = sh_poff[s]; k < sh_poff[s + 1]; ++k)
-      acc += part[c * npart + sh_pidx[k]];
-    kr[c] = acc;
+    for (int c = 0; c < 3; ++c)
+      kr[c] += part[c * npart + q];
   }
-  adj_finish(long(nex_total) + s, kr, j, S);
+  adj_apply(long(nex_total) + s, kr, j, S, nd);
 }
 
 template <typename real>
