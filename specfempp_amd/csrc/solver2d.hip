@@ -563,24 +563,31 @@ __global__ void __launch_bounds__(NODE_THREADS)
   V2 *__restrict__ A2 = reinterpret_cast<V2 *>(A);
   const V2 *__restrict__ M2 = reinterpret_cast<const V2 *>(Minv);
   const int g = nex_total + j;
-  V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off, shp_pos, j);
+  // the point's own 1/M, v (or a) and u first: they do not wait on the
+  // partials
   const V2 m = M2[g];
+  V2 v, u, a0;
+  v.x = v.y = u.x = u.y = a0.x = a0.y = 0;
+  if (MODE == MODE_ASSEMBLE)
+    a0 = A2[g];
+  else
+    v = V2p[g];
+  if (MODE == MODE_STEP || MODE == MODE_RECORD)
+    u = U2[g];
+  V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off, shp_pos, j);
   if (MODE == MODE_ASSEMBLE) {
-    const V2 a0 = A2[g];
     V2 a;
     a.x = (a0.x + acc.x) * m.x;
     a.y = (a0.y + acc.y) * m.y;
     A2[g] = a;
     return;
   }
-  V2 v = V2p[g];
   divide_correct(acc.x, v.x, m.x, dto2);
   divide_correct(acc.y, v.y, m.y, dto2);
   if (MODE == MODE_STEP || MODE == MODE_RECORD) {
     const int jr = MODE == MODE_RECORD ? step % every : 0;
     if (MODE == MODE_RECORD)
       record_ptr<V2>(rec, rec_stride, jr, 2)[g] = acc;
-    V2 u = U2[g];
     predict(u.x, v.x, acc.x, dt, dto2, dt2o2);
     predict(u.y, v.y, acc.y, dt, dto2, dt2o2);
     U2[g] = u;
