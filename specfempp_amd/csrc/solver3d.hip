@@ -815,7 +815,10 @@ __global__ void __launch_bounds__(NODE3_THREADS)
 // accumulator; the tile's exclusive nodes are then updated in place
 // (adj_finish) and its shared nodes get partials.  LDS: accumulator
 // [3][nloc_max], then per round slot rb [3][125], u~ [3][125] and the
-// stress integrand [3][3][125].
+// stress integrand [3][3][125], then the global ids of the tile's shared
+// points [nsh_max] and its round slots [nround_max][RW] (int); each
+// thread's point id of the next round is requested one round ahead, so the
+// gather of a round waits on one global load.
 template <typename real, int GM, bool ME, int RW>
 __global__ void __launch_bounds__(RW * ELEM_THREADS)
     k3_adj_tile(const TileInfo3 *__restrict__ tiles,
@@ -825,7 +828,7 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
                 const float *__restrict__ nodes, Quad3<float> xi3,
                 const real *__restrict__ mat, const real *__restrict__ rec_u,
                 real *__restrict__ part, long npart, real *__restrict__ Gd,
-                real *__restrict__ Ge, int nloc_max, Quad3<real> q,
+                real *__restrict__ Ge, int nloc_max, int nsh_max, Quad3<real> q,
                 const int *__restrict__ adj_step, AdjState3<real> S) {
   extern __shared__ __align__(16) unsigned char lds_raw[];
   real *s_a = reinterpret_cast<real *>(lds_raw);
@@ -838,6 +841,9 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
   real *sr = s_a + 3 * size_t(nloc_max) + size_t(sub) * 15 * NPT3;
   real *su = sr + 3 * NPT3;
   real *sf = su + 3 * NPT3;
+  int *s_sh = reinterpret_cast<int *>(s_a + 3 * size_t(nloc_max) +
+                                      size_t(RW) * 15 * NPT3);
+  int *s_slot = s_sh + nsh_max;
   const long npad = S.npad;
   const int nloc = T.nex + T.nsh;
   const int j = *adj_step % S.every;
@@ -847,20 +853,35 @@ __global__ void __launch_bounds__(RW * ELEM_THREADS)
     s_a[nloc_max + i] = 0;
     s_a[2 * nloc_max + i] = 0;
   }
+  for (int i = tid; i < T.nsh; i += RW * ELEM_THREADS)
+    s_sh[i] = sh_list[T.sh_off + i];
+  for (int i = tid; i < T.nround * RW; i += RW * ELEM_THREADS)
+    s_slot[i] = round_slots[size_t(T.round_off) * RW + i];
   __syncthreads();
+  auto slot_of = [&](int r) { return r < T.nround ? s_slot[r * RW + sub] : -1; };
+  int l_next = 0;
+  {
+    const int s0 = slot_of(0);
+    if (pact && s0 >= 0)
+      l_next = lidx[size_t(s0) * NPT3 + p];
+  }
   for (int r = 0; r < T.nround; ++r) {
-    const int slot = round_slots[(size_t(T.round_off) + r) * RW + sub];
+    const int slot = slot_of(r);
     const bool act = pact && slot >= 0;
     int l = 0;
     if (act) {
-      l = lidx[size_t(slot) * NPT3 + p];
-      const long g = l < T.nex ? long(T.ex_off) + l
-                               : long(sh_list[T.sh_off + l - T.nex]);
+      l = l_next;
+      const long g = l < T.nex ? long(T.ex_off) + l : long(s_sh[l - T.nex]);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         sr[c * NPT3 + p] = S.rb[c * npad + g];
         su[c * NPT3 + p] = Ut[c * npad + g];
       }
+    }
+    {
+      const int s1 = slot_of(r + 1);
+      if (pact && s1 >= 0)
+        l_next = lidx[size_t(s1) * NPT3 + p];
     }
     __syncthreads();
     if (act) {
@@ -1080,8 +1101,19 @@ public:
                            "); use fewer elements per tile");
       adj_lds_ = (size_t(3) * std::max(1, A.nloc_max) +
                   size_t(kRoundWidth) * 15 * NPT3) *
-                 sizeof(real);
+                     sizeof(real) +
+                 (size_t(nsh_max_) + size_t(nround_max_) * kRoundWidth) * sizeof(int);
       adj_tiles_ = adj_lds_ <= prop.sharedMemPerBlock;
+      if (adj_tiles_)
+        for (const void *f :
+             {reinterpret_cast<const void *>(&k3_adj_tile<real, GEOM_ELEM, true, kRoundWidth>),
+              reinterpret_cast<const void *>(&k3_adj_tile<real, GEOM_ELEM, false, kRoundWidth>),
+              reinterpret_cast<const void *>(&k3_adj_tile<real, GEOM_NODES, true, kRoundWidth>),
+              reinterpret_cast<const void *>(&k3_adj_tile<real, GEOM_NODES, false, kRoundWidth>),
+              reinterpret_cast<const void *>(&k3_adj_tile<real, GEOM_POINT, true, kRoundWidth>),
+              reinterpret_cast<const void *>(&k3_adj_tile<real, GEOM_POINT, false, kRoundWidth>)})
+          SMX3_HIP(hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       int(adj_lds_)));
     }
     rec_g_ = upload<int>(A.rec_g);
     rec_h_ = upload<real>(A.rec_h);
@@ -1602,8 +1634,8 @@ private:
         kern<<<dim3(unsigned(A.ntiles)), dim3(kRoundWidth * ELEM_THREADS),
                adj_lds_, stream_>>>(tiles_, round_slots_, lidx_, sh_list_, geom_,
                                     nodes_, xi3_, mat_, rec_u_, part_, npart_,
-                                    gd_, ge_, std::max(1, A.nloc_max), quad_,
-                                    adj_step_, S);
+                                    gd_, ge_, std::max(1, A.nloc_max), nsh_max_,
+                                    quad_, adj_step_, S);
       };
       const bool me = A.mat_elem;
       switch (gmode_) {
