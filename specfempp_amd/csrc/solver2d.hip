@@ -1186,17 +1186,42 @@ __global__ void __launch_bounds__(TILE_THREADS)
     __syncthreads();
   }
 
-  // ---- exclusive points: adjoint node update; shared points: partials
-  for (int i = tid; i < T.nex; i += TILE_THREADS) {
-    const int g = T.ex_off + i;
-    const V2 r = s_r[i], a = At[g];
-    R[g] += r.x * a.x + r.y * a.y;
-    V2 ub = UB2[g], vb = VB2[g], rbn;
-    adj_finish<real>(s_a[i], r, ub, vb, M2[g], bidx[g], bsum, dt, dto2, dt2o2,
-                     rbn);
-    UB2[g] = ub;
-    VB2[g] = vb;
-    RBn[g] = rbn;
+  // ---- exclusive points: adjoint node update (the loads of up to kAdjFin
+  // points per thread requested before the first update); shared points:
+  // partials
+  constexpr int kAdjFin = 4;
+  for (int i0 = tid; i0 < T.nex; i0 += kAdjFin * TILE_THREADS) {
+    V2 a[kAdjFin], ub[kAdjFin], vb[kAdjFin], m[kAdjFin];
+    real rr[kAdjFin];
+    int bi[kAdjFin];
+#pragma unroll
+    for (int k = 0; k < kAdjFin; ++k) {
+      const int i = i0 + k * TILE_THREADS;
+      if (i < T.nex) {
+        const int g = T.ex_off + i;
+        a[k] = At[g];
+        rr[k] = R[g];
+        ub[k] = UB2[g];
+        vb[k] = VB2[g];
+        m[k] = M2[g];
+        bi[k] = bidx[g];
+      }
+    }
+#pragma unroll
+    for (int k = 0; k < kAdjFin; ++k) {
+      const int i = i0 + k * TILE_THREADS;
+      if (i < T.nex) {
+        const int g = T.ex_off + i;
+        const V2 r = s_r[i];
+        R[g] = rr[k] + (r.x * a[k].x + r.y * a[k].y);
+        V2 rbn;
+        adj_finish<real>(s_a[i], r, ub[k], vb[k], m[k], bi[k], bsum, dt, dto2,
+                         dt2o2, rbn);
+        UB2[g] = ub[k];
+        VB2[g] = vb[k];
+        RBn[g] = rbn;
+      }
+    }
   }
   for (int i = T.nex + tid; i < nloc; i += TILE_THREADS)
     P2[T.sh_off + i - T.nex] = s_a[i];
@@ -1235,14 +1260,17 @@ __global__ void __launch_bounds__(NODE_THREADS)
         reinterpret_cast<const V2 *>((step & 1) ? RB1 : RB0);
     V2 *__restrict__ RBn = reinterpret_cast<V2 *>((step & 1) ? RB0 : RB1);
     V2 *__restrict__ VB2 = reinterpret_cast<V2 *>(VB);
-    const V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off,
-                              shp_pos, t);
+    // the node's own state first: it does not wait on the partials
     const V2 r = RBc[g];
     const V2 a = record_ptr<V2>(rec, rec_stride, step % every, 2)[g];
-    R[g] += r.x * a.x + r.y * a.y;
+    const real r0 = R[g];
     V2 ub = UB2[g], vb = VB2[g], rbn;
-    adj_finish<real>(acc, r, ub, vb, reinterpret_cast<const V2 *>(Minv)[g],
-                     bidx[g], bsum, dt, dto2, dt2o2, rbn);
+    const V2 m = reinterpret_cast<const V2 *>(Minv)[g];
+    const int b = bidx[g];
+    const V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off,
+                              shp_pos, t);
+    R[g] = r0 + (r.x * a.x + r.y * a.y);
+    adj_finish<real>(acc, r, ub, vb, m, b, bsum, dt, dto2, dt2o2, rbn);
     const int j = sh_rin[t];
     if (j >= 0 && step > 0)
       adj_inject<real>(ub, j, rin_off, rin_ent, rent_h, gsrc, nrec, step - 1);
