@@ -364,13 +364,17 @@ __global__ void __launch_bounds__(AC_NODE_THREADS)
   if (j >= nshared)
     return;
   const int g = nex_total + j;
-  const real acc = ac_shared_sum(part, shp_off, shp_pos, j);
+  // the point's own 1/M, v (or a) and u first: they do not wait on the
+  // partials
   const real m = Minv[g];
+  const real a0 = MODE == AC_ASSEMBLE ? A[g] : real(0);
+  real v = MODE == AC_ASSEMBLE ? real(0) : V[g];
+  real u = MODE == AC_STEP ? U[g] : real(0);
+  const real acc = ac_shared_sum(part, shp_off, shp_pos, j);
   if (MODE == AC_ASSEMBLE) {
-    A[g] = (A[g] + acc) * m;
+    A[g] = (a0 + acc) * m;
     return;
   }
-  real v = V[g];
   real a = acc * m;
   v += dto2 * a;
   if (sh_iface) {
@@ -379,7 +383,6 @@ __global__ void __launch_bounds__(AC_NODE_THREADS)
       chi_dd[k] = a;
   }
   if (MODE == AC_STEP) {
-    real u = U[g];
     u += dt * v + dt2o2 * a;
     v += dto2 * a;
     U[g] = u;
