@@ -625,40 +625,49 @@ __global__ void k_seis_fused(const real *__restrict__ U,
   // 32 threads per (type, receiver): lane p < 25 evaluates point p, lane 0
   // then sums the 25 terms in point order.
   __shared__ real s_t[SEIS_GROUPS][NPT][2];
-  const int step = *step_ptr;
-  if (step % nsbs != 0)
-    return;
-  const int isig = step / nsbs;
   const int grp = threadIdx.x >> 5, p = threadIdx.x & 31;
   const int pair = blockIdx.x * SEIS_GROUPS + grp;
   const int r = pair % nrec, it = pair / nrec;
   // pressure types are written by k_seis_pressure
   const bool active = it < ntypes && p < NPT && types.t[it] != 3;
+  // the entry's own data is requested before the step counter is known
+  // (all in bounds for every active lane)
+  const int nentries = nrec * NPT;
+  const int id = r * NPT + p;
+  int j = -1, g = 0;
+  real h = 0, t0 = 0, t1 = 0;
+  if (active) {
+    j = rent_shared[id];
+    g = rent_g[id];
+    h = rent_h[id];
+    t0 = seis_terms[(size_t(it) * nentries + id) * 2];
+    t1 = seis_terms[(size_t(it) * nentries + id) * 2 + 1];
+  }
+  const int step = *step_ptr;
+  if (step % nsbs != 0)
+    return;
+  const int isig = step / nsbs;
   if (active) {
     const int type = types.t[it];
-    const int nentries = nrec * NPT;
-    const int id = r * NPT + p;
-    const int j = rent_shared[id];
     real tx, tz;
     if (j < 0) {
-      tx = seis_terms[(size_t(it) * nentries + id) * 2];
-      tz = seis_terms[(size_t(it) * nentries + id) * 2 + 1];
+      tx = t0;
+      tz = t1;
     } else {
-      const int g = rent_g[id];
       real fx, fz;
       if (type == 0) {
         fx = U[2 * size_t(g)];
         fz = U[2 * size_t(g) + 1];
       } else {
+        real vx = V[2 * size_t(g)], vz = V[2 * size_t(g) + 1];
+        const real m0 = Minv[2 * size_t(g)], m1 = Minv[2 * size_t(g) + 1];
         V2 acc = shared_sum(reinterpret_cast<const V2 *>(part), shp_off,
                             shp_pos, j);
-        real vx = V[2 * size_t(g)], vz = V[2 * size_t(g) + 1];
-        divide_correct(acc.x, vx, Minv[2 * size_t(g)], dto2);
-        divide_correct(acc.y, vz, Minv[2 * size_t(g) + 1], dto2);
+        divide_correct(acc.x, vx, m0, dto2);
+        divide_correct(acc.y, vz, m1, dto2);
         fx = type == 1 ? vx : acc.x;
         fz = type == 1 ? vz : acc.y;
       }
-      const real h = rent_h[id];
       tx = h * fx;
       tz = h * fz;
     }
