@@ -66,8 +66,10 @@ def candidates(configs, precision=4):
 
 
 def label(opt):
-    w = opt.get("waves_per_eu", 0) or 7
-    return f"tile{opt['tile_elems']}_graph{opt['graph_steps']}_wpe{w}"
+    """tile{t}_graph{g}_wpe{w}; "default" for the kernel's own occupancy
+    (waves_per_eu 0: 7 waves per SIMD in FP32, unconstrained in FP64)."""
+    w = opt.get("waves_per_eu", 0)
+    return f"tile{opt['tile_elems']}_graph{opt['graph_steps']}_" + (f"wpe{w}" if w else "default")
 
 
 def _context(configs, tile, precision, nstep):

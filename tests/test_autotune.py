@@ -19,7 +19,8 @@ def test_candidates_cover_every_layout():
 def test_fp64_has_the_default_variant_only():
     c = candidates([homogeneous_box(nx=8, nz=6, nstep=10)], precision=8)
     assert len(c) == 6 and {x["waves_per_eu"] for x in c} == {0}
-    assert label(c[0]) == "tile64_graph8_wpe7"
+    assert label(c[0]) == "tile64_graph8_default"
+    assert label({"tile_elems": 32, "graph_steps": 32, "waves_per_eu": 6}) == "tile32_graph32_wpe6"
 
 
 def test_signature_separates_workloads_and_precision():
