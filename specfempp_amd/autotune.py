@@ -53,7 +53,7 @@ def signature(configs, precision):
     for c in configs:
         h.update(f"{c.mesh.nspec}:{c.mesh.ngnod}:{len(c.receivers)}:{len(c.sources)}:"
                  f"{len(c.mesh.abs_ispec)};".encode())
-    h.update(f"p{precision}v2".encode())
+    h.update(f"p{precision}v3".encode())
     return h.hexdigest()[:16]
 
 
