@@ -1054,8 +1054,14 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
 
   // ---- receiver entries (receiver r, point p) -> entry r*NPT + p.  Entries
   // on a tile's exclusive node are evaluated by that tile; entries on shared
-  // nodes by the seismogram pass that runs before the shared-node update.
+  // nodes by the seismogram pass that runs before the shared-node update,
+  // except when an elastic mesh records displacement (and pressure) only:
+  // the tile that owns the receiver's element has every point's
+  // displacement in hand, so it evaluates all 25 entries.
   {
+    bool tile_all = !acoustic;
+    for (int t : A.seis_types)
+      tile_all = tile_all && (t == SMX_SEIS_DISPLACEMENT || t == SMX_SEIS_PRESSURE);
     const int nent = A.nrec * NPT;
     A.rent_g.assign(std::max(1, nent), 0);
     A.rent_shared.assign(std::max(1, nent), -1);
@@ -1068,7 +1074,7 @@ void assemble2d(const smx_mesh2d &m, const smx_sources2d &src,
         const int id = r * NPT + p;
         A.rent_g[id] = g;
         A.rent_h[id] = A.rec_h[size_t(r) * NPT + p];
-        if (g >= A.nex_total)
+        if (g >= A.nex_total && !tile_all)
           A.rent_shared[id] = g - A.nex_total;
         else
           ent.push_back({s / ELEM_TILE, A.lidx[size_t(p) * ld + s], id});
@@ -1499,7 +1505,7 @@ void merge_assemblies(std::vector<HostAssembly2D> &shots, HostAssembly2D &M) {
         M.rec_h[id] = S.rec_h[sid];
         const int g = remap(S.rent_g[sid]);
         M.rent_g[id] = g;
-        M.rent_shared[id] = g >= nex ? g - nex : -1;
+        M.rent_shared[id] = S.rent_shared[sid] >= 0 ? g - nex : -1;
         M.rent_h[id] = S.rent_h[sid];
       }
     }

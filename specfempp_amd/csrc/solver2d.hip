@@ -534,10 +534,51 @@ __global__ void __launch_bounds__(TILE_THREADS) __attribute__((amdgpu_waves_per_
   }
 }
 
+// Seismogram sums folded into k_shared's first workgroup when every term
+// was written by the tiles (displacement and pressure types only: the tiles
+// then evaluate the entries on shared points too, from their gathered
+// displacement).  on = 0: no such workgroup.
+template <typename real> struct SeisSum {
+  const real *terms; // [ntypes][nrec * 25][2]
+  real *out;         // [nsig][ntypes][nrec][2]
+  SeisTypes types;
+  int nrec, ntypes, nsbs, on;
+};
+
+// s = sum of the 25 terms of each displacement (type, receiver) in point
+// order from 0 (as k_seis_fused sums them) on sampled steps; then the step
+// counter advances, after every thread has read it.
+template <typename real>
+__device__ void seis_sum_block(const SeisSum<real> &ss, int *__restrict__ step_ptr) {
+  const int step = *step_ptr;
+  if (step % ss.nsbs == 0) {
+    const int isig = step / ss.nsbs, nent = ss.nrec * NPT;
+    for (int q = threadIdx.x; q < ss.nrec * ss.ntypes; q += blockDim.x) {
+      const int it = q / ss.nrec, r = q % ss.nrec;
+      if (ss.types.t[it] != 0) // pressure: k_seis_pressure
+        continue;
+      const real *t = ss.terms + (size_t(it) * nent + size_t(r) * NPT) * 2;
+      real sx = 0, sz = 0;
+#pragma unroll
+      for (int k = 0; k < NPT; ++k) {
+        sx += t[2 * k];
+        sz += t[2 * k + 1];
+      }
+      const size_t o = ((size_t(isig) * ss.ntypes + it) * ss.nrec + r) * 2;
+      ss.out[o] = sx;
+      ss.out[o + 1] = sz;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    *step_ptr = step + 1;
+}
+
 // ------------------------------------------------------------------------
 // Points shared between tiles: sum the tiles' partials in fixed order and
 // finish the step as k_tile does for exclusive points.  Thread 0 advances
-// the device step counter (fused modes).
+// the device step counter (fused modes), or the seismogram workgroup does
+// when there is one (block 0; the points then start at block 1).
 template <typename real, int MODE>
 __global__ void __launch_bounds__(NODE_THREADS)
     k_shared(real *__restrict__ U, real *__restrict__ V, real *__restrict__ A,
@@ -545,15 +586,24 @@ __global__ void __launch_bounds__(NODE_THREADS)
              const int *__restrict__ shp_off, const int *__restrict__ shp_pos,
              int nshared, int nex_total, real dt, real dto2, real dt2o2,
              int *__restrict__ step_ptr, void *__restrict__ rec,
-             long rec_stride, int every) {
+             long rec_stride, int every, SeisSum<real> ss) {
   using V2 = typename Vec2<real>::T;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  int blk = blockIdx.x;
+  if (MODE == MODE_STEP && ss.on) {
+    if (blk == 0) {
+      seis_sum_block(ss, step_ptr);
+      return;
+    }
+    --blk;
+  }
+  const int j = blk * blockDim.x + threadIdx.x;
   int step = 0;
   if (MODE == MODE_RECORD) {
     step = step_ptr[1];
     if (j == 0)
       step_ptr[0] = step + 1;
-  } else if (j == 0 && step_ptr && MODE != MODE_ASSEMBLE) {
+  } else if (j == 0 && step_ptr && MODE != MODE_ASSEMBLE &&
+             !(MODE == MODE_STEP && ss.on)) {
     *step_ptr += 1;
   }
   if (j >= nshared)
@@ -1616,6 +1666,11 @@ public:
     has_pressure_ = A.nrec > 0 &&
                     std::find(A.seis_types.begin(), A.seis_types.end(),
                               int(SMX_SEIS_PRESSURE)) != A.seis_types.end();
+    seis_in_tiles_ = A.nrec > 0;
+    for (int t : A.seis_types)
+      seis_in_tiles_ = seis_in_tiles_ && (t == 0 || t == int(SMX_SEIS_PRESSURE));
+    for (int id = 0; id < A.nrec * NPT && seis_in_tiles_; ++id)
+      seis_in_tiles_ = A.rent_shared[id] < 0;
     dt_ = A.dt;
     dto2_ = A.dt_over2;
     dt2o2_ = A.dt2_over2;
@@ -2312,17 +2367,24 @@ private:
         seis_);
     SMX_HIP(hipGetLastError());
   }
+  // seis: the step's seismogram sums in the first workgroup (MODE_STEP with
+  // a step counter; every term written by the tiles)
   template <int MODE>
   void launch_shared(real *U, real *V, real *Aout, int *step_ptr,
-                     real *rec = nullptr, real *part = nullptr) {
+                     real *rec = nullptr, real *part = nullptr,
+                     bool seis = false) {
     const HostAssembly2D &A = *asm_;
     const int n = std::max(1, A.nshared);
-    k_shared<real, MODE><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS),
+    SeisSum<real> ss{seis_terms_, seis_, stypes_, A.nrec,
+                     int(A.seis_types.size()), A.nsbs, 0};
+    if (MODE == MODE_STEP && seis && step_ptr)
+      ss.on = 1;
+    k_shared<real, MODE><<<dim3((n + NODE_THREADS - 1) / NODE_THREADS + ss.on),
                            dim3(NODE_THREADS), 0, stream_>>>(
         U, V, Aout, minv_, part ? part : part_, shp_off_, shp_pos_, A.nshared,
         A.nex_total,
         dt_, dto2_, dt2o2_, step_ptr, rec, rec_stride(),
-        std::max(1, rec_every_));
+        std::max(1, rec_every_), ss);
     SMX_HIP(hipGetLastError());
   }
   long rec_stride() const { return long(n2_ * sizeof(real)); }
@@ -2557,7 +2619,10 @@ private:
       launch_pressure(F.u, step_, 0);
     launch_tiles<MODE>(F.u, F.v, F.a, nullptr, 0, step_,
                        seis ? seis_terms_ : nullptr);
-    if (seis) {
+    // every term written by the tiles: the sums ride in k_shared's first
+    // workgroup on the time steps of the graph
+    const bool in_shared = seis && seis_in_tiles_ && MODE == MODE_STEP;
+    if (seis && !in_shared) {
       const int n = A.nrec * ntypes;
       k_seis_fused<real><<<dim3((n + SEIS_GROUPS - 1) / SEIS_GROUPS),
                            dim3(32 * SEIS_GROUPS), 0, stream_>>>(
@@ -2566,7 +2631,7 @@ private:
           seis_);
       SMX_HIP(hipGetLastError());
     }
-    launch_shared<MODE>(F.u, F.v, F.a, step_);
+    launch_shared<MODE>(F.u, F.v, F.a, step_, nullptr, nullptr, in_shared);
   }
   void build_graph(Field &F) {
     capture(graph_, graph_exec_, stream_, [&] {
@@ -2590,6 +2655,9 @@ private:
       *trec_entry_ = nullptr, *rent_g_ = nullptr, *rent_shared_ = nullptr,
       *step_ = nullptr, *rec_slot_ = nullptr;
   bool has_pressure_ = false; // elastic pressure seismograms requested
+  // every receiver term is written by the tiles (displacement and pressure
+  // types only, no entry left to the shared-point pass)
+  bool seis_in_tiles_ = false;
   // device-memory model: slot maps, and whether the host copy is stale
   int *elem_slot_ = nullptr, *bpt_p_ = nullptr, *bpt_bslot_ = nullptr,
       *bpt_slot_ = nullptr, *bnode_off_ = nullptr, *bnode_pts_ = nullptr,
