@@ -412,31 +412,41 @@ __global__ void k_ac_seis_fused(const real *__restrict__ U,
                                 const int *__restrict__ step_ptr, real dto2,
                                 real *__restrict__ out) {
   __shared__ real s_t[AC_SEIS_GROUPS][NPT][2];
-  const int step = *step_ptr;
-  if (step % nsbs != 0)
-    return;
-  const int isig = step / nsbs;
   const int grp = threadIdx.x >> 5, p = threadIdx.x & 31;
   const int pair = blockIdx.x * AC_SEIS_GROUPS + grp;
   const int r = pair % nrec, it = pair / nrec;
   const bool active = it < ntypes && p < NPT;
+  // the entry's own data is requested before the step counter is known
+  // (all in bounds for every active lane)
+  const int id = r * NPT + p;
+  int j = -1, g = 0;
+  real t0 = 0, t1 = 0;
+  if (active) {
+    j = rent_shared[id];
+    g = rent_g[id];
+    const size_t o = (size_t(it) * nrec * NPT + id) * 2;
+    t0 = seis_terms[o];
+    t1 = seis_terms[o + 1];
+  }
+  const int step = *step_ptr;
+  if (step % nsbs != 0)
+    return;
+  const int isig = step / nsbs;
   if (active) {
     const int type = types.t[it];
-    const int id = r * NPT + p;
-    const int j = rent_shared[id];
     real tx, tz;
     if (j < 0) {
-      const size_t o = (size_t(it) * nrec * NPT + id) * 2;
-      tx = seis_terms[o];
-      tz = seis_terms[o + 1];
+      tx = t0;
+      tz = t1;
     } else {
-      const int g = rent_g[id];
       real f;
       if (type == 0) {
         f = U[g];
       } else {
-        real a = ac_shared_sum(part, shp_off, shp_pos, j) * Minv[g];
+        // the point's own 1/M and v before its partial sums
+        const real m = Minv[g];
         real v = V[g];
+        real a = ac_shared_sum(part, shp_off, shp_pos, j) * m;
         v += dto2 * a;
         f = type == 1 ? v : a;
       }
