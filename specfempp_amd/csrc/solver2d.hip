@@ -653,10 +653,14 @@ __global__ void __launch_bounds__(NODE_THREADS)
 }
 
 // ------------------------------------------------------------------------
-// Fused-path seismograms, between the tile launch and k_shared: entries on
-// exclusive points were evaluated by their tile; entries on shared points
-// are evaluated here from the tile partials with the same divide / corrector
-// arithmetic.  Sum over the 25 points in fixed order.
+// Fused-path seismograms, between the tile launch and k_shared: entries the
+// tiles evaluated (those on exclusive points; all of them when only
+// displacement and pressure are recorded, rent_shared = -1) are read from
+// the terms; entries on shared points otherwise are evaluated here from the
+// tile partials with the same divide / corrector arithmetic.  Sum over the
+// 25 points in fixed order.  Runs on every step with velocity or
+// acceleration types, else on the last step of a call only (the other steps
+// sum in k_shared's first workgroup).
 template <typename real>
 __global__ void k_seis_fused(const real *__restrict__ U,
                              const real *__restrict__ V,
