@@ -453,12 +453,19 @@ __global__ void __launch_bounds__(NODE3_THREADS)
               const int *__restrict__ sh_pidx, const real *__restrict__ part,
               long npart, real *__restrict__ U, real *__restrict__ V,
               real *__restrict__ A, const real *__restrict__ Minv, long npad,
-              real dt, real dto2, real dt2o2, const int *__restrict__ step_ptr,
-              Rec3<real> rec, Aux3<real> aux) {
+              real dt, real dto2, real dt2o2, int *__restrict__ step_ptr,
+              Rec3<real> rec, Aux3<real> aux, int advance) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= nsh)
     return;
-  const int step = MODE == M3_ASSEMBLE ? 0 : *step_ptr;
+  // advance: this pass moves the step counter on (no thread of it reads the
+  // counter then: only the recording mode and the velocity / acceleration
+  // samples need the step)
+  const bool need_step =
+      MODE == M3_RECORD || (MODE != M3_ASSEMBLE && aux.v != nullptr);
+  const int step = need_step ? *step_ptr : 0;
+  if (advance && s == 0)
+    *step_ptr += 1;
   const bool sample = MODE != M3_ASSEMBLE && aux.v != nullptr && step % aux.nsbs == 0;
   const int j = MODE == M3_RECORD ? step % rec.every : 0;
   const long g = long(nex_total) + s;
@@ -1814,7 +1821,10 @@ private:
     SMX3_HIP(hipGetLastError());
   }
 
-  template <int MODE> void launch_shared(const Rec3<real> &rec) {
+  // advance: the shared-point pass also moves the step counter on (the
+  // caller checks that nothing in it reads the counter)
+  template <int MODE> void launch_shared(const Rec3<real> &rec,
+                                         bool advance = false) {
     const HostAssembly3D &A = *asm_;
     if (A.nsh_total == 0)
       return;
@@ -1823,7 +1833,7 @@ private:
                                           NODE3_THREADS)),
                             NODE3_THREADS, 0, stream_>>>(
         A.nsh_total, A.nex_total, sh_poff_, sh_pidx_, part_, npart_, U_, V_, A_,
-        minv_, npad_, dt_, dto2_, dt2o2_, step_, rec, aux);
+        minv_, npad_, dt_, dto2_, dt2o2_, step_, rec, aux, int(advance));
     SMX3_HIP(hipGetLastError());
   }
 
@@ -1875,12 +1885,17 @@ private:
   // pass, the shared points, velocity / acceleration seismograms, and the
   // step counter.
   template <bool FINAL> void step() {
+    // without velocity / acceleration samples nothing after the shared-point
+    // pass reads the step counter, so that pass moves it on
+    const bool advance = !aux_v_ && asm_->nsh_total > 0;
     launch_seis<false>();
     launch_tile<FINAL ? M3_FINAL : M3_STEP>(A_, Rec3<real>{});
-    launch_shared<FINAL ? M3_FINAL : M3_STEP>(Rec3<real>{});
+    launch_shared<FINAL ? M3_FINAL : M3_STEP>(Rec3<real>{}, advance);
     launch_seis<true>();
-    k3_step_add<<<1, 64, 0, stream_>>>(step_, 1);
-    SMX3_HIP(hipGetLastError());
+    if (!advance) {
+      k3_step_add<<<1, 64, 0, stream_>>>(step_, 1);
+      SMX3_HIP(hipGetLastError());
+    }
   }
 
   void build_graph() {
